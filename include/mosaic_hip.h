@@ -1,0 +1,238 @@
+/*
+ * mosaic_hip.h -- C ABI of the MI355X (gfx950) tile-matching path of
+ * mosaic-art-rust.
+ *
+ * Every entry point returns an int status (MH_OK = 0, negative on error;
+ * mh_last_error() gives a thread-local message).  Host pointers are
+ * caller-owned; device memory is owned by the context / library handles.
+ * One in-flight call per context.  Material identity is an index
+ * 0..n-1 (the Rust side keeps its PathBuf <-> index maps, as
+ * SimilarityDatabase::path_to_index does, src/similarity.rs:33-36).
+ * Grids are row-major int32 cells, grid[y*grid_w + x], MH_EMPTY = -1 for
+ * None (the reference's Vec<Vec<Option<PathBuf>>>).
+ *
+ * Reference interfaces replaced (paths relative to the mosaic-art-rust
+ * repository):
+ *   mh_library_*              SimilarityDatabase (src/similarity.rs:30-137)
+ *                             + the kd-tree built at src/main.rs:145-151
+ *   mh_average_lab,
+ *   mh_cell_lab_means         MosaicGeneratorImpl::calculate_average_lab
+ *                             (src/lib.rs:94-119) per cell (src/main.rs:536-541)
+ *   mh_nearest_n              kiddo nearest_n::<SquaredEuclidean>
+ *                             (src/main.rs:348-352)
+ *   mh_place                  find_and_use_best_tile_with_position +
+ *                             fallback_tile_selection + UsageTracker
+ *                             (src/main.rs:309-475, src/lib.rs:57-89)
+ *   mh_total_cost             AdjacencyPenaltyCalculator::calculate_total_cost
+ *                             (src/adjacency.rs:91-130)
+ *   mh_penalties              AdjacencyPenaltyCalculator::calculate_penalty
+ *                             (src/adjacency.rs:63-88)
+ *   mh_swap_deltas            AdjacencyPenaltyCalculator::calculate_swap_delta
+ *                             (src/adjacency.rs:133-214)
+ *   mh_anneal                 MosaicOptimizer::optimize / optimize_greedy
+ *                             (src/optimizer.rs:41-198)
+ *   mh_average_rgb            MosaicGenerator::calculate_average_rgb
+ *                             (src/main.rs:687-708)
+ *   mh_optimal_adjustments    calculate_optimal_adjustment
+ *                             (src/color_adjustment.rs:99-154)
+ *   mh_adjust_pixels          ColorAdjustment::apply_to_image / adjust_pixel
+ *                             (src/color_adjustment.rs:40-96)
+ *   mh_resize                 MosaicGenerator::resize_image (src/main.rs:710-738)
+ *   mh_composite              tile compositing (src/main.rs:548-590, :625-664)
+ *   mh_generate               MosaicGenerator::generate_mosaic (src/main.rs:477-685)
+ */
+#ifndef MOSAIC_HIP_H
+#define MOSAIC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MH_OK 0
+#define MH_ERR_INVALID_ARGUMENT (-1)
+#define MH_ERR_NO_DEVICE (-2)
+#define MH_ERR_HIP (-3)
+#define MH_ERR_OUT_OF_MEMORY (-4)
+#define MH_ERR_NOT_FOUND (-5)
+#define MH_ERR_UNSUPPORTED (-6)
+
+#define MH_EMPTY (-1)
+
+/* Largest candidate list the placement keeps per cell: K = min(n, 100),
+ * src/main.rs:348. */
+#define MH_MAX_CANDIDATES 100
+
+typedef struct mh_context mh_context;
+typedef struct mh_library mh_library;
+
+/* ---------------------------------------------------------------- errors */
+const char *mh_last_error(void);
+int32_t mh_abi_version(void);
+
+/* --------------------------------------------------------------- context */
+/* Binds a HIP device and a stream; owns grow-only scratch buffers. */
+int mh_context_create(int32_t device, mh_context **out);
+void mh_context_destroy(mh_context *ctx);
+int mh_context_synchronize(mh_context *ctx);
+/* The context's hipStream_t (as void*), for callers that order their own
+ * work against it. */
+void *mh_context_stream(mh_context *ctx);
+
+/* --------------------------------------------------------------- library */
+/* SimilarityDatabase::new + add_tile x n (src/similarity.rs:51-66): the Lab
+ * table (n x {l,a,b} f32) becomes device resident. n may be 0. */
+int mh_library_from_lab(mh_context *ctx, const float *lab, int32_t n, mh_library **out);
+/* Material ingest (src/main.rs:263-301): n RGB8 images of w x h
+ * (n*h*w*3 bytes, row-major); Lab = calculate_average_lab of each, computed
+ * on the GPU.  The pixels stay resident for compositing. */
+int mh_library_from_pixels(mh_context *ctx, const uint8_t *rgb, int32_t n, int32_t w, int32_t h,
+                           mh_library **out);
+void mh_library_destroy(mh_library *lib);
+int32_t mh_library_size(const mh_library *lib);
+/* SimilarityDatabase::get_lab_color for every material (n x 3 f32). */
+int mh_library_lab(const mh_library *lib, float *out);
+/* build_similarities (src/similarity.rs:69-82): the n x n f32 matrix,
+ * resident in HBM. */
+int mh_library_build_similarities(mh_library *lib);
+/* Read the matrix back: packed = 0 -> full n x n row-major; packed = 1 ->
+ * the reference's upper-triangular Vec<f32>, n(n-1)/2 entries,
+ * position i*n - i*(i+1)/2 + j - i - 1 (src/similarity.rs:100). */
+int mh_library_similarities(mh_library *lib, float *out, int32_t packed);
+/* get_similarity by index (src/similarity.rs:85-103): 0.0 on the diagonal;
+ * MH_ERR_NOT_FOUND when an index is outside 0..n-1 (the reference's None). */
+int mh_library_get_similarity(mh_library *lib, int32_t i, int32_t j, float *out);
+
+/* -------------------------------------------------------------- a1: Lab */
+int mh_average_lab(mh_context *ctx, const uint8_t *rgb, int32_t w, int32_t h, float out[3]);
+/* Average Lab of every cell of a width x height RGB8 target split into
+ * grid_w x grid_h cells of (width/grid_w) x (height/grid_h) pixels;
+ * out = grid_w*grid_h x 3, raster order. */
+int mh_cell_lab_means(mh_context *ctx, const uint8_t *rgb, int32_t width, int32_t height,
+                      int32_t grid_w, int32_t grid_h, float *out);
+
+/* --------------------------------------------------- a3: nearest materials */
+/* For each of nq query Labs, the k = min(k, n) nearest materials by
+ * squared Lab distance, ascending; exact ties ordered by material index.
+ * idx/sqdist: nq x k. */
+int mh_nearest_n(mh_context *ctx, const mh_library *lib, const float *query, int32_t nq, int32_t k,
+                 int32_t *idx, float *sqdist);
+
+/* ------------------------------------------------------ a4-a9: placement */
+typedef struct {
+    int32_t grid_w, grid_h;
+    int32_t max_usage;   /* UsageTracker::new(max_usage_per_image)          */
+    float weight;        /* adjacency_penalty_weight; <= 0 disables penalty */
+} mh_place_params;
+
+typedef struct {
+    int32_t placed, main_path, fallback_ban, fallback_near, resets, empty;
+} mh_place_stats;
+
+/* Places cells in `order` (n_order cell indices y*grid_w+x; NULL = raster
+ * order over the whole grid, as generate_mosaic does) using cell_lab
+ * (grid_w*grid_h x 3).  grid (grid_w*grid_h) and usage (n) are in/out
+ * state; a fresh mosaic passes an all-MH_EMPTY grid and zero usage.
+ * Optional outputs per cell: stage (0 scored scan, 1 fallback
+ * first-placeable, 2 nearest_one, 255 untouched), dist (chosen candidate's
+ * squared distance), score (its total score). */
+int mh_place(mh_context *ctx, const mh_library *lib, const float *cell_lab,
+             const mh_place_params *params, const int32_t *order, int32_t n_order, int32_t *grid,
+             uint32_t *usage, uint8_t *stage, float *dist, float *score, mh_place_stats *stats);
+
+/* ----------------------------------------------- a10-a12: cost, SA */
+int mh_total_cost(mh_context *ctx, const mh_library *lib, const int32_t *grid, int32_t grid_w,
+                  int32_t grid_h, float *out);
+/* calculate_penalty for q (candidate material, x, y) queries against grid;
+ * queries = q x 3 int32 {material, x, y}; out = q f32 (already weighted). */
+int mh_penalties(mh_context *ctx, const mh_library *lib, const int32_t *grid, int32_t grid_w,
+                 int32_t grid_h, float weight, const int32_t *queries, int32_t q, float *out);
+/* calculate_swap_delta for p proposals {x1, y1, x2, y2} (p x 4 int32). */
+int mh_swap_deltas(mh_context *ctx, const mh_library *lib, const int32_t *grid, int32_t grid_w,
+                   int32_t grid_h, const int32_t *pairs, int32_t p, float *out);
+
+typedef struct {
+    int64_t max_iterations;     /* OptimizationConfig::max_iterations        */
+    float initial_temperature;  /* default 100.0                             */
+    float temperature_decay;    /* default 0.99995                           */
+    uint64_t seed;              /* counter-RNG key                           */
+    int32_t greedy;             /* 1 = optimize_greedy                       */
+} mh_anneal_params;
+
+typedef struct {
+    float initial_cost, final_cost, best_cost;
+    int64_t improved_count, accepted_count, iterations;
+    int64_t evaluated;  /* proposals that reached the delta (not skipped)   */
+    int64_t rounds;     /* speculative rounds the device used               */
+} mh_anneal_result;
+
+/* Simulated annealing on grid (in/out).  Proposal it draws x1,y1,x2,y2 and
+ * u from a counter RNG keyed by (seed, it); the device evaluates many
+ * proposals per round and commits them in order up to the first one whose
+ * neighbourhood an earlier accepted swap of the round touched, so the
+ * result equals the one-at-a-time chain bit for bit.  Optional traces
+ * (max_iterations entries): delta (NaN when skipped), accept flag. */
+int mh_anneal(mh_context *ctx, const mh_library *lib, int32_t *grid, int32_t grid_w,
+              int32_t grid_h, const mh_anneal_params *params, mh_anneal_result *result,
+              float *trace_delta, uint8_t *trace_accept);
+
+/* ----------------------------------- a13-a16: colour adjustment, compositing */
+int mh_average_rgb(mh_context *ctx, const uint8_t *rgb, int32_t w, int32_t h, uint8_t out[3]);
+/* calculate_optimal_adjustment for n (tile, target) RGB8 pairs; out = n x 4
+ * {brightness, contrast, hue_shift, saturation} after ColorAdjustment::new's
+ * clamps.  strength is used as given (the generator clamps it to [0,1]). */
+int mh_optimal_adjustments(mh_context *ctx, const uint8_t *tile_rgb, const uint8_t *target_rgb,
+                           int32_t n, float strength, float *out);
+/* ColorAdjustment::apply_to_image over npx RGB8 pixels with adj[4]. */
+int mh_adjust_pixels(mh_context *ctx, const float adj[4], const uint8_t *in, int64_t npx,
+                     uint8_t *out);
+/* Lanczos3 resize of one RGB8 image sw x sh -> dw x dh. */
+int mh_resize(mh_context *ctx, const uint8_t *src, int32_t sw, int32_t sh, uint8_t *dst, int32_t dw,
+              int32_t dh);
+/* Composite a placed grid: lib must hold pixels; out is
+ * (grid_w*tw) x (grid_h*th) x 3 with tw = width/grid_w, th = height/grid_h.
+ * strength > 0 applies the per-cell colour adjustment. */
+int mh_composite(mh_context *ctx, const mh_library *lib, const uint8_t *target, int32_t width,
+                 int32_t height, const int32_t *grid, int32_t grid_w, int32_t grid_h,
+                 float strength, uint8_t *out);
+
+/* ------------------------------------------------------ whole mosaic job */
+typedef struct {
+    int32_t grid_w, grid_h;
+    int32_t max_usage;
+    float weight;                   /* adjacency_penalty_weight              */
+    float color_strength;           /* clamped to [0,1] (src/main.rs:162)    */
+    int32_t enable_optimization;    /* SA runs when set and weight > 0       */
+    int64_t optimization_iterations;
+    float initial_temperature, temperature_decay;
+    uint64_t seed;
+} mh_job_params;
+
+typedef struct {
+    mh_place_stats place;
+    mh_anneal_result anneal;
+    int32_t out_width, out_height;
+} mh_job_result;
+
+/* generate_mosaic minus file I/O: cell Lab means -> candidates ->
+ * placement -> (SA) -> composite.  grid_out (grid_w*grid_h) and image_out
+ * (out_width*out_height*3, out_width = grid_w*(width/grid_w)) are optional. */
+int mh_generate(mh_context *ctx, const mh_library *lib, const uint8_t *target, int32_t width,
+                int32_t height, const mh_job_params *params, int32_t *grid_out, uint8_t *image_out,
+                mh_job_result *result);
+
+/* Device-resident form of mh_generate: target, grid_out and image_out are
+ * device pointers; work is enqueued on the context stream and the call
+ * returns without synchronising.  mh_job_result (if non-NULL) is filled by
+ * mh_generate_device_result after the stream is synchronised. */
+int mh_generate_device(mh_context *ctx, const mh_library *lib, const uint8_t *d_target,
+                       int32_t width, int32_t height, const mh_job_params *params,
+                       int32_t *d_grid_out, uint8_t *d_image_out);
+int mh_generate_device_result(mh_context *ctx, mh_job_result *result);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

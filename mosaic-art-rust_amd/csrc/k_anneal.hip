@@ -1,0 +1,374 @@
+// k_anneal.hip -- adjacency cost, swap deltas and simulated annealing
+// (AdjacencyPenaltyCalculator::calculate_total_cost / calculate_swap_delta /
+// calculate_penalty, src/adjacency.rs:63-214; MosaicOptimizer::optimize and
+// optimize_greedy, src/optimizer.rs:41-198).
+//
+// Annealing keeps the reference's one-proposal-at-a-time semantics exactly
+// while evaluating 64 proposals per round on one wave:
+//   * proposal `it` draws x1,y1,x2,y2,u from a counter RNG keyed by (seed,it),
+//     so any proposal can be evaluated out of order;
+//   * skipped proposals (same cell / empty cell) do not cool, and emptiness
+//     never changes under swaps, so proposal j's temperature index is the
+//     number of non-skipped proposals before it (a ballot prefix count) and
+//     T comes from a table built by the sequential f32 recurrence T *= decay;
+//   * every lane computes its delta on the round-start grid; accepted lanes
+//     stamp the two cells they would write (LDS atomicMax of
+//     round<<6 | 63-lane, i.e. the lowest writer wins); a lane whose read set
+//     (both cells and their 4-neighbours) holds a stamp from a lower lane is a
+//     conflict.  Lanes before the first conflict read exactly what the serial
+//     chain would have read, so they commit; the rest are re-drawn next round.
+//   * the running cost is accumulated in proposal order (f32, as the
+//     reference's `current_cost += delta`).
+// Result: grid, costs and counts are bit-identical to the serial chain.
+#include "mh_device.hpp"
+#include "mh_kernels.hpp"
+
+namespace mh {
+
+// -------------------------------------------------------------- helpers
+template <typename G>
+__device__ __forceinline__ int32_t gget(const G* g, int32_t i) {
+    return (int32_t)g[i];
+}
+
+// calculate_swap_delta (src/adjacency.rs:133-214) on a grid in any address
+// space.  Term order: neighbours of p1 (Up, Down, Left, Right, skipping p2),
+// then of p2 (skipping p1), then the mutual term when adjacent.
+template <typename G>
+__device__ float swap_delta_dev(const G* grid, int32_t gw, int32_t gh,
+                                const float4* __restrict__ lab4, int32_t x1, int32_t y1,
+                                int32_t x2, int32_t y2) {
+    const int32_t m1 = gget(grid, y1 * gw + x1), m2 = gget(grid, y2 * gw + x2);
+    if (m1 == kEmpty || m2 == kEmpty) return 0.0f;
+    if (m1 == m2) return 0.0f;
+    const float4 l1 = lab4[m1], l2 = lab4[m2];
+    float old_cost = 0.0f, new_cost = 0.0f;
+    bool is_adj = false;
+    int32_t ax[4], ay[4], cnt = 0;
+    if (y1 > 0) { ax[cnt] = x1; ay[cnt] = y1 - 1; ++cnt; }
+    if (y1 < gh - 1) { ax[cnt] = x1; ay[cnt] = y1 + 1; ++cnt; }
+    if (x1 > 0) { ax[cnt] = x1 - 1; ay[cnt] = y1; ++cnt; }
+    if (x1 < gw - 1) { ax[cnt] = x1 + 1; ay[cnt] = y1; ++cnt; }
+    for (int32_t i = 0; i < cnt; ++i) {
+        if (ax[i] == x2 && ay[i] == y2) {
+            is_adj = true;
+            continue;
+        }
+        const int32_t a = gget(grid, ay[i] * gw + ax[i]);
+        if (a == kEmpty) continue;
+        const float4 la = lab4[a];
+        old_cost += adj_term(l1, la);
+        new_cost += adj_term(l2, la);
+    }
+    cnt = 0;
+    if (y2 > 0) { ax[cnt] = x2; ay[cnt] = y2 - 1; ++cnt; }
+    if (y2 < gh - 1) { ax[cnt] = x2; ay[cnt] = y2 + 1; ++cnt; }
+    if (x2 > 0) { ax[cnt] = x2 - 1; ay[cnt] = y2; ++cnt; }
+    if (x2 < gw - 1) { ax[cnt] = x2 + 1; ay[cnt] = y2; ++cnt; }
+    for (int32_t i = 0; i < cnt; ++i) {
+        if (ax[i] == x1 && ay[i] == y1) continue;
+        const int32_t a = gget(grid, ay[i] * gw + ax[i]);
+        if (a == kEmpty) continue;
+        const float4 la = lab4[a];
+        old_cost += adj_term(l2, la);
+        new_cost += adj_term(l1, la);
+    }
+    if (is_adj) {
+        const float t = adj_term(l1, l2);
+        old_cost += t;
+        new_cost += t;
+    }
+    return new_cost - old_cost;
+}
+
+// calculate_total_cost (src/adjacency.rs:91-130) by one wave: the terms of
+// `chunk` cells at a time are computed in parallel into scratch (2 per cell,
+// -1 = absent), then lane 0 adds them in the reference's order (raster,
+// right before down).  The result is broadcast to every lane.
+template <typename G>
+__device__ float total_cost_wave(const G* grid, int32_t gw, int32_t gh,
+                                 const float4* __restrict__ lab4, float* scratch, int32_t chunk) {
+    const int lane = threadIdx.x;
+    const int32_t cells = gw * gh;
+    float total = 0.0f;
+    for (int32_t base = 0; base < cells; base += chunk) {
+        const int32_t len = min(chunk, cells - base);
+        for (int32_t i = lane; i < len; i += 64) {
+            const int32_t c = base + i;
+            const int32_t x = c % gw, y = c / gw;
+            const int32_t m = gget(grid, c);
+            float tr = -1.0f, td = -1.0f;
+            if (m != kEmpty) {
+                const float4 lm = lab4[m];
+                if (x < gw - 1) {
+                    const int32_t r = gget(grid, c + 1);
+                    if (r != kEmpty) tr = adj_term(lm, lab4[r]);
+                }
+                if (y < gh - 1) {
+                    const int32_t d = gget(grid, c + gw);
+                    if (d != kEmpty) td = adj_term(lm, lab4[d]);
+                }
+            }
+            scratch[2 * i] = tr;
+            scratch[2 * i + 1] = td;
+        }
+        __syncthreads();
+        if (lane == 0) {
+            for (int32_t i = 0; i < 2 * len; ++i) {
+                const float t = scratch[i];
+                if (t >= 0.0f) total = total + t;
+            }
+        }
+        __syncthreads();
+    }
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(total)));
+}
+
+// ---------------------------------------------------------- annealing
+// Stamp words per job: one per cell, at least 2, even (so the cost scratch
+// of 2 floats per cell always fits a non-empty chunk).
+__host__ __device__ __forceinline__ int32_t stamp_words(int32_t cells) {
+    return (cells + 3) & ~1;
+}
+
+template <typename G>
+__global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
+                                                    const AnnealJob* __restrict__ jobs) {
+    extern __shared__ uint32_t sm[];
+    const AnnealJob job = jobs ? jobs[blockIdx.x] : single;
+    const int lane = threadIdx.x;
+    const int32_t gw = job.grid_w, gh = job.grid_h;
+    const int32_t cells = gw * gh;
+    const int32_t stamps = stamp_words(cells);
+    uint32_t* stamp = sm;
+    G* grid_s = reinterpret_cast<G*>(sm + stamps);
+
+    for (int32_t i = lane; i < cells; i += 64) grid_s[i] = (G)job.grid[i];
+    __syncthreads();
+
+    // the stamp words double as the cost scratch: 2 terms per cell, stamps/2 cells a chunk
+    float cost = total_cost_wave(grid_s, gw, gh, job.lab4, reinterpret_cast<float*>(stamp),
+                                 stamps / 2);
+    for (int32_t i = lane; i < cells; i += 64) stamp[i] = 0u;
+    __syncthreads();
+
+    const float initial = cost;
+    float best = cost;
+    int64_t improved = 0, accepted = 0, evaluated = 0, rounds = 0;
+    int64_t s = 0, kbase = 0;
+    uint32_t tag = 1;
+    const int64_t max_iter = job.max_iter;
+    const uint64_t lanes_below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+
+    while (s < max_iter) {
+        const int64_t it = s + lane;
+        const bool active = it < max_iter;
+        int32_t x1 = 0, y1 = 0, x2 = 0, y2 = 0;
+        bool skip = true;
+        if (active) {
+            x1 = draw_range(job.seed, (uint64_t)it, 0, gw);
+            y1 = draw_range(job.seed, (uint64_t)it, 1, gh);
+            x2 = draw_range(job.seed, (uint64_t)it, 2, gw);
+            y2 = draw_range(job.seed, (uint64_t)it, 3, gh);
+            skip = (x1 == x2 && y1 == y2) || gget(grid_s, y1 * gw + x1) == kEmpty ||
+                   gget(grid_s, y2 * gw + x2) == kEmpty;
+        }
+        const uint64_t nonskip = __ballot(!skip);
+        float delta = 0.0f;
+        bool acc = false;
+        if (!skip) {
+            delta = swap_delta_dev(grid_s, gw, gh, job.lab4, x1, y1, x2, y2);
+            if (job.greedy) {
+                acc = delta < 0.0f;
+            } else if (delta < 0.0f) {
+                acc = true;
+            } else {
+                const int64_t k = kbase + __popcll(nonskip & lanes_below);
+                const float prob = exp_f32(-delta / job.temps[k]);
+                acc = draw_unit(job.seed, (uint64_t)it) < prob;
+            }
+        }
+        const int32_t p1 = y1 * gw + x1, p2 = y2 * gw + x2;
+        const uint32_t mine = (tag << 6) | (uint32_t)(63 - lane);
+        if (acc) {
+            atomicMax(&stamp[p1], mine);
+            atomicMax(&stamp[p2], mine);
+        }
+        __syncthreads();
+        bool conflict = false;
+        if (!skip) {
+            int32_t q[10];
+            int32_t nq = 0;
+            q[nq++] = p1;
+            q[nq++] = p2;
+            if (y1 > 0) q[nq++] = p1 - gw;
+            if (y1 < gh - 1) q[nq++] = p1 + gw;
+            if (x1 > 0) q[nq++] = p1 - 1;
+            if (x1 < gw - 1) q[nq++] = p1 + 1;
+            if (y2 > 0) q[nq++] = p2 - gw;
+            if (y2 < gh - 1) q[nq++] = p2 + gw;
+            if (x2 > 0) q[nq++] = p2 - 1;
+            if (x2 < gw - 1) q[nq++] = p2 + 1;
+            for (int32_t i = 0; i < nq; ++i) {
+                const uint32_t v = stamp[q[i]];
+                if ((v >> 6) == tag && (int)(63 - (v & 63u)) < lane) conflict = true;
+            }
+        }
+        const uint64_t cmask = __ballot(conflict);
+        int32_t first = cmask ? (__ffsll((unsigned long long)cmask) - 1) : 64;
+        const int64_t remain = max_iter - s;
+        if ((int64_t)first > remain) first = (int32_t)remain;
+        const uint64_t commit = first >= 64 ? ~0ull : ((1ull << first) - 1ull);
+        const uint64_t accmask = __ballot(acc) & commit;
+        __syncthreads();
+        if (lane < first) {
+            if (acc) {
+                const G a = grid_s[p1];
+                grid_s[p1] = grid_s[p2];
+                grid_s[p2] = a;
+            }
+            if (job.trace_delta) job.trace_delta[it] = skip ? __int_as_float(0x7fc00000) : delta;
+            if (job.trace_accept) job.trace_accept[it] = acc ? 1 : 0;
+        }
+        // running cost in proposal order; every lane runs the same uniform
+        // loop, so cost/best/improved stay identical across the wave
+        {
+            uint64_t m = accmask;
+            while (m) {
+                const int j = __ffsll((unsigned long long)m) - 1;
+                m &= m - 1;
+                const float dj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(delta), j));
+                cost = cost + dj;
+                if (job.greedy) {
+                    ++improved;
+                } else if (cost < best) {
+                    best = cost;
+                    ++improved;
+                }
+            }
+        }
+        accepted += __popcll(accmask);
+        const int64_t ev = __popcll(nonskip & commit);
+        evaluated += ev;
+        kbase += ev;
+        s += first;
+        ++rounds;
+        ++tag;
+        if (tag >= (1u << 26)) {
+            __syncthreads();
+            for (int32_t i = lane; i < cells; i += 64) stamp[i] = 0u;
+            tag = 1;
+        }
+        __syncthreads();
+    }
+
+    for (int32_t i = lane; i < cells; i += 64) job.grid[i] = (int32_t)grid_s[i];
+    if (lane == 0 && job.result) {
+        AnnealOut r;
+        r.initial_cost = initial;
+        r.final_cost = cost;
+        r.best_cost = job.greedy ? cost : best;
+        r.improved = improved;
+        r.accepted = accepted;
+        r.iterations = max_iter;
+        r.evaluated = evaluated;
+        r.rounds = rounds;
+        *job.result = r;
+    }
+}
+
+size_t anneal_lds_bytes(int32_t cells, bool narrow) {
+    return (size_t)stamp_words(cells) * sizeof(uint32_t) +
+           (size_t)cells * (narrow ? sizeof(int16_t) : sizeof(int32_t));
+}
+
+void launch_anneal_single(const AnnealJob& job, bool narrow, size_t lds_bytes, hipStream_t stream) {
+    if (narrow)
+        hipLaunchKernelGGL(anneal_kernel<int16_t>, dim3(1), dim3(64), lds_bytes, stream, job,
+                           (const AnnealJob*)nullptr);
+    else
+        hipLaunchKernelGGL(anneal_kernel<int32_t>, dim3(1), dim3(64), lds_bytes, stream, job,
+                           (const AnnealJob*)nullptr);
+}
+
+void launch_anneal_batch(const AnnealJob* d_jobs, int32_t n_jobs, bool narrow, size_t lds_bytes,
+                         hipStream_t stream) {
+    if (n_jobs <= 0) return;
+    AnnealJob unused{};
+    if (narrow)
+        hipLaunchKernelGGL(anneal_kernel<int16_t>, dim3(n_jobs), dim3(64), lds_bytes, stream, unused,
+                           d_jobs);
+    else
+        hipLaunchKernelGGL(anneal_kernel<int32_t>, dim3(n_jobs), dim3(64), lds_bytes, stream, unused,
+                           d_jobs);
+}
+
+hipError_t anneal_configure(int max_lds) {
+    hipError_t e = raise_dynamic_lds(reinterpret_cast<const void*>(&anneal_kernel<int16_t>), max_lds);
+    if (e != hipSuccess) return e;
+    return raise_dynamic_lds(reinterpret_cast<const void*>(&anneal_kernel<int32_t>), max_lds);
+}
+
+// ------------------------------------------------ cost / delta / penalty
+__global__ __launch_bounds__(64) void total_cost_kernel(const int32_t* __restrict__ grid,
+                                                        int32_t gw, int32_t gh,
+                                                        const float4* __restrict__ lab4,
+                                                        float* __restrict__ scratch, int32_t chunk,
+                                                        float* __restrict__ out) {
+    const float t = total_cost_wave(grid, gw, gh, lab4, scratch, chunk);
+    if (threadIdx.x == 0) *out = t;
+}
+
+void launch_total_cost(const int32_t* grid, int32_t gw, int32_t gh, const float4* lab4,
+                       float* scratch, int32_t chunk, float* out, hipStream_t stream) {
+    hipLaunchKernelGGL(total_cost_kernel, dim3(1), dim3(64), 0, stream, grid, gw, gh, lab4, scratch,
+                       chunk, out);
+}
+
+__global__ void swap_delta_kernel(const int32_t* __restrict__ grid, int32_t gw, int32_t gh,
+                                  const float4* __restrict__ lab4, const int32_t* __restrict__ pairs,
+                                  int32_t p, float* __restrict__ out) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p) return;
+    const int32_t* q = pairs + 4 * (int64_t)i;
+    out[i] = swap_delta_dev(grid, gw, gh, lab4, q[0], q[1], q[2], q[3]);
+}
+
+void launch_swap_deltas(const int32_t* grid, int32_t gw, int32_t gh, const float4* lab4,
+                        const int32_t* pairs, int32_t p, float* out, hipStream_t stream) {
+    if (p <= 0) return;
+    hipLaunchKernelGGL(swap_delta_kernel, dim3((p + 255) / 256), dim3(256), 0, stream, grid, gw, gh,
+                       lab4, pairs, p, out);
+}
+
+// calculate_penalty (src/adjacency.rs:63-88): Up, Down, Left, Right.
+__global__ void penalty_kernel(const int32_t* __restrict__ grid, int32_t gw, int32_t gh,
+                               const float4* __restrict__ lab4, float weight,
+                               const int32_t* __restrict__ queries, int32_t nq,
+                               float* __restrict__ out) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    const int32_t m = queries[3 * (int64_t)i], x = queries[3 * (int64_t)i + 1],
+                  y = queries[3 * (int64_t)i + 2];
+    const float4 lm = lab4[m];
+    float p = 0.0f;
+    int32_t nb[4], cnt = 0;
+    if (y > 0) nb[cnt++] = (y - 1) * gw + x;
+    if (y < gh - 1) nb[cnt++] = (y + 1) * gw + x;
+    if (x > 0) nb[cnt++] = y * gw + x - 1;
+    if (x < gw - 1) nb[cnt++] = y * gw + x + 1;
+    for (int32_t j = 0; j < cnt; ++j) {
+        const int32_t o = grid[nb[j]];
+        if (o != kEmpty) p += adj_term(lm, lab4[o]);
+    }
+    out[i] = p * weight;
+}
+
+void launch_penalties(const int32_t* grid, int32_t gw, int32_t gh, const float4* lab4, float weight,
+                      const int32_t* queries, int32_t nq, float* out, hipStream_t stream) {
+    if (nq <= 0) return;
+    hipLaunchKernelGGL(penalty_kernel, dim3((nq + 255) / 256), dim3(256), 0, stream, grid, gw, gh,
+                       lab4, weight, queries, nq, out);
+}
+
+}  // namespace mh

@@ -1,0 +1,240 @@
+// k_composite.hip -- tile resize, colour adjustment and compositing
+// (MosaicGenerator::resize_image src/main.rs:710-738,
+// calculate_average_rgb :687-708, calculate_optimal_adjustment
+// src/color_adjustment.rs:99-154, ColorAdjustment::apply_to_image :40-96,
+// and the composite loops src/main.rs:548-590 / :625-664).
+//
+// The resize depends only on (material, cell size), so it runs once per
+// material per job instead of once per cell: a horizontal Lanczos3 pass to a
+// u8 intermediate, then a vertical pass, both integer fixed-point with taps
+// the host computes.  Compositing is one coalesced per-pixel pass: output
+// pixel -> cell -> material -> resized pixel -> per-cell adjustment -> u8.
+// HBM-bound: ~9 B per output pixel (3 B target read for the cell average, 3 B
+// tile read, 3 B written).
+#include "mh_device.hpp"
+#include "mh_kernels.hpp"
+
+namespace mh {
+
+constexpr int kPrec = 22;
+
+__device__ __forceinline__ uint32_t clip8(int32_t acc) {
+    const int32_t v = acc >> kPrec;
+    return (uint32_t)(v < 0 ? 0 : (v > 255 ? 255 : v));
+}
+
+// Horizontal pass: src [n][sh][sw][3] -> tmp [n][sh][dw][3].
+__global__ void resize_h_kernel(const uint8_t* __restrict__ src, int32_t n, int32_t sw, int32_t sh,
+                                int32_t dw, const int32_t* __restrict__ lo,
+                                const int32_t* __restrict__ cnt, const int32_t* __restrict__ kk,
+                                int32_t ks, uint8_t* __restrict__ tmp) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = (int64_t)n * sh * dw;
+    if (idx >= total) return;
+    const int32_t o = (int32_t)(idx % dw);
+    const int64_t row = idx / dw;  // m*sh + y
+    const uint8_t* s = src + (row * sw + lo[o]) * 3;
+    const int32_t* k = kk + (int64_t)o * ks;
+    int32_t a0 = 1 << (kPrec - 1), a1 = a0, a2 = a0;
+    const int32_t c = cnt[o];
+    for (int32_t t = 0; t < c; ++t) {
+        const int32_t w = k[t];
+        a0 += (int32_t)s[3 * t] * w;
+        a1 += (int32_t)s[3 * t + 1] * w;
+        a2 += (int32_t)s[3 * t + 2] * w;
+    }
+    uint8_t* d = tmp + idx * 3;
+    d[0] = (uint8_t)clip8(a0);
+    d[1] = (uint8_t)clip8(a1);
+    d[2] = (uint8_t)clip8(a2);
+}
+
+// Vertical pass: tmp [n][sh][dw][3] -> dst [n][dh][dw][3].
+__global__ void resize_v_kernel(const uint8_t* __restrict__ tmp, int32_t n, int32_t sh, int32_t dw,
+                                int32_t dh, const int32_t* __restrict__ lo,
+                                const int32_t* __restrict__ cnt, const int32_t* __restrict__ kk,
+                                int32_t ks, uint8_t* __restrict__ dst) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = (int64_t)n * dh * dw;
+    if (idx >= total) return;
+    const int32_t x = (int32_t)(idx % dw);
+    const int64_t r = idx / dw;
+    const int32_t o = (int32_t)(r % dh);
+    const int64_t m = r / dh;
+    const uint8_t* s = tmp + ((m * sh + lo[o]) * dw + x) * 3;
+    const int32_t* k = kk + (int64_t)o * ks;
+    const int64_t step = (int64_t)dw * 3;
+    int32_t a0 = 1 << (kPrec - 1), a1 = a0, a2 = a0;
+    const int32_t c = cnt[o];
+    for (int32_t t = 0; t < c; ++t) {
+        const int32_t w = k[t];
+        a0 += (int32_t)s[t * step] * w;
+        a1 += (int32_t)s[t * step + 1] * w;
+        a2 += (int32_t)s[t * step + 2] * w;
+    }
+    uint8_t* d = dst + idx * 3;
+    d[0] = (uint8_t)clip8(a0);
+    d[1] = (uint8_t)clip8(a1);
+    d[2] = (uint8_t)clip8(a2);
+}
+
+void launch_resize(const uint8_t* src, int32_t n, int32_t sw, int32_t sh, int32_t dw, int32_t dh,
+                   const ResizeTaps& h, const ResizeTaps& v, uint8_t* tmp, uint8_t* dst,
+                   hipStream_t stream) {
+    if (n <= 0 || dw <= 0 || dh <= 0) return;
+    const int64_t th = (int64_t)n * sh * dw;
+    hipLaunchKernelGGL(resize_h_kernel, dim3((unsigned)((th + 255) / 256)), dim3(256), 0, stream, src,
+                       n, sw, sh, dw, h.lo, h.cnt, h.k, h.ksize, tmp);
+    const int64_t tv = (int64_t)n * dh * dw;
+    hipLaunchKernelGGL(resize_v_kernel, dim3((unsigned)((tv + 255) / 256)), dim3(256), 0, stream, tmp,
+                       n, sh, dw, dh, v.lo, v.cnt, v.k, v.ksize, dst);
+}
+
+// Integer average RGB of cells (calculate_average_rgb): one 256-thread
+// block per cell; sums are exact so the reduction order is free.  Cell
+// addressing as in the Lab kernel.
+__global__ __launch_bounds__(256) void avg_rgb_kernel(const uint8_t* __restrict__ rgb,
+                                                      int64_t row_stride, int32_t per_row,
+                                                      int64_t block_row_stride,
+                                                      int64_t block_col_stride, int32_t cw,
+                                                      int32_t ch, uint8_t* __restrict__ out) {
+    __shared__ uint64_t part[3][4];
+    const int32_t cell = blockIdx.x;
+    const uint8_t* base = rgb + (int64_t)(cell / per_row) * block_row_stride +
+                          (int64_t)(cell % per_row) * block_col_stride;
+    const int32_t npx = cw * ch;
+    uint64_t s0 = 0, s1 = 0, s2 = 0;
+    for (int32_t p = threadIdx.x; p < npx; p += 256) {
+        const int32_t py = p / cw, px = p - py * cw;
+        const uint8_t* q = base + (int64_t)py * row_stride + (int64_t)px * 3;
+        s0 += q[0];
+        s1 += q[1];
+        s2 += q[2];
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        s0 += __shfl_xor(s0, off, 64);
+        s1 += __shfl_xor(s1, off, 64);
+        s2 += __shfl_xor(s2, off, 64);
+    }
+    const int wid = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        part[0][wid] = s0;
+        part[1][wid] = s1;
+        part[2][wid] = s2;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const uint64_t t = part[threadIdx.x][0] + part[threadIdx.x][1] + part[threadIdx.x][2] +
+                           part[threadIdx.x][3];
+        out[3 * (int64_t)cell + threadIdx.x] = (uint8_t)(t / (uint64_t)npx);
+    }
+}
+
+void launch_avg_rgb(const uint8_t* rgb, int64_t row_stride, int32_t per_row,
+                    int64_t block_row_stride, int64_t block_col_stride, int32_t cw, int32_t ch,
+                    int32_t n_cells, uint8_t* out, hipStream_t stream) {
+    if (n_cells <= 0 || cw <= 0 || ch <= 0) return;
+    hipLaunchKernelGGL(avg_rgb_kernel, dim3(n_cells), dim3(256), 0, stream, rgb, row_stride, per_row,
+                       block_row_stride, block_col_stride, cw, ch, out);
+}
+
+// Per-cell adjustment: calculate_optimal_adjustment(tile_avg[grid[c]],
+// target_avg[c], strength).  Empty cells get a NaN marker in .y.
+__global__ void cell_adjust_kernel(const int32_t* __restrict__ grid, int32_t cells,
+                                   const uint8_t* __restrict__ tile_avg,
+                                   const uint8_t* __restrict__ target_avg, float strength,
+                                   float4* __restrict__ out) {
+    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cells) return;
+    const int32_t m = grid[c];
+    if (m == kEmpty) {
+        out[c] = make_float4(0.0f, __int_as_float(0x7fc00000), 0.0f, 1.0f);
+        return;
+    }
+    const uint8_t* t = tile_avg + 3 * (int64_t)m;
+    const uint8_t* g = target_avg + 3 * (int64_t)c;
+    out[c] = optimal_adjustment(t[0], t[1], t[2], g[0], g[1], g[2], strength);
+}
+
+void launch_cell_adjust(const int32_t* grid, int32_t cells, const uint8_t* tile_avg,
+                        const uint8_t* target_avg, float strength, float4* out,
+                        hipStream_t stream) {
+    if (cells <= 0) return;
+    hipLaunchKernelGGL(cell_adjust_kernel, dim3((cells + 255) / 256), dim3(256), 0, stream, grid,
+                       cells, tile_avg, target_avg, strength, out);
+}
+
+// One thread per output pixel (ow = gw*tw, oh = gh*th).
+__global__ void composite_kernel(const int32_t* __restrict__ grid, int32_t gw, int32_t tw,
+                                 int32_t th, int32_t ow, int32_t oh,
+                                 const uint8_t* __restrict__ tiles,
+                                 const float4* __restrict__ adj, int32_t adjust,
+                                 uint8_t* __restrict__ out) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)ow * oh) return;
+    const int32_t X = (int32_t)(idx % ow), Y = (int32_t)(idx / ow);
+    const int32_t cx = X / tw, cy = Y / th;
+    const int32_t cell = cy * gw + cx;
+    const int32_t m = grid[cell];
+    uint32_t r = 0, g = 0, b = 0;
+    if (m != kEmpty) {
+        const int32_t dx = X - cx * tw, dy = Y - cy * th;
+        const uint8_t* p = tiles + (((int64_t)m * th + dy) * tw + dx) * 3;
+        r = p[0];
+        g = p[1];
+        b = p[2];
+        if (adjust) adjust_pixel(adj[cell], r, g, b, r, g, b);
+    }
+    uint8_t* d = out + idx * 3;
+    d[0] = (uint8_t)r;
+    d[1] = (uint8_t)g;
+    d[2] = (uint8_t)b;
+}
+
+void launch_composite(const int32_t* grid, int32_t gw, int32_t gh, int32_t tw, int32_t th,
+                      const uint8_t* tiles, const float4* adj, int32_t adjust, uint8_t* out,
+                      hipStream_t stream) {
+    const int64_t total = (int64_t)gw * tw * gh * th;
+    if (total <= 0) return;
+    hipLaunchKernelGGL(composite_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream,
+                       grid, gw, tw, th, gw * tw, gh * th, tiles, adj, adjust, out);
+}
+
+// ColorAdjustment::apply_to_image over a flat pixel array.
+__global__ void adjust_pixels_kernel(float4 adj, const uint8_t* __restrict__ in, int64_t npx,
+                                     uint8_t* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npx) return;
+    uint32_t r, g, b;
+    adjust_pixel(adj, in[3 * i], in[3 * i + 1], in[3 * i + 2], r, g, b);
+    out[3 * i] = (uint8_t)r;
+    out[3 * i + 1] = (uint8_t)g;
+    out[3 * i + 2] = (uint8_t)b;
+}
+
+void launch_adjust_pixels(float4 adj, const uint8_t* in, int64_t npx, uint8_t* out,
+                          hipStream_t stream) {
+    if (npx <= 0) return;
+    hipLaunchKernelGGL(adjust_pixels_kernel, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0,
+                       stream, adj, in, npx, out);
+}
+
+__global__ void optimal_adjust_kernel(const uint8_t* __restrict__ tile,
+                                      const uint8_t* __restrict__ target, int32_t n, float k,
+                                      float4* __restrict__ out) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint8_t* t = tile + 3 * (int64_t)i;
+    const uint8_t* g = target + 3 * (int64_t)i;
+    out[i] = optimal_adjustment(t[0], t[1], t[2], g[0], g[1], g[2], k);
+}
+
+void launch_optimal_adjust(const uint8_t* tile, const uint8_t* target, int32_t n, float k,
+                           float4* out, hipStream_t stream) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(optimal_adjust_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, tile,
+                       target, n, k, out);
+}
+
+}  // namespace mh

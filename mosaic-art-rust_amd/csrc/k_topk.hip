@@ -1,0 +1,193 @@
+// k_topk.hip -- K nearest materials per cell (the kiddo k-d tree query
+// nearest_n::<SquaredEuclidean>(lab, min(N,100)) at src/main.rs:348-352).
+//
+// Brute force over all N materials, exact: each material gets the 64-bit
+// key (float_bits(sqdist) << 32 | index).  Squared distances are >= +0, so
+// the bit pattern orders like the value and keys are unique; the K smallest
+// keys are the K nearest materials ascending with ties broken by index.
+//
+// One 256-thread block per query cell:
+//   1. squared distances for all N (VALU; cached in LDS when N <= 16384),
+//   2. radix select of the K-th smallest key over 11-bit digits (LDS
+//      histograms, usually 2-3 passes; early exit once a digit bin holds
+//      exactly the remaining count),
+//   3. gather the K keys <= threshold, bitonic sort in LDS, write ascending.
+// VALU-bound: 8 flops + 1 LDS atomic per material per pass.
+#include "mh_device.hpp"
+#include "mh_kernels.hpp"
+
+namespace mh {
+
+constexpr int kTopkThreads = 256;
+constexpr int kDigitBits = 11;
+constexpr int kBins = 1 << kDigitBits;
+constexpr int kBinsPerThread = kBins / kTopkThreads;
+constexpr int32_t kDistCacheMax = 16384;
+
+__device__ __forceinline__ uint64_t make_key(uint32_t dbits, int32_t i) {
+    return ((uint64_t)dbits << 32) | (uint32_t)i;
+}
+
+// Exclusive prefix sum over the block (256 threads = 4 waves).
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* wsum) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t inc = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t o = __shfl_up(inc, off, 64);
+        if (lane >= off) inc += o;
+    }
+    if (lane == 63) wsum[wid] = inc;
+    __syncthreads();
+    uint32_t base = 0;
+    for (int w = 0; w < wid; ++w) base += wsum[w];
+    __syncthreads();
+    return base + inc - v;
+}
+
+template <bool kCache>
+__global__ __launch_bounds__(kTopkThreads) void topk_kernel(
+    const float* __restrict__ mL, const float* __restrict__ mA, const float* __restrict__ mB,
+    int32_t n, const float* __restrict__ query, int32_t k, int32_t* __restrict__ out_idx,
+    float* __restrict__ out_d, int32_t out_stride) {
+    extern __shared__ uint32_t dcache[];
+    __shared__ uint32_t hist[kBins];
+    __shared__ uint64_t sel[128];
+    __shared__ uint32_t wsum[4];
+    __shared__ uint32_t s_digit, s_need, s_cnt, s_fill;
+
+    const int tid = threadIdx.x;
+    const int32_t q = blockIdx.x;
+    const float ql = query[3 * (int64_t)q], qa = query[3 * (int64_t)q + 1],
+                qb = query[3 * (int64_t)q + 2];
+
+    auto dist_bits = [&](int32_t i) -> uint32_t {
+        if (kCache) return dcache[i];
+        return __float_as_uint(sq_distance(ql, qa, qb, mL[i], mA[i], mB[i]));
+    };
+
+    if (kCache) {
+        for (int32_t i = tid; i < n; i += kTopkThreads)
+            dcache[i] = __float_as_uint(sq_distance(ql, qa, qb, mL[i], mA[i], mB[i]));
+    }
+    if (tid < 128) sel[tid] = ~0ull;
+    __syncthreads();
+
+    if (n <= 128) {
+        if (tid < n) sel[tid] = make_key(dist_bits(tid), tid);
+        __syncthreads();
+    } else {
+        uint64_t prefix = 0;
+        uint32_t need = (uint32_t)k;
+        int lo = 0;
+        for (int hi = 62; hi >= 0; hi -= kDigitBits) {
+            const int lo_w = hi - kDigitBits + 1 > 0 ? hi - kDigitBits + 1 : 0;
+            const int width = hi - lo_w + 1;
+            const uint32_t mask = (1u << width) - 1u;
+            for (int b = tid; b < kBins; b += kTopkThreads) hist[b] = 0;
+            __syncthreads();
+            for (int32_t i = tid; i < n; i += kTopkThreads) {
+                const uint64_t key = make_key(dist_bits(i), i);
+                if ((key >> (hi + 1)) == prefix) atomicAdd(&hist[(uint32_t)(key >> lo_w) & mask], 1u);
+            }
+            __syncthreads();
+            uint32_t local[kBinsPerThread];
+            uint32_t sum = 0;
+#pragma unroll
+            for (int b = 0; b < kBinsPerThread; ++b) {
+                local[b] = hist[tid * kBinsPerThread + b];
+                sum += local[b];
+            }
+            const uint32_t before = block_exclusive_scan(sum, wsum);
+            if (before < need && need <= before + sum) {
+                uint32_t c = before;
+#pragma unroll
+                for (int b = 0; b < kBinsPerThread; ++b) {
+                    if (need > c && need <= c + local[b]) {
+                        s_digit = tid * kBinsPerThread + b;
+                        s_need = need - c;
+                        s_cnt = local[b];
+                    }
+                    c += local[b];
+                }
+            }
+            __syncthreads();
+            prefix = (prefix << width) | s_digit;
+            need = s_need;
+            lo = lo_w;
+            const bool done = s_cnt == need;
+            __syncthreads();
+            if (done) break;
+        }
+        if (tid == 0) s_fill = 0;
+        __syncthreads();
+        for (int32_t i = tid; i < n; i += kTopkThreads) {
+            const uint64_t key = make_key(dist_bits(i), i);
+            if ((key >> lo) <= prefix) {
+                const uint32_t slot = atomicAdd(&s_fill, 1u);
+                if (slot < 128) sel[slot] = key;
+            }
+        }
+        __syncthreads();
+    }
+
+    // bitonic sort of 128 keys, ascending
+    for (int size = 2; size <= 128; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            if (tid < 64) {
+                const int i = 2 * stride * (tid / stride) + (tid % stride);
+                const int j = i + stride;
+                const bool asc = (i & size) == 0;
+                const uint64_t a = sel[i], b = sel[j];
+                if ((a > b) == asc) {
+                    sel[i] = b;
+                    sel[j] = a;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    const int32_t kk = k < n ? k : n;
+    if (tid < kk) {
+        const uint64_t key = sel[tid];
+        out_idx[(int64_t)q * out_stride + tid] = (int32_t)(uint32_t)(key & 0xffffffffu);
+        out_d[(int64_t)q * out_stride + tid] = __uint_as_float((uint32_t)(key >> 32));
+    }
+}
+
+void launch_topk(const float* mL, const float* mA, const float* mB, int32_t n, const float* query,
+                 int32_t nq, int32_t k, int32_t* out_idx, float* out_d, int32_t out_stride,
+                 hipStream_t stream) {
+    if (k > n) k = n;
+    if (nq <= 0 || n <= 0 || k <= 0) return;
+    if (n <= kDistCacheMax) {
+        const size_t shmem = sizeof(uint32_t) * (size_t)n;
+        hipLaunchKernelGGL(topk_kernel<true>, dim3(nq), dim3(kTopkThreads), shmem, stream, mL, mA,
+                           mB, n, query, k, out_idx, out_d, out_stride);
+    } else {
+        hipLaunchKernelGGL(topk_kernel<false>, dim3(nq), dim3(kTopkThreads), 0, stream, mL, mA, mB,
+                           n, query, k, out_idx, out_d, out_stride);
+    }
+}
+
+hipError_t raise_dynamic_lds(const void* kernel, int max_lds) {
+    hipFuncAttributes attr;
+    hipError_t e = hipFuncGetAttributes(&attr, kernel);
+    if (e != hipSuccess) return e;
+    const int dyn = max_lds - (int)attr.sharedSizeBytes;
+    if (dyn <= 0) return hipErrorInvalidValue;
+    return hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+}
+
+hipError_t topk_configure(int max_lds) {
+    return raise_dynamic_lds(reinterpret_cast<const void*>(&topk_kernel<true>), max_lds);
+}
+
+hipError_t configure_kernel_limits(int max_lds) {
+    hipError_t e = topk_configure(max_lds);
+    if (e == hipSuccess) e = place_configure(max_lds);
+    if (e == hipSuccess) e = anneal_configure(max_lds);
+    return e;
+}
+
+}  // namespace mh

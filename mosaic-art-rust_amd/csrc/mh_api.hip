@@ -1,0 +1,1062 @@
+// mh_api.hip -- the C ABI (include/mosaic_hip.h) over the gfx950 kernels.
+//
+// Host entry points take caller-owned host buffers, stage them through
+// context-owned device scratch (grow-only), enqueue the kernels on the
+// context stream and synchronise.  mh_generate_device is the asynchronous,
+// device-resident form used by the benchmark and by batching callers.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/mosaic_hip.h"
+#include "mh_kernels.hpp"
+
+using namespace mh;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(call)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(e_ == hipErrorOutOfMemory ? MH_ERR_OUT_OF_MEMORY : MH_ERR_HIP,        \
+                        "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__,     \
+                        __LINE__);                                                           \
+    } while (0)
+
+// Device buffer that only grows.
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+};
+
+enum Slot {
+    S_IN0, S_IN1, S_IN2, S_OUT0, S_OUT1, S_OUT2, S_OUT3,
+    S_CELL_LAB, S_CAND_IDX, S_CAND_D, S_GRID, S_USAGE, S_ORDER,
+    S_TILE_TMP, S_TILES, S_TILE_AVG, S_TARGET_AVG, S_ADJ, S_STATS, S_ANNEAL_OUT,
+    S_COUNT
+};
+
+struct TapsBuf {
+    DevBuf lo, cnt, k;
+    int32_t ksize = 0;
+};
+
+}  // namespace
+
+struct mh_context {
+    int32_t device = 0;
+    hipStream_t stream = nullptr;
+    float* d_lut = nullptr;
+    size_t max_lds = 65536;
+    DevBuf slots[S_COUNT];
+    // temperature table cache
+    float t0 = NAN, decay = NAN;
+    int64_t t_len = -1;
+    DevBuf temps;
+    // Lanczos taps keyed by (in, out)
+    std::map<std::pair<int32_t, int32_t>, TapsBuf> taps;
+    // last device job geometry for mh_generate_device_result
+    int32_t last_out_w = 0, last_out_h = 0;
+    int32_t last_ran_anneal = 0;
+    int64_t last_iters = 0;
+};
+
+struct mh_library {
+    mh_context* ctx = nullptr;
+    int32_t n = 0;
+    std::vector<float> h_lab;  // n x 3
+    float4* d_lab4 = nullptr;
+    float* d_L = nullptr;
+    float* d_A = nullptr;
+    float* d_B = nullptr;
+    float* d_sim = nullptr;    // n x n after build_similarities
+    uint8_t* d_pixels = nullptr;
+    int32_t pw = 0, ph = 0;
+};
+
+namespace {
+
+int set_device(mh_context* ctx) {
+    HIPCHK(hipSetDevice(ctx->device));
+    return MH_OK;
+}
+
+int ensure(mh_context* ctx, DevBuf& b, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (b.cap >= bytes) return MH_OK;
+    if (b.p) HIPCHK(hipFree(b.p));
+    b.p = nullptr;
+    b.cap = 0;
+    size_t want = bytes + bytes / 4;
+    HIPCHK(hipMalloc(&b.p, want));
+    b.cap = want;
+    (void)ctx;
+    return MH_OK;
+}
+
+template <typename T>
+int slot(mh_context* ctx, Slot s, size_t count, T** out) {
+    int rc = ensure(ctx, ctx->slots[s], sizeof(T) * count);
+    if (rc) return rc;
+    *out = reinterpret_cast<T*>(ctx->slots[s].p);
+    return MH_OK;
+}
+
+bool finite3(const float* v, int64_t count) {
+    for (int64_t i = 0; i < count; ++i)
+        if (!isfinite(v[i])) return false;
+    return true;
+}
+
+// Lanczos3 fixed-point taps for one axis (same construction as the
+// kernels assume): support 3*max(scale,1), weights normalised in double and
+// rounded half away from zero to Q22.
+double sinc(double x) {
+    if (x == 0.0) return 1.0;
+    x = x * M_PI;
+    return sin(x) / x;
+}
+
+double lanczos3(double x) {
+    if (x > -3.0 && x < 3.0) return sinc(x) * sinc(x / 3.0);
+    return 0.0;
+}
+
+int get_taps(mh_context* ctx, int32_t in_size, int32_t out_size, ResizeTaps* t) {
+    auto key = std::make_pair(in_size, out_size);
+    auto it = ctx->taps.find(key);
+    if (it == ctx->taps.end()) {
+        const double scale = (double)in_size / (double)out_size;
+        const double fs = scale < 1.0 ? 1.0 : scale;
+        const double support = 3.0 * fs;
+        const int32_t ks = (int32_t)ceil(support) * 2 + 1;
+        std::vector<int32_t> lo(out_size), cnt(out_size), k((size_t)out_size * ks, 0);
+        std::vector<double> w(ks);
+        for (int32_t o = 0; o < out_size; ++o) {
+            const double center = ((double)o + 0.5) * scale;
+            int32_t xmin = (int32_t)(center - support + 0.5);
+            if (xmin < 0) xmin = 0;
+            int32_t xmax = (int32_t)(center + support + 0.5);
+            if (xmax > in_size) xmax = in_size;
+            xmax -= xmin;
+            double ww = 0.0;
+            for (int32_t x = 0; x < xmax; ++x) {
+                w[x] = lanczos3(((double)(x + xmin) - center + 0.5) / fs);
+                ww += w[x];
+            }
+            for (int32_t x = 0; x < xmax; ++x) {
+                const double v = ww != 0.0 ? w[x] / ww : 0.0;
+                const double f = v * (double)(1 << 22);
+                k[(size_t)o * ks + x] = (int32_t)(f < 0.0 ? f - 0.5 : f + 0.5);
+            }
+            lo[o] = xmin;
+            cnt[o] = xmax;
+        }
+        TapsBuf tb;
+        tb.ksize = ks;
+        int rc;
+        if ((rc = ensure(ctx, tb.lo, sizeof(int32_t) * out_size))) return rc;
+        if ((rc = ensure(ctx, tb.cnt, sizeof(int32_t) * out_size))) return rc;
+        if ((rc = ensure(ctx, tb.k, sizeof(int32_t) * k.size()))) return rc;
+        HIPCHK(hipMemcpy(tb.lo.p, lo.data(), sizeof(int32_t) * out_size, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(tb.cnt.p, cnt.data(), sizeof(int32_t) * out_size, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(tb.k.p, k.data(), sizeof(int32_t) * k.size(), hipMemcpyHostToDevice));
+        it = ctx->taps.emplace(key, tb).first;
+    }
+    t->lo = (const int32_t*)it->second.lo.p;
+    t->cnt = (const int32_t*)it->second.cnt.p;
+    t->k = (const int32_t*)it->second.k.p;
+    t->ksize = it->second.ksize;
+    return MH_OK;
+}
+
+// Temperature after k non-skipped proposals: T_0 = T0, T_{k+1} = T_k * decay
+// in f32 (src/optimizer.rs:54, :102).
+int get_temps(mh_context* ctx, float t0, float decay, int64_t len, const float** out) {
+    if (len < 1) len = 1;
+    bool same = ctx->t_len >= len && memcmp(&ctx->t0, &t0, 4) == 0 &&
+                memcmp(&ctx->decay, &decay, 4) == 0;
+    if (!same) {
+        // kernels already queued may still read the old table
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        std::vector<float> t((size_t)len);
+        float v = t0;
+        for (int64_t i = 0; i < len; ++i) {
+            t[(size_t)i] = v;
+            v = v * decay;
+        }
+        int rc = ensure(ctx, ctx->temps, sizeof(float) * (size_t)len);
+        if (rc) return rc;
+        HIPCHK(hipMemcpy(ctx->temps.p, t.data(), sizeof(float) * (size_t)len,
+                         hipMemcpyHostToDevice));
+        ctx->t0 = t0;
+        ctx->decay = decay;
+        ctx->t_len = len;
+    }
+    *out = (const float*)ctx->temps.p;
+    return MH_OK;
+}
+
+__global__ void fill_i32_kernel(int32_t* p, int64_t n, int32_t v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+void fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+}
+
+// Device-resident placement of every cell of a grid in raster order (or
+// `order`), from cell Labs already on the device.
+int run_place(mh_context* ctx, const mh_library* lib, const float* d_cell_lab, int32_t gw,
+              int32_t gh, int32_t max_usage, float weight, const int32_t* d_order,
+              int32_t n_order, int32_t* d_grid, uint32_t* d_usage, uint8_t* d_stage,
+              float* d_dist, float* d_score, int32_t* d_stats) {
+    const int32_t cells = gw * gh;
+    const size_t lds = place_lds_bytes(cells, lib->n);
+    if (lds > ctx->max_lds)
+        return fail(MH_ERR_UNSUPPORTED,
+                    "placement state (%d cells + %d materials) needs %zu B of LDS, limit %zu",
+                    cells, lib->n, lds, ctx->max_lds);
+    int32_t* cidx = nullptr;
+    float* cd = nullptr;
+    int rc;
+    if ((rc = slot(ctx, S_CAND_IDX, (size_t)cells * kCandStride, &cidx))) return rc;
+    if ((rc = slot(ctx, S_CAND_D, (size_t)cells * kCandStride, &cd))) return rc;
+    const int32_t K = lib->n < kMaxCand ? lib->n : kMaxCand;
+    launch_topk(lib->d_L, lib->d_A, lib->d_B, lib->n, d_cell_lab, cells, K, cidx, cd, kCandStride,
+                ctx->stream);
+    PlaceJob job;
+    job.lab4 = lib->d_lab4;
+    job.n = lib->n;
+    job.cand_idx = cidx;
+    job.cand_d = cd;
+    job.order = d_order;
+    job.n_order = n_order;
+    job.grid = d_grid;
+    job.usage = d_usage;
+    job.grid_w = gw;
+    job.grid_h = gh;
+    job.max_usage = max_usage;
+    job.weight = weight;
+    job.stage = d_stage;
+    job.dist = d_dist;
+    job.score = d_score;
+    job.stats = d_stats;
+    launch_place_single(job, lds, ctx->stream);
+    HIPCHK(hipGetLastError());
+    return MH_OK;
+}
+
+int run_anneal(mh_context* ctx, const mh_library* lib, int32_t* d_grid, int32_t gw, int32_t gh,
+               const mh_anneal_params* p, float* d_trace_delta, uint8_t* d_trace_accept,
+               AnnealOut* d_out) {
+    const int32_t cells = gw * gh;
+    const bool narrow = lib->n < 32768;
+    const size_t lds = anneal_lds_bytes(cells, narrow);
+    if (lds > ctx->max_lds)
+        return fail(MH_ERR_UNSUPPORTED, "annealing state (%d cells) needs %zu B of LDS, limit %zu",
+                    cells, lds, ctx->max_lds);
+    const float* temps = nullptr;
+    int rc;
+    if (!p->greedy) {
+        if ((rc = get_temps(ctx, p->initial_temperature, p->temperature_decay, p->max_iterations,
+                            &temps)))
+            return rc;
+    }
+    AnnealJob job;
+    job.lab4 = lib->d_lab4;
+    job.n = lib->n;
+    job.grid = d_grid;
+    job.grid_w = gw;
+    job.grid_h = gh;
+    job.max_iter = p->max_iterations;
+    job.temps = temps;
+    job.seed = p->seed;
+    job.greedy = p->greedy;
+    job.trace_delta = d_trace_delta;
+    job.trace_accept = d_trace_accept;
+    job.result = d_out;
+    launch_anneal_single(job, narrow, lds, ctx->stream);
+    HIPCHK(hipGetLastError());
+    return MH_OK;
+}
+
+// Resized tiles + their average RGB for cell size tw x th (device).
+int run_tiles(mh_context* ctx, const mh_library* lib, int32_t tw, int32_t th, uint8_t** tiles,
+              uint8_t** tile_avg) {
+    ResizeTaps h, v;
+    int rc;
+    if ((rc = get_taps(ctx, lib->pw, tw, &h))) return rc;
+    if ((rc = get_taps(ctx, lib->ph, th, &v))) return rc;
+    uint8_t *tmp, *dst, *avg;
+    const size_t n = (size_t)lib->n;
+    if ((rc = slot(ctx, S_TILE_TMP, n * lib->ph * tw * 3, &tmp))) return rc;
+    if ((rc = slot(ctx, S_TILES, n * th * tw * 3, &dst))) return rc;
+    if ((rc = slot(ctx, S_TILE_AVG, n * 3, &avg))) return rc;
+    launch_resize(lib->d_pixels, lib->n, lib->pw, lib->ph, tw, th, h, v, tmp, dst, ctx->stream);
+    launch_avg_rgb(dst, (int64_t)tw * 3, 1, (int64_t)tw * th * 3, 0, tw, th, lib->n, avg,
+                   ctx->stream);
+    *tiles = dst;
+    *tile_avg = avg;
+    return MH_OK;
+}
+
+int run_composite(mh_context* ctx, const mh_library* lib, const uint8_t* d_target, int32_t width,
+                  int32_t height, const int32_t* d_grid, int32_t gw, int32_t gh, float strength,
+                  uint8_t* d_out) {
+    const int32_t tw = width / gw, th = height / gh;
+    if (tw <= 0 || th <= 0) return MH_OK;
+    uint8_t *tiles, *tile_avg;
+    int rc;
+    if ((rc = run_tiles(ctx, lib, tw, th, &tiles, &tile_avg))) return rc;
+    const int32_t cells = gw * gh;
+    float4* adj = nullptr;
+    const int32_t adjust = strength > 0.0f ? 1 : 0;
+    if (adjust) {
+        uint8_t* tavg;
+        if ((rc = slot(ctx, S_TARGET_AVG, (size_t)cells * 3, &tavg))) return rc;
+        if ((rc = slot(ctx, S_ADJ, (size_t)cells, &adj))) return rc;
+        launch_avg_rgb(d_target, (int64_t)width * 3, gw, (int64_t)th * width * 3, (int64_t)tw * 3, tw,
+                       th, cells, tavg, ctx->stream);
+        launch_cell_adjust(d_grid, cells, tile_avg, tavg, strength, adj, ctx->stream);
+    }
+    launch_composite(d_grid, gw, gh, tw, th, tiles, adj, adjust, d_out, ctx->stream);
+    HIPCHK(hipGetLastError());
+    return MH_OK;
+}
+
+template <typename T>
+int upload(mh_context* ctx, Slot s, const T* host, size_t count, T** dev) {
+    int rc = slot(ctx, s, count, dev);
+    if (rc) return rc;
+    if (count) HIPCHK(hipMemcpyAsync(*dev, host, sizeof(T) * count, hipMemcpyHostToDevice, ctx->stream));
+    return MH_OK;
+}
+
+template <typename T>
+int download(mh_context* ctx, T* host, const T* dev, size_t count) {
+    if (host && count)
+        HIPCHK(hipMemcpyAsync(host, dev, sizeof(T) * count, hipMemcpyDeviceToHost, ctx->stream));
+    return MH_OK;
+}
+
+int sync(mh_context* ctx) {
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MH_OK;
+}
+
+#define CHK(expr)                 \
+    do {                          \
+        int rc_ = (expr);         \
+        if (rc_) return rc_;      \
+    } while (0)
+
+int check_grid_values(const int32_t* grid, int64_t cells, int32_t n) {
+    for (int64_t i = 0; i < cells; ++i)
+        if (grid[i] < -1 || grid[i] >= n)
+            return fail(MH_ERR_INVALID_ARGUMENT, "grid[%lld] = %d is not MH_EMPTY or a material index < %d",
+                        (long long)i, grid[i], n);
+    return MH_OK;
+}
+
+int check_geometry(int32_t gw, int32_t gh) {
+    if (gw <= 0 || gh <= 0)
+        return fail(MH_ERR_INVALID_ARGUMENT, "grid %d x %d must be positive", gw, gh);
+    if ((int64_t)gw * gh > (int64_t)1 << 30)
+        return fail(MH_ERR_INVALID_ARGUMENT, "grid %d x %d too large", gw, gh);
+    return MH_OK;
+}
+
+}  // namespace
+
+// =================================================================== ABI
+extern "C" {
+
+const char* mh_last_error(void) { return g_err.c_str(); }
+
+int32_t mh_abi_version(void) { return 1; }
+
+int mh_context_create(int32_t device, mh_context** out) {
+    if (!out) return fail(MH_ERR_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+        return fail(MH_ERR_NO_DEVICE, "no HIP device available");
+    if (device < 0 || device >= count)
+        return fail(MH_ERR_INVALID_ARGUMENT, "device %d out of range (%d devices)", device, count);
+    mh_context* ctx = new mh_context();
+    ctx->device = device;
+    if (set_device(ctx)) {
+        delete ctx;
+        return MH_ERR_HIP;
+    }
+    hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete ctx;
+        return fail(MH_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess &&
+        lds > 0)
+        ctx->max_lds = (size_t)lds;
+    float lut[256];
+    for (int v = 0; v < 256; ++v) {
+        const float c = (float)v / 255.0f;
+        lut[v] = c <= 0.04045f ? c / 12.92f : (float)pow(((double)c + 0.055) / 1.055, 2.4);
+    }
+    e = hipMalloc(&ctx->d_lut, sizeof(lut));
+    if (e == hipSuccess) e = hipMemcpy(ctx->d_lut, lut, sizeof(lut), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        mh_context_destroy(ctx);
+        return fail(MH_ERR_HIP, "context init: %s", hipGetErrorString(e));
+    }
+    e = configure_kernel_limits((int)ctx->max_lds);
+    if (e != hipSuccess) {
+        mh_context_destroy(ctx);
+        return fail(MH_ERR_HIP, "raising the dynamic LDS limit to %zu B: %s", ctx->max_lds,
+                    hipGetErrorString(e));
+    }
+    *out = ctx;
+    return MH_OK;
+}
+
+void mh_context_destroy(mh_context* ctx) {
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    for (auto& b : ctx->slots)
+        if (b.p) hipFree(b.p);
+    if (ctx->temps.p) hipFree(ctx->temps.p);
+    for (auto& kv : ctx->taps) {
+        if (kv.second.lo.p) hipFree(kv.second.lo.p);
+        if (kv.second.cnt.p) hipFree(kv.second.cnt.p);
+        if (kv.second.k.p) hipFree(kv.second.k.p);
+    }
+    if (ctx->d_lut) hipFree(ctx->d_lut);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int mh_context_synchronize(mh_context* ctx) {
+    if (!ctx) return fail(MH_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    CHK(set_device(ctx));
+    return sync(ctx);
+}
+
+void* mh_context_stream(mh_context* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+// ---------------------------------------------------------------- library
+static int library_alloc(mh_context* ctx, int32_t n, mh_library** out) {
+    mh_library* lib = new mh_library();
+    lib->ctx = ctx;
+    lib->n = n;
+    lib->h_lab.assign((size_t)n * 3, 0.0f);
+    const size_t m = n > 0 ? (size_t)n : 1;
+    hipError_t e = hipMalloc(&lib->d_lab4, sizeof(float4) * m);
+    if (e == hipSuccess) e = hipMalloc(&lib->d_L, sizeof(float) * m);
+    if (e == hipSuccess) e = hipMalloc(&lib->d_A, sizeof(float) * m);
+    if (e == hipSuccess) e = hipMalloc(&lib->d_B, sizeof(float) * m);
+    if (e != hipSuccess) {
+        mh_library_destroy(lib);
+        return fail(e == hipErrorOutOfMemory ? MH_ERR_OUT_OF_MEMORY : MH_ERR_HIP, "library alloc: %s",
+                    hipGetErrorString(e));
+    }
+    *out = lib;
+    return MH_OK;
+}
+
+static int library_upload_lab(mh_library* lib) {
+    const int32_t n = lib->n;
+    if (n == 0) return MH_OK;
+    std::vector<float4> l4((size_t)n);
+    std::vector<float> L((size_t)n), A((size_t)n), B((size_t)n);
+    for (int32_t i = 0; i < n; ++i) {
+        L[i] = lib->h_lab[3 * i];
+        A[i] = lib->h_lab[3 * i + 1];
+        B[i] = lib->h_lab[3 * i + 2];
+        l4[i] = make_float4(L[i], A[i], B[i], 0.0f);
+    }
+    HIPCHK(hipMemcpy(lib->d_lab4, l4.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(lib->d_L, L.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(lib->d_A, A.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(lib->d_B, B.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    return MH_OK;
+}
+
+int mh_library_from_lab(mh_context* ctx, const float* lab, int32_t n, mh_library** out) {
+    if (!ctx || !out || n < 0 || (n > 0 && !lab))
+        return fail(MH_ERR_INVALID_ARGUMENT, "mh_library_from_lab: bad arguments");
+    if (!finite3(lab, (int64_t)n * 3))
+        return fail(MH_ERR_INVALID_ARGUMENT, "material Lab values must be finite");
+    CHK(set_device(ctx));
+    mh_library* lib = nullptr;
+    CHK(library_alloc(ctx, n, &lib));
+    if (n) memcpy(lib->h_lab.data(), lab, sizeof(float) * 3 * (size_t)n);
+    int rc = library_upload_lab(lib);
+    if (rc) {
+        mh_library_destroy(lib);
+        return rc;
+    }
+    *out = lib;
+    return MH_OK;
+}
+
+int mh_library_from_pixels(mh_context* ctx, const uint8_t* rgb, int32_t n, int32_t w, int32_t h,
+                           mh_library** out) {
+    if (!ctx || !out || n < 0 || w <= 0 || h <= 0 || (n > 0 && !rgb))
+        return fail(MH_ERR_INVALID_ARGUMENT, "mh_library_from_pixels: bad arguments");
+    CHK(set_device(ctx));
+    mh_library* lib = nullptr;
+    CHK(library_alloc(ctx, n, &lib));
+    const size_t bytes = (size_t)n * w * h * 3;
+    lib->pw = w;
+    lib->ph = h;
+    int rc = MH_OK;
+    hipError_t e = hipMalloc(&lib->d_pixels, bytes ? bytes : 16);
+    if (e == hipSuccess && bytes) e = hipMemcpy(lib->d_pixels, rgb, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        mh_library_destroy(lib);
+        return fail(MH_ERR_HIP, "library pixels: %s", hipGetErrorString(e));
+    }
+    if (n > 0) {
+        float* d_lab = nullptr;
+        rc = slot(ctx, S_OUT0, (size_t)n * 3, &d_lab);
+        if (!rc) {
+            launch_cell_lab(lib->d_pixels, (int64_t)w * 3, 1, (int64_t)w * h * 3, 0, w, h, n, ctx->d_lut,
+                            d_lab, ctx->stream);
+            e = hipMemcpyAsync(lib->h_lab.data(), d_lab, sizeof(float) * 3 * n, hipMemcpyDeviceToHost,
+                               ctx->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+            if (e != hipSuccess) rc = fail(MH_ERR_HIP, "material Lab: %s", hipGetErrorString(e));
+        }
+        if (!rc) rc = library_upload_lab(lib);
+    }
+    if (rc) {
+        mh_library_destroy(lib);
+        return rc;
+    }
+    *out = lib;
+    return MH_OK;
+}
+
+void mh_library_destroy(mh_library* lib) {
+    if (!lib) return;
+    if (lib->ctx) {
+        hipSetDevice(lib->ctx->device);
+        hipStreamSynchronize(lib->ctx->stream);
+    }
+    if (lib->d_lab4) hipFree(lib->d_lab4);
+    if (lib->d_L) hipFree(lib->d_L);
+    if (lib->d_A) hipFree(lib->d_A);
+    if (lib->d_B) hipFree(lib->d_B);
+    if (lib->d_sim) hipFree(lib->d_sim);
+    if (lib->d_pixels) hipFree(lib->d_pixels);
+    delete lib;
+}
+
+int32_t mh_library_size(const mh_library* lib) { return lib ? lib->n : -1; }
+
+int mh_library_lab(const mh_library* lib, float* out) {
+    if (!lib || (!out && lib->n > 0)) return fail(MH_ERR_INVALID_ARGUMENT, "bad arguments");
+    if (lib->n) memcpy(out, lib->h_lab.data(), sizeof(float) * 3 * (size_t)lib->n);
+    return MH_OK;
+}
+
+int mh_library_build_similarities(mh_library* lib) {
+    if (!lib) return fail(MH_ERR_INVALID_ARGUMENT, "lib is NULL");
+    mh_context* ctx = lib->ctx;
+    CHK(set_device(ctx));
+    if (lib->n == 0) return MH_OK;
+    if (!lib->d_sim) HIPCHK(hipMalloc(&lib->d_sim, sizeof(float) * (size_t)lib->n * lib->n));
+    launch_sim_full(lib->d_lab4, lib->n, lib->d_sim, ctx->stream);
+    HIPCHK(hipGetLastError());
+    return sync(ctx);
+}
+
+int mh_library_similarities(mh_library* lib, float* out, int32_t packed) {
+    if (!lib || (!out && lib->n > 1)) return fail(MH_ERR_INVALID_ARGUMENT, "bad arguments");
+    mh_context* ctx = lib->ctx;
+    CHK(set_device(ctx));
+    const int64_t n = lib->n;
+    if (n == 0) return MH_OK;
+    if (packed) {
+        const int64_t cnt = n * (n - 1) / 2;
+        if (cnt == 0) return MH_OK;
+        float* d = nullptr;
+        CHK(slot(ctx, S_OUT0, (size_t)cnt, &d));
+        launch_sim_packed(lib->d_lab4, lib->n, d, ctx->stream);
+        HIPCHK(hipGetLastError());
+        CHK(download(ctx, out, d, (size_t)cnt));
+        return sync(ctx);
+    }
+    if (!lib->d_sim) CHK(mh_library_build_similarities(lib));
+    CHK(download(ctx, out, lib->d_sim, (size_t)(n * n)));
+    return sync(ctx);
+}
+
+int mh_library_get_similarity(mh_library* lib, int32_t i, int32_t j, float* out) {
+    if (!lib || !out) return fail(MH_ERR_INVALID_ARGUMENT, "bad arguments");
+    if (i < 0 || j < 0 || i >= lib->n || j >= lib->n)
+        return fail(MH_ERR_NOT_FOUND, "material index (%d, %d) not in library of %d", i, j, lib->n);
+    if (i == j) {
+        *out = 0.0f;
+        return MH_OK;
+    }
+    mh_context* ctx = lib->ctx;
+    CHK(set_device(ctx));
+    if (!lib->d_sim) CHK(mh_library_build_similarities(lib));
+    const int32_t a = i < j ? i : j, b = i < j ? j : i;
+    HIPCHK(hipMemcpy(out, lib->d_sim + (size_t)a * lib->n + b, sizeof(float), hipMemcpyDeviceToHost));
+    return MH_OK;
+}
+
+// -------------------------------------------------------------- a1: Lab
+int mh_average_lab(mh_context* ctx, const uint8_t* rgb, int32_t w, int32_t h, float out[3]) {
+    if (!ctx || !rgb || !out || w <= 0 || h <= 0)
+        return fail(MH_ERR_INVALID_ARGUMENT, "mh_average_lab: bad arguments");
+    CHK(set_device(ctx));
+    uint8_t* d_in;
+    float* d_out;
+    CHK(upload(ctx, S_IN0, rgb, (size_t)w * h * 3, &d_in));
+    CHK(slot(ctx, S_OUT0, 3, &d_out));
+    launch_cell_lab(d_in, (int64_t)w * 3, 1, 0, 0, w, h, 1, ctx->d_lut, d_out, ctx->stream);
+    HIPCHK(hipGetLastError());
+    CHK(download(ctx, out, d_out, 3));
+    return sync(ctx);
+}
+
+int mh_cell_lab_means(mh_context* ctx, const uint8_t* rgb, int32_t width, int32_t height,
+                      int32_t gw, int32_t gh, float* out) {
+    if (!ctx || !rgb || !out) return fail(MH_ERR_INVALID_ARGUMENT, "mh_cell_lab_means: NULL argument");
+    CHK(check_geometry(gw, gh));
+    const int32_t tw = width / gw, th = height / gh;
+    if (tw <= 0 || th <= 0)
+        return fail(MH_ERR_INVALID_ARGUMENT, "image %d x %d smaller than grid %d x %d", width, height,
+                    gw, gh);
+    CHK(set_device(ctx));
+    uint8_t* d_in;
+    float* d_out;
+    const int32_t cells = gw * gh;
+    CHK(upload(ctx, S_IN0, rgb, (size_t)width * height * 3, &d_in));
+    CHK(slot(ctx, S_OUT0, (size_t)cells * 3, &d_out));
+    launch_cell_lab(d_in, (int64_t)width * 3, gw, (int64_t)th * width * 3, (int64_t)tw * 3, tw, th,
+                    cells, ctx->d_lut, d_out, ctx->stream);
+    HIPCHK(hipGetLastError());
+    CHK(download(ctx, out, d_out, (size_t)cells * 3));
+    return sync(ctx);
+}
+
+// ------------------------------------------------------------- a3: top-K
+int mh_nearest_n(mh_context* ctx, const mh_library* lib, const float* query, int32_t nq, int32_t k,
+                 int32_t* idx, float* sqdist) {
+    if (!ctx || !lib || nq < 0 || k < 0 || k > 128 || (nq > 0 && (!query || !idx || !sqdist)))
+        return fail(MH_ERR_INVALID_ARGUMENT, "mh_nearest_n: bad arguments (k must be <= 128)");
+    if (!finite3(query, (int64_t)nq * 3))
+        return fail(MH_ERR_INVALID_ARGUMENT, "query Lab values must be finite");
+    const int32_t kk = k < lib->n ? k : lib->n;
+    if (nq == 0 || kk == 0) return MH_OK;
+    CHK(set_device(ctx));
+    float* d_q;
+    int32_t* d_i;
+    float* d_d;
+    CHK(upload(ctx, S_IN0, query, (size_t)nq * 3, &d_q));
+    CHK(slot(ctx, S_OUT0, (size_t)nq * kk, &d_i));
+    CHK(slot(ctx, S_OUT1, (size_t)nq * kk, &d_d));
+    launch_topk(lib->d_L, lib->d_A, lib->d_B, lib->n, d_q, nq, kk, d_i, d_d, kk, ctx->stream);
+    HIPCHK(hipGetLastError());
+    CHK(download(ctx, idx, d_i, (size_t)nq * kk));
+    CHK(download(ctx, sqdist, d_d, (size_t)nq * kk));
+    return sync(ctx);
+}
+
+// ----------------------------------------------------------- a4-a9: place
+int mh_place(mh_context* ctx, const mh_library* lib, const float* cell_lab,
+             const mh_place_params* p, const int32_t* order, int32_t n_order, int32_t* grid,
+             uint32_t* usage, uint8_t* stage, float* dist, float* score, mh_place_stats* stats) {
+    if (!ctx || !lib || !p || !cell_lab || !grid || (!usage && lib->n > 0))
+        return fail(MH_ERR_INVALID_ARGUMENT, "mh_place: NULL argument");
+    CHK(check_geometry(p->grid_w, p->grid_h));
+    if (p->max_usage < 0) return fail(MH_ERR_INVALID_ARGUMENT, "max_usage must be >= 0");
+    const int32_t cells = p->grid_w * p->grid_h;
+    if (!finite3(cell_lab, (int64_t)cells * 3))
+        return fail(MH_ERR_INVALID_ARGUMENT, "cell Lab values must be finite");
+    if (order) {
+        if (n_order < 0) return fail(MH_ERR_INVALID_ARGUMENT, "n_order < 0");
+        for (int32_t i = 0; i < n_order; ++i)
+            if (order[i] < 0 || order[i] >= cells)
+                return fail(MH_ERR_INVALID_ARGUMENT, "order[%d] = %d outside the grid", i, order[i]);
+    }
+    CHK(check_grid_values(grid, cells, lib->n));
+    CHK(set_device(ctx));
+    float* d_lab;
+    int32_t *d_grid, *d_order = nullptr, *d_stats;
+    uint32_t* d_usage;
+    CHK(upload(ctx, S_CELL_LAB, cell_lab, (size_t)cells * 3, &d_lab));
+    CHK(upload(ctx, S_GRID, grid, (size_t)cells, &d_grid));
+    CHK(upload(ctx, S_USAGE, usage, (size_t)lib->n, &d_usage));
+    if (order) CHK(upload(ctx, S_ORDER, order, (size_t)(n_order > 0 ? n_order : 1), &d_order));
+    uint8_t* d_stage = nullptr;
+    float *d_dist = nullptr, *d_score = nullptr;
+    if (stage) {
+        CHK(upload(ctx, S_OUT0, stage, (size_t)cells, &d_stage));
+    }
+    if (dist) CHK(upload(ctx, S_OUT1, dist, (size_t)cells, &d_dist));
+    if (score) CHK(upload(ctx, S_OUT2, score, (size_t)cells, &d_score));
+    CHK(slot(ctx, S_STATS, 8, &d_stats));
+    CHK(run_place(ctx, lib, d_lab, p->grid_w, p->grid_h, p->max_usage, p->weight, d_order,
+                  order ? n_order : cells, d_grid, d_usage, d_stage, d_dist, d_score, d_stats));
+    int32_t st[6];
+    CHK(download(ctx, grid, d_grid, (size_t)cells));
+    if (usage) CHK(download(ctx, usage, d_usage, (size_t)lib->n));
+    CHK(download(ctx, stage, d_stage, (size_t)cells));
+    CHK(download(ctx, dist, d_dist, (size_t)cells));
+    CHK(download(ctx, score, d_score, (size_t)cells));
+    CHK(download(ctx, st, d_stats, 6));
+    CHK(sync(ctx));
+    if (stats) {
+        stats->placed = st[0];
+        stats->main_path = st[1];
+        stats->fallback_ban = st[2];
+        stats->fallback_near = st[3];
+        stats->resets = st[4];
+        stats->empty = st[5];
+    }
+    return MH_OK;
+}
+
+// ------------------------------------------------------- a10-a12: cost/SA
+int mh_total_cost(mh_context* ctx, const mh_library* lib, const int32_t* grid, int32_t gw,
+                  int32_t gh, float* out) {
+    if (!ctx || !lib || !out) return fail(MH_ERR_INVALID_ARGUMENT, "mh_total_cost: NULL argument");
+    if (gw == 0 || gh == 0) {
+        *out = 0.0f;
+        return MH_OK;
+    }
+    CHK(check_geometry(gw, gh));
+    if (!grid) return fail(MH_ERR_INVALID_ARGUMENT, "grid is NULL");
+    const int32_t cells = gw * gh;
+    CHK(check_grid_values(grid, cells, lib->n));
+    CHK(set_device(ctx));
+    int32_t* d_grid;
+    float *d_scratch, *d_out;
+    const int32_t chunk = cells < 65536 ? cells : 65536;
+    CHK(upload(ctx, S_GRID, grid, (size_t)cells, &d_grid));
+    CHK(slot(ctx, S_OUT0, (size_t)chunk * 2, &d_scratch));
+    CHK(slot(ctx, S_OUT1, 1, &d_out));
+    launch_total_cost(d_grid, gw, gh, lib->d_lab4, d_scratch, chunk, d_out, ctx->stream);
+    HIPCHK(hipGetLastError());
+    CHK(download(ctx, out, d_out, 1));
+    return sync(ctx);
+}
+
+int mh_penalties(mh_context* ctx, const mh_library* lib, const int32_t* grid, int32_t gw,
+                 int32_t gh, float weight, const int32_t* queries, int32_t q, float* out) {
+    if (!ctx || !lib || !grid || q < 0 || (q > 0 && (!queries || !out)))
+        return fail(MH_ERR_INVALID_ARGUMENT, "mh_penalties: bad arguments");
+    CHK(check_geometry(gw, gh));
+    const int32_t cells = gw * gh;
+    CHK(check_grid_values(grid, cells, lib->n));
+    for (int32_t i = 0; i < q; ++i) {
+        const int32_t m = queries[3 * i], x = queries[3 * i + 1], y = queries[3 * i + 2];
+        if (m < 0 || m >= lib->n || x < 0 || x >= gw || y < 0 || y >= gh)
+            return fail(MH_ERR_INVALID_ARGUMENT, "penalty query %d out of range", i);
+    }
+    if (q == 0) return MH_OK;
+    CHK(set_device(ctx));
+    int32_t *d_grid, *d_q;
+    float* d_out;
+    CHK(upload(ctx, S_GRID, grid, (size_t)cells, &d_grid));
+    CHK(upload(ctx, S_IN0, queries, (size_t)q * 3, &d_q));
+    CHK(slot(ctx, S_OUT0, (size_t)q, &d_out));
+    launch_penalties(d_grid, gw, gh, lib->d_lab4, weight, d_q, q, d_out, ctx->stream);
+    HIPCHK(hipGetLastError());
+    CHK(download(ctx, out, d_out, (size_t)q));
+    return sync(ctx);
+}
+
+int mh_swap_deltas(mh_context* ctx, const mh_library* lib, const int32_t* grid, int32_t gw,
+                   int32_t gh, const int32_t* pairs, int32_t p, float* out) {
+    if (!ctx || !lib || !grid || p < 0 || (p > 0 && (!pairs || !out)))
+        return fail(MH_ERR_INVALID_ARGUMENT, "mh_swap_deltas: bad arguments");
+    CHK(check_geometry(gw, gh));
+    const int32_t cells = gw * gh;
+    CHK(check_grid_values(grid, cells, lib->n));
+    for (int32_t i = 0; i < p; ++i) {
+        const int32_t* q = pairs + 4 * i;
+        if (q[0] < 0 || q[0] >= gw || q[1] < 0 || q[1] >= gh || q[2] < 0 || q[2] >= gw || q[3] < 0 ||
+            q[3] >= gh)
+            return fail(MH_ERR_INVALID_ARGUMENT, "swap pair %d out of range", i);
+    }
+    if (p == 0) return MH_OK;
+    CHK(set_device(ctx));
+    int32_t *d_grid, *d_pairs;
+    float* d_out;
+    CHK(upload(ctx, S_GRID, grid, (size_t)cells, &d_grid));
+    CHK(upload(ctx, S_IN0, pairs, (size_t)p * 4, &d_pairs));
+    CHK(slot(ctx, S_OUT0, (size_t)p, &d_out));
+    launch_swap_deltas(d_grid, gw, gh, lib->d_lab4, d_pairs, p, d_out, ctx->stream);
+    HIPCHK(hipGetLastError());
+    CHK(download(ctx, out, d_out, (size_t)p));
+    return sync(ctx);
+}
+
+static void to_result(const AnnealOut& o, mh_anneal_result* r) {
+    r->initial_cost = o.initial_cost;
+    r->final_cost = o.final_cost;
+    r->best_cost = o.best_cost;
+    r->improved_count = o.improved;
+    r->accepted_count = o.accepted;
+    r->iterations = o.iterations;
+    r->evaluated = o.evaluated;
+    r->rounds = o.rounds;
+}
+
+int mh_anneal(mh_context* ctx, const mh_library* lib, int32_t* grid, int32_t gw, int32_t gh,
+              const mh_anneal_params* p, mh_anneal_result* result, float* trace_delta,
+              uint8_t* trace_accept) {
+    if (!ctx || !lib || !p) return fail(MH_ERR_INVALID_ARGUMENT, "mh_anneal: NULL argument");
+    if (p->max_iterations < 0) return fail(MH_ERR_INVALID_ARGUMENT, "max_iterations < 0");
+    if (gw == 0 || gh == 0) {  // empty grid: OptimizationResult::default()
+        if (result) memset(result, 0, sizeof(*result));
+        return MH_OK;
+    }
+    CHK(check_geometry(gw, gh));
+    if (!grid) return fail(MH_ERR_INVALID_ARGUMENT, "grid is NULL");
+    const int32_t cells = gw * gh;
+    CHK(check_grid_values(grid, cells, lib->n));
+    CHK(set_device(ctx));
+    int32_t* d_grid;
+    AnnealOut* d_out;
+    float* d_td = nullptr;
+    uint8_t* d_ta = nullptr;
+    const size_t iters = (size_t)p->max_iterations;
+    CHK(upload(ctx, S_GRID, grid, (size_t)cells, &d_grid));
+    CHK(slot(ctx, S_ANNEAL_OUT, 1, &d_out));
+    if (trace_delta) CHK(slot(ctx, S_OUT0, iters, &d_td));
+    if (trace_accept) CHK(slot(ctx, S_OUT1, iters, &d_ta));
+    CHK(run_anneal(ctx, lib, d_grid, gw, gh, p, d_td, d_ta, d_out));
+    AnnealOut o;
+    CHK(download(ctx, grid, d_grid, (size_t)cells));
+    CHK(download(ctx, &o, d_out, 1));
+    CHK(download(ctx, trace_delta, d_td, iters));
+    CHK(download(ctx, trace_accept, d_ta, iters));
+    CHK(sync(ctx));
+    if (result) to_result(o, result);
+    return MH_OK;
+}
+
+// ----------------------------------------------- a13-a16: adjust/composite
+int mh_average_rgb(mh_context* ctx, const uint8_t* rgb, int32_t w, int32_t h, uint8_t out[3]) {
+    if (!ctx || !rgb || !out || w <= 0 || h <= 0)
+        return fail(MH_ERR_INVALID_ARGUMENT, "mh_average_rgb: bad arguments");
+    CHK(set_device(ctx));
+    uint8_t *d_in, *d_out;
+    CHK(upload(ctx, S_IN0, rgb, (size_t)w * h * 3, &d_in));
+    CHK(slot(ctx, S_OUT0, 3, &d_out));
+    launch_avg_rgb(d_in, (int64_t)w * 3, 1, 0, 0, w, h, 1, d_out, ctx->stream);
+    HIPCHK(hipGetLastError());
+    CHK(download(ctx, out, d_out, 3));
+    return sync(ctx);
+}
+
+int mh_optimal_adjustments(mh_context* ctx, const uint8_t* tile_rgb, const uint8_t* target_rgb,
+                           int32_t n, float strength, float* out) {
+    if (!ctx || n < 0 || (n > 0 && (!tile_rgb || !target_rgb || !out)))
+        return fail(MH_ERR_INVALID_ARGUMENT, "mh_optimal_adjustments: bad arguments");
+    if (n == 0) return MH_OK;
+    CHK(set_device(ctx));
+    uint8_t *d_t, *d_g;
+    float4* d_out;
+    CHK(upload(ctx, S_IN0, tile_rgb, (size_t)n * 3, &d_t));
+    CHK(upload(ctx, S_IN1, target_rgb, (size_t)n * 3, &d_g));
+    CHK(slot(ctx, S_OUT0, (size_t)n, &d_out));
+    launch_optimal_adjust(d_t, d_g, n, strength, d_out, ctx->stream);
+    HIPCHK(hipGetLastError());
+    CHK(download(ctx, reinterpret_cast<float4*>(out), d_out, (size_t)n));
+    return sync(ctx);
+}
+
+int mh_adjust_pixels(mh_context* ctx, const float adj[4], const uint8_t* in, int64_t npx,
+                     uint8_t* out) {
+    if (!ctx || !adj || npx < 0 || (npx > 0 && (!in || !out)))
+        return fail(MH_ERR_INVALID_ARGUMENT, "mh_adjust_pixels: bad arguments");
+    if (npx == 0) return MH_OK;
+    CHK(set_device(ctx));
+    uint8_t *d_in, *d_out;
+    CHK(upload(ctx, S_IN0, in, (size_t)npx * 3, &d_in));
+    CHK(slot(ctx, S_OUT0, (size_t)npx * 3, &d_out));
+    launch_adjust_pixels(make_float4(adj[0], adj[1], adj[2], adj[3]), d_in, npx, d_out, ctx->stream);
+    HIPCHK(hipGetLastError());
+    CHK(download(ctx, out, d_out, (size_t)npx * 3));
+    return sync(ctx);
+}
+
+int mh_resize(mh_context* ctx, const uint8_t* src, int32_t sw, int32_t sh, uint8_t* dst, int32_t dw,
+              int32_t dh) {
+    if (!ctx || !src || !dst || sw <= 0 || sh <= 0 || dw <= 0 || dh <= 0)
+        return fail(MH_ERR_INVALID_ARGUMENT, "mh_resize: bad arguments");
+    CHK(set_device(ctx));
+    ResizeTaps h, v;
+    CHK(get_taps(ctx, sw, dw, &h));
+    CHK(get_taps(ctx, sh, dh, &v));
+    uint8_t *d_src, *d_tmp, *d_dst;
+    CHK(upload(ctx, S_IN0, src, (size_t)sw * sh * 3, &d_src));
+    CHK(slot(ctx, S_OUT1, (size_t)sh * dw * 3, &d_tmp));
+    CHK(slot(ctx, S_OUT0, (size_t)dw * dh * 3, &d_dst));
+    launch_resize(d_src, 1, sw, sh, dw, dh, h, v, d_tmp, d_dst, ctx->stream);
+    HIPCHK(hipGetLastError());
+    CHK(download(ctx, dst, d_dst, (size_t)dw * dh * 3));
+    return sync(ctx);
+}
+
+int mh_composite(mh_context* ctx, const mh_library* lib, const uint8_t* target, int32_t width,
+                 int32_t height, const int32_t* grid, int32_t gw, int32_t gh, float strength,
+                 uint8_t* out) {
+    if (!ctx || !lib || !target || !grid || !out)
+        return fail(MH_ERR_INVALID_ARGUMENT, "mh_composite: NULL argument");
+    if (!lib->d_pixels) return fail(MH_ERR_INVALID_ARGUMENT, "library holds no pixels");
+    CHK(check_geometry(gw, gh));
+    const int32_t cells = gw * gh;
+    CHK(check_grid_values(grid, cells, lib->n));
+    const int32_t tw = width / gw, th = height / gh;
+    const size_t out_bytes = (size_t)gw * tw * gh * th * 3;
+    if (out_bytes == 0) return MH_OK;
+    CHK(set_device(ctx));
+    uint8_t *d_t, *d_out;
+    int32_t* d_grid;
+    CHK(upload(ctx, S_IN0, target, (size_t)width * height * 3, &d_t));
+    CHK(upload(ctx, S_GRID, grid, (size_t)cells, &d_grid));
+    CHK(slot(ctx, S_OUT0, out_bytes, &d_out));
+    CHK(run_composite(ctx, lib, d_t, width, height, d_grid, gw, gh, strength, d_out));
+    CHK(download(ctx, out, d_out, out_bytes));
+    return sync(ctx);
+}
+
+// -------------------------------------------------------------- whole job
+int mh_generate_device(mh_context* ctx, const mh_library* lib, const uint8_t* d_target,
+                       int32_t width, int32_t height, const mh_job_params* p,
+                       int32_t* d_grid_out, uint8_t* d_image_out) {
+    if (!ctx || !lib || !p || !d_target) return fail(MH_ERR_INVALID_ARGUMENT, "mh_generate: NULL argument");
+    CHK(check_geometry(p->grid_w, p->grid_h));
+    if (p->max_usage < 0) return fail(MH_ERR_INVALID_ARGUMENT, "max_usage must be >= 0");
+    if (p->optimization_iterations < 0)
+        return fail(MH_ERR_INVALID_ARGUMENT, "optimization_iterations < 0");
+    const int32_t gw = p->grid_w, gh = p->grid_h, cells = gw * gh;
+    const int32_t tw = width / gw, th = height / gh;
+    if (tw <= 0 || th <= 0)
+        return fail(MH_ERR_INVALID_ARGUMENT, "image %d x %d smaller than grid %d x %d", width, height,
+                    gw, gh);
+    if (d_image_out && !lib->d_pixels)
+        return fail(MH_ERR_INVALID_ARGUMENT, "compositing needs a library with pixels");
+    CHK(set_device(ctx));
+    float* d_lab;
+    int32_t* d_grid = d_grid_out;
+    uint32_t* d_usage;
+    int32_t* d_stats;
+    AnnealOut* d_aout;
+    CHK(slot(ctx, S_CELL_LAB, (size_t)cells * 3, &d_lab));
+    if (!d_grid) CHK(slot(ctx, S_GRID, (size_t)cells, &d_grid));
+    CHK(slot(ctx, S_USAGE, (size_t)(lib->n > 0 ? lib->n : 1), &d_usage));
+    CHK(slot(ctx, S_STATS, 8, &d_stats));
+    CHK(slot(ctx, S_ANNEAL_OUT, 1, &d_aout));
+    // a1: target cell Lab means (src/main.rs:536-541)
+    launch_cell_lab(d_target, (int64_t)width * 3, gw, (int64_t)th * width * 3, (int64_t)tw * 3, tw,
+                    th, cells, ctx->d_lut, d_lab, ctx->stream);
+    fill_i32(d_grid, cells, kEmpty, ctx->stream);
+    fill_i32(reinterpret_cast<int32_t*>(d_usage), lib->n, 0, ctx->stream);
+    // a3-a9: candidates + placement chain
+    CHK(run_place(ctx, lib, d_lab, gw, gh, p->max_usage, p->weight, nullptr, cells, d_grid, d_usage,
+                  nullptr, nullptr, nullptr, d_stats));
+    // a10-a12: SA when enabled and weight > 0 (src/main.rs:608)
+    const bool sa = p->enable_optimization && p->weight > 0.0f;
+    ctx->last_ran_anneal = sa ? 1 : 0;
+    ctx->last_iters = p->optimization_iterations;
+    if (sa) {
+        mh_anneal_params ap;
+        ap.max_iterations = p->optimization_iterations;
+        ap.initial_temperature = p->initial_temperature;
+        ap.temperature_decay = p->temperature_decay;
+        ap.seed = p->seed;
+        ap.greedy = 0;
+        CHK(run_anneal(ctx, lib, d_grid, gw, gh, &ap, nullptr, nullptr, d_aout));
+    }
+    // a13-a16: composite (strength clamped, src/main.rs:162)
+    float strength = p->color_strength;
+    strength = strength < 0.0f ? 0.0f : (strength > 1.0f ? 1.0f : strength);
+    if (d_image_out)
+        CHK(run_composite(ctx, lib, d_target, width, height, d_grid, gw, gh, strength, d_image_out));
+    ctx->last_out_w = gw * tw;
+    ctx->last_out_h = gh * th;
+    HIPCHK(hipGetLastError());
+    return MH_OK;
+}
+
+int mh_generate_device_result(mh_context* ctx, mh_job_result* r) {
+    if (!ctx || !r) return fail(MH_ERR_INVALID_ARGUMENT, "NULL argument");
+    CHK(set_device(ctx));
+    int32_t st[6];
+    AnnealOut o;
+    memset(&o, 0, sizeof(o));
+    CHK(download(ctx, st, (const int32_t*)ctx->slots[S_STATS].p, 6));
+    if (ctx->last_ran_anneal) CHK(download(ctx, &o, (const AnnealOut*)ctx->slots[S_ANNEAL_OUT].p, 1));
+    CHK(sync(ctx));
+    memset(r, 0, sizeof(*r));
+    r->place.placed = st[0];
+    r->place.main_path = st[1];
+    r->place.fallback_ban = st[2];
+    r->place.fallback_near = st[3];
+    r->place.resets = st[4];
+    r->place.empty = st[5];
+    if (ctx->last_ran_anneal) to_result(o, &r->anneal);
+    r->out_width = ctx->last_out_w;
+    r->out_height = ctx->last_out_h;
+    return MH_OK;
+}
+
+int mh_generate(mh_context* ctx, const mh_library* lib, const uint8_t* target, int32_t width,
+                int32_t height, const mh_job_params* p, int32_t* grid_out, uint8_t* image_out,
+                mh_job_result* result) {
+    if (!ctx || !lib || !p || !target) return fail(MH_ERR_INVALID_ARGUMENT, "mh_generate: NULL argument");
+    CHK(check_geometry(p->grid_w, p->grid_h));
+    if (width <= 0 || height <= 0) return fail(MH_ERR_INVALID_ARGUMENT, "bad image size");
+    CHK(set_device(ctx));
+    const int32_t cells = p->grid_w * p->grid_h;
+    const int32_t tw = width / p->grid_w, th = height / p->grid_h;
+    const size_t out_bytes = (size_t)p->grid_w * tw * p->grid_h * th * 3;
+    uint8_t *d_t, *d_img = nullptr;
+    int32_t* d_grid;
+    CHK(upload(ctx, S_IN0, target, (size_t)width * height * 3, &d_t));
+    CHK(slot(ctx, S_IN1, (size_t)cells, &d_grid));
+    if (image_out && out_bytes) CHK(slot(ctx, S_OUT3, out_bytes, &d_img));
+    CHK(mh_generate_device(ctx, lib, d_t, width, height, p, d_grid, d_img));
+    CHK(download(ctx, grid_out, d_grid, (size_t)cells));
+    if (d_img) CHK(download(ctx, image_out, d_img, out_bytes));
+    mh_job_result r;
+    CHK(mh_generate_device_result(ctx, &r));
+    if (result) *result = r;
+    return MH_OK;
+}
+
+}  // extern "C"

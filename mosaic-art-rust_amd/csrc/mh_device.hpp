@@ -1,0 +1,259 @@
+// mh_device.hpp -- device-side arithmetic shared by the gfx950 kernels.
+//
+// Bit-level contract: IEEE f32/f64 with the reference's operation order and
+// no FMA contraction (the whole library is built with -ffp-contract=off; the
+// pragma below keeps it so for anyone including this header).  f32 divide and
+// sqrt are HIP's correctly rounded defaults.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mh_kernels.hpp"
+
+#pragma clang fp contract(off)
+
+namespace mh {
+
+// ----------------------------------------------------------------- helpers
+__device__ __forceinline__ float clampf(float v, float lo, float hi) {
+    // Rust f32::clamp: NaN passes through.
+    if (v < lo) return lo;
+    if (v > hi) return hi;
+    return v;
+}
+
+// Cube root for Lab's f(t): exponent/3 seed on the double's high word, four
+// Newton steps t <- t - (t^3 - x)/(3 t^2) in double, one rounding to f32.
+// Uses only IEEE +,-,*,/ so host and device produce the same bits.
+__device__ __forceinline__ float cbrt_f32(float xf) {
+    double x = (double)xf;
+    uint64_t bits = (uint64_t)__double_as_longlong(x);
+    uint32_t hi = (uint32_t)(bits >> 32);
+    hi = hi / 3u + 715094163u;
+    double t = __longlong_as_double((long long)((uint64_t)hi << 32));
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        double t2 = t * t;
+        t = t - (t2 * t - x) / (3.0 * t2);
+    }
+    return (float)t;
+}
+
+// exp for f32 arguments: Cody-Waite reduction with fdlibm's ln2 split,
+// degree-13 Taylor polynomial in double (Horner), exact 2^n scaling.
+__device__ __forceinline__ float exp_f32(float xf) {
+    if (xf != xf) return xf;
+    if (xf > 88.8f) return __int_as_float(0x7f800000);
+    if (xf < -104.0f) return 0.0f;
+    double x = (double)xf;
+    double n = floor(x * 1.4426950408889634 + 0.5);
+    double r = (x - n * 6.93147180369123816490e-01) - n * 1.90821492927058770002e-10;
+    double p = 1.0 / 6227020800.0;
+    p = p * r + 1.0 / 479001600.0;
+    p = p * r + 1.0 / 39916800.0;
+    p = p * r + 1.0 / 3628800.0;
+    p = p * r + 1.0 / 362880.0;
+    p = p * r + 1.0 / 40320.0;
+    p = p * r + 1.0 / 5040.0;
+    p = p * r + 1.0 / 720.0;
+    p = p * r + 1.0 / 120.0;
+    p = p * r + 1.0 / 24.0;
+    p = p * r + 1.0 / 6.0;
+    p = p * r + 0.5;
+    p = p * r + 1.0;
+    p = p * r + 1.0;
+    int ni = (int)n;
+    double s = __longlong_as_double((long long)((uint64_t)(int64_t)(ni + 1023) << 52));
+    return (float)(p * s);
+}
+
+// ------------------------------------------------------------ sRGB -> Lab
+__device__ __forceinline__ float lab_f(float t) {
+    const float eps = (float)(216.0 / 24389.0);
+    const float kappa = (float)(841.0 / 108.0);
+    const float delta = (float)(4.0 / 29.0);
+    if (t > eps) return cbrt_f32(t);
+    return kappa * t + delta;
+}
+
+// palette Lab::from_color(Srgb<f32>) on a u8 triple (src/lib.rs:102-107):
+// lut[] is the linearised u8 ramp; IEC 61966-2-1 matrix, D65 white point,
+// CIE f(t), FromColor clamp of L/a/b.
+__device__ __forceinline__ void rgb8_to_lab(const float* lut, uint32_t r8, uint32_t g8, uint32_t b8,
+                                            float& L, float& A, float& B) {
+    float r = lut[r8], g = lut[g8], b = lut[b8];
+    float x = 0.4124564f * r + 0.3575761f * g + 0.1804375f * b;
+    float y = 0.2126729f * r + 0.7151522f * g + 0.0721750f * b;
+    float z = 0.0193339f * r + 0.1191920f * g + 0.9503041f * b;
+    x = x / 0.95047f;
+    y = y / 1.0f;
+    z = z / 1.08883f;
+    float fx = lab_f(x), fy = lab_f(y), fz = lab_f(z);
+    L = clampf(fy * 116.0f - 16.0f, 0.0f, 100.0f);
+    A = clampf((fx - fy) * 500.0f, -128.0f, 127.0f);
+    B = clampf((fy - fz) * 200.0f, -128.0f, 127.0f);
+}
+
+// ------------------------------------------------------------- distances
+// calculate_lab_distance (src/similarity.rs:141-146).  (x-y)^2 == (y-x)^2
+// exactly, so dist(i,j) and dist(j,i) have the same bits.
+__device__ __forceinline__ float lab_distance(float4 p, float4 q) {
+    float dl = p.x - q.x, da = p.y - q.y, db = p.z - q.z;
+    return sqrtf(dl * dl + da * da + db * db);
+}
+
+// kiddo SquaredEuclidean: ((0 + dl^2) + da^2) + db^2; 0 + x is exact.
+__device__ __forceinline__ float sq_distance(float ql, float qa, float qb, float ml, float ma,
+                                             float mb) {
+    float d0 = ql - ml, d1 = qa - ma, d2 = qb - mb;
+    return d0 * d0 + d1 * d1 + d2 * d2;
+}
+
+// One adjacency term 1/(sim+1) (src/adjacency.rs:82, :110, :174).
+// get_similarity(i,i) = 0.0, which the distance formula also gives.
+__device__ __forceinline__ float adj_term(float4 p, float4 q) {
+    return 1.0f / (lab_distance(p, q) + 1.0f);
+}
+
+// ----------------------------------------------------------- counter RNG
+// SplitMix64 finaliser (Steele, Lea, Flood 2014) of
+// seed + golden * (8*it + k + 1); high 32 bits.
+__device__ __forceinline__ uint32_t draw(uint64_t seed, uint64_t it, uint32_t k) {
+    uint64_t z = seed + 0x9E3779B97F4A7C15ULL * (it * 8u + k + 1u);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z = z ^ (z >> 31);
+    return (uint32_t)(z >> 32);
+}
+
+__device__ __forceinline__ int32_t draw_range(uint64_t seed, uint64_t it, uint32_t k,
+                                              int32_t bound) {
+    return (int32_t)(((uint64_t)draw(seed, it, k) * (uint64_t)(uint32_t)bound) >> 32);
+}
+
+__device__ __forceinline__ float draw_unit(uint64_t seed, uint64_t it) {
+    return (float)(draw(seed, it, 4) >> 8) * (1.0f / 16777216.0f);
+}
+
+// ------------------------------------------------------------ HSV helpers
+// palette Hsv::from_color(Srgb<f32>) (src/color_adjustment.rs:70, :122).
+__device__ __forceinline__ void rgb_to_hsv(float r, float g, float b, float& h, float& s,
+                                           float& v) {
+    r = r > 0.0f ? r : 0.0f;
+    g = g > 0.0f ? g : 0.0f;
+    b = b > 0.0f ? b : 0.0f;
+    float mx, mn, sep, coeff;
+    if (r > g) {
+        mx = r; mn = g; sep = g - b; coeff = 0.0f;
+    } else {
+        mx = g; mn = r; sep = b - r; coeff = 2.0f;
+    }
+    if (b > mx) {
+        mx = b; sep = r - g; coeff = 4.0f;
+    } else if (b < mn) {
+        mn = b;
+    }
+    float hh = 0.0f, ss = 0.0f;
+    if (mx != mn) {
+        float d = mx - mn;
+        hh = (sep / d + coeff) * 60.0f;
+        ss = d / mx;
+    }
+    h = hh;
+    s = clampf(ss, 0.0f, 1.0f);
+    v = clampf(mx, 0.0f, 1.0f);
+}
+
+// palette Srgb::from_color(Hsv) (src/color_adjustment.rs:76).
+__device__ __forceinline__ void hsv_to_rgb(float hdeg, float s, float v, float& r, float& g,
+                                           float& b) {
+    float c = v * s;
+    float hp = hdeg - floorf(hdeg / 360.0f) * 360.0f;
+    float h = hp / 60.0f;
+    float hm2 = h - 2.0f * floorf(h * 0.5f);
+    float x = c * (1.0f - fabsf(hm2 - 1.0f));
+    float m = v - c;
+    float rr, gg, bb;
+    if (h >= 0.0f && h < 1.0f) { rr = c; gg = x; bb = 0.0f; }
+    else if (h >= 1.0f && h < 2.0f) { rr = x; gg = c; bb = 0.0f; }
+    else if (h >= 2.0f && h < 3.0f) { rr = 0.0f; gg = c; bb = x; }
+    else if (h >= 3.0f && h < 4.0f) { rr = 0.0f; gg = x; bb = c; }
+    else if (h >= 4.0f && h < 5.0f) { rr = x; gg = 0.0f; bb = c; }
+    else { rr = c; gg = 0.0f; bb = x; }
+    r = clampf(rr + m, 0.0f, 1.0f);
+    g = clampf(gg + m, 0.0f, 1.0f);
+    b = clampf(bb + m, 0.0f, 1.0f);
+}
+
+__device__ __forceinline__ float bright_contrast(float v, float brightness, float contrast) {
+    float c = clampf((v - 0.5f) * contrast + 0.5f, 0.0f, 1.0f);
+    return clampf(c + brightness, 0.0f, 1.0f);
+}
+
+__device__ __forceinline__ uint32_t to_u8(float v) {
+    float t = clampf(v * 255.0f, 0.0f, 255.0f);
+    if (t != t) return 0;
+    return (uint32_t)t;
+}
+
+// ColorAdjustment::adjust_pixel (src/color_adjustment.rs:53-87).
+// adj = {brightness, contrast, hue_shift, saturation}.
+__device__ __forceinline__ void adjust_pixel(float4 adj, uint32_t r8, uint32_t g8, uint32_t b8,
+                                             uint32_t& ro, uint32_t& go, uint32_t& bo) {
+    float r = (float)r8 / 255.0f, g = (float)g8 / 255.0f, b = (float)b8 / 255.0f;
+    float br = bright_contrast(r, adj.x, adj.y);
+    float bg = bright_contrast(g, adj.x, adj.y);
+    float bb = bright_contrast(b, adj.x, adj.y);
+    if (adj.z != 0.0f || adj.w != 1.0f) {
+        float h, s, v;
+        rgb_to_hsv(br, bg, bb, h, s, v);
+        float hr = fmodf(h + adj.z, 360.0f);  // rem_euclid(360.0)
+        if (hr < 0.0f) hr = hr + 360.0f;
+        float s2 = clampf(s * adj.w, 0.0f, 1.0f);
+        hsv_to_rgb(hr, s2, v, br, bg, bb);
+    }
+    ro = to_u8(br);
+    go = to_u8(bg);
+    bo = to_u8(bb);
+}
+
+// calculate_optimal_adjustment (src/color_adjustment.rs:99-154), followed
+// by ColorAdjustment::new's clamps (:30-37).
+__device__ __forceinline__ float4 optimal_adjustment(uint32_t t0, uint32_t t1, uint32_t t2,
+                                                     uint32_t g0, uint32_t g1, uint32_t g2,
+                                                     float k) {
+    float tr = (float)t0 / 255.0f, tg = (float)t1 / 255.0f, tb = (float)t2 / 255.0f;
+    float gr = (float)g0 / 255.0f, gg = (float)g1 / 255.0f, gb = (float)g2 / 255.0f;
+    float tile_luma = 0.299f * tr + 0.587f * tg + 0.114f * tb;
+    float target_luma = 0.299f * gr + 0.587f * gg + 0.114f * gb;
+    float brightness = (target_luma - tile_luma) * k;
+    float th, ts, tv, gh, gs, gv;
+    rgb_to_hsv(tr, tg, tb, th, ts, tv);
+    rgb_to_hsv(gr, gg, gb, gh, gs, gv);
+    float hue = 0.0f;
+    if (ts > 0.1f && gs > 0.1f) {
+        float diff = gh - th;
+        float wrapped = diff > 180.0f ? diff - 360.0f : (diff < -180.0f ? diff + 360.0f : diff);
+        hue = wrapped * k * 0.5f;
+    }
+    float sat = 1.0f;
+    if (ts > 0.01f) {
+        float ratio = gs / ts;
+        sat = 1.0f + (ratio - 1.0f) * k * 0.7f;
+    }
+    return make_float4(clampf(brightness, -1.0f, 1.0f), clampf(1.0f, 0.0f, 2.0f),
+                       clampf(hue, -180.0f, 180.0f), clampf(sat, 0.0f, 2.0f));
+}
+
+// ------------------------------------------------------ wave reductions
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        uint64_t o = __shfl_xor(v, off, 64);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+}  // namespace mh

@@ -1,0 +1,112 @@
+// mh_kernels.hpp -- job descriptors and host-side launchers of the gfx950
+// kernels (one translation unit per kernel family; the C ABI in mh_api.hip
+// only calls these launchers).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace mh {
+
+constexpr int32_t kEmpty = -1;       // MH_EMPTY
+constexpr int32_t kMaxCand = 100;    // K = min(n, 100), src/main.rs:348
+constexpr int32_t kCandStride = 128; // per-cell candidate row (padded)
+
+// One placement chain (one wave).  All pointers are device pointers.
+struct PlaceJob {
+    const float4* lab4;      // n material Labs {l,a,b,0}
+    int32_t n;
+    const int32_t* cand_idx;  // cells x kCandStride, ascending (sqdist, idx)
+    const float* cand_d;
+    const int32_t* order;     // n_order cell indices, or nullptr = raster
+    int32_t n_order;
+    int32_t* grid;            // cells, in/out
+    uint32_t* usage;          // n, in/out
+    int32_t grid_w, grid_h;
+    int32_t max_usage;
+    float weight;
+    uint8_t* stage;           // optional per-cell outputs
+    float* dist;
+    float* score;
+    int32_t* stats;           // 6 ints: placed, main, ban, near, resets, empty
+};
+
+struct AnnealOut {
+    float initial_cost, final_cost, best_cost;
+    int64_t improved, accepted, iterations, evaluated, rounds;
+};
+
+// One annealing chain (one wave).
+struct AnnealJob {
+    const float4* lab4;
+    int32_t n;
+    int32_t* grid;            // cells, in/out
+    int32_t grid_w, grid_h;
+    int64_t max_iter;
+    const float* temps;       // temperature after k non-skipped proposals
+    uint64_t seed;
+    int32_t greedy;
+    float* trace_delta;       // optional, max_iter entries
+    uint8_t* trace_accept;
+    AnnealOut* result;
+};
+
+// Fixed-point Lanczos3 taps for one axis (device pointers).
+struct ResizeTaps {
+    const int32_t* lo;
+    const int32_t* cnt;
+    const int32_t* k;
+    int32_t ksize;
+};
+
+void launch_cell_lab(const uint8_t* rgb, int64_t row_stride, int32_t per_row,
+                     int64_t block_row_stride, int64_t block_col_stride, int32_t cw, int32_t ch,
+                     int32_t n_cells, const float* lut, float* out, hipStream_t stream);
+void launch_sim_full(const float4* lab, int32_t n, float* out, hipStream_t stream);
+void launch_sim_packed(const float4* lab, int32_t n, float* out, hipStream_t stream);
+void launch_topk(const float* mL, const float* mA, const float* mB, int32_t n, const float* query,
+                 int32_t nq, int32_t k, int32_t* out_idx, float* out_d, int32_t out_stride,
+                 hipStream_t stream);
+size_t place_lds_bytes(int32_t cells, int32_t n);
+// One job passed by value (kernel argument), or n_jobs jobs from a device
+// array (one wave each).
+void launch_place_single(const PlaceJob& job, size_t lds_bytes, hipStream_t stream);
+void launch_place_batch(const PlaceJob* d_jobs, int32_t n_jobs, size_t lds_bytes,
+                        hipStream_t stream);
+size_t anneal_lds_bytes(int32_t cells, bool narrow);
+void launch_anneal_single(const AnnealJob& job, bool narrow, size_t lds_bytes, hipStream_t stream);
+void launch_anneal_batch(const AnnealJob* d_jobs, int32_t n_jobs, bool narrow, size_t lds_bytes,
+                         hipStream_t stream);
+void launch_total_cost(const int32_t* grid, int32_t gw, int32_t gh, const float4* lab4,
+                       float* scratch, int32_t chunk, float* out, hipStream_t stream);
+void launch_swap_deltas(const int32_t* grid, int32_t gw, int32_t gh, const float4* lab4,
+                        const int32_t* pairs, int32_t p, float* out, hipStream_t stream);
+void launch_penalties(const int32_t* grid, int32_t gw, int32_t gh, const float4* lab4, float weight,
+                      const int32_t* queries, int32_t nq, float* out, hipStream_t stream);
+void launch_resize(const uint8_t* src, int32_t n, int32_t sw, int32_t sh, int32_t dw, int32_t dh,
+                   const ResizeTaps& h, const ResizeTaps& v, uint8_t* tmp, uint8_t* dst,
+                   hipStream_t stream);
+void launch_avg_rgb(const uint8_t* rgb, int64_t row_stride, int32_t per_row,
+                    int64_t block_row_stride, int64_t block_col_stride, int32_t cw, int32_t ch,
+                    int32_t n_cells, uint8_t* out, hipStream_t stream);
+void launch_cell_adjust(const int32_t* grid, int32_t cells, const uint8_t* tile_avg,
+                        const uint8_t* target_avg, float strength, float4* out,
+                        hipStream_t stream);
+void launch_composite(const int32_t* grid, int32_t gw, int32_t gh, int32_t tw, int32_t th,
+                      const uint8_t* tiles, const float4* adj, int32_t adjust, uint8_t* out,
+                      hipStream_t stream);
+void launch_adjust_pixels(float4 adj, const uint8_t* in, int64_t npx, uint8_t* out,
+                          hipStream_t stream);
+void launch_optimal_adjust(const uint8_t* tile, const uint8_t* target, int32_t n, float k,
+                           float4* out, hipStream_t stream);
+
+// Kernels whose dynamic LDS may exceed 64 KB get their limit raised once
+// (to the per-block maximum minus the kernel's static LDS).
+hipError_t raise_dynamic_lds(const void* kernel, int max_lds);
+hipError_t configure_kernel_limits(int max_lds);
+hipError_t place_configure(int max_lds);
+hipError_t anneal_configure(int max_lds);
+hipError_t topk_configure(int max_lds);
+
+}  // namespace mh

@@ -1,0 +1,392 @@
+"""ctypes binding of libmosaic_hip.so (include/mosaic_hip.h) with numpy I/O.
+
+This is plumbing for tests, the benchmark and Python callers; the product is
+the C ABI and its gfx950 kernels.  Loading fails loudly when the library is
+missing -- there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmosaic_hip.so")
+
+MH_OK = 0
+MH_ERR_INVALID_ARGUMENT = -1
+MH_ERR_NO_DEVICE = -2
+MH_ERR_HIP = -3
+MH_ERR_OUT_OF_MEMORY = -4
+MH_ERR_NOT_FOUND = -5
+MH_ERR_UNSUPPORTED = -6
+MH_EMPTY = -1
+
+# Every symbol include/mosaic_hip.h declares.
+EXPORTS = [
+    "mh_last_error", "mh_abi_version", "mh_context_create", "mh_context_destroy",
+    "mh_context_synchronize", "mh_context_stream", "mh_library_from_lab",
+    "mh_library_from_pixels", "mh_library_destroy", "mh_library_size", "mh_library_lab",
+    "mh_library_build_similarities", "mh_library_similarities", "mh_library_get_similarity",
+    "mh_average_lab", "mh_cell_lab_means", "mh_nearest_n", "mh_place", "mh_total_cost",
+    "mh_penalties", "mh_swap_deltas", "mh_anneal", "mh_average_rgb", "mh_optimal_adjustments",
+    "mh_adjust_pixels", "mh_resize", "mh_composite", "mh_generate", "mh_generate_device",
+    "mh_generate_device_result",
+]
+
+
+class MhError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"mosaic_hip error {code}: {msg}")
+        self.code = code
+
+
+class PlaceParams(ctypes.Structure):
+    _fields_ = [("grid_w", ctypes.c_int32), ("grid_h", ctypes.c_int32),
+                ("max_usage", ctypes.c_int32), ("weight", ctypes.c_float)]
+
+
+class PlaceStats(ctypes.Structure):
+    _fields_ = [("placed", ctypes.c_int32), ("main_path", ctypes.c_int32),
+                ("fallback_ban", ctypes.c_int32), ("fallback_near", ctypes.c_int32),
+                ("resets", ctypes.c_int32), ("empty", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class AnnealParams(ctypes.Structure):
+    _fields_ = [("max_iterations", ctypes.c_int64), ("initial_temperature", ctypes.c_float),
+                ("temperature_decay", ctypes.c_float), ("seed", ctypes.c_uint64),
+                ("greedy", ctypes.c_int32)]
+
+
+class AnnealResult(ctypes.Structure):
+    _fields_ = [("initial_cost", ctypes.c_float), ("final_cost", ctypes.c_float),
+                ("best_cost", ctypes.c_float), ("improved_count", ctypes.c_int64),
+                ("accepted_count", ctypes.c_int64), ("iterations", ctypes.c_int64),
+                ("evaluated", ctypes.c_int64), ("rounds", ctypes.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class JobParams(ctypes.Structure):
+    _fields_ = [("grid_w", ctypes.c_int32), ("grid_h", ctypes.c_int32),
+                ("max_usage", ctypes.c_int32), ("weight", ctypes.c_float),
+                ("color_strength", ctypes.c_float), ("enable_optimization", ctypes.c_int32),
+                ("optimization_iterations", ctypes.c_int64),
+                ("initial_temperature", ctypes.c_float), ("temperature_decay", ctypes.c_float),
+                ("seed", ctypes.c_uint64)]
+
+
+class JobResult(ctypes.Structure):
+    _fields_ = [("place", PlaceStats), ("anneal", AnnealResult),
+                ("out_width", ctypes.c_int32), ("out_height", ctypes.c_int32)]
+
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Loads the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: run __graft_entry__.build() first")
+    lib = ctypes.CDLL(path)
+    P = ctypes.c_void_p
+    i32, i64, f32, u64 = ctypes.c_int32, ctypes.c_int64, ctypes.c_float, ctypes.c_uint64
+    sig = {
+        "mh_last_error": (ctypes.c_char_p, []),
+        "mh_abi_version": (i32, []),
+        "mh_context_create": (ctypes.c_int, [i32, ctypes.POINTER(P)]),
+        "mh_context_destroy": (None, [P]),
+        "mh_context_synchronize": (ctypes.c_int, [P]),
+        "mh_context_stream": (P, [P]),
+        "mh_library_from_lab": (ctypes.c_int, [P, P, i32, ctypes.POINTER(P)]),
+        "mh_library_from_pixels": (ctypes.c_int, [P, P, i32, i32, i32, ctypes.POINTER(P)]),
+        "mh_library_destroy": (None, [P]),
+        "mh_library_size": (i32, [P]),
+        "mh_library_lab": (ctypes.c_int, [P, P]),
+        "mh_library_build_similarities": (ctypes.c_int, [P]),
+        "mh_library_similarities": (ctypes.c_int, [P, P, i32]),
+        "mh_library_get_similarity": (ctypes.c_int, [P, i32, i32, ctypes.POINTER(f32)]),
+        "mh_average_lab": (ctypes.c_int, [P, P, i32, i32, P]),
+        "mh_cell_lab_means": (ctypes.c_int, [P, P, i32, i32, i32, i32, P]),
+        "mh_nearest_n": (ctypes.c_int, [P, P, P, i32, i32, P, P]),
+        "mh_place": (ctypes.c_int, [P, P, P, ctypes.POINTER(PlaceParams), P, i32, P, P, P, P, P,
+                                    ctypes.POINTER(PlaceStats)]),
+        "mh_total_cost": (ctypes.c_int, [P, P, P, i32, i32, ctypes.POINTER(f32)]),
+        "mh_penalties": (ctypes.c_int, [P, P, P, i32, i32, f32, P, i32, P]),
+        "mh_swap_deltas": (ctypes.c_int, [P, P, P, i32, i32, P, i32, P]),
+        "mh_anneal": (ctypes.c_int, [P, P, P, i32, i32, ctypes.POINTER(AnnealParams),
+                                     ctypes.POINTER(AnnealResult), P, P]),
+        "mh_average_rgb": (ctypes.c_int, [P, P, i32, i32, P]),
+        "mh_optimal_adjustments": (ctypes.c_int, [P, P, P, i32, f32, P]),
+        "mh_adjust_pixels": (ctypes.c_int, [P, P, P, i64, P]),
+        "mh_resize": (ctypes.c_int, [P, P, i32, i32, P, i32, i32]),
+        "mh_composite": (ctypes.c_int, [P, P, P, i32, i32, P, i32, i32, f32, P]),
+        "mh_generate": (ctypes.c_int, [P, P, P, i32, i32, ctypes.POINTER(JobParams), P, P,
+                                       ctypes.POINTER(JobResult)]),
+        "mh_generate_device": (ctypes.c_int, [P, P, P, i32, i32, ctypes.POINTER(JobParams), P, P]),
+        "mh_generate_device_result": (ctypes.c_int, [P, ctypes.POINTER(JobResult)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc != MH_OK:
+        raise MhError(rc, load().mh_last_error().decode())
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+class Context:
+    """One HIP device + stream (mh_context)."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        _check(self.lib.mh_context_create(device, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.mh_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        _check(self.lib.mh_context_synchronize(self.h))
+
+    @property
+    def stream(self):
+        return self.lib.mh_context_stream(self.h)
+
+    # ---------------------------------------------------------------- a1
+    def average_lab(self, rgb):
+        rgb = _c(rgb, np.uint8)
+        h, w = rgb.shape[:2]
+        out = np.zeros(3, np.float32)
+        _check(self.lib.mh_average_lab(self.h, _ptr(rgb), w, h, _ptr(out)))
+        return out
+
+    def cell_lab_means(self, rgb, grid_w, grid_h):
+        rgb = _c(rgb, np.uint8)
+        h, w = rgb.shape[:2]
+        out = np.zeros((grid_w * grid_h, 3), np.float32)
+        _check(self.lib.mh_cell_lab_means(self.h, _ptr(rgb), w, h, grid_w, grid_h, _ptr(out)))
+        return out
+
+    # ------------------------------------------------------------ library
+    def library_from_lab(self, lab):
+        return Library(self, lab=lab)
+
+    def library_from_pixels(self, rgb):
+        return Library(self, pixels=rgb)
+
+    # --------------------------------------------------------------- a3
+    def nearest_n(self, lib, query, k):
+        q = _c(query, np.float32).reshape(-1, 3)
+        kk = min(k, lib.n)
+        idx = np.zeros((q.shape[0], kk), np.int32)
+        d = np.zeros((q.shape[0], kk), np.float32)
+        _check(self.lib.mh_nearest_n(self.h, lib.h, _ptr(q), q.shape[0], k, _ptr(idx), _ptr(d)))
+        return idx, d
+
+    # ------------------------------------------------------------ a4-a9
+    def place(self, lib, cell_lab, grid_w, grid_h, max_usage, weight, order=None, grid=None,
+              usage=None):
+        cells = grid_w * grid_h
+        cl = _c(cell_lab, np.float32).reshape(cells, 3)
+        g = np.full(cells, MH_EMPTY, np.int32) if grid is None else _c(grid, np.int32).copy()
+        u = np.zeros(max(lib.n, 1), np.uint32) if usage is None else _c(usage, np.uint32).copy()
+        o = None if order is None else _c(order, np.int32)
+        stage = np.full(cells, 255, np.uint8)
+        dist = np.full(cells, np.nan, np.float32)
+        score = np.full(cells, np.nan, np.float32)
+        st = PlaceStats()
+        prm = PlaceParams(grid_w, grid_h, max_usage, weight)
+        _check(self.lib.mh_place(self.h, lib.h, _ptr(cl), ctypes.byref(prm), _ptr(o),
+                                 0 if o is None else len(o), _ptr(g), _ptr(u), _ptr(stage),
+                                 _ptr(dist), _ptr(score), ctypes.byref(st)))
+        return {"grid": g, "usage": u[:lib.n], "stage": stage, "dist": dist, "score": score,
+                "stats": st.as_dict()}
+
+    # ----------------------------------------------------------- a10-a12
+    def total_cost(self, lib, grid, grid_w, grid_h):
+        g = _c(grid, np.int32)
+        out = ctypes.c_float()
+        _check(self.lib.mh_total_cost(self.h, lib.h, _ptr(g), grid_w, grid_h, ctypes.byref(out)))
+        return out.value
+
+    def penalties(self, lib, grid, grid_w, grid_h, weight, queries):
+        g = _c(grid, np.int32)
+        q = _c(queries, np.int32).reshape(-1, 3)
+        out = np.zeros(q.shape[0], np.float32)
+        _check(self.lib.mh_penalties(self.h, lib.h, _ptr(g), grid_w, grid_h, weight, _ptr(q),
+                                     q.shape[0], _ptr(out)))
+        return out
+
+    def swap_deltas(self, lib, grid, grid_w, grid_h, pairs):
+        g = _c(grid, np.int32)
+        p = _c(pairs, np.int32).reshape(-1, 4)
+        out = np.zeros(p.shape[0], np.float32)
+        _check(self.lib.mh_swap_deltas(self.h, lib.h, _ptr(g), grid_w, grid_h, _ptr(p),
+                                       p.shape[0], _ptr(out)))
+        return out
+
+    def anneal(self, lib, grid, grid_w, grid_h, max_iterations, initial_temperature=100.0,
+               temperature_decay=0.99995, seed=0, greedy=False, trace=False):
+        g = _c(grid, np.int32).copy()
+        prm = AnnealParams(max_iterations, initial_temperature, temperature_decay, seed,
+                           1 if greedy else 0)
+        res = AnnealResult()
+        td = np.zeros(max(max_iterations, 1), np.float32) if trace else None
+        ta = np.zeros(max(max_iterations, 1), np.uint8) if trace else None
+        _check(self.lib.mh_anneal(self.h, lib.h, _ptr(g), grid_w, grid_h, ctypes.byref(prm),
+                                  ctypes.byref(res), _ptr(td), _ptr(ta)))
+        out = {"grid": g, "result": res.as_dict()}
+        if trace:
+            out["trace_delta"] = td[:max_iterations]
+            out["trace_accept"] = ta[:max_iterations]
+        return out
+
+    # ----------------------------------------------------------- a13-a16
+    def average_rgb(self, rgb):
+        rgb = _c(rgb, np.uint8)
+        h, w = rgb.shape[:2]
+        out = np.zeros(3, np.uint8)
+        _check(self.lib.mh_average_rgb(self.h, _ptr(rgb), w, h, _ptr(out)))
+        return out
+
+    def optimal_adjustments(self, tile_rgb, target_rgb, strength):
+        t = _c(tile_rgb, np.uint8).reshape(-1, 3)
+        g = _c(target_rgb, np.uint8).reshape(-1, 3)
+        out = np.zeros((t.shape[0], 4), np.float32)
+        _check(self.lib.mh_optimal_adjustments(self.h, _ptr(t), _ptr(g), t.shape[0], strength,
+                                               _ptr(out)))
+        return out
+
+    def adjust_pixels(self, adj, rgb):
+        a = _c(adj, np.float32)
+        p = _c(rgb, np.uint8)
+        out = np.zeros_like(p)
+        _check(self.lib.mh_adjust_pixels(self.h, _ptr(a), _ptr(p), p.size // 3, _ptr(out)))
+        return out
+
+    def resize(self, rgb, dw, dh):
+        s = _c(rgb, np.uint8)
+        sh, sw = s.shape[:2]
+        out = np.zeros((dh, dw, 3), np.uint8)
+        _check(self.lib.mh_resize(self.h, _ptr(s), sw, sh, _ptr(out), dw, dh))
+        return out
+
+    def composite(self, lib, target, grid, grid_w, grid_h, strength):
+        t = _c(target, np.uint8)
+        h, w = t.shape[:2]
+        tw, th = w // grid_w, h // grid_h
+        out = np.zeros((grid_h * th, grid_w * tw, 3), np.uint8)
+        g = _c(grid, np.int32)
+        _check(self.lib.mh_composite(self.h, lib.h, _ptr(t), w, h, _ptr(g), grid_w, grid_h,
+                                     strength, _ptr(out)))
+        return out
+
+    # ---------------------------------------------------------- whole job
+    def generate(self, lib, target, grid_w, grid_h, max_usage, weight, color_strength,
+                 enable_optimization=True, optimization_iterations=1000,
+                 initial_temperature=100.0, temperature_decay=0.99995, seed=0, image=True):
+        t = _c(target, np.uint8)
+        h, w = t.shape[:2]
+        tw, th = w // grid_w, h // grid_h
+        grid = np.zeros(grid_w * grid_h, np.int32)
+        img = np.zeros((grid_h * th, grid_w * tw, 3), np.uint8) if image else None
+        prm = JobParams(grid_w, grid_h, max_usage, weight, color_strength,
+                        1 if enable_optimization else 0, optimization_iterations,
+                        initial_temperature, temperature_decay, seed)
+        res = JobResult()
+        _check(self.lib.mh_generate(self.h, lib.h, _ptr(t), w, h, ctypes.byref(prm), _ptr(grid),
+                                    _ptr(img), ctypes.byref(res)))
+        return {"grid": grid, "image": img, "place": res.place.as_dict(),
+                "anneal": res.anneal.as_dict(), "out_size": (res.out_width, res.out_height)}
+
+    def generate_device(self, lib, d_target, width, height, params, d_grid, d_image):
+        """Device-pointer form; d_* are integer device addresses (e.g. torch
+        tensor.data_ptr()).  Enqueues on the context stream, no sync."""
+        _check(self.lib.mh_generate_device(self.h, lib.h, ctypes.c_void_p(d_target), width, height,
+                                           ctypes.byref(params), ctypes.c_void_p(d_grid),
+                                           ctypes.c_void_p(d_image) if d_image else None))
+
+    def generate_device_result(self):
+        res = JobResult()
+        _check(self.lib.mh_generate_device_result(self.h, ctypes.byref(res)))
+        return {"place": res.place.as_dict(), "anneal": res.anneal.as_dict(),
+                "out_size": (res.out_width, res.out_height)}
+
+
+class Library:
+    """Device-resident material library (SimilarityDatabase + Lab table)."""
+
+    def __init__(self, ctx, lab=None, pixels=None):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        h = ctypes.c_void_p()
+        if pixels is not None:
+            px = _c(pixels, np.uint8)
+            n, mh, mw = px.shape[:3]
+            _check(self.lib.mh_library_from_pixels(ctx.h, _ptr(px), n, mw, mh, ctypes.byref(h)))
+        else:
+            lab = _c(lab, np.float32).reshape(-1, 3)
+            _check(self.lib.mh_library_from_lab(ctx.h, _ptr(lab), lab.shape[0], ctypes.byref(h)))
+        self.h = h
+        self.n = self.lib.mh_library_size(h)
+
+    def close(self):
+        if self.h:
+            self.lib.mh_library_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def lab(self):
+        out = np.zeros((self.n, 3), np.float32)
+        _check(self.lib.mh_library_lab(self.h, _ptr(out)))
+        return out
+
+    def build_similarities(self):
+        _check(self.lib.mh_library_build_similarities(self.h))
+
+    def similarities(self, packed=False):
+        n = self.n
+        out = np.zeros(n * (n - 1) // 2 if packed else n * n, np.float32)
+        _check(self.lib.mh_library_similarities(self.h, _ptr(out), 1 if packed else 0))
+        return out if packed else out.reshape(n, n)
+
+    def get_similarity(self, i, j):
+        v = ctypes.c_float()
+        rc = self.lib.mh_library_get_similarity(self.h, i, j, ctypes.byref(v))
+        if rc == MH_ERR_NOT_FOUND:
+            return None
+        _check(rc)
+        return v.value

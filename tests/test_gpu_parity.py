@@ -1,0 +1,380 @@
+"""Parity of the HIP path (through the C ABI) against the C oracle.
+
+Bar: bit-exact for Lab means, similarities, candidate lists, placement
+(indices, usage, stages, distances, scores, stats), costs, swap deltas and
+annealing (the counter-RNG chain is reproduced exactly); composited pixels
+within +-1 LSB of u8 (they are expected to be exact).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _rng(seed):
+    return np.random.default_rng(seed)
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def assert_bits_equal(a, b):
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float32)
+    assert a.shape == b.shape
+    diff = np.nonzero(_bits(a) != _bits(b))
+    assert diff[0].size == 0, f"{diff[0].size} mismatches, first at {diff[0][:5]}: {a[diff][:5]} vs {b[diff][:5]}"
+
+
+# ------------------------------------------------------------------- a1 Lab
+def test_average_lab_known_answers(ctx, orc):
+    # src/lib.rs:133-158
+    red = np.zeros((10, 10, 3), np.uint8)
+    red[..., 0] = 255
+    lab = ctx.average_lab(red)
+    assert abs(lab[0] - 53.24) < 1.0 and abs(lab[1] - 80.09) < 1.0 and abs(lab[2] - 67.20) < 1.0
+    assert_bits_equal(lab, orc.average_lab(red))
+    grey = np.full((10, 10, 3), 128, np.uint8)
+    lab = ctx.average_lab(grey)
+    assert abs(lab[0] - 53.59) < 1.0 and abs(lab[1]) < 1.0 and abs(lab[2]) < 1.0
+    assert_bits_equal(lab, orc.average_lab(grey))
+
+
+def test_every_rgb_value_converts_like_oracle(ctx, orc):
+    # all 256 grey levels + 4096 random colours as 1-pixel images in a strip
+    rng = _rng(1)
+    cols = np.concatenate([np.repeat(np.arange(256, dtype=np.uint8)[:, None], 3, 1),
+                           rng.integers(0, 256, (4096, 3)).astype(np.uint8)])
+    img = cols[None, :, :]
+    got = ctx.cell_lab_means(img, cols.shape[0], 1)
+    want = orc.cell_lab_means(img, cols.shape[0], 1)
+    assert_bits_equal(got, want)
+
+
+@pytest.mark.parametrize("w,h,gw,gh", [(640, 480, 40, 30), (101, 77, 7, 5), (64, 64, 1, 1),
+                                       (1000, 37, 250, 37)])
+def test_cell_lab_means(ctx, orc, w, h, gw, gh):
+    img = _rng(w * h).integers(0, 256, (h, w, 3)).astype(np.uint8)
+    assert_bits_equal(ctx.cell_lab_means(img, gw, gh), orc.cell_lab_means(img, gw, gh))
+
+
+def test_material_lab_from_pixels(ctx, orc, synth):
+    mats = synth.flat_materials(300, 64, _rng(2))
+    lib = ctx.library_from_pixels(mats)
+    assert_bits_equal(lib.lab(), orc.stack_lab_means(mats))
+    lib.close()
+
+
+# ------------------------------------------------------------ a2 similarity
+@pytest.mark.parametrize("n", [1, 3, 200, 201, 1500])
+def test_similarity_matrix(ctx, orc, n):
+    lab = (_rng(n).random((n, 3)) * [100, 200, 200] - [0, 100, 100]).astype(np.float32)
+    lib = ctx.library_from_lab(lab)
+    full = lib.similarities()
+    assert_bits_equal(full, orc.sim_full(lab))
+    assert_bits_equal(full, full.T)
+    assert np.all(np.diag(full) == 0.0)
+    if n > 1:
+        assert_bits_equal(lib.similarities(packed=True), orc.sim_packed(lab))
+    lib.close()
+
+
+def test_similarity_known_answers(ctx):
+    # src/similarity.rs:186-205, :320-362
+    lab = np.array([[50, 0, 0], [60, 10, 10], [40, -10, -10]], np.float32)
+    lib = ctx.library_from_lab(lab)
+    s12 = lib.get_similarity(0, 1)
+    assert s12 == np.float32(np.sqrt(np.float32(300.0)))
+    assert lib.get_similarity(1, 0) == s12
+    assert lib.get_similarity(0, 0) == 0.0
+    assert lib.get_similarity(0, 3) is None
+    lib.close()
+
+
+# -------------------------------------------------------------- a3 top-K
+@pytest.mark.parametrize("n,k", [(200, 100), (1500, 100), (50, 100), (128, 100), (129, 100),
+                                 (10000, 100), (3000, 7)])
+def test_nearest_n(ctx, orc, n, k):
+    rng = _rng(n + k)
+    lab = (rng.random((n, 3)) * [100, 200, 200] - [0, 100, 100]).astype(np.float32)
+    q = (rng.random((64, 3)) * [100, 200, 200] - [0, 100, 100]).astype(np.float32)
+    lib = ctx.library_from_lab(lab)
+    gi, gd = ctx.nearest_n(lib, q, k)
+    oi, od = orc.nearest_n(lab, q, k)
+    np.testing.assert_array_equal(gi, oi)
+    assert_bits_equal(gd, od)
+    lib.close()
+
+
+def test_nearest_n_ties_by_index(ctx, orc):
+    # many exact duplicates: equal squared distances must come out by index
+    rng = _rng(7)
+    base = (rng.random((40, 3)) * 50).astype(np.float32)
+    lab = np.concatenate([base] * 25)  # 1000 materials, every colour 25 times
+    perm = rng.permutation(lab.shape[0])
+    lab = lab[perm]
+    q = base[:16] + np.float32(0.25)
+    lib = ctx.library_from_lab(lab)
+    gi, gd = ctx.nearest_n(lib, q, 100)
+    oi, od = orc.nearest_n(lab, q, 100)
+    np.testing.assert_array_equal(gi, oi)
+    assert_bits_equal(gd, od)
+    for r in range(gi.shape[0]):
+        keys = list(zip(gd[r].tolist(), gi[r].tolist()))
+        assert keys == sorted(keys)
+    lib.close()
+
+
+def test_nearest_one_known_answer(ctx):
+    # src/lib.rs:213-227
+    lab = np.array([[50, 0, 0], [53.24, 80.09, 67.20], [87.74, -86.18, 83.18],
+                    [32.30, 79.20, -107.86]], np.float32)
+    lib = ctx.library_from_lab(lab)
+    gi, _ = ctx.nearest_n(lib, np.array([[53.0, 80.0, 67.0]], np.float32), 1)
+    assert gi[0, 0] == 1
+    lib.close()
+
+
+# ------------------------------------------------------------ a4-a9 place
+def _check_place(got, want):
+    np.testing.assert_array_equal(got["grid"], want["grid"])
+    np.testing.assert_array_equal(got["usage"], want["usage"])
+    np.testing.assert_array_equal(got["stage"], want["stage"])
+    assert_bits_equal(got["dist"], want["dist"])
+    assert_bits_equal(got["score"], want["score"])
+    assert got["stats"] == want["stats"]
+
+
+@pytest.mark.parametrize("name", ["small", "medium"])
+def test_place_workloads(ctx, orc, synth, name):
+    p, target, mats = synth.make(name)
+    lab = orc.stack_lab_means(mats)
+    cl = orc.cell_lab_means(target, p["grid_w"], p["grid_h"])
+    lib = ctx.library_from_lab(lab)
+    got = ctx.place(lib, cl, p["grid_w"], p["grid_h"], p["max_usage"], p["weight"])
+    want = orc.place(lab, cl, p["grid_w"], p["grid_h"], p["max_usage"], p["weight"])
+    _check_place(got, want)
+    if name == "small":  # 200 x 3 = 600 < 1200 cells: the reset path must run
+        assert want["stats"]["resets"] >= 1
+    lib.close()
+
+
+@pytest.mark.parametrize("n,gw,gh,usage,weight", [
+    (12, 6, 5, 2, 0.25),   # forces resets (24 < 30)
+    (12, 6, 5, 0, 0.25),   # max_usage = 0: fallback on every cell
+    (1, 4, 3, 3, 0.3),     # single material: nearest_one after the ban
+    (2, 5, 4, 1, 0.5),     # two materials, ban + usage
+    (3, 7, 7, 2, 0.0),     # weight 0: no penalty
+    (150, 13, 11, 1, -1.0),  # negative weight behaves like 0
+    (500, 20, 20, 3, 2.0),
+])
+def test_place_edge_cases(ctx, orc, n, gw, gh, usage, weight):
+    rng = _rng(n * 131 + gw)
+    lab = (rng.random((n, 3)) * [100, 60, 60] - [0, 30, 30]).astype(np.float32)
+    cl = (rng.random((gw * gh, 3)) * [100, 60, 60] - [0, 30, 30]).astype(np.float32)
+    lib = ctx.library_from_lab(lab)
+    got = ctx.place(lib, cl, gw, gh, usage, weight)
+    want = orc.place(lab, cl, gw, gh, usage, weight)
+    _check_place(got, want)
+    lib.close()
+
+
+def test_place_empty_library(ctx, orc):
+    lib = ctx.library_from_lab(np.zeros((0, 3), np.float32))
+    cl = np.zeros((12, 3), np.float32)
+    got = ctx.place(lib, cl, 4, 3, 3, 0.25)
+    assert np.all(got["grid"] == -1)
+    assert got["stats"]["empty"] == 12 and got["stats"]["placed"] == 0
+    lib.close()
+
+
+def test_place_order_and_prefilled_state(ctx, orc):
+    # find_and_use_best_tile_with_position at arbitrary cells with a
+    # pre-filled grid and usage (src/main.rs:1250-1283): all 4 neighbours count
+    rng = _rng(11)
+    n, gw, gh = 40, 6, 6
+    lab = (rng.random((n, 3)) * [100, 60, 60] - [0, 30, 30]).astype(np.float32)
+    cl = (rng.random((gw * gh, 3)) * [100, 60, 60] - [0, 30, 30]).astype(np.float32)
+    grid = np.full(gw * gh, -1, np.int32)
+    grid[rng.choice(gw * gh, 10, replace=False)] = rng.integers(0, n, 10)
+    usage = rng.integers(0, 3, n).astype(np.uint32)
+    order = rng.permutation(np.nonzero(grid < 0)[0]).astype(np.int32)
+    lib = ctx.library_from_lab(lab)
+    got = ctx.place(lib, cl, gw, gh, 3, 0.5, order=order, grid=grid, usage=usage)
+    want = orc.place(lab, cl, gw, gh, 3, 0.5, order=order, grid=grid, usage=usage)
+    _check_place(got, want)
+    lib.close()
+
+
+# ------------------------------------------------- a10-a12 cost, delta, SA
+def _random_grid(rng, n, gw, gh, empty_frac=0.1):
+    g = rng.integers(0, n, gw * gh).astype(np.int32)
+    g[rng.random(gw * gh) < empty_frac] = -1
+    return g
+
+
+@pytest.mark.parametrize("n,gw,gh", [(5, 4, 4), (200, 40, 30), (1500, 80, 60), (3, 1, 9)])
+def test_total_cost_and_penalties(ctx, orc, n, gw, gh):
+    rng = _rng(n + gw)
+    lab = (rng.random((n, 3)) * [100, 60, 60] - [0, 30, 30]).astype(np.float32)
+    g = _random_grid(rng, n, gw, gh)
+    lib = ctx.library_from_lab(lab)
+    assert _bits(np.float32(ctx.total_cost(lib, g, gw, gh))) == _bits(orc.total_cost(lab, g, gw, gh))
+    q = np.stack([rng.integers(0, n, 256), rng.integers(0, gw, 256), rng.integers(0, gh, 256)], 1)
+    got = ctx.penalties(lib, g, gw, gh, 0.25, q)
+    sim = orc.sim_full(lab)
+    want = []
+    for m, x, y in q:  # calculate_penalty restated (Up, Down, Left, Right)
+        p = np.float32(0.0)
+        for (nx, ny) in [(x, y - 1), (x, y + 1), (x - 1, y), (x + 1, y)]:
+            if 0 <= nx < gw and 0 <= ny < gh and g[ny * gw + nx] >= 0:
+                p = np.float32(p + np.float32(1.0) / np.float32(sim[m, g[ny * gw + nx]] + np.float32(1.0)))
+        want.append(np.float32(p * np.float32(0.25)))
+    assert_bits_equal(got, np.array(want, np.float32))
+    lib.close()
+
+
+@pytest.mark.parametrize("n,gw,gh", [(4, 2, 2), (25, 5, 5), (200, 40, 30), (6, 3, 1)])
+def test_swap_deltas(ctx, orc, n, gw, gh):
+    rng = _rng(n * 7 + gh)
+    lab = (rng.random((n, 3)) * [100, 60, 60] - [0, 30, 30]).astype(np.float32)
+    g = _random_grid(rng, n, gw, gh)
+    pairs = np.stack([rng.integers(0, gw, 500), rng.integers(0, gh, 500),
+                      rng.integers(0, gw, 500), rng.integers(0, gh, 500)], 1).astype(np.int32)
+    lib = ctx.library_from_lab(lab)
+    got = ctx.swap_deltas(lib, g, gw, gh, pairs)
+    want = np.array([orc.swap_delta(lab, g, gw, gh, *map(int, p)) for p in pairs], np.float32)
+    assert_bits_equal(got, want)
+    lib.close()
+
+
+def _check_anneal(got, want):
+    np.testing.assert_array_equal(got["grid"], want["grid"])
+    for key in ["initial_cost", "final_cost", "best_cost"]:
+        assert _bits(np.float32(got["result"][key])) == _bits(np.float32(want["result"][key])), key
+    for key in ["improved_count", "accepted_count", "iterations", "evaluated"]:
+        assert got["result"][key] == want["result"][key], key
+    if "trace_delta" in want:
+        assert_bits_equal(got["trace_delta"], want["trace_delta"])
+        np.testing.assert_array_equal(got["trace_accept"], want["trace_accept"])
+
+
+@pytest.mark.parametrize("n,gw,gh,iters,t0,decay,greedy,seed", [
+    (200, 40, 30, 500, 100.0, 0.99995, False, 0),
+    (1500, 80, 60, 2000, 100.0, 0.99995, False, 1),
+    (25, 5, 5, 300, 100.0, 0.99995, False, 2),
+    (4, 2, 2, 50, 0.001, 0.99, False, 3),       # low temperature
+    (4, 2, 2, 10, 100.0, 0.0, False, 4),        # T -> 0: -d/0 edge cases
+    (4, 2, 2, 50, 1000.0, 0.5, False, 5),       # very fast cooling
+    (200, 40, 30, 3000, 100.0, 0.99995, True, 6),  # optimize_greedy
+    (9, 3, 3, 0, 100.0, 0.99995, False, 7),     # zero iterations
+    (1, 1, 1, 100, 100.0, 0.99995, False, 8),   # single tile
+    (30, 17, 9, 4000, 5.0, 0.999, False, 9),
+])
+def test_anneal_matches_sequential_chain(ctx, orc, n, gw, gh, iters, t0, decay, greedy, seed):
+    rng = _rng(seed + 100)
+    lab = (rng.random((n, 3)) * [100, 60, 60] - [0, 30, 30]).astype(np.float32)
+    g = _random_grid(rng, n, gw, gh, 0.05 if gw * gh > 4 else 0.0)
+    lib = ctx.library_from_lab(lab)
+    got = ctx.anneal(lib, g, gw, gh, iters, t0, decay, seed, greedy, trace=True)
+    want = orc.anneal(lab, g, gw, gh, iters, t0, decay, seed, greedy, trace=True)
+    _check_anneal(got, want)
+    # the recorded cost equals the cost function on the final grid up to f32 drift
+    final = orc.total_cost(lab, got["grid"], gw, gh)
+    assert abs(final - got["result"]["final_cost"]) <= 1e-3 * max(1.0, abs(final))
+    lib.close()
+
+
+def test_anneal_sparse_grid(ctx, orc):
+    # src/optimizer.rs:336-357: non-adjacent tiles have zero cost
+    lab = np.array([[50, 0, 0], [60, 10, 10]], np.float32)
+    g = np.full(9, -1, np.int32)
+    g[0], g[8] = 0, 1
+    lib = ctx.library_from_lab(lab)
+    got = ctx.anneal(lib, g, 3, 3, 50, seed=3, trace=True)
+    want = orc.anneal(lab, g, 3, 3, 50, seed=3, trace=True)
+    _check_anneal(got, want)
+    assert got["result"]["initial_cost"] == 0.0
+    lib.close()
+
+
+# ------------------------------------------------ a13-a16 adjust/composite
+def test_average_rgb_known_answer(ctx):
+    # src/main.rs:877-889
+    img = np.array([[[255, 0, 0], [0, 255, 0]], [[0, 0, 255], [255, 255, 255]]], np.uint8)
+    np.testing.assert_array_equal(ctx.average_rgb(img), [127, 127, 127])
+
+
+def test_optimal_adjustments(ctx, orc):
+    rng = _rng(21)
+    t = rng.integers(0, 256, (4000, 3)).astype(np.uint8)
+    g = rng.integers(0, 256, (4000, 3)).astype(np.uint8)
+    t[:256] = np.arange(256)[:, None]  # greys (saturation 0 branches)
+    for k in [0.0, 0.4, 1.0]:
+        got = ctx.optimal_adjustments(t, g, k)
+        want = np.stack([orc.optimal_adjustment(t[i], g[i], k) for i in range(t.shape[0])])
+        assert_bits_equal(got, want)
+
+
+def test_adjust_pixels(ctx, orc):
+    rng = _rng(22)
+    px = rng.integers(0, 256, (20000, 3)).astype(np.uint8)
+    adjs = [orc.adjustment_new(0.0, 1.0, 0.0, 1.0), orc.adjustment_new(0.2, 1.0, 0.0, 1.0),
+            orc.adjustment_new(0.0, 1.5, 0.0, 1.0), orc.adjustment_new(0.0, 1.0, 120.0, 1.0),
+            orc.adjustment_new(-0.3, 0.7, -47.5, 0.5), orc.adjustment_new(0.5, 1.5, 60.0, 1.5),
+            orc.adjustment_new(0.05, 1.0, 179.9, 2.0)]
+    for a in adjs:
+        got = ctx.adjust_pixels(a, px)
+        want = orc.apply_adjustment(a, px)
+        assert np.max(np.abs(got.astype(int) - want.astype(int))) <= 1
+        np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("sw,sh,dw,dh", [(64, 64, 16, 16), (128, 128, 24, 18), (64, 64, 20, 20),
+                                         (100, 100, 50, 50), (100, 100, 200, 100), (7, 5, 13, 3)])
+def test_resize(ctx, orc, sw, sh, dw, dh):
+    img = _rng(sw * dh).integers(0, 256, (sh, sw, 3)).astype(np.uint8)
+    np.testing.assert_array_equal(ctx.resize(img, dw, dh), orc.resize(img, dw, dh))
+
+
+def test_resize_solid_colour(ctx):
+    # src/main.rs:892-908
+    red = np.zeros((100, 100, 3), np.uint8)
+    red[..., 0] = 255
+    out = ctx.resize(red, 50, 50)
+    assert out.shape == (50, 50, 3)
+    assert out[25, 25, 0] > 200 and out[25, 25, 1] < 50 and out[25, 25, 2] < 50
+    assert np.all(out == out[0, 0])
+
+
+@pytest.mark.parametrize("strength", [0.0, 0.4, 1.0])
+def test_composite(ctx, orc, synth, strength):
+    p, target, mats = synth.make("small")
+    lab = orc.stack_lab_means(mats)
+    cl = orc.cell_lab_means(target, p["grid_w"], p["grid_h"])
+    grid = orc.place(lab, cl, p["grid_w"], p["grid_h"], p["max_usage"], p["weight"])["grid"]
+    grid[5] = -1  # an empty cell stays black
+    lib = ctx.library_from_pixels(mats)
+    got = ctx.composite(lib, target, grid, p["grid_w"], p["grid_h"], strength)
+    want = orc.composite(target, mats, grid, p["grid_w"], p["grid_h"], strength)
+    assert np.max(np.abs(got.astype(int) - want.astype(int))) <= 1
+    np.testing.assert_array_equal(got, want)
+    lib.close()
+
+
+# ------------------------------------------------------------ whole job
+@pytest.mark.parametrize("name", ["small", "medium"])
+def test_generate_matches_oracle(ctx, orc, synth, name):
+    p, target, mats = synth.make(name)
+    lib = ctx.library_from_pixels(mats)
+    got = ctx.generate(lib, target, p["grid_w"], p["grid_h"], p["max_usage"], p["weight"],
+                       p["color"], True, p["iters"], seed=0)
+    want = orc.generate(target, mats, p["grid_w"], p["grid_h"], p["max_usage"], p["weight"],
+                        p["color"], True, p["iters"], seed=0)
+    np.testing.assert_array_equal(got["grid"], want["grid"])
+    assert got["place"] == want["place"]["stats"]
+    for key in ["initial_cost", "final_cost", "best_cost"]:
+        assert _bits(np.float32(got["anneal"][key])) == _bits(np.float32(want["anneal"]["result"][key]))
+    assert np.max(np.abs(got["image"].astype(int) - want["image"].astype(int))) <= 1
+    lib.close()
