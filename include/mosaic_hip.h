@@ -81,6 +81,31 @@ int mh_context_synchronize(mh_context *ctx);
  * work against it. */
 void *mh_context_stream(mh_context *ctx);
 
+/* Per-phase device timing: when enabled, every phase the library enqueues
+ * is bracketed by HIP events on the context stream.  mh_context_timing
+ * synchronises, returns the accumulated milliseconds and launch counts per
+ * phase since the last call, and clears them.  Phase names: mh_phase_name. */
+#define MH_PHASE_CELL_LAB 0
+#define MH_PHASE_TOPK 1
+#define MH_PHASE_PLACE 2
+#define MH_PHASE_ANNEAL 3
+#define MH_PHASE_RESIZE 4
+#define MH_PHASE_ADJUST 5
+#define MH_PHASE_COMPOSITE 6
+#define MH_PHASE_SIM 7
+#define MH_PHASES 8
+int mh_context_set_timing(mh_context *ctx, int32_t enabled);
+int mh_context_timing(mh_context *ctx, double ms[MH_PHASES], int64_t launches[MH_PHASES]);
+const char *mh_phase_name(int32_t phase);
+
+/* Device buffers for the device-resident entry points (mh_generate_device):
+ * plain hipMalloc'd memory on the context's device, and copies ordered on the
+ * context stream (mh_copy_* block until the copy is done). */
+int mh_device_alloc(mh_context *ctx, size_t bytes, void **out);
+int mh_device_free(mh_context *ctx, void *ptr);
+int mh_copy_to_device(mh_context *ctx, void *dst, const void *src, size_t bytes);
+int mh_copy_to_host(mh_context *ctx, void *dst, const void *src, size_t bytes);
+
 /* --------------------------------------------------------------- library */
 /* SimilarityDatabase::new + add_tile x n (src/similarity.rs:51-66): the Lab
  * table (n x {l,a,b} f32) becomes device resident. n may be 0. */

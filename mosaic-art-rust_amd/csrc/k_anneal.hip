@@ -41,7 +41,7 @@ __device__ float swap_delta_dev(const G* grid, int32_t gw, int32_t gh,
     const int32_t m1 = gget(grid, y1 * gw + x1), m2 = gget(grid, y2 * gw + x2);
     if (m1 == kEmpty || m2 == kEmpty) return 0.0f;
     if (m1 == m2) return 0.0f;
-    const float4 l1 = lab4[m1], l2 = lab4[m2];
+    const float4 l1 = gload4(lab4, m1), l2 = gload4(lab4, m2);
     float old_cost = 0.0f, new_cost = 0.0f;
     bool is_adj = false;
     int32_t ax[4], ay[4], cnt = 0;
@@ -56,7 +56,7 @@ __device__ float swap_delta_dev(const G* grid, int32_t gw, int32_t gh,
         }
         const int32_t a = gget(grid, ay[i] * gw + ax[i]);
         if (a == kEmpty) continue;
-        const float4 la = lab4[a];
+        const float4 la = gload4(lab4, a);
         old_cost += adj_term(l1, la);
         new_cost += adj_term(l2, la);
     }
@@ -69,7 +69,7 @@ __device__ float swap_delta_dev(const G* grid, int32_t gw, int32_t gh,
         if (ax[i] == x1 && ay[i] == y1) continue;
         const int32_t a = gget(grid, ay[i] * gw + ax[i]);
         if (a == kEmpty) continue;
-        const float4 la = lab4[a];
+        const float4 la = gload4(lab4, a);
         old_cost += adj_term(l2, la);
         new_cost += adj_term(l1, la);
     }
@@ -82,9 +82,13 @@ __device__ float swap_delta_dev(const G* grid, int32_t gw, int32_t gh,
 }
 
 // calculate_total_cost (src/adjacency.rs:91-130) by one wave: the terms of
-// `chunk` cells at a time are computed in parallel into scratch (2 per cell,
-// -1 = absent), then lane 0 adds them in the reference's order (raster,
-// right before down).  The result is broadcast to every lane.
+// `chunk` cells at a time are computed in parallel into scratch (right term,
+// down term per cell), then lane 0 adds them in the reference's order
+// (raster, right before down).  An absent term is stored as +0.0: the running
+// sum starts at +0 and only grows, and x + (+0) == x bit for bit, so adding
+// it leaves the sum exactly as the reference's skipped addition does.
+// scratch holds 2*chunk + 2 floats and is 16-byte aligned.
+// The result is broadcast to every lane.
 template <typename G>
 __device__ float total_cost_wave(const G* grid, int32_t gw, int32_t gh,
                                  const float4* __restrict__ lab4, float* scratch, int32_t chunk) {
@@ -97,26 +101,35 @@ __device__ float total_cost_wave(const G* grid, int32_t gw, int32_t gh,
             const int32_t c = base + i;
             const int32_t x = c % gw, y = c / gw;
             const int32_t m = gget(grid, c);
-            float tr = -1.0f, td = -1.0f;
+            float tr = 0.0f, td = 0.0f;
             if (m != kEmpty) {
-                const float4 lm = lab4[m];
+                const float4 lm = gload4(lab4, m);
                 if (x < gw - 1) {
                     const int32_t r = gget(grid, c + 1);
-                    if (r != kEmpty) tr = adj_term(lm, lab4[r]);
+                    if (r != kEmpty) tr = adj_term(lm, gload4(lab4, r));
                 }
                 if (y < gh - 1) {
                     const int32_t d = gget(grid, c + gw);
-                    if (d != kEmpty) td = adj_term(lm, lab4[d]);
+                    if (d != kEmpty) td = adj_term(lm, gload4(lab4, d));
                 }
             }
             scratch[2 * i] = tr;
             scratch[2 * i + 1] = td;
         }
+        if ((len & 1) != 0 && lane == 0) {  // pad to a whole float4
+            scratch[2 * len] = 0.0f;
+            scratch[2 * len + 1] = 0.0f;
+        }
         __syncthreads();
         if (lane == 0) {
-            for (int32_t i = 0; i < 2 * len; ++i) {
-                const float t = scratch[i];
-                if (t >= 0.0f) total = total + t;
+            const float4* v = reinterpret_cast<const float4*>(scratch);
+            const int32_t n4 = (2 * len + 3) / 4;
+            for (int32_t i = 0; i < n4; ++i) {
+                const float4 t = v[i];
+                total = total + t.x;
+                total = total + t.y;
+                total = total + t.z;
+                total = total + t.w;
             }
         }
         __syncthreads();
@@ -125,16 +138,16 @@ __device__ float total_cost_wave(const G* grid, int32_t gw, int32_t gh,
 }
 
 // ---------------------------------------------------------- annealing
-// Stamp words per job: one per cell, at least 2, even (so the cost scratch
-// of 2 floats per cell always fits a non-empty chunk).
+// Stamp words per job: one per cell plus slack, a multiple of 4 (the words
+// double as the cost scratch: 2 floats per cell of a chunk + 2 of padding).
 __host__ __device__ __forceinline__ int32_t stamp_words(int32_t cells) {
-    return (cells + 3) & ~1;
+    return (cells + 8) & ~3;
 }
 
 template <typename G>
 __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
                                                     const AnnealJob* __restrict__ jobs) {
-    extern __shared__ uint32_t sm[];
+    extern __shared__ __align__(16) uint32_t sm[];
     const AnnealJob job = jobs ? jobs[blockIdx.x] : single;
     const int lane = threadIdx.x;
     const int32_t gw = job.grid_w, gh = job.grid_h;
@@ -143,12 +156,12 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
     uint32_t* stamp = sm;
     G* grid_s = reinterpret_cast<G*>(sm + stamps);
 
-    for (int32_t i = lane; i < cells; i += 64) grid_s[i] = (G)job.grid[i];
+    for (int32_t i = lane; i < cells; i += 64) grid_s[i] = (G)gload(job.grid, i);
     __syncthreads();
 
-    // the stamp words double as the cost scratch: 2 terms per cell, stamps/2 cells a chunk
+    // the stamp words double as the cost scratch (2 terms per cell + padding)
     float cost = total_cost_wave(grid_s, gw, gh, job.lab4, reinterpret_cast<float*>(stamp),
-                                 stamps / 2);
+                                 (stamps / 2 - 1) & ~1);
     for (int32_t i = lane; i < cells; i += 64) stamp[i] = 0u;
     __syncthreads();
 
@@ -184,7 +197,7 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
                 acc = true;
             } else {
                 const int64_t k = kbase + __popcll(nonskip & lanes_below);
-                const float prob = exp_f32(-delta / job.temps[k]);
+                const float prob = exp_f32(-delta / gload(job.temps, k));
                 acc = draw_unit(job.seed, (uint64_t)it) < prob;
             }
         }
@@ -227,8 +240,8 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
                 grid_s[p1] = grid_s[p2];
                 grid_s[p2] = a;
             }
-            if (job.trace_delta) job.trace_delta[it] = skip ? __int_as_float(0x7fc00000) : delta;
-            if (job.trace_accept) job.trace_accept[it] = acc ? 1 : 0;
+            if (job.trace_delta) gstore(job.trace_delta, it, skip ? __int_as_float(0x7fc00000) : delta);
+            if (job.trace_accept) gstore(job.trace_accept, it, (uint8_t)(acc ? 1 : 0));
         }
         // running cost in proposal order; every lane runs the same uniform
         // loop, so cost/best/improved stay identical across the wave
@@ -262,7 +275,7 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
         __syncthreads();
     }
 
-    for (int32_t i = lane; i < cells; i += 64) job.grid[i] = (int32_t)grid_s[i];
+    for (int32_t i = lane; i < cells; i += 64) gstore(job.grid, i, (int32_t)grid_s[i]);
     if (lane == 0 && job.result) {
         AnnealOut r;
         r.initial_cost = initial;

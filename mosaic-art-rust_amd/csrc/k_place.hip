@@ -8,17 +8,29 @@
 // usage reset, and the just-placed neighbours), so one job is one wave64
 // walking its cells in order.  Per cell the wave evaluates the K <= 100
 // candidates in parallel (2 per lane): usage < max, neighbour ban, weighted
-// penalty (similarities recomputed from the Lab table -- bit-identical to the
-// stored matrix), score = sqdist + penalty, and the reference's first
-// strict-< winner becomes a min over (score_bits << 32 | rank).  Usage counts
-// and the grid live in LDS.  Latency-bound: one cell is a few hundred cycles.
+// penalty (similarities recomputed from Labs -- bit-identical to the stored
+// matrix), score = sqdist + penalty; the reference's first strict-< winner
+// is the minimum of (score, rank).  Many jobs run as many independent waves.
 //
-// Many jobs run as many independent waves (one block per job).
+// Two variants:
+//  * place_raster_kernel -- what generate_mosaic runs: raster order from an
+//    empty grid, so only Up and Left can be filled.  The left neighbour is
+//    the previous winner (registers), the up neighbour comes from a per-column
+//    LDS row (material + Lab of the last placement in that column), usage
+//    counts live in LDS, candidate records are prefetched 8 cells ahead into
+//    registers, the min is a DPP reduction, and the winner's id / Lab / count
+//    are read from its lane.  Winners are buffered 64 at a time in one
+//    register and stored coalesced.  Latency-bound: a few hundred cycles per
+//    cell.
+//  * place_chain_kernel -- any cell order over a pre-filled grid and usage
+//    (find_and_use_best_tile_with_position at arbitrary cells); grid and usage
+//    in LDS, all four neighbours read per cell.
 #include "mh_device.hpp"
 #include "mh_kernels.hpp"
 
 namespace mh {
 
+// ------------------------------------------------------------ general chain
 __global__ __launch_bounds__(64) void place_chain_kernel(PlaceJob single,
                                                          const PlaceJob* __restrict__ jobs) {
     extern __shared__ int32_t sm[];
@@ -29,8 +41,8 @@ __global__ __launch_bounds__(64) void place_chain_kernel(PlaceJob single,
     int32_t* grid_s = sm;
     uint32_t* usage_s = reinterpret_cast<uint32_t*>(sm + cells);
 
-    for (int32_t i = lane; i < cells; i += 64) grid_s[i] = job.grid[i];
-    for (int32_t i = lane; i < n; i += 64) usage_s[i] = job.usage[i];
+    for (int32_t i = lane; i < cells; i += 64) grid_s[i] = gload(job.grid, i);
+    for (int32_t i = lane; i < n; i += 64) usage_s[i] = job.zero_usage ? 0u : gload(job.usage, i);
     __syncthreads();
 
     const int32_t K = n < kMaxCand ? n : kMaxCand;
@@ -41,7 +53,7 @@ __global__ __launch_bounds__(64) void place_chain_kernel(PlaceJob single,
     const int32_t steps = job.order ? job.n_order : cells;
 
     for (int32_t c = 0; c < steps; ++c) {
-        const int32_t cell = job.order ? job.order[c] : c;
+        const int32_t cell = job.order ? gload(job.order, c) : c;
         if (n == 0) {
             ++st_empty;
             continue;
@@ -54,32 +66,28 @@ __global__ __launch_bounds__(64) void place_chain_kernel(PlaceJob single,
         const int32_t nb_right = x < gw - 1 ? grid_s[cell + 1] : kEmpty;
         float4 lu = make_float4(0, 0, 0, 0), ld = lu, ll = lu, lr = lu;
         if (use_pen) {
-            if (nb_up != kEmpty) lu = job.lab4[nb_up];
-            if (nb_down != kEmpty) ld = job.lab4[nb_down];
-            if (nb_left != kEmpty) ll = job.lab4[nb_left];
-            if (nb_right != kEmpty) lr = job.lab4[nb_right];
+            if (nb_up != kEmpty) lu = gload4(job.lab4, nb_up);
+            if (nb_down != kEmpty) ld = gload4(job.lab4, nb_down);
+            if (nb_left != kEmpty) ll = gload4(job.lab4, nb_left);
+            if (nb_right != kEmpty) lr = gload4(job.lab4, nb_right);
         }
         const int32_t* cidx = job.cand_idx + (int64_t)cell * kCandStride;
-        const float* cd = job.cand_d + (int64_t)cell * kCandStride;
+        const float4* clab = job.cand_lab + (int64_t)cell * kCandStride;
 
         uint64_t best = ~0ull;
-        int32_t m_slot[2];
         bool placeable[2];
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
             const int32_t r = lane + 64 * s;
-            m_slot[s] = kEmpty;
             placeable[s] = false;
             if (r < K) {
-                const int32_t m = cidx[r];
-                const float d = cd[r];
-                m_slot[s] = m;
+                const int32_t m = gload(cidx, r);
+                const float4 lm = gload4(clab, r);
                 const bool pl = m != nb_up && m != nb_down && m != nb_left && m != nb_right;
                 placeable[s] = pl;
                 if (pl && usage_s[m] < max_usage) {
                     float pen = 0.0f;
                     if (use_pen) {
-                        const float4 lm = job.lab4[m];
                         float p = 0.0f;
                         if (nb_up != kEmpty) p += adj_term(lm, lu);
                         if (nb_down != kEmpty) p += adj_term(lm, ld);
@@ -87,7 +95,7 @@ __global__ __launch_bounds__(64) void place_chain_kernel(PlaceJob single,
                         if (nb_right != kEmpty) p += adj_term(lm, lr);
                         pen = p * weight;
                     }
-                    const float score = d + pen;
+                    const float score = lm.w + pen;
                     const uint64_t key = ((uint64_t)__float_as_uint(score) << 32) | (uint32_t)r;
                     best = key < best ? key : best;
                 }
@@ -120,31 +128,280 @@ __global__ __launch_bounds__(64) void place_chain_kernel(PlaceJob single,
                 ++st_near;
             }
         }
-        const int32_t w = cidx[rank];
+        const int32_t w = gload(cidx, rank);
+        const float wd = gload4(clab, rank).w;
         __syncthreads();
         if (lane == 0) {
             usage_s[w] = usage_s[w] + 1u;
             grid_s[cell] = w;
-            if (job.stage) job.stage[cell] = (uint8_t)stage;
-            if (job.dist) job.dist[cell] = cd[rank];
-            if (job.score) {
-                float sc = cd[rank];
-                if (stage == 0) sc = __uint_as_float((uint32_t)(best >> 32));
-                job.score[cell] = sc;
-            }
+            if (job.stage) gstore(job.stage, cell, (uint8_t)stage);
+            if (job.dist) gstore(job.dist, cell, wd);
+            if (job.score)
+                gstore(job.score, cell, stage == 0 ? __uint_as_float((uint32_t)(best >> 32)) : wd);
         }
         __syncthreads();
     }
 
-    for (int32_t i = lane; i < cells; i += 64) job.grid[i] = grid_s[i];
-    for (int32_t i = lane; i < n; i += 64) job.usage[i] = usage_s[i];
+    for (int32_t i = lane; i < cells; i += 64) gstore(job.grid, i, grid_s[i]);
+    for (int32_t i = lane; i < n; i += 64) gstore(job.usage, i, usage_s[i]);
     if (lane == 0 && job.stats) {
-        job.stats[0] = st_main + st_ban + st_near;
-        job.stats[1] = st_main;
-        job.stats[2] = st_ban;
-        job.stats[3] = st_near;
-        job.stats[4] = st_resets;
-        job.stats[5] = st_empty;
+        gstore(job.stats, 0, st_main + st_ban + st_near);
+        gstore(job.stats, 1, st_main);
+        gstore(job.stats, 2, st_ban);
+        gstore(job.stats, 3, st_near);
+        gstore(job.stats, 4, st_resets);
+        gstore(job.stats, 5, st_empty);
+    }
+}
+
+// ------------------------------------------------------------- raster chain
+struct CandSet {
+    int32_t m0, m1;   // material of rank lane / lane+64 (-1: no candidate)
+    float4 c0, c1;    // {l, a, b, sqdist}
+};
+
+__device__ __forceinline__ void load_cands(CandSet& s, const PlaceJob& job, int32_t cell,
+                                           int32_t cells, int32_t K, int lane) {
+    s.m0 = -1;
+    s.m1 = -1;
+    s.c0 = make_float4(0, 0, 0, 0);
+    s.c1 = s.c0;
+    if (cell < cells) {
+        const int64_t base = (int64_t)cell * kCandStride;
+        if (lane < K) {
+            s.m0 = gload(job.cand_idx, base + lane);
+            s.c0 = gload4(job.cand_lab, base + lane);
+        }
+        if (lane + 64 < K) {
+            s.m1 = gload(job.cand_idx, base + lane + 64);
+            s.c1 = gload4(job.cand_lab, base + lane + 64);
+        }
+    }
+}
+
+__device__ __forceinline__ float4 readlane4(float4 v, int lane) {
+    return make_float4(readlane_f(v.x, lane), readlane_f(v.y, lane), readlane_f(v.z, lane),
+                       readlane_f(v.w, lane));
+}
+
+struct RasterState {
+    int32_t x, y;          // current cell position
+    int32_t left_m;        // previous winner (valid when x > 0)
+    float4 left_lab;
+    uint32_t u0, u1;       // usage counts of the current cell's candidates
+    int32_t up_m;          // material above the current cell (valid when y > 0)
+    float4 up_lab;
+    uint32_t win_buf;      // winners of the current 64-cell block, one per lane
+    int32_t st_main, st_ban, st_near, st_resets;
+};
+
+// One cell of the raster chain.  The next cell's usage counts and up
+// neighbour are read from LDS before this cell commits (so their latency
+// overlaps the decision) and then corrected in registers for this cell's
+// winner: +1 for its candidate that equals the winner, or counts of 0/1
+// after a usage reset; when the grid is one column wide the next cell's up
+// neighbour is this cell's winner.
+template <bool kOutputs>
+__device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn, int32_t c,
+                                            RasterState& st, const PlaceJob& job,
+                                            uint32_t* usage_s, int32_t* row_mat, float4* row_lab,
+                                            int32_t n, int lane) {
+    const uint32_t kNone = 0xffffffffu;
+    const int32_t gw = job.grid_w;
+    const int32_t x = st.x, y = st.y;
+    const int32_t xn = x + 1 == gw ? 0 : x + 1;
+    const int32_t yn = x + 1 == gw ? y + 1 : y;
+    // pre-read for the next cell (sees every commit up to the previous cell)
+    const uint32_t nu0 = usage_s[sn.m0 >= 0 ? sn.m0 : 0];
+    const uint32_t nu1 = usage_s[sn.m1 >= 0 ? sn.m1 : 0];
+    int32_t nup_m = kEmpty;
+    float4 nup_lab = make_float4(0, 0, 0, 0);
+    if (yn > 0) {
+        nup_m = row_mat[xn];
+        nup_lab = row_lab[xn];
+    }
+
+    const int32_t up_m = y > 0 ? st.up_m : kEmpty;
+    const int32_t left_m = x > 0 ? st.left_m : kEmpty;
+    const bool pl0 = s.m0 >= 0 && s.m0 != up_m && s.m0 != left_m;
+    const bool pl1 = s.m1 >= 0 && s.m1 != up_m && s.m1 != left_m;
+    const uint32_t max_usage = (uint32_t)job.max_usage;
+    const bool ok0 = pl0 && st.u0 < max_usage;
+    const bool ok1 = pl1 && st.u1 < max_usage;
+    const uint64_t ob0 = __ballot(ok0), ob1 = __ballot(ok1);
+
+    int32_t rank, stage;
+    uint32_t win_score = 0;
+    if (ob0 | ob1) {
+        // First usable candidate f (ranks are in ascending (sqdist, idx) order).
+        // The penalty is at most fl(2w) (two terms <= 1, times w), so
+        // score_f <= fl(d_f + 2w) =: bound, and a candidate with d > bound
+        // scores strictly worse than f.  Only candidates with d <= bound
+        // ("contenders") need their penalty terms.
+        const int32_t f = ob0 ? __ffsll((unsigned long long)ob0) - 1
+                              : 64 + __ffsll((unsigned long long)ob1) - 1;
+        const float d_f = f < 64 ? readlane_f(s.c0.w, f) : readlane_f(s.c1.w, f & 63);
+        rank = f;
+        if (job.weight > 0.0f) {
+            const float bound = d_f + 2.0f * job.weight;
+            const bool ct0 = ok0 && s.c0.w <= bound;
+            const bool ct1 = ok1 && s.c1.w <= bound;
+            const uint64_t cb0 = __ballot(ct0), cb1 = __ballot(ct1);
+            if (__popcll(cb0) + __popcll(cb1) > 1 || (kOutputs && job.score)) {
+                // calculate_penalty for the contenders: Up then Left (Down and
+                // Right are still empty in raster order)
+                uint32_t k0 = kNone, k1 = kNone;
+                if (cb0) {
+                    float p = 0.0f;
+                    if (y > 0) p += adj_term(s.c0, st.up_lab);
+                    if (x > 0) p += adj_term(s.c0, st.left_lab);
+                    const float sc = s.c0.w + p * job.weight;
+                    k0 = ct0 ? __float_as_uint(sc) : kNone;
+                }
+                if (cb1) {
+                    float p = 0.0f;
+                    if (y > 0) p += adj_term(s.c1, st.up_lab);
+                    if (x > 0) p += adj_term(s.c1, st.left_lab);
+                    const float sc = s.c1.w + p * job.weight;
+                    k1 = ct1 ? __float_as_uint(sc) : kNone;
+                }
+                const uint32_t kmin = wave_min_u32(k0 < k1 ? k0 : k1);
+                const uint64_t b0 = __ballot(k0 == kmin);
+                rank = b0 ? __ffsll((unsigned long long)b0) - 1
+                          : 64 + __ffsll((unsigned long long)__ballot(k1 == kmin)) - 1;
+                win_score = kmin;
+            }
+        } else {
+            win_score = __float_as_uint(d_f);  // penalty 0: scores are the distances
+        }
+        stage = 0;
+        ++st.st_main;
+    } else {
+        // fallback_tile_selection: reset every count, first placeable, else nearest_one
+        for (int32_t i = lane; i < n; i += 64) usage_s[i] = 0u;
+        ++st.st_resets;
+        const uint64_t b0 = __ballot(pl0);
+        const uint64_t b1 = __ballot(pl1);
+        if (b0) {
+            rank = __ffsll((unsigned long long)b0) - 1;
+            stage = 1;
+            ++st.st_ban;
+        } else if (b1) {
+            rank = 64 + __ffsll((unsigned long long)b1) - 1;
+            stage = 1;
+            ++st.st_ban;
+        } else {
+            rank = 0;
+            stage = 2;
+            ++st.st_near;
+        }
+    }
+    const int wl = rank & 63;
+    int32_t w;
+    float4 wlab;
+    uint32_t cnt = 0;
+    if (rank < 64) {
+        w = __builtin_amdgcn_readlane(s.m0, wl);
+        wlab = readlane4(s.c0, wl);
+        if (stage == 0) cnt = (uint32_t)__builtin_amdgcn_readlane((int)st.u0, wl);
+    } else {
+        w = __builtin_amdgcn_readlane(s.m1, wl);
+        wlab = readlane4(s.c1, wl);
+        if (stage == 0) cnt = (uint32_t)__builtin_amdgcn_readlane((int)st.u1, wl);
+    }
+    if (lane == 0) {
+        usage_s[w] = cnt + 1u;
+        row_mat[x] = w;
+        row_lab[x] = wlab;
+        if (kOutputs) {
+            if (job.stage) gstore(job.stage, c, (uint8_t)stage);
+            if (job.dist) gstore(job.dist, c, wlab.w);
+            if (job.score) gstore(job.score, c, stage == 0 ? __uint_as_float(win_score) : wlab.w);
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (lane == (c & 63)) st.win_buf = (uint32_t)w;
+    if ((c & 63) == 63) gstore(job.grid, c - 63 + lane, (int32_t)st.win_buf);
+    // carry the next cell's state, corrected for this commit
+    const uint32_t h0 = sn.m0 == w ? 1u : 0u, h1 = sn.m1 == w ? 1u : 0u;
+    st.u0 = stage == 0 ? nu0 + h0 : h0;
+    st.u1 = stage == 0 ? nu1 + h1 : h1;
+    st.up_m = gw == 1 ? w : nup_m;
+    st.up_lab = gw == 1 ? wlab : nup_lab;
+    st.left_m = w;
+    st.left_lab = wlab;
+    st.x = xn;
+    st.y = yn;
+}
+
+template <bool kOutputs>
+__global__ __launch_bounds__(64) void place_raster_kernel(PlaceJob single,
+                                                          const PlaceJob* __restrict__ jobs) {
+    extern __shared__ __align__(16) uint32_t smr[];
+    const PlaceJob job = jobs ? jobs[blockIdx.x] : single;
+    const int lane = threadIdx.x;
+    const int32_t gw = job.grid_w, n = job.n;
+    const int32_t cells = gw * job.grid_h;
+    float4* row_lab = reinterpret_cast<float4*>(smr);
+    int32_t* row_mat = reinterpret_cast<int32_t*>(row_lab + gw);
+    uint32_t* usage_s = reinterpret_cast<uint32_t*>(row_mat + gw);
+
+    if (n == 0) {  // src/main.rs:342-345: no tiles, every cell stays empty
+        for (int32_t i = lane; i < cells; i += 64) gstore(job.grid, i, kEmpty);
+        if (lane == 0 && job.stats) {
+            for (int i = 0; i < 5; ++i) gstore(job.stats, i, 0);
+            gstore(job.stats, 5, cells);
+        }
+        return;
+    }
+    for (int32_t i = lane; i < n; i += 64) usage_s[i] = job.zero_usage ? 0u : gload(job.usage, i);
+    __syncthreads();
+
+    const int32_t K = n < kMaxCand ? n : kMaxCand;
+    // candidate records are prefetched kAhead cells ahead into a register ring
+    constexpr int kAhead = 8;
+    CandSet q[kAhead];
+#pragma unroll
+    for (int i = 0; i < kAhead; ++i) load_cands(q[i], job, i, cells, K, lane);
+
+    RasterState st;
+    st.x = 0;
+    st.y = 0;
+    st.left_m = kEmpty;
+    st.left_lab = make_float4(0, 0, 0, 0);
+    st.u0 = usage_s[q[0].m0 >= 0 ? q[0].m0 : 0];
+    st.u1 = usage_s[q[0].m1 >= 0 ? q[0].m1 : 0];
+    st.up_m = kEmpty;
+    st.up_lab = make_float4(0, 0, 0, 0);
+    st.win_buf = 0;
+    st.st_main = st.st_ban = st.st_near = st.st_resets = 0;
+
+    int32_t c = 0;
+    for (; c + kAhead <= cells; c += kAhead) {
+#pragma unroll
+        for (int i = 0; i < kAhead; ++i) {
+            raster_step<kOutputs>(q[i], q[(i + 1) % kAhead], c + i, st, job, usage_s, row_mat,
+                                  row_lab, n, lane);
+            load_cands(q[i], job, c + i + kAhead, cells, K, lane);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < kAhead - 1; ++i)
+        if (c + i < cells)
+            raster_step<kOutputs>(q[i], q[(i + 1) % kAhead], c + i, st, job, usage_s, row_mat,
+                                  row_lab, n, lane);
+    // flush the partial winner block
+    const int32_t tail = cells & 63;
+    if (tail != 0 && lane < tail) gstore(job.grid, cells - tail + lane, (int32_t)st.win_buf);
+    __syncthreads();
+    for (int32_t i = lane; i < n; i += 64) gstore(job.usage, i, usage_s[i]);
+    if (lane == 0 && job.stats) {
+        gstore(job.stats, 0, st.st_main + st.st_ban + st.st_near);
+        gstore(job.stats, 1, st.st_main);
+        gstore(job.stats, 2, st.st_ban);
+        gstore(job.stats, 3, st.st_near);
+        gstore(job.stats, 4, st.st_resets);
+        gstore(job.stats, 5, 0);
     }
 }
 
@@ -152,20 +409,43 @@ size_t place_lds_bytes(int32_t cells, int32_t n) {
     return sizeof(int32_t) * ((size_t)cells + (size_t)n);
 }
 
-void launch_place_single(const PlaceJob& job, size_t lds_bytes, hipStream_t stream) {
-    hipLaunchKernelGGL(place_chain_kernel, dim3(1), dim3(64), lds_bytes, stream, job,
-                       (const PlaceJob*)nullptr);
+size_t place_raster_lds_bytes(int32_t grid_w, int32_t n) {
+    return sizeof(float4) * (size_t)grid_w + sizeof(int32_t) * (size_t)grid_w +
+           sizeof(uint32_t) * (size_t)(n > 0 ? n : 1);
 }
 
-void launch_place_batch(const PlaceJob* d_jobs, int32_t n_jobs, size_t lds_bytes,
+void launch_place_single(const PlaceJob& job, bool raster, size_t lds_bytes, hipStream_t stream) {
+    const bool outputs = job.stage || job.dist || job.score;
+    if (raster && outputs)
+        hipLaunchKernelGGL(place_raster_kernel<true>, dim3(1), dim3(64), lds_bytes, stream, job,
+                           (const PlaceJob*)nullptr);
+    else if (raster)
+        hipLaunchKernelGGL(place_raster_kernel<false>, dim3(1), dim3(64), lds_bytes, stream, job,
+                           (const PlaceJob*)nullptr);
+    else
+        hipLaunchKernelGGL(place_chain_kernel, dim3(1), dim3(64), lds_bytes, stream, job,
+                           (const PlaceJob*)nullptr);
+}
+
+void launch_place_batch(const PlaceJob* d_jobs, int32_t n_jobs, bool raster, size_t lds_bytes,
                         hipStream_t stream) {
     if (n_jobs <= 0) return;
     PlaceJob unused{};
-    hipLaunchKernelGGL(place_chain_kernel, dim3(n_jobs), dim3(64), lds_bytes, stream, unused, d_jobs);
+    if (raster)
+        hipLaunchKernelGGL(place_raster_kernel<false>, dim3(n_jobs), dim3(64), lds_bytes, stream,
+                           unused, d_jobs);
+    else
+        hipLaunchKernelGGL(place_chain_kernel, dim3(n_jobs), dim3(64), lds_bytes, stream, unused,
+                           d_jobs);
 }
 
 hipError_t place_configure(int max_lds) {
-    return raise_dynamic_lds(reinterpret_cast<const void*>(&place_chain_kernel), max_lds);
+    hipError_t e = raise_dynamic_lds(reinterpret_cast<const void*>(&place_chain_kernel), max_lds);
+    if (e == hipSuccess)
+        e = raise_dynamic_lds(reinterpret_cast<const void*>(&place_raster_kernel<true>), max_lds);
+    if (e == hipSuccess)
+        e = raise_dynamic_lds(reinterpret_cast<const void*>(&place_raster_kernel<false>), max_lds);
+    return e;
 }
 
 }  // namespace mh

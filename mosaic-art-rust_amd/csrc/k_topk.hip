@@ -49,7 +49,7 @@ template <bool kCache>
 __global__ __launch_bounds__(kTopkThreads) void topk_kernel(
     const float* __restrict__ mL, const float* __restrict__ mA, const float* __restrict__ mB,
     int32_t n, const float* __restrict__ query, int32_t k, int32_t* __restrict__ out_idx,
-    float* __restrict__ out_d, int32_t out_stride) {
+    float* __restrict__ out_d, float4* __restrict__ out_lab, int32_t out_stride) {
     extern __shared__ uint32_t dcache[];
     __shared__ uint32_t hist[kBins];
     __shared__ uint64_t sel[128];
@@ -150,23 +150,27 @@ __global__ __launch_bounds__(kTopkThreads) void topk_kernel(
     const int32_t kk = k < n ? k : n;
     if (tid < kk) {
         const uint64_t key = sel[tid];
-        out_idx[(int64_t)q * out_stride + tid] = (int32_t)(uint32_t)(key & 0xffffffffu);
-        out_d[(int64_t)q * out_stride + tid] = __uint_as_float((uint32_t)(key >> 32));
+        const int32_t m = (int32_t)(uint32_t)(key & 0xffffffffu);
+        const float d = __uint_as_float((uint32_t)(key >> 32));
+        const int64_t o = (int64_t)q * out_stride + tid;
+        out_idx[o] = m;
+        if (out_d) out_d[o] = d;
+        if (out_lab) out_lab[o] = make_float4(mL[m], mA[m], mB[m], d);
     }
 }
 
 void launch_topk(const float* mL, const float* mA, const float* mB, int32_t n, const float* query,
-                 int32_t nq, int32_t k, int32_t* out_idx, float* out_d, int32_t out_stride,
-                 hipStream_t stream) {
+                 int32_t nq, int32_t k, int32_t* out_idx, float* out_d, float4* out_lab,
+                 int32_t out_stride, hipStream_t stream) {
     if (k > n) k = n;
     if (nq <= 0 || n <= 0 || k <= 0) return;
     if (n <= kDistCacheMax) {
         const size_t shmem = sizeof(uint32_t) * (size_t)n;
         hipLaunchKernelGGL(topk_kernel<true>, dim3(nq), dim3(kTopkThreads), shmem, stream, mL, mA,
-                           mB, n, query, k, out_idx, out_d, out_stride);
+                           mB, n, query, k, out_idx, out_d, out_lab, out_stride);
     } else {
         hipLaunchKernelGGL(topk_kernel<false>, dim3(nq), dim3(kTopkThreads), 0, stream, mL, mA, mB,
-                           n, query, k, out_idx, out_d, out_stride);
+                           n, query, k, out_idx, out_d, out_lab, out_stride);
     }
 }
 

@@ -63,8 +63,20 @@ struct TapsBuf {
 
 }  // namespace
 
+struct PhaseMark {
+    int32_t phase;
+    hipEvent_t a, b;
+};
+
 struct mh_context {
     int32_t device = 0;
+    // per-phase timing (HIP events on `stream`)
+    bool timing = false;
+    std::vector<hipEvent_t> event_pool;
+    std::vector<PhaseMark> marks;
+    hipEvent_t open_event[MH_PHASES] = {};
+    double phase_ms[MH_PHASES] = {};
+    int64_t phase_n[MH_PHASES] = {};
     hipStream_t stream = nullptr;
     float* d_lut = nullptr;
     size_t max_lds = 65536;
@@ -99,6 +111,32 @@ namespace {
 int set_device(mh_context* ctx) {
     HIPCHK(hipSetDevice(ctx->device));
     return MH_OK;
+}
+
+hipEvent_t take_event(mh_context* ctx) {
+    if (!ctx->event_pool.empty()) {
+        hipEvent_t e = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Bracket the kernels of one phase with events (no-ops unless timing is on).
+void phase_begin(mh_context* ctx, int phase) {
+    if (!ctx->timing) return;
+    hipEvent_t e = take_event(ctx);
+    if (e && hipEventRecord(e, ctx->stream) == hipSuccess) ctx->open_event[phase] = e;
+}
+
+void phase_end(mh_context* ctx, int phase) {
+    if (!ctx->timing || !ctx->open_event[phase]) return;
+    hipEvent_t e = take_event(ctx);
+    if (e && hipEventRecord(e, ctx->stream) == hipSuccess)
+        ctx->marks.push_back(PhaseMark{phase, ctx->open_event[phase], e});
+    ctx->open_event[phase] = nullptr;
 }
 
 int ensure(mh_context* ctx, DevBuf& b, size_t bytes) {
@@ -227,35 +265,39 @@ void fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s) {
     hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
 }
 
-// Device-resident placement of every cell of a grid in raster order (or
-// `order`), from cell Labs already on the device.
+// Device-resident placement from cell Labs already on the device.  raster:
+// the whole grid in raster order from an empty grid (generate_mosaic's case,
+// the fast chain); otherwise `order` (or raster) over the given grid/usage.
 int run_place(mh_context* ctx, const mh_library* lib, const float* d_cell_lab, int32_t gw,
-              int32_t gh, int32_t max_usage, float weight, const int32_t* d_order,
-              int32_t n_order, int32_t* d_grid, uint32_t* d_usage, uint8_t* d_stage,
-              float* d_dist, float* d_score, int32_t* d_stats) {
+              int32_t gh, int32_t max_usage, float weight, bool raster, const int32_t* d_order,
+              int32_t n_order, int32_t* d_grid, uint32_t* d_usage, bool zero_usage,
+              uint8_t* d_stage, float* d_dist, float* d_score, int32_t* d_stats) {
     const int32_t cells = gw * gh;
-    const size_t lds = place_lds_bytes(cells, lib->n);
+    const size_t lds = raster ? place_raster_lds_bytes(gw, lib->n) : place_lds_bytes(cells, lib->n);
     if (lds > ctx->max_lds)
         return fail(MH_ERR_UNSUPPORTED,
                     "placement state (%d cells + %d materials) needs %zu B of LDS, limit %zu",
                     cells, lib->n, lds, ctx->max_lds);
     int32_t* cidx = nullptr;
-    float* cd = nullptr;
+    float4* clab = nullptr;
     int rc;
     if ((rc = slot(ctx, S_CAND_IDX, (size_t)cells * kCandStride, &cidx))) return rc;
-    if ((rc = slot(ctx, S_CAND_D, (size_t)cells * kCandStride, &cd))) return rc;
+    if ((rc = slot(ctx, S_CAND_D, (size_t)cells * kCandStride, &clab))) return rc;
     const int32_t K = lib->n < kMaxCand ? lib->n : kMaxCand;
-    launch_topk(lib->d_L, lib->d_A, lib->d_B, lib->n, d_cell_lab, cells, K, cidx, cd, kCandStride,
-                ctx->stream);
+    phase_begin(ctx, MH_PHASE_TOPK);
+    launch_topk(lib->d_L, lib->d_A, lib->d_B, lib->n, d_cell_lab, cells, K, cidx, nullptr, clab,
+                kCandStride, ctx->stream);
+    phase_end(ctx, MH_PHASE_TOPK);
     PlaceJob job;
     job.lab4 = lib->d_lab4;
     job.n = lib->n;
     job.cand_idx = cidx;
-    job.cand_d = cd;
+    job.cand_lab = clab;
     job.order = d_order;
     job.n_order = n_order;
     job.grid = d_grid;
     job.usage = d_usage;
+    job.zero_usage = zero_usage ? 1 : 0;
     job.grid_w = gw;
     job.grid_h = gh;
     job.max_usage = max_usage;
@@ -264,7 +306,9 @@ int run_place(mh_context* ctx, const mh_library* lib, const float* d_cell_lab, i
     job.dist = d_dist;
     job.score = d_score;
     job.stats = d_stats;
-    launch_place_single(job, lds, ctx->stream);
+    phase_begin(ctx, MH_PHASE_PLACE);
+    launch_place_single(job, raster, lds, ctx->stream);
+    phase_end(ctx, MH_PHASE_PLACE);
     HIPCHK(hipGetLastError());
     return MH_OK;
 }
@@ -298,7 +342,9 @@ int run_anneal(mh_context* ctx, const mh_library* lib, int32_t* d_grid, int32_t 
     job.trace_delta = d_trace_delta;
     job.trace_accept = d_trace_accept;
     job.result = d_out;
+    phase_begin(ctx, MH_PHASE_ANNEAL);
     launch_anneal_single(job, narrow, lds, ctx->stream);
+    phase_end(ctx, MH_PHASE_ANNEAL);
     HIPCHK(hipGetLastError());
     return MH_OK;
 }
@@ -315,7 +361,10 @@ int run_tiles(mh_context* ctx, const mh_library* lib, int32_t tw, int32_t th, ui
     if ((rc = slot(ctx, S_TILE_TMP, n * lib->ph * tw * 3, &tmp))) return rc;
     if ((rc = slot(ctx, S_TILES, n * th * tw * 3, &dst))) return rc;
     if ((rc = slot(ctx, S_TILE_AVG, n * 3, &avg))) return rc;
+    phase_begin(ctx, MH_PHASE_RESIZE);
     launch_resize(lib->d_pixels, lib->n, lib->pw, lib->ph, tw, th, h, v, tmp, dst, ctx->stream);
+    phase_end(ctx, MH_PHASE_RESIZE);
+    phase_begin(ctx, MH_PHASE_ADJUST);
     launch_avg_rgb(dst, (int64_t)tw * 3, 1, (int64_t)tw * th * 3, 0, tw, th, lib->n, avg,
                    ctx->stream);
     *tiles = dst;
@@ -342,7 +391,10 @@ int run_composite(mh_context* ctx, const mh_library* lib, const uint8_t* d_targe
                        th, cells, tavg, ctx->stream);
         launch_cell_adjust(d_grid, cells, tile_avg, tavg, strength, adj, ctx->stream);
     }
+    phase_end(ctx, MH_PHASE_ADJUST);
+    phase_begin(ctx, MH_PHASE_COMPOSITE);
     launch_composite(d_grid, gw, gh, tw, th, tiles, adj, adjust, d_out, ctx->stream);
+    phase_end(ctx, MH_PHASE_COMPOSITE);
     HIPCHK(hipGetLastError());
     return MH_OK;
 }
@@ -455,6 +507,13 @@ void mh_context_destroy(mh_context* ctx) {
         if (kv.second.k.p) hipFree(kv.second.k.p);
     }
     if (ctx->d_lut) hipFree(ctx->d_lut);
+    for (const PhaseMark& m : ctx->marks) {
+        hipEventDestroy(m.a);
+        hipEventDestroy(m.b);
+    }
+    for (hipEvent_t e : ctx->event_pool) hipEventDestroy(e);
+    for (hipEvent_t e : ctx->open_event)
+        if (e) hipEventDestroy(e);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -466,6 +525,71 @@ int mh_context_synchronize(mh_context* ctx) {
 }
 
 void* mh_context_stream(mh_context* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int mh_context_set_timing(mh_context* ctx, int32_t enabled) {
+    if (!ctx) return fail(MH_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    ctx->timing = enabled != 0;
+    return MH_OK;
+}
+
+int mh_context_timing(mh_context* ctx, double ms[MH_PHASES], int64_t launches[MH_PHASES]) {
+    if (!ctx || !ms || !launches) return fail(MH_ERR_INVALID_ARGUMENT, "NULL argument");
+    CHK(set_device(ctx));
+    CHK(sync(ctx));
+    for (const PhaseMark& m : ctx->marks) {
+        float t = 0.0f;
+        HIPCHK(hipEventElapsedTime(&t, m.a, m.b));
+        ctx->phase_ms[m.phase] += t;
+        ctx->phase_n[m.phase] += 1;
+        ctx->event_pool.push_back(m.a);
+        ctx->event_pool.push_back(m.b);
+    }
+    ctx->marks.clear();
+    for (int i = 0; i < MH_PHASES; ++i) {
+        ms[i] = ctx->phase_ms[i];
+        launches[i] = ctx->phase_n[i];
+        ctx->phase_ms[i] = 0.0;
+        ctx->phase_n[i] = 0;
+    }
+    return MH_OK;
+}
+
+const char* mh_phase_name(int32_t phase) {
+    static const char* names[MH_PHASES] = {"cell_lab", "topk", "place", "anneal",
+                                           "resize", "adjust", "composite", "sim_matrix"};
+    return (phase >= 0 && phase < MH_PHASES) ? names[phase] : "unknown";
+}
+
+int mh_device_alloc(mh_context* ctx, size_t bytes, void** out) {
+    if (!ctx || !out) return fail(MH_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    CHK(set_device(ctx));
+    HIPCHK(hipMalloc(out, bytes ? bytes : 16));
+    return MH_OK;
+}
+
+int mh_device_free(mh_context* ctx, void* ptr) {
+    if (!ctx) return fail(MH_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (!ptr) return MH_OK;
+    CHK(set_device(ctx));
+    CHK(sync(ctx));
+    HIPCHK(hipFree(ptr));
+    return MH_OK;
+}
+
+int mh_copy_to_device(mh_context* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return fail(MH_ERR_INVALID_ARGUMENT, "NULL argument");
+    CHK(set_device(ctx));
+    if (bytes) HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return sync(ctx);
+}
+
+int mh_copy_to_host(mh_context* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return fail(MH_ERR_INVALID_ARGUMENT, "NULL argument");
+    CHK(set_device(ctx));
+    if (bytes) HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    return sync(ctx);
+}
 
 // ---------------------------------------------------------------- library
 static int library_alloc(mh_context* ctx, int32_t n, mh_library** out) {
@@ -590,7 +714,9 @@ int mh_library_build_similarities(mh_library* lib) {
     CHK(set_device(ctx));
     if (lib->n == 0) return MH_OK;
     if (!lib->d_sim) HIPCHK(hipMalloc(&lib->d_sim, sizeof(float) * (size_t)lib->n * lib->n));
+    phase_begin(ctx, MH_PHASE_SIM);
     launch_sim_full(lib->d_lab4, lib->n, lib->d_sim, ctx->stream);
+    phase_end(ctx, MH_PHASE_SIM);
     HIPCHK(hipGetLastError());
     return sync(ctx);
 }
@@ -684,7 +810,8 @@ int mh_nearest_n(mh_context* ctx, const mh_library* lib, const float* query, int
     CHK(upload(ctx, S_IN0, query, (size_t)nq * 3, &d_q));
     CHK(slot(ctx, S_OUT0, (size_t)nq * kk, &d_i));
     CHK(slot(ctx, S_OUT1, (size_t)nq * kk, &d_d));
-    launch_topk(lib->d_L, lib->d_A, lib->d_B, lib->n, d_q, nq, kk, d_i, d_d, kk, ctx->stream);
+    launch_topk(lib->d_L, lib->d_A, lib->d_B, lib->n, d_q, nq, kk, d_i, d_d, nullptr, kk,
+                ctx->stream);
     HIPCHK(hipGetLastError());
     CHK(download(ctx, idx, d_i, (size_t)nq * kk));
     CHK(download(ctx, sqdist, d_d, (size_t)nq * kk));
@@ -697,6 +824,7 @@ int mh_place(mh_context* ctx, const mh_library* lib, const float* cell_lab,
              uint32_t* usage, uint8_t* stage, float* dist, float* score, mh_place_stats* stats) {
     if (!ctx || !lib || !p || !cell_lab || !grid || (!usage && lib->n > 0))
         return fail(MH_ERR_INVALID_ARGUMENT, "mh_place: NULL argument");
+    if (!isfinite(p->weight)) return fail(MH_ERR_INVALID_ARGUMENT, "weight must be finite");
     CHK(check_geometry(p->grid_w, p->grid_h));
     if (p->max_usage < 0) return fail(MH_ERR_INVALID_ARGUMENT, "max_usage must be >= 0");
     const int32_t cells = p->grid_w * p->grid_h;
@@ -725,8 +853,12 @@ int mh_place(mh_context* ctx, const mh_library* lib, const float* cell_lab,
     if (dist) CHK(upload(ctx, S_OUT1, dist, (size_t)cells, &d_dist));
     if (score) CHK(upload(ctx, S_OUT2, score, (size_t)cells, &d_score));
     CHK(slot(ctx, S_STATS, 8, &d_stats));
-    CHK(run_place(ctx, lib, d_lab, p->grid_w, p->grid_h, p->max_usage, p->weight, d_order,
-                  order ? n_order : cells, d_grid, d_usage, d_stage, d_dist, d_score, d_stats));
+    bool empty_grid = true;
+    for (int32_t i = 0; i < cells && empty_grid; ++i) empty_grid = grid[i] == MH_EMPTY;
+    const bool raster = !order && empty_grid;
+    CHK(run_place(ctx, lib, d_lab, p->grid_w, p->grid_h, p->max_usage, p->weight, raster, d_order,
+                  order ? n_order : cells, d_grid, d_usage, false, d_stage, d_dist, d_score,
+                  d_stats));
     int32_t st[6];
     CHK(download(ctx, grid, d_grid, (size_t)cells));
     if (usage) CHK(download(ctx, usage, d_usage, (size_t)lib->n));
@@ -763,7 +895,7 @@ int mh_total_cost(mh_context* ctx, const mh_library* lib, const int32_t* grid, i
     float *d_scratch, *d_out;
     const int32_t chunk = cells < 65536 ? cells : 65536;
     CHK(upload(ctx, S_GRID, grid, (size_t)cells, &d_grid));
-    CHK(slot(ctx, S_OUT0, (size_t)chunk * 2, &d_scratch));
+    CHK(slot(ctx, S_OUT0, (size_t)chunk * 2 + 4, &d_scratch));
     CHK(slot(ctx, S_OUT1, 1, &d_out));
     launch_total_cost(d_grid, gw, gh, lib->d_lab4, d_scratch, chunk, d_out, ctx->stream);
     HIPCHK(hipGetLastError());
@@ -963,6 +1095,7 @@ int mh_generate_device(mh_context* ctx, const mh_library* lib, const uint8_t* d_
     if (p->max_usage < 0) return fail(MH_ERR_INVALID_ARGUMENT, "max_usage must be >= 0");
     if (p->optimization_iterations < 0)
         return fail(MH_ERR_INVALID_ARGUMENT, "optimization_iterations < 0");
+    if (!isfinite(p->weight)) return fail(MH_ERR_INVALID_ARGUMENT, "weight must be finite");
     const int32_t gw = p->grid_w, gh = p->grid_h, cells = gw * gh;
     const int32_t tw = width / gw, th = height / gh;
     if (tw <= 0 || th <= 0)
@@ -982,13 +1115,13 @@ int mh_generate_device(mh_context* ctx, const mh_library* lib, const uint8_t* d_
     CHK(slot(ctx, S_STATS, 8, &d_stats));
     CHK(slot(ctx, S_ANNEAL_OUT, 1, &d_aout));
     // a1: target cell Lab means (src/main.rs:536-541)
+    phase_begin(ctx, MH_PHASE_CELL_LAB);
     launch_cell_lab(d_target, (int64_t)width * 3, gw, (int64_t)th * width * 3, (int64_t)tw * 3, tw,
                     th, cells, ctx->d_lut, d_lab, ctx->stream);
-    fill_i32(d_grid, cells, kEmpty, ctx->stream);
-    fill_i32(reinterpret_cast<int32_t*>(d_usage), lib->n, 0, ctx->stream);
-    // a3-a9: candidates + placement chain
-    CHK(run_place(ctx, lib, d_lab, gw, gh, p->max_usage, p->weight, nullptr, cells, d_grid, d_usage,
-                  nullptr, nullptr, nullptr, d_stats));
+    phase_end(ctx, MH_PHASE_CELL_LAB);
+    // a3-a9: candidates + placement chain (raster order, empty grid, zero usage)
+    CHK(run_place(ctx, lib, d_lab, gw, gh, p->max_usage, p->weight, true, nullptr, cells, d_grid,
+                  d_usage, true, nullptr, nullptr, nullptr, d_stats));
     // a10-a12: SA when enabled and weight > 0 (src/main.rs:608)
     const bool sa = p->enable_optimization && p->weight > 0.0f;
     ctx->last_ran_anneal = sa ? 1 : 0;

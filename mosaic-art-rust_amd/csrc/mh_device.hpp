@@ -15,6 +15,27 @@
 
 namespace mh {
 
+// ------------------------------------------------------- global pointers
+// Pointers read out of job descriptors are generic to the compiler, which
+// then emits flat loads; on gfx9 a flat load also counts against lgkmcnt, so
+// every LDS wait would wait for it too.  These views pin the global address
+// space (global_load / global_store).
+typedef float v4f_t __attribute__((ext_vector_type(4)));
+#define MH_GAS __attribute__((address_space(1)))
+
+__device__ __forceinline__ float4 gload4(const float4* p, int64_t i) {
+    const v4f_t v = ((const MH_GAS v4f_t*)p)[i];
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+template <typename T>
+__device__ __forceinline__ T gload(const T* p, int64_t i) {
+    return ((const MH_GAS T*)p)[i];
+}
+template <typename T>
+__device__ __forceinline__ void gstore(T* p, int64_t i, T v) {
+    ((MH_GAS T*)p)[i] = v;
+}
+
 // ----------------------------------------------------------------- helpers
 __device__ __forceinline__ float clampf(float v, float lo, float hi) {
     // Rust f32::clamp: NaN passes through.
@@ -247,6 +268,30 @@ __device__ __forceinline__ float4 optimal_adjustment(uint32_t t0, uint32_t t1, u
 }
 
 // ------------------------------------------------------ wave reductions
+// Minimum of a u32 over the 64 lanes with DPP lane moves (quad swaps, half-row
+// and row mirrors, row broadcasts 15 and 31); the total lands in lane 63 and
+// is returned uniformly.  All 64 lanes must be active.
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+    uint32_t t;
+    t = (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0xB1, 0xF, 0xF, false);
+    v = v < t ? v : t;
+    t = (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x4E, 0xF, 0xF, false);
+    v = v < t ? v : t;
+    t = (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x141, 0xF, 0xF, false);
+    v = v < t ? v : t;
+    t = (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x140, 0xF, 0xF, false);
+    v = v < t ? v : t;
+    t = (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x142, 0xA, 0xF, false);
+    v = v < t ? v : t;
+    t = (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x143, 0xC, 0xF, false);
+    v = v < t ? v : t;
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+__device__ __forceinline__ float readlane_f(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
 __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {

@@ -18,11 +18,12 @@ struct PlaceJob {
     const float4* lab4;      // n material Labs {l,a,b,0}
     int32_t n;
     const int32_t* cand_idx;  // cells x kCandStride, ascending (sqdist, idx)
-    const float* cand_d;
+    const float4* cand_lab;   // same layout: {l, a, b, sqdist} of each candidate
     const int32_t* order;     // n_order cell indices, or nullptr = raster
     int32_t n_order;
     int32_t* grid;            // cells, in/out
     uint32_t* usage;          // n, in/out
+    int32_t zero_usage;       // 1: start from zero counts instead of reading `usage`
     int32_t grid_w, grid_h;
     int32_t max_usage;
     float weight;
@@ -65,14 +66,19 @@ void launch_cell_lab(const uint8_t* rgb, int64_t row_stride, int32_t per_row,
                      int32_t n_cells, const float* lut, float* out, hipStream_t stream);
 void launch_sim_full(const float4* lab, int32_t n, float* out, hipStream_t stream);
 void launch_sim_packed(const float4* lab, int32_t n, float* out, hipStream_t stream);
+// out_d / out_lab ({l, a, b, sqdist} of each candidate) may be null.
 void launch_topk(const float* mL, const float* mA, const float* mB, int32_t n, const float* query,
-                 int32_t nq, int32_t k, int32_t* out_idx, float* out_d, int32_t out_stride,
-                 hipStream_t stream);
+                 int32_t nq, int32_t k, int32_t* out_idx, float* out_d, float4* out_lab,
+                 int32_t out_stride, hipStream_t stream);
+// General chain: any cell order, pre-filled grid and usage (state in LDS).
 size_t place_lds_bytes(int32_t cells, int32_t n);
+// Raster chain: generate_mosaic's raster order from an empty grid, where
+// only the up and left neighbours can be filled.
+size_t place_raster_lds_bytes(int32_t grid_w, int32_t n);
 // One job passed by value (kernel argument), or n_jobs jobs from a device
 // array (one wave each).
-void launch_place_single(const PlaceJob& job, size_t lds_bytes, hipStream_t stream);
-void launch_place_batch(const PlaceJob* d_jobs, int32_t n_jobs, size_t lds_bytes,
+void launch_place_single(const PlaceJob& job, bool raster, size_t lds_bytes, hipStream_t stream);
+void launch_place_batch(const PlaceJob* d_jobs, int32_t n_jobs, bool raster, size_t lds_bytes,
                         hipStream_t stream);
 size_t anneal_lds_bytes(int32_t cells, bool narrow);
 void launch_anneal_single(const AnnealJob& job, bool narrow, size_t lds_bytes, hipStream_t stream);

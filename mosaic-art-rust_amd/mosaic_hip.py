@@ -20,11 +20,14 @@ MH_ERR_OUT_OF_MEMORY = -4
 MH_ERR_NOT_FOUND = -5
 MH_ERR_UNSUPPORTED = -6
 MH_EMPTY = -1
+MH_PHASES = 8
 
 # Every symbol include/mosaic_hip.h declares.
 EXPORTS = [
     "mh_last_error", "mh_abi_version", "mh_context_create", "mh_context_destroy",
-    "mh_context_synchronize", "mh_context_stream", "mh_library_from_lab",
+    "mh_context_synchronize", "mh_context_stream", "mh_context_set_timing",
+    "mh_context_timing", "mh_phase_name", "mh_device_alloc", "mh_device_free",
+    "mh_copy_to_device", "mh_copy_to_host", "mh_library_from_lab",
     "mh_library_from_pixels", "mh_library_destroy", "mh_library_size", "mh_library_lab",
     "mh_library_build_similarities", "mh_library_similarities", "mh_library_get_similarity",
     "mh_average_lab", "mh_cell_lab_means", "mh_nearest_n", "mh_place", "mh_total_cost",
@@ -104,6 +107,13 @@ def load(path=LIB_PATH):
         "mh_context_destroy": (None, [P]),
         "mh_context_synchronize": (ctypes.c_int, [P]),
         "mh_context_stream": (P, [P]),
+        "mh_context_set_timing": (ctypes.c_int, [P, i32]),
+        "mh_context_timing": (ctypes.c_int, [P, P, P]),
+        "mh_phase_name": (ctypes.c_char_p, [i32]),
+        "mh_device_alloc": (ctypes.c_int, [P, ctypes.c_size_t, ctypes.POINTER(P)]),
+        "mh_device_free": (ctypes.c_int, [P, P]),
+        "mh_copy_to_device": (ctypes.c_int, [P, P, P, ctypes.c_size_t]),
+        "mh_copy_to_host": (ctypes.c_int, [P, P, P, ctypes.c_size_t]),
         "mh_library_from_lab": (ctypes.c_int, [P, P, i32, ctypes.POINTER(P)]),
         "mh_library_from_pixels": (ctypes.c_int, [P, P, i32, i32, i32, ctypes.POINTER(P)]),
         "mh_library_destroy": (None, [P]),
@@ -179,6 +189,38 @@ class Context:
     @property
     def stream(self):
         return self.lib.mh_context_stream(self.h)
+
+    def set_timing(self, enabled):
+        _check(self.lib.mh_context_set_timing(self.h, 1 if enabled else 0))
+
+    def timing(self):
+        """{phase: (total_ms, launches)} since the last call (synchronises)."""
+        ms = np.zeros(MH_PHASES, np.float64)
+        n = np.zeros(MH_PHASES, np.int64)
+        _check(self.lib.mh_context_timing(self.h, _ptr(ms), _ptr(n)))
+        return {self.lib.mh_phase_name(i).decode(): (float(ms[i]), int(n[i]))
+                for i in range(MH_PHASES)}
+
+    # ------------------------------------------------------ device buffers
+    def device_alloc(self, nbytes):
+        p = ctypes.c_void_p()
+        _check(self.lib.mh_device_alloc(self.h, nbytes, ctypes.byref(p)))
+        return p.value
+
+    def device_free(self, ptr):
+        _check(self.lib.mh_device_free(self.h, ctypes.c_void_p(ptr)))
+
+    def to_device(self, array):
+        """Copies a numpy array into a new device buffer; returns its address."""
+        a = np.ascontiguousarray(array)
+        ptr = self.device_alloc(a.nbytes)
+        _check(self.lib.mh_copy_to_device(self.h, ctypes.c_void_p(ptr), _ptr(a), a.nbytes))
+        return ptr
+
+    def to_host(self, ptr, shape, dtype):
+        out = np.empty(shape, dtype)
+        _check(self.lib.mh_copy_to_host(self.h, _ptr(out), ctypes.c_void_p(ptr), out.nbytes))
+        return out
 
     # ---------------------------------------------------------------- a1
     def average_lab(self, rgb):
