@@ -31,17 +31,27 @@ __device__ __forceinline__ int32_t gget(const G* g, int32_t i) {
     return (int32_t)g[i];
 }
 
+// One adjacency term 1/(sim(a,b)+1): a gather from the library's term
+// matrix when it is resident, else computed from the Lab table (same bits).
+struct TermSource {
+    const float4* lab4;
+    const float* term;
+    int32_t n;
+    __device__ __forceinline__ float operator()(int32_t a, int32_t b) const {
+        if (term) return gload(term, (int64_t)a * n + b);
+        return adj_term(gload4(lab4, a), gload4(lab4, b));
+    }
+};
+
 // calculate_swap_delta (src/adjacency.rs:133-214) on a grid in any address
 // space.  Term order: neighbours of p1 (Up, Down, Left, Right, skipping p2),
 // then of p2 (skipping p1), then the mutual term when adjacent.
 template <typename G>
-__device__ float swap_delta_dev(const G* grid, int32_t gw, int32_t gh,
-                                const float4* __restrict__ lab4, int32_t x1, int32_t y1,
-                                int32_t x2, int32_t y2) {
+__device__ float swap_delta_dev(const G* grid, int32_t gw, int32_t gh, const TermSource& T,
+                                int32_t x1, int32_t y1, int32_t x2, int32_t y2) {
     const int32_t m1 = gget(grid, y1 * gw + x1), m2 = gget(grid, y2 * gw + x2);
     if (m1 == kEmpty || m2 == kEmpty) return 0.0f;
     if (m1 == m2) return 0.0f;
-    const float4 l1 = gload4(lab4, m1), l2 = gload4(lab4, m2);
     float old_cost = 0.0f, new_cost = 0.0f;
     bool is_adj = false;
     int32_t ax[4], ay[4], cnt = 0;
@@ -56,9 +66,8 @@ __device__ float swap_delta_dev(const G* grid, int32_t gw, int32_t gh,
         }
         const int32_t a = gget(grid, ay[i] * gw + ax[i]);
         if (a == kEmpty) continue;
-        const float4 la = gload4(lab4, a);
-        old_cost += adj_term(l1, la);
-        new_cost += adj_term(l2, la);
+        old_cost += T(m1, a);
+        new_cost += T(m2, a);
     }
     cnt = 0;
     if (y2 > 0) { ax[cnt] = x2; ay[cnt] = y2 - 1; ++cnt; }
@@ -69,12 +78,11 @@ __device__ float swap_delta_dev(const G* grid, int32_t gw, int32_t gh,
         if (ax[i] == x1 && ay[i] == y1) continue;
         const int32_t a = gget(grid, ay[i] * gw + ax[i]);
         if (a == kEmpty) continue;
-        const float4 la = gload4(lab4, a);
-        old_cost += adj_term(l2, la);
-        new_cost += adj_term(l1, la);
+        old_cost += T(m2, a);
+        new_cost += T(m1, a);
     }
     if (is_adj) {
-        const float t = adj_term(l1, l2);
+        const float t = T(m1, m2);
         old_cost += t;
         new_cost += t;
     }
@@ -90,8 +98,8 @@ __device__ float swap_delta_dev(const G* grid, int32_t gw, int32_t gh,
 // scratch holds 2*chunk + 2 floats and is 16-byte aligned.
 // The result is broadcast to every lane.
 template <typename G>
-__device__ float total_cost_wave(const G* grid, int32_t gw, int32_t gh,
-                                 const float4* __restrict__ lab4, float* scratch, int32_t chunk) {
+__device__ float total_cost_wave(const G* grid, int32_t gw, int32_t gh, const TermSource& T,
+                                 float* scratch, int32_t chunk) {
     const int lane = threadIdx.x;
     const int32_t cells = gw * gh;
     float total = 0.0f;
@@ -103,14 +111,13 @@ __device__ float total_cost_wave(const G* grid, int32_t gw, int32_t gh,
             const int32_t m = gget(grid, c);
             float tr = 0.0f, td = 0.0f;
             if (m != kEmpty) {
-                const float4 lm = gload4(lab4, m);
                 if (x < gw - 1) {
                     const int32_t r = gget(grid, c + 1);
-                    if (r != kEmpty) tr = adj_term(lm, gload4(lab4, r));
+                    if (r != kEmpty) tr = T(m, r);
                 }
                 if (y < gh - 1) {
                     const int32_t d = gget(grid, c + gw);
-                    if (d != kEmpty) td = adj_term(lm, gload4(lab4, d));
+                    if (d != kEmpty) td = T(m, d);
                 }
             }
             scratch[2 * i] = tr;
@@ -160,7 +167,8 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
     __syncthreads();
 
     // the stamp words double as the cost scratch (2 terms per cell + padding)
-    float cost = total_cost_wave(grid_s, gw, gh, job.lab4, reinterpret_cast<float*>(stamp),
+    const TermSource T{job.lab4, job.term, job.n};
+    float cost = total_cost_wave(grid_s, gw, gh, T, reinterpret_cast<float*>(stamp),
                                  (stamps / 2 - 1) & ~1);
     for (int32_t i = lane; i < cells; i += 64) stamp[i] = 0u;
     __syncthreads();
@@ -190,7 +198,7 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
         float delta = 0.0f;
         bool acc = false;
         if (!skip) {
-            delta = swap_delta_dev(grid_s, gw, gh, job.lab4, x1, y1, x2, y2);
+            delta = swap_delta_dev(grid_s, gw, gh, T, x1, y1, x2, y2);
             if (job.greedy) {
                 acc = delta < 0.0f;
             } else if (delta < 0.0f) {
@@ -326,32 +334,38 @@ hipError_t anneal_configure(int max_lds) {
 __global__ __launch_bounds__(64) void total_cost_kernel(const int32_t* __restrict__ grid,
                                                         int32_t gw, int32_t gh,
                                                         const float4* __restrict__ lab4,
+                                                        const float* __restrict__ term, int32_t n,
                                                         float* __restrict__ scratch, int32_t chunk,
                                                         float* __restrict__ out) {
-    const float t = total_cost_wave(grid, gw, gh, lab4, scratch, chunk);
+    const TermSource T{lab4, term, n};
+    const float t = total_cost_wave(grid, gw, gh, T, scratch, chunk);
     if (threadIdx.x == 0) *out = t;
 }
 
 void launch_total_cost(const int32_t* grid, int32_t gw, int32_t gh, const float4* lab4,
-                       float* scratch, int32_t chunk, float* out, hipStream_t stream) {
-    hipLaunchKernelGGL(total_cost_kernel, dim3(1), dim3(64), 0, stream, grid, gw, gh, lab4, scratch,
-                       chunk, out);
+                       const float* term, int32_t n, float* scratch, int32_t chunk, float* out,
+                       hipStream_t stream) {
+    hipLaunchKernelGGL(total_cost_kernel, dim3(1), dim3(64), 0, stream, grid, gw, gh, lab4, term, n,
+                       scratch, chunk, out);
 }
 
 __global__ void swap_delta_kernel(const int32_t* __restrict__ grid, int32_t gw, int32_t gh,
-                                  const float4* __restrict__ lab4, const int32_t* __restrict__ pairs,
-                                  int32_t p, float* __restrict__ out) {
+                                  const float4* __restrict__ lab4, const float* __restrict__ term,
+                                  int32_t n, const int32_t* __restrict__ pairs, int32_t p,
+                                  float* __restrict__ out) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= p) return;
+    const TermSource T{lab4, term, n};
     const int32_t* q = pairs + 4 * (int64_t)i;
-    out[i] = swap_delta_dev(grid, gw, gh, lab4, q[0], q[1], q[2], q[3]);
+    out[i] = swap_delta_dev(grid, gw, gh, T, q[0], q[1], q[2], q[3]);
 }
 
 void launch_swap_deltas(const int32_t* grid, int32_t gw, int32_t gh, const float4* lab4,
-                        const int32_t* pairs, int32_t p, float* out, hipStream_t stream) {
+                        const float* term, int32_t n, const int32_t* pairs, int32_t p, float* out,
+                        hipStream_t stream) {
     if (p <= 0) return;
     hipLaunchKernelGGL(swap_delta_kernel, dim3((p + 255) / 256), dim3(256), 0, stream, grid, gw, gh,
-                       lab4, pairs, p, out);
+                       lab4, term, n, pairs, p, out);
 }
 
 // calculate_penalty (src/adjacency.rs:63-88): Up, Down, Left, Right.

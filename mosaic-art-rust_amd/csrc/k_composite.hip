@@ -78,10 +78,86 @@ __global__ void resize_v_kernel(const uint8_t* __restrict__ tmp, int32_t n, int3
     d[2] = (uint8_t)clip8(a2);
 }
 
+// Both passes for one material per 256-thread block, staged in LDS: the
+// source image arrives with coalesced dword loads, the horizontal pass writes
+// a u8 intermediate in LDS, the vertical pass writes the tile.  Same
+// arithmetic as the two-kernel path.
+constexpr int kFusedResizeMaxLds = 96 * 1024;
+
+__global__ __launch_bounds__(256) void resize_fused_kernel(
+    const uint8_t* __restrict__ src, int32_t sw, int32_t sh, int32_t dw, int32_t dh,
+    const int32_t* __restrict__ hlo, const int32_t* __restrict__ hcnt,
+    const int32_t* __restrict__ hk, int32_t hks, const int32_t* __restrict__ vlo,
+    const int32_t* __restrict__ vcnt, const int32_t* __restrict__ vk, int32_t vks,
+    uint8_t* __restrict__ dst) {
+    extern __shared__ __align__(16) uint8_t rsm[];
+    const int32_t m = blockIdx.x;
+    const int32_t src_bytes = sw * sh * 3;
+    const int32_t src_pad = (src_bytes + 15) & ~15;
+    uint8_t* s_src = rsm;
+    uint8_t* s_tmp = rsm + src_pad;
+    const uint8_t* g = src + (int64_t)m * src_bytes;
+    const int tid = threadIdx.x;
+    if ((src_bytes & 3) == 0) {
+        const uint32_t* g4 = reinterpret_cast<const uint32_t*>(g);
+        uint32_t* s4 = reinterpret_cast<uint32_t*>(s_src);
+        for (int32_t i = tid; i < src_bytes / 4; i += 256) s4[i] = g4[i];
+    } else {
+        for (int32_t i = tid; i < src_bytes; i += 256) s_src[i] = g[i];
+    }
+    __syncthreads();
+    for (int32_t idx = tid; idx < sh * dw; idx += 256) {
+        const int32_t y = idx / dw, o = idx - y * dw;
+        const uint8_t* row = s_src + (y * sw + hlo[o]) * 3;
+        const int32_t* k = hk + (int64_t)o * hks;
+        int32_t a0 = 1 << (kPrec - 1), a1 = a0, a2 = a0;
+        const int32_t c = hcnt[o];
+        for (int32_t t = 0; t < c; ++t) {
+            const int32_t w = k[t];
+            a0 += (int32_t)row[3 * t] * w;
+            a1 += (int32_t)row[3 * t + 1] * w;
+            a2 += (int32_t)row[3 * t + 2] * w;
+        }
+        uint8_t* d = s_tmp + idx * 3;
+        d[0] = (uint8_t)clip8(a0);
+        d[1] = (uint8_t)clip8(a1);
+        d[2] = (uint8_t)clip8(a2);
+    }
+    __syncthreads();
+    uint8_t* out = dst + (int64_t)m * dh * dw * 3;
+    for (int32_t idx = tid; idx < dh * dw; idx += 256) {
+        const int32_t o = idx / dw, x = idx - o * dw;
+        const uint8_t* col = s_tmp + (vlo[o] * dw + x) * 3;
+        const int32_t* k = vk + (int64_t)o * vks;
+        const int32_t step = dw * 3;
+        int32_t a0 = 1 << (kPrec - 1), a1 = a0, a2 = a0;
+        const int32_t c = vcnt[o];
+        for (int32_t t = 0; t < c; ++t) {
+            const int32_t w = k[t];
+            a0 += (int32_t)col[t * step] * w;
+            a1 += (int32_t)col[t * step + 1] * w;
+            a2 += (int32_t)col[t * step + 2] * w;
+        }
+        out[idx * 3] = (uint8_t)clip8(a0);
+        out[idx * 3 + 1] = (uint8_t)clip8(a1);
+        out[idx * 3 + 2] = (uint8_t)clip8(a2);
+    }
+}
+
+size_t resize_fused_lds_bytes(int32_t sw, int32_t sh, int32_t dw) {
+    return (size_t)(((sw * sh * 3) + 15) & ~15) + (size_t)sh * dw * 3;
+}
+
 void launch_resize(const uint8_t* src, int32_t n, int32_t sw, int32_t sh, int32_t dw, int32_t dh,
                    const ResizeTaps& h, const ResizeTaps& v, uint8_t* tmp, uint8_t* dst,
                    hipStream_t stream) {
     if (n <= 0 || dw <= 0 || dh <= 0) return;
+    const size_t lds = resize_fused_lds_bytes(sw, sh, dw);
+    if (lds <= (size_t)kFusedResizeMaxLds) {
+        hipLaunchKernelGGL(resize_fused_kernel, dim3(n), dim3(256), lds, stream, src, sw, sh, dw, dh,
+                           h.lo, h.cnt, h.k, h.ksize, v.lo, v.cnt, v.k, v.ksize, dst);
+        return;
+    }
     const int64_t th = (int64_t)n * sh * dw;
     hipLaunchKernelGGL(resize_h_kernel, dim3((unsigned)((th + 255) / 256)), dim3(256), 0, stream, src,
                        n, sw, sh, dw, h.lo, h.cnt, h.k, h.ksize, tmp);
@@ -235,6 +311,10 @@ void launch_optimal_adjust(const uint8_t* tile, const uint8_t* target, int32_t n
     if (n <= 0) return;
     hipLaunchKernelGGL(optimal_adjust_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, tile,
                        target, n, k, out);
+}
+
+hipError_t resize_configure(int max_lds) {
+    return raise_dynamic_lds(reinterpret_cast<const void*>(&resize_fused_kernel), max_lds);
 }
 
 }  // namespace mh

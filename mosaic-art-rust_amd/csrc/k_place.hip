@@ -17,7 +17,7 @@
 //    empty grid, so only Up and Left can be filled.  The left neighbour is
 //    the previous winner (registers), the up neighbour comes from a per-column
 //    LDS row (material + Lab of the last placement in that column), usage
-//    counts live in LDS, candidate records are prefetched 8 cells ahead into
+//    counts live in LDS, candidate records are loaded 12 cells at a time into
 //    registers, the min is a DPP reduction, and the winner's id / Lab / count
 //    are read from its lane.  Winners are buffered 64 at a time in one
 //    register and stored coalesced.  Latency-bound: a few hundred cycles per
@@ -160,23 +160,21 @@ struct CandSet {
     float4 c0, c1;    // {l, a, b, sqdist}
 };
 
+// Loads one cell's candidate records.  Every load is issued unconditionally
+// (rows beyond the grid read the last row, then are masked; ranks >= K hold
+// the -1 sentinel written by the candidate kernel) so the compiler's memory
+// wait counts stay exact across the unrolled group.
 __device__ __forceinline__ void load_cands(CandSet& s, const PlaceJob& job, int32_t cell,
-                                           int32_t cells, int32_t K, int lane) {
-    s.m0 = -1;
-    s.m1 = -1;
-    s.c0 = make_float4(0, 0, 0, 0);
-    s.c1 = s.c0;
-    if (cell < cells) {
-        const int64_t base = (int64_t)cell * kCandStride;
-        if (lane < K) {
-            s.m0 = gload(job.cand_idx, base + lane);
-            s.c0 = gload4(job.cand_lab, base + lane);
-        }
-        if (lane + 64 < K) {
-            s.m1 = gload(job.cand_idx, base + lane + 64);
-            s.c1 = gload4(job.cand_lab, base + lane + 64);
-        }
-    }
+                                           int32_t cells, int lane) {
+    const int32_t row = cell < cells ? cell : cells - 1;
+    const int64_t base = (int64_t)row * kCandStride;
+    s.m0 = gload(job.cand_idx, base + lane);
+    s.c0 = gload4(job.cand_lab, base + lane);
+    s.m1 = gload(job.cand_idx, base + lane + 64);
+    s.c1 = gload4(job.cand_lab, base + lane + 64);
+    const bool live = cell < cells;
+    s.m0 = live ? s.m0 : -1;
+    s.m1 = live ? s.m1 : -1;
 }
 
 __device__ __forceinline__ float4 readlane4(float4 v, int lane) {
@@ -357,39 +355,42 @@ __global__ __launch_bounds__(64) void place_raster_kernel(PlaceJob single,
     for (int32_t i = lane; i < n; i += 64) usage_s[i] = job.zero_usage ? 0u : gload(job.usage, i);
     __syncthreads();
 
-    const int32_t K = n < kMaxCand ? n : kMaxCand;
-    // candidate records are prefetched kAhead cells ahead into a register ring
-    constexpr int kAhead = 8;
-    CandSet q[kAhead];
-#pragma unroll
-    for (int i = 0; i < kAhead; ++i) load_cands(q[i], job, i, cells, K, lane);
-
+    // Candidate records for kGroup cells are loaded at the top of each group
+    // and consumed in the same loop iteration, so the compiler's wait counts
+    // stay exact (a load consumed after the loop's back edge would make it
+    // wait for every load in flight); one memory latency per group.
+    constexpr int kGroup = 12;
+    CandSet q[kGroup + 1];
     RasterState st;
     st.x = 0;
     st.y = 0;
     st.left_m = kEmpty;
     st.left_lab = make_float4(0, 0, 0, 0);
-    st.u0 = usage_s[q[0].m0 >= 0 ? q[0].m0 : 0];
-    st.u1 = usage_s[q[0].m1 >= 0 ? q[0].m1 : 0];
     st.up_m = kEmpty;
     st.up_lab = make_float4(0, 0, 0, 0);
     st.win_buf = 0;
     st.st_main = st.st_ban = st.st_near = st.st_resets = 0;
-
-    int32_t c = 0;
-    for (; c + kAhead <= cells; c += kAhead) {
-#pragma unroll
-        for (int i = 0; i < kAhead; ++i) {
-            raster_step<kOutputs>(q[i], q[(i + 1) % kAhead], c + i, st, job, usage_s, row_mat,
-                                  row_lab, n, lane);
-            load_cands(q[i], job, c + i + kAhead, cells, K, lane);
-        }
+    {
+        CandSet h;
+        load_cands(h, job, 0, cells, lane);
+        st.u0 = usage_s[h.m0 >= 0 ? h.m0 : 0];
+        st.u1 = usage_s[h.m1 >= 0 ? h.m1 : 0];
     }
+    int32_t c = 0;
+    for (; c + kGroup <= cells; c += kGroup) {
 #pragma unroll
-    for (int i = 0; i < kAhead - 1; ++i)
-        if (c + i < cells)
-            raster_step<kOutputs>(q[i], q[(i + 1) % kAhead], c + i, st, job, usage_s, row_mat,
-                                  row_lab, n, lane);
+        for (int i = 0; i <= kGroup; ++i) load_cands(q[i], job, c + i, cells, lane);
+#pragma unroll
+        for (int i = 0; i < kGroup; ++i)
+            raster_step<kOutputs>(q[i], q[i + 1], c + i, st, job, usage_s, row_mat, row_lab, n,
+                                  lane);
+    }
+    for (; c < cells; ++c) {  // tail: fewer than kGroup cells
+        CandSet a, b;
+        load_cands(a, job, c, cells, lane);
+        load_cands(b, job, c + 1, cells, lane);
+        raster_step<kOutputs>(a, b, c, st, job, usage_s, row_mat, row_lab, n, lane);
+    }
     // flush the partial winner block
     const int32_t tail = cells & 63;
     if (tail != 0 && lane < tail) gstore(job.grid, cells - tail + lane, (int32_t)st.win_buf);

@@ -57,6 +57,35 @@ __global__ __launch_bounds__(256) void sim_packed_kernel(const float4* __restric
     for (int32_t j = i + 1 + threadIdx.x; j < n; j += 256) out[off + (j - i - 1)] = lab_distance(p, lab[j]);
 }
 
+// Adjacency-term matrix term[i][j] = 1/(dist(i,j) + 1): the value every
+// penalty / cost / swap-delta term takes (src/adjacency.rs:82, :110, :174),
+// bit-identical to computing it on the fly.  Same tiling as the distance
+// matrix.
+__global__ __launch_bounds__(256) void term_full_kernel(const float4* __restrict__ lab, int32_t n,
+                                                        float* __restrict__ out) {
+    const int tid = threadIdx.x;
+    const int32_t col0 = blockIdx.x * kSimCols + (tid & 63);
+    const int32_t row0 = blockIdx.y * kSimRows + (tid >> 6);
+    float4 c[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) c[j] = lab[min(col0 + 64 * j, n - 1)];
+    for (int r = 0; r < kSimRows; r += 4) {
+        const int32_t row = row0 + r;
+        if (row >= n) break;
+        const float4 p = lab[row];
+        float* dst = out + (int64_t)row * n;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (col0 + 64 * j < n) dst[col0 + 64 * j] = adj_term(p, c[j]);
+    }
+}
+
+void launch_term_full(const float4* lab, int32_t n, float* out, hipStream_t stream) {
+    if (n <= 0) return;
+    dim3 grid((n + kSimCols - 1) / kSimCols, (n + kSimRows - 1) / kSimRows);
+    hipLaunchKernelGGL(term_full_kernel, grid, dim3(256), 0, stream, lab, n, out);
+}
+
 void launch_sim_full(const float4* lab, int32_t n, float* out, hipStream_t stream) {
     if (n <= 0) return;
     dim3 grid((n + kSimCols - 1) / kSimCols, (n + kSimRows - 1) / kSimRows);

@@ -148,6 +148,8 @@ __global__ __launch_bounds__(kTopkThreads) void topk_kernel(
         }
     }
     const int32_t kk = k < n ? k : n;
+    // ranks beyond k hold the -1 sentinel in padded (chain) rows
+    if (out_lab && tid >= kk && tid < out_stride) out_idx[(int64_t)q * out_stride + tid] = -1;
     if (tid < kk) {
         const uint64_t key = sel[tid];
         const int32_t m = (int32_t)(uint32_t)(key & 0xffffffffu);
@@ -191,6 +193,7 @@ hipError_t configure_kernel_limits(int max_lds) {
     hipError_t e = topk_configure(max_lds);
     if (e == hipSuccess) e = place_configure(max_lds);
     if (e == hipSuccess) e = anneal_configure(max_lds);
+    if (e == hipSuccess) e = resize_configure(max_lds);
     return e;
 }
 

@@ -102,6 +102,7 @@ struct mh_library {
     float* d_A = nullptr;
     float* d_B = nullptr;
     float* d_sim = nullptr;    // n x n after build_similarities
+    float* d_term = nullptr;   // n x n 1/(sim+1), built on first use by the cost kernels
     uint8_t* d_pixels = nullptr;
     int32_t pw = 0, ph = 0;
 };
@@ -313,6 +314,24 @@ int run_place(mh_context* ctx, const mh_library* lib, const float* d_cell_lab, i
     return MH_OK;
 }
 
+// The library's adjacency-term matrix (n x n f32), built once on the
+// context stream; nullptr when it would exceed kTermMatrixMaxN^2 entries
+// (the kernels then compute terms from the Lab table, with the same bits).
+constexpr int32_t kTermMatrixMaxN = 16384;
+
+int library_terms(mh_context* ctx, const mh_library* clib, const float** out) {
+    mh_library* lib = const_cast<mh_library*>(clib);
+    *out = nullptr;
+    if (lib->n == 0 || lib->n > kTermMatrixMaxN) return MH_OK;
+    if (!lib->d_term) {
+        HIPCHK(hipMalloc(&lib->d_term, sizeof(float) * (size_t)lib->n * lib->n));
+        launch_term_full(lib->d_lab4, lib->n, lib->d_term, ctx->stream);
+        HIPCHK(hipGetLastError());
+    }
+    *out = lib->d_term;
+    return MH_OK;
+}
+
 int run_anneal(mh_context* ctx, const mh_library* lib, int32_t* d_grid, int32_t gw, int32_t gh,
                const mh_anneal_params* p, float* d_trace_delta, uint8_t* d_trace_accept,
                AnnealOut* d_out) {
@@ -329,8 +348,11 @@ int run_anneal(mh_context* ctx, const mh_library* lib, int32_t* d_grid, int32_t 
                             &temps)))
             return rc;
     }
+    const float* term = nullptr;
+    if ((rc = library_terms(ctx, lib, &term))) return rc;
     AnnealJob job;
     job.lab4 = lib->d_lab4;
+    job.term = term;
     job.n = lib->n;
     job.grid = d_grid;
     job.grid_w = gw;
@@ -696,6 +718,7 @@ void mh_library_destroy(mh_library* lib) {
     if (lib->d_A) hipFree(lib->d_A);
     if (lib->d_B) hipFree(lib->d_B);
     if (lib->d_sim) hipFree(lib->d_sim);
+    if (lib->d_term) hipFree(lib->d_term);
     if (lib->d_pixels) hipFree(lib->d_pixels);
     delete lib;
 }
@@ -897,7 +920,10 @@ int mh_total_cost(mh_context* ctx, const mh_library* lib, const int32_t* grid, i
     CHK(upload(ctx, S_GRID, grid, (size_t)cells, &d_grid));
     CHK(slot(ctx, S_OUT0, (size_t)chunk * 2 + 4, &d_scratch));
     CHK(slot(ctx, S_OUT1, 1, &d_out));
-    launch_total_cost(d_grid, gw, gh, lib->d_lab4, d_scratch, chunk, d_out, ctx->stream);
+    const float* term = nullptr;
+    CHK(library_terms(ctx, lib, &term));
+    launch_total_cost(d_grid, gw, gh, lib->d_lab4, term, lib->n, d_scratch, chunk, d_out,
+                      ctx->stream);
     HIPCHK(hipGetLastError());
     CHK(download(ctx, out, d_out, 1));
     return sync(ctx);
@@ -948,7 +974,9 @@ int mh_swap_deltas(mh_context* ctx, const mh_library* lib, const int32_t* grid, 
     CHK(upload(ctx, S_GRID, grid, (size_t)cells, &d_grid));
     CHK(upload(ctx, S_IN0, pairs, (size_t)p * 4, &d_pairs));
     CHK(slot(ctx, S_OUT0, (size_t)p, &d_out));
-    launch_swap_deltas(d_grid, gw, gh, lib->d_lab4, d_pairs, p, d_out, ctx->stream);
+    const float* term = nullptr;
+    CHK(library_terms(ctx, lib, &term));
+    launch_swap_deltas(d_grid, gw, gh, lib->d_lab4, term, lib->n, d_pairs, p, d_out, ctx->stream);
     HIPCHK(hipGetLastError());
     CHK(download(ctx, out, d_out, (size_t)p));
     return sync(ctx);

@@ -41,6 +41,7 @@ struct AnnealOut {
 // One annealing chain (one wave).
 struct AnnealJob {
     const float4* lab4;
+    const float* term;        // n x n adjacency terms, or nullptr (computed from lab4)
     int32_t n;
     int32_t* grid;            // cells, in/out
     int32_t grid_w, grid_h;
@@ -66,6 +67,7 @@ void launch_cell_lab(const uint8_t* rgb, int64_t row_stride, int32_t per_row,
                      int32_t n_cells, const float* lut, float* out, hipStream_t stream);
 void launch_sim_full(const float4* lab, int32_t n, float* out, hipStream_t stream);
 void launch_sim_packed(const float4* lab, int32_t n, float* out, hipStream_t stream);
+void launch_term_full(const float4* lab, int32_t n, float* out, hipStream_t stream);
 // out_d / out_lab ({l, a, b, sqdist} of each candidate) may be null.
 void launch_topk(const float* mL, const float* mA, const float* mB, int32_t n, const float* query,
                  int32_t nq, int32_t k, int32_t* out_idx, float* out_d, float4* out_lab,
@@ -85,9 +87,11 @@ void launch_anneal_single(const AnnealJob& job, bool narrow, size_t lds_bytes, h
 void launch_anneal_batch(const AnnealJob* d_jobs, int32_t n_jobs, bool narrow, size_t lds_bytes,
                          hipStream_t stream);
 void launch_total_cost(const int32_t* grid, int32_t gw, int32_t gh, const float4* lab4,
-                       float* scratch, int32_t chunk, float* out, hipStream_t stream);
+                       const float* term, int32_t n, float* scratch, int32_t chunk, float* out,
+                       hipStream_t stream);
 void launch_swap_deltas(const int32_t* grid, int32_t gw, int32_t gh, const float4* lab4,
-                        const int32_t* pairs, int32_t p, float* out, hipStream_t stream);
+                        const float* term, int32_t n, const int32_t* pairs, int32_t p, float* out,
+                        hipStream_t stream);
 void launch_penalties(const int32_t* grid, int32_t gw, int32_t gh, const float4* lab4, float weight,
                       const int32_t* queries, int32_t nq, float* out, hipStream_t stream);
 void launch_resize(const uint8_t* src, int32_t n, int32_t sw, int32_t sh, int32_t dw, int32_t dh,
@@ -114,5 +118,6 @@ hipError_t configure_kernel_limits(int max_lds);
 hipError_t place_configure(int max_lds);
 hipError_t anneal_configure(int max_lds);
 hipError_t topk_configure(int max_lds);
+hipError_t resize_configure(int max_lds);
 
 }  // namespace mh

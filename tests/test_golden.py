@@ -93,12 +93,11 @@ def test_hip_matches_fixtures(ctx):
 
     lp = G["lab_pixels"]
     img = np.array(lp["rgb"], np.uint8)[None]
-    assert bits(ctx.cell_lab_means(img, len(lp["rgb"]), 1)) == [x for row in lp["lab_bits"] for x in row]
+    assert bits(ctx.cell_lab_means(img, len(lp["rgb"]), 1).ravel()) == [x for row in lp["lab_bits"] for x in row]
     ap = G["adjust_pixels"]
     for a_, row in zip(ap["adj"], ap["out"]):
         assert ctx.adjust_pixels(f(a_), np.array(ap["rgb"], np.uint8)).tolist() == row
     oa = G["optimal_adjustment"]
     t = np.array([p[0] for p in oa["pairs"]], np.uint8)
     u = np.array([p[1] for p in oa["pairs"]], np.uint8)
-    assert bits(ctx.optimal_adjustments(t, u, 1.0)).__len__() == len(oa["adj_bits"])
     assert [bits(r_) for r_ in ctx.optimal_adjustments(t, u, 1.0)] == oa["adj_bits"]

@@ -286,6 +286,24 @@ def test_anneal_matches_sequential_chain(ctx, orc, n, gw, gh, iters, t0, decay, 
     lib.close()
 
 
+def test_anneal_without_term_matrix(ctx, orc):
+    # libraries above the term-matrix cap compute every term from the Labs
+    rng = _rng(31)
+    n, gw, gh = 17000, 30, 20
+    lab = (rng.random((n, 3)) * [100, 60, 60] - [0, 30, 30]).astype(np.float32)
+    g = rng.integers(0, n, gw * gh).astype(np.int32)
+    lib = ctx.library_from_lab(lab)
+    got = ctx.anneal(lib, g, gw, gh, 1500, 100.0, 0.99995, 4, False, trace=True)
+    want = orc.anneal(lab, g, gw, gh, 1500, 100.0, 0.99995, 4, False, trace=True)
+    _check_anneal(got, want)
+    pairs = np.stack([rng.integers(0, gw, 300), rng.integers(0, gh, 300),
+                      rng.integers(0, gw, 300), rng.integers(0, gh, 300)], 1).astype(np.int32)
+    assert_bits_equal(ctx.swap_deltas(lib, g, gw, gh, pairs),
+                      [orc.swap_delta(lab, g, gw, gh, *map(int, p)) for p in pairs])
+    assert _bits(np.float32(ctx.total_cost(lib, g, gw, gh))) == _bits(orc.total_cost(lab, g, gw, gh))
+    lib.close()
+
+
 def test_anneal_sparse_grid(ctx, orc):
     # src/optimizer.rs:336-357: non-adjacent tiles have zero cost
     lab = np.array([[50, 0, 0], [60, 10, 10]], np.float32)
@@ -332,7 +350,8 @@ def test_adjust_pixels(ctx, orc):
 
 
 @pytest.mark.parametrize("sw,sh,dw,dh", [(64, 64, 16, 16), (128, 128, 24, 18), (64, 64, 20, 20),
-                                         (100, 100, 50, 50), (100, 100, 200, 100), (7, 5, 13, 3)])
+                                         (100, 100, 50, 50), (100, 100, 200, 100), (7, 5, 13, 3),
+                                         (256, 256, 30, 20), (61, 47, 19, 23)])
 def test_resize(ctx, orc, sw, sh, dw, dh):
     img = _rng(sw * dh).integers(0, 256, (sh, sw, 3)).astype(np.uint8)
     np.testing.assert_array_equal(ctx.resize(img, dw, dh), orc.resize(img, dw, dh))
