@@ -59,6 +59,21 @@ def algorithmic_work(phase, p, n):
     return 0, "B", "hbm"
 
 
+def measured_traffic(phase, workload):
+    """HBM bytes per launch of a phase's kernel from the committed counter
+    summary (profiles/traffic.json: FETCH_SIZE + WRITE_SIZE per dispatch,
+    each scaled by the factor a copy of known size reads), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        k = t["workloads"][workload][phase]
+        return int(round((k["fetch_kib"] * t["fetch_factor"] + k["write_kib"] * t["write_factor"])
+                         * 1024))
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(oracle, p, target, mats, budget_s):
     """The oracle (single-threaded C restatement of the reference's CPU path)
     running whole jobs of the same workload until ~budget_s has passed."""
@@ -81,6 +96,8 @@ def main():
     ap.add_argument("--workload", default="medium", choices=["small", "medium", "large"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="also run a device-to-device copy of BYTES (counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -131,6 +148,15 @@ def main():
     sim_phase = ctx.timing()["sim_matrix"]
     ctx.set_timing(False)
 
+    if args.calibrate_copy > 0:
+        src = ctx.device_alloc(args.calibrate_copy)
+        dst = ctx.device_alloc(args.calibrate_copy)
+        for _ in range(3):
+            ctx.copy_on_device(dst, src, args.calibrate_copy)
+        ctx.synchronize()
+        ctx.device_free(src)
+        ctx.device_free(dst)
+
     jobs_per_s = args.steps / elapsed
     ms_per_step = 1000.0 * elapsed / args.steps
     avg = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in phases.items()}
@@ -146,16 +172,17 @@ def main():
     dominant = max(kernels, key=lambda k: kernels[k]["avg_ms"])
     d_amount, d_unit, d_bound = algorithmic_work(dominant, p, n)
     d_ms = kernels[dominant]["avg_ms"]
+    traffic = measured_traffic(dominant, args.workload)
     if d_unit == "B":
         achieved = d_amount / (d_ms * 1e-3) / 1e9
         roof = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 3),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
-                "traffic": None, "avg_ms": d_ms}
+                "traffic": traffic, "algorithmic_bytes": d_amount, "avg_ms": d_ms}
     else:
         achieved = d_amount / (d_ms * 1e-3) / 1e12
         roof = {"kernel": dominant, "bound": "mfma", "achieved": round(achieved, 4),
                 "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP32_PEAK_TFLOPS, 6), "traffic": None, "avg_ms": d_ms}
+                "frac": round(achieved / FP32_PEAK_TFLOPS, 6), "traffic": traffic, "avg_ms": d_ms}
     sim_ms = sim_phase[0] / max(sim_phase[1], 1)
     sim_gbs = 4.0 * n * n / (sim_ms * 1e-3) / 1e9
 

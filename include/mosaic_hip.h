@@ -105,6 +105,9 @@ int mh_device_alloc(mh_context *ctx, size_t bytes, void **out);
 int mh_device_free(mh_context *ctx, void *ptr);
 int mh_copy_to_device(mh_context *ctx, void *dst, const void *src, size_t bytes);
 int mh_copy_to_host(mh_context *ctx, void *dst, const void *src, size_t bytes);
+/* Device-to-device copy by a plain streaming kernel, enqueued on the context
+ * stream (also the known-size copy used to calibrate HBM byte counters). */
+int mh_copy_on_device(mh_context *ctx, void *dst, const void *src, size_t bytes);
 
 /* --------------------------------------------------------------- library */
 /* SimilarityDatabase::new + add_tile x n (src/similarity.rs:51-66): the Lab

@@ -606,6 +606,38 @@ int mh_copy_to_device(mh_context* ctx, void* dst, const void* src, size_t bytes)
     return sync(ctx);
 }
 
+__global__ void device_copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                   int64_t n16) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+__global__ void device_copy_tail_kernel(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst,
+                                        int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+int mh_copy_on_device(mh_context* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return fail(MH_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (bytes == 0) return MH_OK;
+    CHK(set_device(ctx));
+    const bool aligned = (((uintptr_t)dst | (uintptr_t)src) & 15) == 0;
+    const int64_t n16 = aligned ? (int64_t)(bytes / 16) : 0;
+    if (n16 > 0) {
+        const int64_t blocks = (n16 + 255) / 256 < 8192 ? (n16 + 255) / 256 : 8192;
+        hipLaunchKernelGGL(device_copy_kernel, dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
+                           (const uint4*)src, (uint4*)dst, n16);
+    }
+    const int64_t done = n16 * 16, rest = (int64_t)bytes - done;
+    if (rest > 0)
+        hipLaunchKernelGGL(device_copy_tail_kernel, dim3((unsigned)((rest + 255) / 256)), dim3(256), 0,
+                           ctx->stream, (const uint8_t*)src + done, (uint8_t*)dst + done, rest);
+    HIPCHK(hipGetLastError());
+    return MH_OK;
+}
+
 int mh_copy_to_host(mh_context* ctx, void* dst, const void* src, size_t bytes) {
     if (!ctx || (bytes && (!dst || !src))) return fail(MH_ERR_INVALID_ARGUMENT, "NULL argument");
     CHK(set_device(ctx));
