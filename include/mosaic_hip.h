@@ -40,6 +40,9 @@
  *   mh_resize                 MosaicGenerator::resize_image (src/main.rs:710-738)
  *   mh_composite              tile compositing (src/main.rs:548-590, :625-664)
  *   mh_generate               MosaicGenerator::generate_mosaic (src/main.rs:477-685)
+ *   mh_batch_*, mh_server_*   no counterpart: the reference runs one job per
+ *                             process (src/main.rs:117-163); these run many
+ *                             generate_mosaic jobs per launch / as a request loop
  */
 #ifndef MOSAIC_HIP_H
 #define MOSAIC_HIP_H
@@ -259,6 +262,79 @@ int mh_generate_device(mh_context *ctx, const mh_library *lib, const uint8_t *d_
                        int32_t width, int32_t height, const mh_job_params *params,
                        int32_t *d_grid_out, uint8_t *d_image_out);
 int mh_generate_device_result(mh_context *ctx, mh_job_result *result);
+
+/* ------------------------------------------- batched and served jobs */
+/* One mosaic job for the batch and server entry points.  d_target is a
+ * device RGB8 image with target_pitch bytes per row (0 = width*3).  Outputs
+ * are device buffers (grid_w*grid_h int32; (grid_w*tw) x (grid_h*th) x 3
+ * bytes); when NULL the library composites into working memory of its own
+ * (readable through mh_batch_job_outputs for batches, discarded for served
+ * jobs).  The library must hold pixels and belong to the same context.
+ * Each job's result equals mh_generate_device's for the same job alone. */
+typedef struct {
+    const mh_library *lib;
+    const uint8_t *d_target;
+    int64_t target_pitch;
+    int32_t width, height;
+    mh_job_params params;
+    int32_t *d_grid_out;
+    uint8_t *d_image_out;
+} mh_job;
+
+/* Device working memory the job needs while it runs (bytes), or < 0 on a
+ * validation error. */
+int64_t mh_job_footprint(mh_context *ctx, const mh_job *job);
+
+/* A fixed set of jobs run as one fused launch per stage (cell Lab, nearest
+ * materials, placement chains, annealing chains, adjustment, composite) on
+ * the context stream.  Creation validates the jobs, builds the libraries'
+ * caches and allocates all working memory; runs allocate nothing.  With the
+ * graph enabled the first run captures the launch sequence into a hipGraph
+ * and later runs replay it (inputs are re-read from the same addresses). */
+typedef struct mh_batch mh_batch;
+int mh_batch_create(mh_context *ctx, const mh_job *jobs, int32_t n_jobs, mh_batch **out);
+int mh_batch_set_graph(mh_batch *batch, int32_t enabled);
+int mh_batch_run(mh_batch *batch);
+/* Synchronises the context stream; results has n_jobs entries. */
+int mh_batch_results(mh_batch *batch, mh_job_result *results);
+/* Device grid and image of job i (the caller's buffers or the batch's own). */
+int mh_batch_job_outputs(mh_batch *batch, int32_t i, int32_t **d_grid, uint8_t **d_image);
+void mh_batch_destroy(mh_batch *batch);
+
+/* Continuous-batching request loop.  A scheduler thread admits queued jobs
+ * (FIFO) while their working memory fits memory_budget, and launches the
+ * admitted jobs as one batch on a free stream slot (at most streams batches
+ * in flight, at most max_batch jobs each); jobs that arrive while batches
+ * run join the next one as slots and memory free up.  Libraries stay
+ * resident with their term matrices and resized tiles.  The context's other
+ * entry points must not run concurrently with a live server. */
+typedef struct {
+    int64_t memory_budget;  /* bytes of device working memory (0 = 64 GiB) */
+    int32_t max_batch;      /* jobs per launch (0 = 256)                    */
+    int32_t streams;        /* stream slots, 1..16 (0 = 4)                  */
+} mh_server_params;
+
+typedef struct {
+    int64_t submitted, completed, batches;
+    int32_t max_batch_jobs;
+    int64_t memory_in_use, memory_peak, memory_budget;
+} mh_server_stats;
+
+typedef struct mh_server mh_server;
+int mh_server_create(mh_context *ctx, const mh_server_params *params, mh_server **out);
+/* Queues a job (its buffers must stay valid until it completes); returns a
+ * ticket.  Fails at once if the job is invalid or larger than the budget. */
+int mh_server_submit(mh_server *server, const mh_job *job, int64_t *ticket);
+/* Blocks until the job is done; latency_ms = completion - submission. */
+int mh_server_wait(mh_server *server, int64_t ticket, mh_job_result *result, double *latency_ms);
+int mh_server_poll(mh_server *server, int64_t ticket, int32_t *done);
+/* Blocks until every submitted job is done. */
+int mh_server_drain(mh_server *server);
+/* Host steady-clock times (ms) of submission, launch and completion. */
+int mh_server_job_times(mh_server *server, int64_t ticket, double times_ms[3]);
+int mh_server_get_stats(mh_server *server, mh_server_stats *stats);
+/* Finishes queued and running jobs, then frees the server. */
+void mh_server_destroy(mh_server *server);
 
 #ifdef __cplusplus
 }

@@ -169,20 +169,16 @@ void launch_resize(const uint8_t* src, int32_t n, int32_t sw, int32_t sh, int32_
 // Integer average RGB of cells (calculate_average_rgb): one 256-thread
 // block per cell; sums are exact so the reduction order is free.  Cell
 // addressing as in the Lab kernel.
-__global__ __launch_bounds__(256) void avg_rgb_kernel(const uint8_t* __restrict__ rgb,
-                                                      int64_t row_stride, int32_t per_row,
-                                                      int64_t block_row_stride,
-                                                      int64_t block_col_stride, int32_t cw,
-                                                      int32_t ch, uint8_t* __restrict__ out) {
+__device__ __forceinline__ void avg_rgb_block(const CellSeg& s, int32_t cell) {
     __shared__ uint64_t part[3][4];
-    const int32_t cell = blockIdx.x;
-    const uint8_t* base = rgb + (int64_t)(cell / per_row) * block_row_stride +
-                          (int64_t)(cell % per_row) * block_col_stride;
-    const int32_t npx = cw * ch;
+    const int32_t cw = s.cw;
+    const uint8_t* base = s.rgb + (int64_t)(cell / s.per_row) * s.block_row_stride +
+                          (int64_t)(cell % s.per_row) * s.block_col_stride;
+    const int32_t npx = cw * s.ch;
     uint64_t s0 = 0, s1 = 0, s2 = 0;
     for (int32_t p = threadIdx.x; p < npx; p += 256) {
         const int32_t py = p / cw, px = p - py * cw;
-        const uint8_t* q = base + (int64_t)py * row_stride + (int64_t)px * 3;
+        const uint8_t* q = base + (int64_t)py * s.row_stride + (int64_t)px * 3;
         s0 += q[0];
         s1 += q[1];
         s2 += q[2];
@@ -203,26 +199,42 @@ __global__ __launch_bounds__(256) void avg_rgb_kernel(const uint8_t* __restrict_
     if (threadIdx.x < 3) {
         const uint64_t t = part[threadIdx.x][0] + part[threadIdx.x][1] + part[threadIdx.x][2] +
                            part[threadIdx.x][3];
-        out[3 * (int64_t)cell + threadIdx.x] = (uint8_t)(t / (uint64_t)npx);
+        s.out_rgb[3 * (int64_t)cell + threadIdx.x] = (uint8_t)(t / (uint64_t)npx);
     }
+}
+
+__global__ __launch_bounds__(256) void avg_rgb_kernel(CellSeg s) { avg_rgb_block(s, blockIdx.x); }
+
+__global__ __launch_bounds__(256) void avg_rgb_batch_kernel(const CellSeg* __restrict__ segs,
+                                                            const int32_t* __restrict__ starts,
+                                                            int32_t n_segs) {
+    const int32_t j = find_segment(starts, n_segs, blockIdx.x);
+    avg_rgb_block(segs[j], blockIdx.x - starts[j]);
 }
 
 void launch_avg_rgb(const uint8_t* rgb, int64_t row_stride, int32_t per_row,
                     int64_t block_row_stride, int64_t block_col_stride, int32_t cw, int32_t ch,
                     int32_t n_cells, uint8_t* out, hipStream_t stream) {
     if (n_cells <= 0 || cw <= 0 || ch <= 0) return;
-    hipLaunchKernelGGL(avg_rgb_kernel, dim3(n_cells), dim3(256), 0, stream, rgb, row_stride, per_row,
-                       block_row_stride, block_col_stride, cw, ch, out);
+    const CellSeg s{rgb, row_stride, block_row_stride, block_col_stride, per_row, cw, ch, n_cells,
+                    nullptr, out};
+    hipLaunchKernelGGL(avg_rgb_kernel, dim3(n_cells), dim3(256), 0, stream, s);
+}
+
+// One block per cell; segments with n_cells > 0 only.
+void launch_avg_rgb_batch(const CellSeg* d_segs, const int32_t* d_starts, int32_t n_segs,
+                          int32_t total_cells, hipStream_t stream) {
+    if (n_segs <= 0 || total_cells <= 0) return;
+    hipLaunchKernelGGL(avg_rgb_batch_kernel, dim3(total_cells), dim3(256), 0, stream, d_segs,
+                       d_starts, n_segs);
 }
 
 // Per-cell adjustment: calculate_optimal_adjustment(tile_avg[grid[c]],
 // target_avg[c], strength).  Empty cells get a NaN marker in .y.
-__global__ void cell_adjust_kernel(const int32_t* __restrict__ grid, int32_t cells,
-                                   const uint8_t* __restrict__ tile_avg,
-                                   const uint8_t* __restrict__ target_avg, float strength,
-                                   float4* __restrict__ out) {
-    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= cells) return;
+__device__ __forceinline__ void cell_adjust_one(const int32_t* __restrict__ grid, int32_t c,
+                                                const uint8_t* __restrict__ tile_avg,
+                                                const uint8_t* __restrict__ target_avg,
+                                                float strength, float4* __restrict__ out) {
     const int32_t m = grid[c];
     if (m == kEmpty) {
         out[c] = make_float4(0.0f, __int_as_float(0x7fc00000), 0.0f, 1.0f);
@@ -233,6 +245,15 @@ __global__ void cell_adjust_kernel(const int32_t* __restrict__ grid, int32_t cel
     out[c] = optimal_adjustment(t[0], t[1], t[2], g[0], g[1], g[2], strength);
 }
 
+__global__ void cell_adjust_kernel(const int32_t* __restrict__ grid, int32_t cells,
+                                   const uint8_t* __restrict__ tile_avg,
+                                   const uint8_t* __restrict__ target_avg, float strength,
+                                   float4* __restrict__ out) {
+    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cells) return;
+    cell_adjust_one(grid, c, tile_avg, target_avg, strength, out);
+}
+
 void launch_cell_adjust(const int32_t* grid, int32_t cells, const uint8_t* tile_avg,
                         const uint8_t* target_avg, float strength, float4* out,
                         hipStream_t stream) {
@@ -241,14 +262,12 @@ void launch_cell_adjust(const int32_t* grid, int32_t cells, const uint8_t* tile_
                        cells, tile_avg, target_avg, strength, out);
 }
 
-// One thread per output pixel (ow = gw*tw, oh = gh*th).
-__global__ void composite_kernel(const int32_t* __restrict__ grid, int32_t gw, int32_t tw,
-                                 int32_t th, int32_t ow, int32_t oh,
-                                 const uint8_t* __restrict__ tiles,
-                                 const float4* __restrict__ adj, int32_t adjust,
-                                 uint8_t* __restrict__ out) {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (int64_t)ow * oh) return;
+// Output pixel idx of a gw*tw x gh*th mosaic.
+__device__ __forceinline__ void composite_pixel(const int32_t* __restrict__ grid, int32_t gw,
+                                                int32_t tw, int32_t th, int32_t ow, int64_t idx,
+                                                const uint8_t* __restrict__ tiles,
+                                                const float4* __restrict__ adj, int32_t adjust,
+                                                uint8_t* __restrict__ out) {
     const int32_t X = (int32_t)(idx % ow), Y = (int32_t)(idx / ow);
     const int32_t cx = X / tw, cy = Y / th;
     const int32_t cell = cy * gw + cx;
@@ -268,6 +287,17 @@ __global__ void composite_kernel(const int32_t* __restrict__ grid, int32_t gw, i
     d[2] = (uint8_t)b;
 }
 
+// One thread per output pixel (ow = gw*tw, oh = gh*th).
+__global__ void composite_kernel(const int32_t* __restrict__ grid, int32_t gw, int32_t tw,
+                                 int32_t th, int32_t ow, int32_t oh,
+                                 const uint8_t* __restrict__ tiles,
+                                 const float4* __restrict__ adj, int32_t adjust,
+                                 uint8_t* __restrict__ out) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)ow * oh) return;
+    composite_pixel(grid, gw, tw, th, ow, idx, tiles, adj, adjust, out);
+}
+
 void launch_composite(const int32_t* grid, int32_t gw, int32_t gh, int32_t tw, int32_t th,
                       const uint8_t* tiles, const float4* adj, int32_t adjust, uint8_t* out,
                       hipStream_t stream) {
@@ -275,6 +305,51 @@ void launch_composite(const int32_t* grid, int32_t gw, int32_t gh, int32_t tw, i
     if (total <= 0) return;
     hipLaunchKernelGGL(composite_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream,
                        grid, gw, tw, th, gw * tw, gh * th, tiles, adj, adjust, out);
+}
+
+// Batched finish of many jobs: per-cell adjustments (a thread per cell), then
+// compositing (a thread per output pixel).  Block b of a launch belongs to
+// segment j with starts[j] <= b < starts[j+1]; the host sizes each
+// segment's block count from its cells (adjust) or output pixels (composite).
+__global__ __launch_bounds__(256) void cell_adjust_batch_kernel(const CompSeg* __restrict__ segs,
+                                                                const int32_t* __restrict__ starts,
+                                                                int32_t n_segs) {
+    const int32_t j = find_segment(starts, n_segs, blockIdx.x);
+    const CompSeg& s = segs[j];
+    const int32_t c = (blockIdx.x - starts[j]) * 256 + threadIdx.x;
+    if (c >= s.gw * s.gh) return;
+    cell_adjust_one(s.grid, c, s.tile_avg, s.target_avg, s.strength, s.adj);
+}
+
+__global__ __launch_bounds__(256) void composite_batch_kernel(const CompSeg* __restrict__ segs,
+                                                              const int32_t* __restrict__ starts,
+                                                              int32_t n_segs) {
+    const int32_t j = find_segment(starts, n_segs, blockIdx.x);
+    const CompSeg& s = segs[j];
+    const int32_t ow = s.gw * s.tw;
+    const int64_t idx = (int64_t)(blockIdx.x - starts[j]) * 256 + threadIdx.x;
+    if (idx >= (int64_t)ow * s.gh * s.th) return;
+    composite_pixel(s.grid, s.gw, s.tw, s.th, ow, idx, s.tiles, s.adj, s.adjust, s.out);
+}
+
+int32_t cell_adjust_blocks(int32_t cells) { return cells > 0 ? (cells + 255) / 256 : 0; }
+
+int32_t composite_blocks(int64_t pixels) {
+    return pixels > 0 ? (int32_t)((pixels + 255) / 256) : 0;
+}
+
+void launch_cell_adjust_batch(const CompSeg* d_segs, const int32_t* d_starts, int32_t n_segs,
+                              int32_t total_blocks, hipStream_t stream) {
+    if (n_segs <= 0 || total_blocks <= 0) return;
+    hipLaunchKernelGGL(cell_adjust_batch_kernel, dim3(total_blocks), dim3(256), 0, stream, d_segs,
+                       d_starts, n_segs);
+}
+
+void launch_composite_batch(const CompSeg* d_segs, const int32_t* d_starts, int32_t n_segs,
+                            int32_t total_blocks, hipStream_t stream) {
+    if (n_segs <= 0 || total_blocks <= 0) return;
+    hipLaunchKernelGGL(composite_batch_kernel, dim3(total_blocks), dim3(256), 0, stream, d_segs,
+                       d_starts, n_segs);
 }
 
 // ColorAdjustment::apply_to_image over a flat pixel array.

@@ -8,6 +8,9 @@
 // chunk of kChunk pixels per cell to Lab (the expensive part: LUT + 3x cbrt)
 // into LDS, then one lane per (cell, channel) adds the chunk in pixel order
 // to a register accumulator.  Chunks repeat until the cell is done.
+//
+// The batched kernel runs the same blocks for many images at once: block b
+// belongs to the segment j with starts[j] <= b < starts[j+1].
 #include "mh_device.hpp"
 #include "mh_kernels.hpp"
 
@@ -18,20 +21,21 @@ constexpr int kChunk = 256;
 constexpr int kPitch = kChunk + 1;  // odd pitch: the 48 summing lanes hit distinct banks
 constexpr int kLabThreads = 256;
 
+struct LabShared {
+    float lut[256];
+    float lab[3][kCellsPerBlock][kPitch];
+};
+
 // Cell c's pixel (px, py) lives at
 //   rgb + (c / per_row) * block_row_stride + (c % per_row) * block_col_stride
 //       + py * row_stride + px * 3.
-__global__ __launch_bounds__(kLabThreads) void cell_lab_kernel(
-    const uint8_t* __restrict__ rgb, int64_t row_stride, int32_t per_row, int64_t block_row_stride,
-    int64_t block_col_stride, int32_t cw, int32_t ch, int32_t n_cells,
-    const float* __restrict__ lut_g, float* __restrict__ out) {
-    __shared__ float lut[256];
-    __shared__ float lab_s[3][kCellsPerBlock][kPitch];
-
+__device__ __forceinline__ void cell_lab_block(const CellSeg& s, int32_t blk,
+                                               const float* __restrict__ lut_g, LabShared& sh) {
     const int tid = threadIdx.x;
-    lut[tid] = lut_g[tid];
+    sh.lut[tid] = lut_g[tid];
 
-    const int32_t cell0 = blockIdx.x * kCellsPerBlock;
+    const int32_t cell0 = blk * kCellsPerBlock;
+    const int32_t cw = s.cw, ch = s.ch, n_cells = s.n_cells, per_row = s.per_row;
     const int32_t npx = cw * ch;
     // summing lane -> (cell, channel)
     const int sum_cell = tid / 3, sum_ch = tid % 3;
@@ -47,36 +51,62 @@ __global__ __launch_bounds__(kLabThreads) void cell_lab_kernel(
             if (cell < n_cells && p < len) {
                 const int32_t pix = base + p;
                 const int32_t py = pix / cw, px = pix - py * cw;
-                const uint8_t* src = rgb + (int64_t)(cell / per_row) * block_row_stride +
-                                     (int64_t)(cell % per_row) * block_col_stride +
-                                     (int64_t)py * row_stride + (int64_t)px * 3;
+                const uint8_t* src = s.rgb + (int64_t)(cell / per_row) * s.block_row_stride +
+                                     (int64_t)(cell % per_row) * s.block_col_stride +
+                                     (int64_t)py * s.row_stride + (int64_t)px * 3;
                 float L, A, B;
-                rgb8_to_lab(lut, src[0], src[1], src[2], L, A, B);
-                lab_s[0][g][p] = L;
-                lab_s[1][g][p] = A;
-                lab_s[2][g][p] = B;
+                rgb8_to_lab(sh.lut, src[0], src[1], src[2], L, A, B);
+                sh.lab[0][g][p] = L;
+                sh.lab[1][g][p] = A;
+                sh.lab[2][g][p] = B;
             }
         }
         __syncthreads();
         if (summer) {
-            const float* v = lab_s[sum_ch][sum_cell];
+            const float* v = sh.lab[sum_ch][sum_cell];
             for (int32_t p = 0; p < len; ++p) acc = acc + v[p];
         }
         __syncthreads();
     }
     if (summer) {
         const float total = (float)((uint32_t)cw * (uint32_t)ch);
-        out[(int64_t)(cell0 + sum_cell) * 3 + sum_ch] = acc / total;
+        s.out[(int64_t)(cell0 + sum_cell) * 3 + sum_ch] = acc / total;
     }
+}
+
+__global__ __launch_bounds__(kLabThreads) void cell_lab_kernel(CellSeg s,
+                                                               const float* __restrict__ lut) {
+    __shared__ LabShared sh;
+    cell_lab_block(s, blockIdx.x, lut, sh);
+}
+
+__global__ __launch_bounds__(kLabThreads) void cell_lab_batch_kernel(
+    const CellSeg* __restrict__ segs, const int32_t* __restrict__ starts, int32_t n_segs,
+    const float* __restrict__ lut) {
+    __shared__ LabShared sh;
+    const int32_t j = find_segment(starts, n_segs, blockIdx.x);
+    cell_lab_block(segs[j], blockIdx.x - starts[j], lut, sh);
+}
+
+int32_t cell_lab_blocks(int32_t n_cells) {
+    return n_cells > 0 ? (n_cells + kCellsPerBlock - 1) / kCellsPerBlock : 0;
 }
 
 void launch_cell_lab(const uint8_t* rgb, int64_t row_stride, int32_t per_row,
                      int64_t block_row_stride, int64_t block_col_stride, int32_t cw, int32_t ch,
                      int32_t n_cells, const float* lut, float* out, hipStream_t stream) {
     if (n_cells <= 0) return;
-    const int blocks = (n_cells + kCellsPerBlock - 1) / kCellsPerBlock;
-    hipLaunchKernelGGL(cell_lab_kernel, dim3(blocks), dim3(kLabThreads), 0, stream, rgb, row_stride,
-                       per_row, block_row_stride, block_col_stride, cw, ch, n_cells, lut, out);
+    const CellSeg s{rgb, row_stride, block_row_stride, block_col_stride, per_row, cw, ch, n_cells,
+                    out, nullptr};
+    hipLaunchKernelGGL(cell_lab_kernel, dim3(cell_lab_blocks(n_cells)), dim3(kLabThreads), 0,
+                       stream, s, lut);
+}
+
+void launch_cell_lab_batch(const CellSeg* d_segs, const int32_t* d_starts, int32_t n_segs,
+                           int32_t total_blocks, const float* lut, hipStream_t stream) {
+    if (n_segs <= 0 || total_blocks <= 0) return;
+    hipLaunchKernelGGL(cell_lab_batch_kernel, dim3(total_blocks), dim3(kLabThreads), 0, stream,
+                       d_segs, d_starts, n_segs, lut);
 }
 
 }  // namespace mh

@@ -4,25 +4,17 @@
 // context-owned device scratch (grow-only), enqueue the kernels on the
 // context stream and synchronise.  mh_generate_device is the asynchronous,
 // device-resident form used by the benchmark and by batching callers.
-#include <math.h>
-#include <stdarg.h>
-#include <stdio.h>
-#include <string.h>
-
-#include <map>
-#include <mutex>
-#include <string>
-#include <tuple>
-#include <vector>
-
-#include "../../include/mosaic_hip.h"
-#include "mh_kernels.hpp"
+#include "mh_internal.hpp"
 
 using namespace mh;
+using namespace mh::host;
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
+
+namespace mh {
+namespace host {
 
 int fail(int code, const char* fmt, ...) {
     char buf[512];
@@ -33,81 +25,6 @@ int fail(int code, const char* fmt, ...) {
     g_err = buf;
     return code;
 }
-
-#define HIPCHK(call)                                                                         \
-    do {                                                                                     \
-        hipError_t e_ = (call);                                                              \
-        if (e_ != hipSuccess)                                                                \
-            return fail(e_ == hipErrorOutOfMemory ? MH_ERR_OUT_OF_MEMORY : MH_ERR_HIP,        \
-                        "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__,     \
-                        __LINE__);                                                           \
-    } while (0)
-
-// Device buffer that only grows.
-struct DevBuf {
-    void* p = nullptr;
-    size_t cap = 0;
-};
-
-enum Slot {
-    S_IN0, S_IN1, S_IN2, S_OUT0, S_OUT1, S_OUT2, S_OUT3,
-    S_CELL_LAB, S_CAND_IDX, S_CAND_D, S_GRID, S_USAGE, S_ORDER,
-    S_TILE_TMP, S_TILES, S_TILE_AVG, S_TARGET_AVG, S_ADJ, S_STATS, S_ANNEAL_OUT,
-    S_COUNT
-};
-
-struct TapsBuf {
-    DevBuf lo, cnt, k;
-    int32_t ksize = 0;
-};
-
-}  // namespace
-
-struct PhaseMark {
-    int32_t phase;
-    hipEvent_t a, b;
-};
-
-struct mh_context {
-    int32_t device = 0;
-    // per-phase timing (HIP events on `stream`)
-    bool timing = false;
-    std::vector<hipEvent_t> event_pool;
-    std::vector<PhaseMark> marks;
-    hipEvent_t open_event[MH_PHASES] = {};
-    double phase_ms[MH_PHASES] = {};
-    int64_t phase_n[MH_PHASES] = {};
-    hipStream_t stream = nullptr;
-    float* d_lut = nullptr;
-    size_t max_lds = 65536;
-    DevBuf slots[S_COUNT];
-    // temperature table cache
-    float t0 = NAN, decay = NAN;
-    int64_t t_len = -1;
-    DevBuf temps;
-    // Lanczos taps keyed by (in, out)
-    std::map<std::pair<int32_t, int32_t>, TapsBuf> taps;
-    // last device job geometry for mh_generate_device_result
-    int32_t last_out_w = 0, last_out_h = 0;
-    int32_t last_ran_anneal = 0;
-    int64_t last_iters = 0;
-};
-
-struct mh_library {
-    mh_context* ctx = nullptr;
-    int32_t n = 0;
-    std::vector<float> h_lab;  // n x 3
-    float4* d_lab4 = nullptr;
-    float* d_L = nullptr;
-    float* d_A = nullptr;
-    float* d_B = nullptr;
-    float* d_sim = nullptr;    // n x n after build_similarities
-    float* d_term = nullptr;   // n x n 1/(sim+1), built on first use by the cost kernels
-    uint8_t* d_pixels = nullptr;
-    int32_t pw = 0, ph = 0;
-};
-
-namespace {
 
 int set_device(mh_context* ctx) {
     HIPCHK(hipSetDevice(ctx->device));
@@ -153,13 +70,6 @@ int ensure(mh_context* ctx, DevBuf& b, size_t bytes) {
     return MH_OK;
 }
 
-template <typename T>
-int slot(mh_context* ctx, Slot s, size_t count, T** out) {
-    int rc = ensure(ctx, ctx->slots[s], sizeof(T) * count);
-    if (rc) return rc;
-    *out = reinterpret_cast<T*>(ctx->slots[s].p);
-    return MH_OK;
-}
 
 bool finite3(const float* v, int64_t count) {
     for (int64_t i = 0; i < count; ++i)
@@ -182,6 +92,7 @@ double lanczos3(double x) {
 }
 
 int get_taps(mh_context* ctx, int32_t in_size, int32_t out_size, ResizeTaps* t) {
+    std::lock_guard<std::mutex> g(ctx->taps_mu);
     auto key = std::make_pair(in_size, out_size);
     auto it = ctx->taps.find(key);
     if (it == ctx->taps.end()) {
@@ -233,26 +144,30 @@ int get_taps(mh_context* ctx, int32_t in_size, int32_t out_size, ResizeTaps* t) 
 // in f32 (src/optimizer.rs:54, :102).
 int get_temps(mh_context* ctx, float t0, float decay, int64_t len, const float** out) {
     if (len < 1) len = 1;
-    bool same = ctx->t_len >= len && memcmp(&ctx->t0, &t0, 4) == 0 &&
-                memcmp(&ctx->decay, &decay, 4) == 0;
-    if (!same) {
-        // kernels already queued may still read the old table
-        HIPCHK(hipStreamSynchronize(ctx->stream));
+    std::lock_guard<std::mutex> g(ctx->temps_mu);
+    uint32_t kt, kd;
+    memcpy(&kt, &t0, 4);
+    memcpy(&kd, &decay, 4);
+    TempTable& tt = ctx->temps[std::make_pair(kt, kd)];
+    if (tt.len < len) {
         std::vector<float> t((size_t)len);
         float v = t0;
         for (int64_t i = 0; i < len; ++i) {
             t[(size_t)i] = v;
             v = v * decay;
         }
-        int rc = ensure(ctx, ctx->temps, sizeof(float) * (size_t)len);
-        if (rc) return rc;
-        HIPCHK(hipMemcpy(ctx->temps.p, t.data(), sizeof(float) * (size_t)len,
-                         hipMemcpyHostToDevice));
-        ctx->t0 = t0;
-        ctx->decay = decay;
-        ctx->t_len = len;
+        float* p = nullptr;
+        HIPCHK(hipMalloc(&p, sizeof(float) * (size_t)len));
+        hipError_t e = hipMemcpy(p, t.data(), sizeof(float) * (size_t)len, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            hipFree(p);
+            return fail(MH_ERR_HIP, "temperature table: %s", hipGetErrorString(e));
+        }
+        if (tt.p) ctx->retired.push_back(tt.p);
+        tt.p = p;
+        tt.len = len;
     }
-    *out = (const float*)ctx->temps.p;
+    *out = tt.p;
     return MH_OK;
 }
 
@@ -314,21 +229,67 @@ int run_place(mh_context* ctx, const mh_library* lib, const float* d_cell_lab, i
     return MH_OK;
 }
 
-// The library's adjacency-term matrix (n x n f32), built once on the
-// context stream; nullptr when it would exceed kTermMatrixMaxN^2 entries
-// (the kernels then compute terms from the Lab table, with the same bits).
-constexpr int32_t kTermMatrixMaxN = 16384;
-
-int library_terms(mh_context* ctx, const mh_library* clib, const float** out) {
+int library_terms(const mh_library* clib, hipStream_t stream, const float** out) {
     mh_library* lib = const_cast<mh_library*>(clib);
     *out = nullptr;
     if (lib->n == 0 || lib->n > kTermMatrixMaxN) return MH_OK;
+    std::lock_guard<std::mutex> g(lib->cache_mu);
     if (!lib->d_term) {
-        HIPCHK(hipMalloc(&lib->d_term, sizeof(float) * (size_t)lib->n * lib->n));
-        launch_term_full(lib->d_lab4, lib->n, lib->d_term, ctx->stream);
-        HIPCHK(hipGetLastError());
+        float* t = nullptr;
+        HIPCHK(hipMalloc(&t, sizeof(float) * (size_t)lib->n * lib->n));
+        launch_term_full(lib->d_lab4, lib->n, t, stream);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) {
+            hipFree(t);
+            return fail(MH_ERR_HIP, "term matrix: %s", hipGetErrorString(e));
+        }
+        lib->d_term = t;
     }
     *out = lib->d_term;
+    return MH_OK;
+}
+
+// Resized tiles + their average RGB for cell size tw x th, cached per
+// library (the resize depends only on material and cell size).
+int library_tiles(const mh_library* clib, int32_t tw, int32_t th, hipStream_t stream,
+                  const uint8_t** tiles, const uint8_t** avg) {
+    mh_library* lib = const_cast<mh_library*>(clib);
+    mh_context* ctx = lib->ctx;
+    std::lock_guard<std::mutex> g(lib->cache_mu);
+    auto key = std::make_pair(tw, th);
+    auto it = lib->tiles.find(key);
+    if (it == lib->tiles.end()) {
+        ResizeTaps h, v;
+        CHK(get_taps(ctx, lib->pw, tw, &h));
+        CHK(get_taps(ctx, lib->ph, th, &v));
+        const size_t n = (size_t)(lib->n > 0 ? lib->n : 1);
+        TileSet ts;
+        uint8_t* tmp = nullptr;
+        hipError_t e = hipMalloc(&ts.tiles, n * th * tw * 3);
+        if (e == hipSuccess) e = hipMalloc(&ts.avg, n * 3);
+        if (e == hipSuccess) e = hipMalloc(&tmp, n * lib->ph * tw * 3);
+        if (e == hipSuccess && lib->n > 0) {
+            const bool timed = stream == ctx->stream;
+            if (timed) phase_begin(ctx, MH_PHASE_RESIZE);
+            launch_resize(lib->d_pixels, lib->n, lib->pw, lib->ph, tw, th, h, v, tmp, ts.tiles, stream);
+            if (timed) phase_end(ctx, MH_PHASE_RESIZE);
+            launch_avg_rgb(ts.tiles, (int64_t)tw * 3, 1, (int64_t)tw * th * 3, 0, tw, th, lib->n, ts.avg,
+                           stream);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (tmp) hipFree(tmp);
+        if (e != hipSuccess) {
+            if (ts.tiles) hipFree(ts.tiles);
+            if (ts.avg) hipFree(ts.avg);
+            return fail(e == hipErrorOutOfMemory ? MH_ERR_OUT_OF_MEMORY : MH_ERR_HIP,
+                        "resized tiles %d x %d: %s", tw, th, hipGetErrorString(e));
+        }
+        it = lib->tiles.emplace(key, ts).first;
+    }
+    *tiles = it->second.tiles;
+    *avg = it->second.avg;
     return MH_OK;
 }
 
@@ -349,7 +310,7 @@ int run_anneal(mh_context* ctx, const mh_library* lib, int32_t* d_grid, int32_t 
             return rc;
     }
     const float* term = nullptr;
-    if ((rc = library_terms(ctx, lib, &term))) return rc;
+    if ((rc = library_terms(lib, ctx->stream, &term))) return rc;
     AnnealJob job;
     job.lab4 = lib->d_lab4;
     job.term = term;
@@ -371,37 +332,16 @@ int run_anneal(mh_context* ctx, const mh_library* lib, int32_t* d_grid, int32_t 
     return MH_OK;
 }
 
-// Resized tiles + their average RGB for cell size tw x th (device).
-int run_tiles(mh_context* ctx, const mh_library* lib, int32_t tw, int32_t th, uint8_t** tiles,
-              uint8_t** tile_avg) {
-    ResizeTaps h, v;
-    int rc;
-    if ((rc = get_taps(ctx, lib->pw, tw, &h))) return rc;
-    if ((rc = get_taps(ctx, lib->ph, th, &v))) return rc;
-    uint8_t *tmp, *dst, *avg;
-    const size_t n = (size_t)lib->n;
-    if ((rc = slot(ctx, S_TILE_TMP, n * lib->ph * tw * 3, &tmp))) return rc;
-    if ((rc = slot(ctx, S_TILES, n * th * tw * 3, &dst))) return rc;
-    if ((rc = slot(ctx, S_TILE_AVG, n * 3, &avg))) return rc;
-    phase_begin(ctx, MH_PHASE_RESIZE);
-    launch_resize(lib->d_pixels, lib->n, lib->pw, lib->ph, tw, th, h, v, tmp, dst, ctx->stream);
-    phase_end(ctx, MH_PHASE_RESIZE);
-    phase_begin(ctx, MH_PHASE_ADJUST);
-    launch_avg_rgb(dst, (int64_t)tw * 3, 1, (int64_t)tw * th * 3, 0, tw, th, lib->n, avg,
-                   ctx->stream);
-    *tiles = dst;
-    *tile_avg = avg;
-    return MH_OK;
-}
 
 int run_composite(mh_context* ctx, const mh_library* lib, const uint8_t* d_target, int32_t width,
                   int32_t height, const int32_t* d_grid, int32_t gw, int32_t gh, float strength,
                   uint8_t* d_out) {
     const int32_t tw = width / gw, th = height / gh;
     if (tw <= 0 || th <= 0) return MH_OK;
-    uint8_t *tiles, *tile_avg;
+    const uint8_t *tiles, *tile_avg;
     int rc;
-    if ((rc = run_tiles(ctx, lib, tw, th, &tiles, &tile_avg))) return rc;
+    if ((rc = library_tiles(lib, tw, th, ctx->stream, &tiles, &tile_avg))) return rc;
+    phase_begin(ctx, MH_PHASE_ADJUST);
     const int32_t cells = gw * gh;
     float4* adj = nullptr;
     const int32_t adjust = strength > 0.0f ? 1 : 0;
@@ -441,11 +381,6 @@ int sync(mh_context* ctx) {
     return MH_OK;
 }
 
-#define CHK(expr)                 \
-    do {                          \
-        int rc_ = (expr);         \
-        if (rc_) return rc_;      \
-    } while (0)
 
 int check_grid_values(const int32_t* grid, int64_t cells, int32_t n) {
     for (int64_t i = 0; i < cells; ++i)
@@ -463,7 +398,19 @@ int check_geometry(int32_t gw, int32_t gh) {
     return MH_OK;
 }
 
-}  // namespace
+void to_result(const AnnealOut& o, mh_anneal_result* r) {
+    r->initial_cost = o.initial_cost;
+    r->final_cost = o.final_cost;
+    r->best_cost = o.best_cost;
+    r->improved_count = o.improved;
+    r->accepted_count = o.accepted;
+    r->iterations = o.iterations;
+    r->evaluated = o.evaluated;
+    r->rounds = o.rounds;
+}
+
+}  // namespace host
+}  // namespace mh
 
 // =================================================================== ABI
 extern "C" {
@@ -522,7 +469,9 @@ void mh_context_destroy(mh_context* ctx) {
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
     for (auto& b : ctx->slots)
         if (b.p) hipFree(b.p);
-    if (ctx->temps.p) hipFree(ctx->temps.p);
+    for (auto& kv : ctx->temps)
+        if (kv.second.p) hipFree(kv.second.p);
+    for (void* p : ctx->retired) hipFree(p);
     for (auto& kv : ctx->taps) {
         if (kv.second.lo.p) hipFree(kv.second.lo.p);
         if (kv.second.cnt.p) hipFree(kv.second.cnt.p);
@@ -752,6 +701,10 @@ void mh_library_destroy(mh_library* lib) {
     if (lib->d_sim) hipFree(lib->d_sim);
     if (lib->d_term) hipFree(lib->d_term);
     if (lib->d_pixels) hipFree(lib->d_pixels);
+    for (auto& kv : lib->tiles) {
+        if (kv.second.tiles) hipFree(kv.second.tiles);
+        if (kv.second.avg) hipFree(kv.second.avg);
+    }
     delete lib;
 }
 
@@ -953,7 +906,7 @@ int mh_total_cost(mh_context* ctx, const mh_library* lib, const int32_t* grid, i
     CHK(slot(ctx, S_OUT0, (size_t)chunk * 2 + 4, &d_scratch));
     CHK(slot(ctx, S_OUT1, 1, &d_out));
     const float* term = nullptr;
-    CHK(library_terms(ctx, lib, &term));
+    CHK(library_terms(lib, ctx->stream, &term));
     launch_total_cost(d_grid, gw, gh, lib->d_lab4, term, lib->n, d_scratch, chunk, d_out,
                       ctx->stream);
     HIPCHK(hipGetLastError());
@@ -1007,23 +960,13 @@ int mh_swap_deltas(mh_context* ctx, const mh_library* lib, const int32_t* grid, 
     CHK(upload(ctx, S_IN0, pairs, (size_t)p * 4, &d_pairs));
     CHK(slot(ctx, S_OUT0, (size_t)p, &d_out));
     const float* term = nullptr;
-    CHK(library_terms(ctx, lib, &term));
+    CHK(library_terms(lib, ctx->stream, &term));
     launch_swap_deltas(d_grid, gw, gh, lib->d_lab4, term, lib->n, d_pairs, p, d_out, ctx->stream);
     HIPCHK(hipGetLastError());
     CHK(download(ctx, out, d_out, (size_t)p));
     return sync(ctx);
 }
 
-static void to_result(const AnnealOut& o, mh_anneal_result* r) {
-    r->initial_cost = o.initial_cost;
-    r->final_cost = o.final_cost;
-    r->best_cost = o.best_cost;
-    r->improved_count = o.improved;
-    r->accepted_count = o.accepted;
-    r->iterations = o.iterations;
-    r->evaluated = o.evaluated;
-    r->rounds = o.rounds;
-}
 
 int mh_anneal(mh_context* ctx, const mh_library* lib, int32_t* grid, int32_t gw, int32_t gh,
               const mh_anneal_params* p, mh_anneal_result* result, float* trace_delta,

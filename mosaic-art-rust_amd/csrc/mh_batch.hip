@@ -1,0 +1,843 @@
+// mh_batch.hip -- many mosaic jobs per fused launch (mh_batch_*) and the
+// continuous-batching request loop (mh_server_*).
+//
+// A batch runs every job's whole pipeline with one launch per stage:
+//   cell Lab means -> K nearest -> placement chains (a wave per job) ->
+//   annealing chains (a wave per job) -> cell averages + adjustments ->
+//   composite.
+// Wide stages find their job through a segment table (block -> job by
+// binary search over per-job block offsets); chain stages take a job array.
+// Every job's result is what mh_generate_device gives for it alone: the
+// kernels are the same and no state crosses jobs.
+//
+// The server keeps libraries resident (Lab table, term matrix, resized
+// tiles), admits queued jobs while their device footprint fits a memory
+// budget, and launches what has been admitted as one batch on a free stream
+// slot; batches on different slots overlap on the device.  A scheduler
+// thread does admission, launches and completion (event queries), so jobs
+// that arrive while others run join the next batch as slots and memory free.
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <thread>
+
+#include "mh_internal.hpp"
+
+using namespace mh;
+using namespace mh::host;
+
+namespace {
+
+constexpr size_t kAlign = 256;
+size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
+
+double now_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+// A job after validation, with its library caches resolved and the layout of
+// its working memory (offsets into one block of `bytes`).
+struct JobSpec {
+    mh_job job;
+    int32_t gw = 0, gh = 0, cells = 0, tw = 0, th = 0, n = 0;
+    int64_t pitch = 0;
+    bool sa = false, adjust = false;
+    float strength = 0.0f;
+    const float* term = nullptr;
+    const float* temps = nullptr;
+    const uint8_t* tiles = nullptr;
+    const uint8_t* tile_avg = nullptr;
+    size_t o_cell_lab = 0, o_cand_idx = 0, o_cand_lab = 0, o_usage = 0, o_tavg = 0, o_adj = 0;
+    size_t o_grid = 0, o_image = 0;  // SIZE_MAX: the caller's buffer
+    size_t bytes = 0;
+};
+
+// Host-only checks (no GPU work) and the working-memory layout.
+int check_job(const mh_context* ctx, const mh_job& j, JobSpec* s) {
+    if (!j.lib || !j.d_target) return fail(MH_ERR_INVALID_ARGUMENT, "job: NULL library or target");
+    const mh_job_params& p = j.params;
+    CHK(check_geometry(p.grid_w, p.grid_h));
+    if (p.max_usage < 0) return fail(MH_ERR_INVALID_ARGUMENT, "max_usage must be >= 0");
+    if (p.optimization_iterations < 0)
+        return fail(MH_ERR_INVALID_ARGUMENT, "optimization_iterations < 0");
+    if (!isfinite(p.weight)) return fail(MH_ERR_INVALID_ARGUMENT, "weight must be finite");
+    if (j.width <= 0 || j.height <= 0) return fail(MH_ERR_INVALID_ARGUMENT, "bad image size");
+    const mh_library* lib = j.lib;
+    if (lib->ctx != ctx) return fail(MH_ERR_INVALID_ARGUMENT, "library belongs to another context");
+    s->job = j;
+    s->gw = p.grid_w;
+    s->gh = p.grid_h;
+    s->cells = p.grid_w * p.grid_h;
+    s->tw = j.width / p.grid_w;
+    s->th = j.height / p.grid_h;
+    s->n = lib->n;
+    if (s->tw <= 0 || s->th <= 0)
+        return fail(MH_ERR_INVALID_ARGUMENT, "image %d x %d smaller than grid %d x %d", j.width,
+                    j.height, p.grid_w, p.grid_h);
+    s->pitch = j.target_pitch ? j.target_pitch : (int64_t)j.width * 3;
+    if (s->pitch < (int64_t)j.width * 3)
+        return fail(MH_ERR_INVALID_ARGUMENT, "target pitch %lld < width*3", (long long)s->pitch);
+    if (!lib->d_pixels) return fail(MH_ERR_INVALID_ARGUMENT, "compositing needs a library with pixels");
+    s->sa = p.enable_optimization && p.weight > 0.0f;
+    float k = p.color_strength;
+    s->strength = k < 0.0f ? 0.0f : (k > 1.0f ? 1.0f : k);  // src/main.rs:162
+    s->adjust = s->strength > 0.0f;
+    const size_t place_lds = place_raster_lds_bytes(s->gw, s->n);
+    if (place_lds > ctx->max_lds)
+        return fail(MH_ERR_UNSUPPORTED, "placement state (%d columns + %d materials) needs %zu B of LDS",
+                    s->gw, s->n, place_lds);
+    if (s->sa && anneal_lds_bytes(s->cells, s->n < 32768) > ctx->max_lds)
+        return fail(MH_ERR_UNSUPPORTED, "annealing state (%d cells) exceeds LDS", s->cells);
+    // working memory
+    const size_t cells = (size_t)s->cells;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = o;
+        o = align_up(o + bytes);
+        return at;
+    };
+    s->o_cell_lab = take(cells * 3 * sizeof(float));
+    s->o_cand_idx = take(cells * kCandStride * sizeof(int32_t));
+    s->o_cand_lab = take(cells * kCandStride * sizeof(float4));
+    s->o_usage = take((size_t)(s->n > 0 ? s->n : 1) * sizeof(uint32_t));
+    s->o_tavg = s->adjust ? take(cells * 3) : 0;
+    s->o_adj = s->adjust ? take(cells * sizeof(float4)) : 0;
+    s->o_grid = j.d_grid_out ? SIZE_MAX : take(cells * sizeof(int32_t));
+    s->o_image = j.d_image_out ? SIZE_MAX
+                               : take((size_t)s->gw * s->tw * s->gh * s->th * 3);
+    s->bytes = o;
+    return MH_OK;
+}
+
+// Library-resident inputs of a job: term matrix, temperature table, resized
+// tiles (each built once, on `stream`).
+int prepare_job(mh_context* ctx, JobSpec* s, hipStream_t stream) {
+    const mh_job_params& p = s->job.params;
+    if (s->sa) {
+        CHK(library_terms(s->job.lib, stream, &s->term));
+        CHK(get_temps(ctx, p.initial_temperature, p.temperature_decay, p.optimization_iterations,
+                      &s->temps));
+    }
+    CHK(library_tiles(s->job.lib, s->tw, s->th, stream, &s->tiles, &s->tile_avg));
+    return MH_OK;
+}
+
+// One batch's descriptor tables and per-job results, laid out in one block.
+struct BatchPlan {
+    int32_t n_jobs = 0;
+    size_t o_lab = 0, o_lab_st = 0, o_topk = 0, o_topk_st = 0, o_place = 0, o_anneal = 0;
+    size_t o_avg = 0, o_avg_st = 0, o_comp = 0, o_adj_st = 0, o_comp_st = 0;
+    size_t o_stats = 0, o_aout = 0, bytes = 0;
+    int32_t lab_blocks = 0;
+    int32_t n_topk = 0, topk_blocks = 0, max_n = 0;
+    size_t place_lds = 0;
+    int32_t n_anneal = 0;
+    size_t anneal_lds = 0;
+    bool narrow = true;
+    int32_t n_avg = 0, avg_blocks = 0, adj_blocks = 0;
+    int32_t n_comp = 0, comp_blocks = 0;
+    std::vector<int32_t> anneal_job;  // job index of each anneal entry
+};
+
+constexpr int32_t kStatsInts = 8;
+
+size_t plan_bytes(int32_t n, BatchPlan* p) {
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = o;
+        o = align_up(o + bytes);
+        return at;
+    };
+    const size_t nn = (size_t)(n > 0 ? n : 1);
+    p->o_lab = take(nn * sizeof(CellSeg));
+    p->o_lab_st = take((nn + 1) * sizeof(int32_t));
+    p->o_topk = take(nn * sizeof(TopkSeg));
+    p->o_topk_st = take((nn + 1) * sizeof(int32_t));
+    p->o_place = take(nn * sizeof(PlaceJob));
+    p->o_anneal = take(nn * sizeof(AnnealJob));
+    p->o_avg = take(nn * sizeof(CellSeg));
+    p->o_avg_st = take((nn + 1) * sizeof(int32_t));
+    p->o_comp = take(nn * sizeof(CompSeg));
+    p->o_adj_st = take((nn + 1) * sizeof(int32_t));
+    p->o_comp_st = take((nn + 1) * sizeof(int32_t));
+    p->o_stats = take(nn * kStatsInts * sizeof(int32_t));
+    p->o_aout = take(nn * sizeof(AnnealOut));
+    p->bytes = o;
+    return o;
+}
+
+template <typename T>
+T* at(uint8_t* base, size_t off) {
+    return reinterpret_cast<T*>(base + off);
+}
+
+// Fill the host image `h` of a batch block whose device copy lives at `d`;
+// job i's working memory is at job_mem[i].
+int build_plan(const std::vector<JobSpec>& specs, const std::vector<uint8_t*>& job_mem, uint8_t* h,
+               uint8_t* d, BatchPlan* p) {
+    const int32_t n = (int32_t)specs.size();
+    p->n_jobs = n;
+    plan_bytes(n, p);
+    memset(h, 0, p->bytes);
+    CellSeg* lab = at<CellSeg>(h, p->o_lab);
+    int32_t* lab_st = at<int32_t>(h, p->o_lab_st);
+    TopkSeg* topk = at<TopkSeg>(h, p->o_topk);
+    int32_t* topk_st = at<int32_t>(h, p->o_topk_st);
+    PlaceJob* place = at<PlaceJob>(h, p->o_place);
+    AnnealJob* anneal = at<AnnealJob>(h, p->o_anneal);
+    CellSeg* avg = at<CellSeg>(h, p->o_avg);
+    int32_t* avg_st = at<int32_t>(h, p->o_avg_st);
+    CompSeg* comp = at<CompSeg>(h, p->o_comp);
+    int32_t* adj_st = at<int32_t>(h, p->o_adj_st);
+    int32_t* comp_st = at<int32_t>(h, p->o_comp_st);
+    int32_t* d_stats = at<int32_t>(d, p->o_stats);
+    AnnealOut* d_aout = at<AnnealOut>(d, p->o_aout);
+
+    int64_t lab_blocks = 0, topk_blocks = 0, avg_blocks = 0, adj_blocks = 0, comp_blocks = 0;
+    p->n_topk = p->n_anneal = p->n_avg = p->n_comp = 0;
+    p->max_n = 0;
+    p->place_lds = 0;
+    p->anneal_lds = 0;
+    p->narrow = true;
+    p->anneal_job.clear();
+    for (int32_t i = 0; i < n; ++i) {
+        const JobSpec& s = specs[i];
+        const mh_job& j = s.job;
+        const mh_library* lib = j.lib;
+        uint8_t* m = job_mem[i];
+        float* cell_lab = at<float>(m, s.o_cell_lab);
+        int32_t* cand_idx = at<int32_t>(m, s.o_cand_idx);
+        float4* cand_lab = at<float4>(m, s.o_cand_lab);
+        int32_t* grid = s.o_grid == SIZE_MAX ? j.d_grid_out : at<int32_t>(m, s.o_grid);
+        uint8_t* image = s.o_image == SIZE_MAX ? j.d_image_out : at<uint8_t>(m, s.o_image);
+        // a1: cell Lab means of the target
+        lab_st[i] = (int32_t)lab_blocks;
+        lab[i] = CellSeg{j.d_target, s.pitch, (int64_t)s.th * s.pitch, (int64_t)s.tw * 3, s.gw, s.tw,
+                         s.th, s.cells, cell_lab, nullptr};
+        lab_blocks += cell_lab_blocks(s.cells);
+        // a3: K nearest (libraries with materials only)
+        if (s.n > 0) {
+            topk_st[p->n_topk] = (int32_t)topk_blocks;
+            topk[p->n_topk] = TopkSeg{lib->d_L, lib->d_A, lib->d_B, cell_lab, cand_idx, nullptr,
+                                      cand_lab, s.n, s.cells, s.n < kMaxCand ? s.n : kMaxCand,
+                                      kCandStride};
+            ++p->n_topk;
+            topk_blocks += s.cells;
+            if (s.n > p->max_n) p->max_n = s.n;
+        }
+        // a4-a9: placement chain
+        PlaceJob pj{};
+        pj.lab4 = lib->d_lab4;
+        pj.n = s.n;
+        pj.cand_idx = cand_idx;
+        pj.cand_lab = cand_lab;
+        pj.order = nullptr;
+        pj.n_order = s.cells;
+        pj.grid = grid;
+        pj.usage = at<uint32_t>(m, s.o_usage);
+        pj.zero_usage = 1;
+        pj.grid_w = s.gw;
+        pj.grid_h = s.gh;
+        pj.max_usage = j.params.max_usage;
+        pj.weight = j.params.weight;
+        pj.stats = d_stats + (size_t)i * kStatsInts;
+        place[i] = pj;
+        const size_t pl = place_raster_lds_bytes(s.gw, s.n);
+        if (pl > p->place_lds) p->place_lds = pl;
+        // a10-a12: annealing chain
+        if (s.sa) {
+            AnnealJob aj{};
+            aj.lab4 = lib->d_lab4;
+            aj.term = s.term;
+            aj.n = s.n;
+            aj.grid = grid;
+            aj.grid_w = s.gw;
+            aj.grid_h = s.gh;
+            aj.max_iter = j.params.optimization_iterations;
+            aj.temps = s.temps;
+            aj.seed = j.params.seed;
+            aj.greedy = 0;
+            aj.result = d_aout + i;
+            anneal[p->n_anneal] = aj;
+            p->anneal_job.push_back(i);
+            ++p->n_anneal;
+            if (s.n >= 32768) p->narrow = false;
+        }
+        // a13-a16: cell averages, adjustments, composite
+        uint8_t* tavg = s.adjust ? at<uint8_t>(m, s.o_tavg) : nullptr;
+        float4* adj = s.adjust ? at<float4>(m, s.o_adj) : nullptr;
+        if (s.adjust) {
+            avg_st[p->n_avg] = (int32_t)avg_blocks;
+            avg[p->n_avg] = CellSeg{j.d_target, s.pitch, (int64_t)s.th * s.pitch, (int64_t)s.tw * 3,
+                                    s.gw, s.tw, s.th, s.cells, nullptr, tavg};
+            ++p->n_avg;
+            avg_blocks += s.cells;
+        }
+        adj_st[p->n_comp] = (int32_t)adj_blocks;
+        comp_st[p->n_comp] = (int32_t)comp_blocks;
+        comp[p->n_comp] = CompSeg{grid, s.gw, s.gh, s.tw, s.th, s.tiles, s.tile_avg, tavg, s.strength,
+                                  s.adjust ? 1 : 0, adj, image};
+        ++p->n_comp;
+        adj_blocks += s.adjust ? cell_adjust_blocks(s.cells) : 0;
+        comp_blocks += composite_blocks((int64_t)s.gw * s.tw * s.gh * s.th);
+    }
+    for (int32_t a = 0; a < p->n_anneal; ++a) {
+        const JobSpec& s = specs[p->anneal_job[a]];
+        const size_t al = anneal_lds_bytes(s.cells, p->narrow);
+        if (al > p->anneal_lds) p->anneal_lds = al;
+    }
+    if (lab_blocks > INT32_MAX || topk_blocks > INT32_MAX || avg_blocks > INT32_MAX ||
+        comp_blocks > INT32_MAX)
+        return fail(MH_ERR_UNSUPPORTED, "batch too large for one launch");
+    lab_st[n] = (int32_t)lab_blocks;
+    topk_st[p->n_topk] = (int32_t)topk_blocks;
+    avg_st[p->n_avg] = (int32_t)avg_blocks;
+    adj_st[p->n_comp] = (int32_t)adj_blocks;
+    comp_st[p->n_comp] = (int32_t)comp_blocks;
+    p->lab_blocks = (int32_t)lab_blocks;
+    p->topk_blocks = (int32_t)topk_blocks;
+    p->avg_blocks = (int32_t)avg_blocks;
+    p->adj_blocks = (int32_t)adj_blocks;
+    p->comp_blocks = (int32_t)comp_blocks;
+    return MH_OK;
+}
+
+void enqueue_plan(const BatchPlan& p, uint8_t* d, const float* lut, hipStream_t s) {
+    launch_cell_lab_batch(at<CellSeg>(d, p.o_lab), at<int32_t>(d, p.o_lab_st), p.n_jobs, p.lab_blocks,
+                          lut, s);
+    launch_topk_batch(at<TopkSeg>(d, p.o_topk), at<int32_t>(d, p.o_topk_st), p.n_topk, p.topk_blocks,
+                      p.max_n, s);
+    launch_place_batch(at<PlaceJob>(d, p.o_place), p.n_jobs, true, p.place_lds, s);
+    if (p.n_anneal > 0)
+        launch_anneal_batch(at<AnnealJob>(d, p.o_anneal), p.n_anneal, p.narrow, p.anneal_lds, s);
+    if (p.n_avg > 0) {
+        launch_avg_rgb_batch(at<CellSeg>(d, p.o_avg), at<int32_t>(d, p.o_avg_st), p.n_avg,
+                             p.avg_blocks, s);
+        launch_cell_adjust_batch(at<CompSeg>(d, p.o_comp), at<int32_t>(d, p.o_adj_st), p.n_comp,
+                                 p.adj_blocks, s);
+    }
+    launch_composite_batch(at<CompSeg>(d, p.o_comp), at<int32_t>(d, p.o_comp_st), p.n_comp,
+                           p.comp_blocks, s);
+}
+
+void fill_result(const JobSpec& s, const int32_t* st, const AnnealOut* ao, mh_job_result* r) {
+    memset(r, 0, sizeof(*r));
+    r->place.placed = st[0];
+    r->place.main_path = st[1];
+    r->place.fallback_ban = st[2];
+    r->place.fallback_near = st[3];
+    r->place.resets = st[4];
+    r->place.empty = st[5];
+    if (s.sa) to_result(*ao, &r->anneal);
+    r->out_width = s.gw * s.tw;
+    r->out_height = s.gh * s.th;
+}
+
+}  // namespace
+
+// ================================================================= batch
+struct mh_batch {
+    mh_context* ctx = nullptr;
+    std::vector<JobSpec> specs;
+    std::vector<uint8_t*> job_mem;
+    uint8_t* d_jobs = nullptr;   // all jobs' working memory
+    uint8_t* d_plan = nullptr;   // descriptor tables + results
+    std::vector<uint8_t> h_plan;
+    BatchPlan plan;
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+int64_t mh_job_footprint(mh_context* ctx, const mh_job* job) {
+    if (!ctx || !job) return fail(MH_ERR_INVALID_ARGUMENT, "NULL argument");
+    JobSpec s;
+    CHK(check_job(ctx, *job, &s));
+    return (int64_t)s.bytes;
+}
+
+void mh_batch_destroy(mh_batch* b) {
+    if (!b) return;
+    hipSetDevice(b->ctx->device);
+    hipStreamSynchronize(b->ctx->stream);
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    if (b->d_jobs) hipFree(b->d_jobs);
+    if (b->d_plan) hipFree(b->d_plan);
+    delete b;
+}
+
+int mh_batch_create(mh_context* ctx, const mh_job* jobs, int32_t n_jobs, mh_batch** out) {
+    if (!ctx || !out || n_jobs <= 0 || !jobs)
+        return fail(MH_ERR_INVALID_ARGUMENT, "mh_batch_create: bad arguments");
+    *out = nullptr;
+    CHK(set_device(ctx));
+    mh_batch* b = new mh_batch();
+    b->ctx = ctx;
+    b->specs.resize((size_t)n_jobs);
+    size_t total = 0;
+    std::vector<size_t> offs((size_t)n_jobs);
+    int rc = MH_OK;
+    for (int32_t i = 0; i < n_jobs && !rc; ++i) {
+        rc = check_job(ctx, jobs[i], &b->specs[i]);
+        if (rc) {
+            std::string m = mh_last_error();
+            rc = fail(rc, "job %d: %s", i, m.c_str());
+            break;
+        }
+        rc = prepare_job(ctx, &b->specs[i], ctx->stream);
+        offs[i] = total;
+        total = align_up(total + b->specs[i].bytes);
+    }
+    hipError_t e = hipSuccess;
+    if (!rc) {
+        e = hipMalloc(&b->d_jobs, total ? total : kAlign);
+        if (e == hipSuccess) e = hipMalloc(&b->d_plan, plan_bytes(n_jobs, &b->plan));
+        if (e != hipSuccess)
+            rc = fail(e == hipErrorOutOfMemory ? MH_ERR_OUT_OF_MEMORY : MH_ERR_HIP, "batch memory: %s",
+                      hipGetErrorString(e));
+    }
+    if (!rc) {
+        b->job_mem.resize((size_t)n_jobs);
+        for (int32_t i = 0; i < n_jobs; ++i) b->job_mem[i] = b->d_jobs + offs[i];
+        b->h_plan.assign(b->plan.bytes, 0);
+        rc = build_plan(b->specs, b->job_mem, b->h_plan.data(), b->d_plan, &b->plan);
+    }
+    if (!rc) {
+        e = hipMemcpy(b->d_plan, b->h_plan.data(), b->plan.bytes, hipMemcpyHostToDevice);
+        if (e != hipSuccess) rc = fail(MH_ERR_HIP, "batch tables: %s", hipGetErrorString(e));
+    }
+    if (rc) {
+        mh_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return MH_OK;
+}
+
+int mh_batch_set_graph(mh_batch* b, int32_t enabled) {
+    if (!b) return fail(MH_ERR_INVALID_ARGUMENT, "batch is NULL");
+    b->use_graph = enabled != 0;
+    return MH_OK;
+}
+
+int mh_batch_run(mh_batch* b) {
+    if (!b) return fail(MH_ERR_INVALID_ARGUMENT, "batch is NULL");
+    mh_context* ctx = b->ctx;
+    CHK(set_device(ctx));
+    if (!b->use_graph) {
+        enqueue_plan(b->plan, b->d_plan, ctx->d_lut, ctx->stream);
+        HIPCHK(hipGetLastError());
+        return MH_OK;
+    }
+    if (!b->exec) {
+        HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_plan(b->plan, b->d_plan, ctx->d_lut, ctx->stream);
+        hipError_t le = hipGetLastError();
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        if (le != hipSuccess || e != hipSuccess) {
+            if (g) hipGraphDestroy(g);
+            return fail(MH_ERR_HIP, "batch capture: %s",
+                        hipGetErrorString(le != hipSuccess ? le : e));
+        }
+        b->graph = g;
+        HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    HIPCHK(hipGraphLaunch(b->exec, ctx->stream));
+    return MH_OK;
+}
+
+int mh_batch_results(mh_batch* b, mh_job_result* results) {
+    if (!b || !results) return fail(MH_ERR_INVALID_ARGUMENT, "NULL argument");
+    mh_context* ctx = b->ctx;
+    CHK(set_device(ctx));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const int32_t n = b->plan.n_jobs;
+    std::vector<int32_t> st((size_t)n * kStatsInts);
+    std::vector<AnnealOut> ao((size_t)n);
+    HIPCHK(hipMemcpy(st.data(), b->d_plan + b->plan.o_stats, st.size() * sizeof(int32_t),
+                     hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ao.data(), b->d_plan + b->plan.o_aout, ao.size() * sizeof(AnnealOut),
+                     hipMemcpyDeviceToHost));
+    for (int32_t i = 0; i < n; ++i)
+        fill_result(b->specs[i], &st[(size_t)i * kStatsInts], &ao[i], &results[i]);
+    return MH_OK;
+}
+
+int mh_batch_job_outputs(mh_batch* b, int32_t i, int32_t** d_grid, uint8_t** d_image) {
+    if (!b || i < 0 || i >= b->plan.n_jobs) return fail(MH_ERR_INVALID_ARGUMENT, "bad job index");
+    const JobSpec& s = b->specs[i];
+    if (d_grid)
+        *d_grid = s.o_grid == SIZE_MAX ? s.job.d_grid_out : at<int32_t>(b->job_mem[i], s.o_grid);
+    if (d_image)
+        *d_image = s.o_image == SIZE_MAX ? s.job.d_image_out : at<uint8_t>(b->job_mem[i], s.o_image);
+    return MH_OK;
+}
+
+}  // extern "C"
+
+// ================================================================ server
+namespace {
+
+// First-fit sub-allocator over one device allocation (offsets only).
+struct Arena {
+    size_t size = 0, used = 0, peak = 0;
+    std::map<size_t, size_t> free_;  // offset -> bytes
+
+    void reset(size_t bytes) {
+        size = bytes;
+        used = peak = 0;
+        free_.clear();
+        free_[0] = bytes;
+    }
+    bool alloc(size_t bytes, size_t* off) {
+        bytes = align_up(bytes ? bytes : 1);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < bytes) continue;
+            *off = it->first;
+            const size_t rest = it->second - bytes;
+            free_.erase(it);
+            if (rest) free_[*off + bytes] = rest;
+            used += bytes;
+            if (used > peak) peak = used;
+            return true;
+        }
+        return false;
+    }
+    void release(size_t off, size_t bytes) {
+        bytes = align_up(bytes ? bytes : 1);
+        used -= bytes;
+        auto it = free_.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
+
+enum JobState { J_QUEUED = 0, J_RUNNING = 1, J_DONE = 2, J_FAILED = 3 };
+
+struct JobRec {
+    JobSpec spec;
+    int state = J_QUEUED;
+    int err = MH_OK;
+    std::string msg;
+    mh_job_result result{};
+    double t_submit = 0, t_launch = 0, t_done = 0;
+    size_t off = 0;
+    bool holds_mem = false;
+};
+
+struct StreamSlot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    uint8_t* d_plan = nullptr;    // descriptor tables + results (device)
+    uint8_t* h_plan = nullptr;    // pinned staging of the tables
+    uint8_t* h_res = nullptr;     // pinned copy of stats + anneal results
+    BatchPlan plan;
+    std::vector<int64_t> tickets;
+    bool busy = false;
+};
+
+}  // namespace
+
+struct mh_server {
+    mh_context* ctx = nullptr;
+    int32_t max_batch = 256;
+    uint8_t* d_arena = nullptr;
+    Arena arena;
+    std::vector<StreamSlot> slots;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<int64_t> queue;
+    std::deque<JobRec> jobs;  // by ticket
+    std::thread worker;
+    std::atomic<bool> stop{false};
+    int64_t batches = 0, batched_jobs = 0, completed = 0;
+    int32_t max_seen_batch = 0;
+    int fatal = MH_OK;
+    std::string fatal_msg;
+};
+
+namespace {
+
+// Launch the admitted jobs `tickets` as one batch on slot `sl`.  Called by
+// the worker without the lock held (the records it reads are not touched by
+// other threads while RUNNING).
+int launch_batch(mh_server* sv, StreamSlot& sl, const std::vector<JobRec*>& recs) {
+    mh_context* ctx = sv->ctx;
+    std::vector<JobSpec> specs;
+    std::vector<uint8_t*> mem;
+    specs.reserve(recs.size());
+    mem.reserve(recs.size());
+    for (JobRec* r : recs) {
+        CHK(prepare_job(ctx, &r->spec, sl.stream));
+        specs.push_back(r->spec);
+        mem.push_back(sv->d_arena + r->off);
+    }
+    CHK(build_plan(specs, mem, sl.h_plan, sl.d_plan, &sl.plan));
+    HIPCHK(hipMemcpyAsync(sl.d_plan, sl.h_plan, sl.plan.o_stats, hipMemcpyHostToDevice, sl.stream));
+    enqueue_plan(sl.plan, sl.d_plan, ctx->d_lut, sl.stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(sl.h_res, sl.d_plan + sl.plan.o_stats, sl.plan.bytes - sl.plan.o_stats,
+                          hipMemcpyDeviceToHost, sl.stream));
+    HIPCHK(hipEventRecord(sl.done, sl.stream));
+    return MH_OK;
+}
+
+void worker_loop(mh_server* sv) {
+    hipSetDevice(sv->ctx->device);
+    std::unique_lock<std::mutex> lk(sv->mu);
+    for (;;) {
+        bool progressed = false;
+        // 1. completions
+        for (StreamSlot& sl : sv->slots) {
+            if (!sl.busy) continue;
+            lk.unlock();
+            const hipError_t q = hipEventQuery(sl.done);
+            lk.lock();
+            if (q == hipErrorNotReady) continue;
+            const double t = now_ms();
+            const int32_t* st = reinterpret_cast<const int32_t*>(sl.h_res);
+            const AnnealOut* ao =
+                reinterpret_cast<const AnnealOut*>(sl.h_res + (sl.plan.o_aout - sl.plan.o_stats));
+            for (size_t i = 0; i < sl.tickets.size(); ++i) {
+                JobRec& r = sv->jobs[(size_t)sl.tickets[i]];
+                if (q == hipSuccess) {
+                    fill_result(r.spec, st + i * kStatsInts, ao + i, &r.result);
+                    r.state = J_DONE;
+                } else {
+                    r.state = J_FAILED;
+                    r.err = MH_ERR_HIP;
+                    r.msg = std::string("batch failed: ") + hipGetErrorString(q);
+                }
+                r.t_done = t;
+                if (r.holds_mem) sv->arena.release(r.off, r.spec.bytes);
+                r.holds_mem = false;
+                ++sv->completed;
+            }
+            sl.tickets.clear();
+            sl.busy = false;
+            progressed = true;
+            sv->cv_done.notify_all();
+        }
+        // 2. admission + launch on a free slot
+        StreamSlot* free_slot = nullptr;
+        for (StreamSlot& sl : sv->slots)
+            if (!sl.busy) {
+                free_slot = &sl;
+                break;
+            }
+        if (free_slot && !sv->queue.empty()) {
+            std::vector<int64_t> take;
+            std::vector<JobRec*> recs;
+            while (!sv->queue.empty() && (int32_t)take.size() < sv->max_batch) {
+                const int64_t t = sv->queue.front();
+                JobRec& r = sv->jobs[(size_t)t];
+                size_t off = 0;
+                if (!sv->arena.alloc(r.spec.bytes, &off)) break;  // FIFO: wait for memory
+                r.off = off;
+                r.holds_mem = true;
+                r.state = J_RUNNING;
+                r.t_launch = now_ms();
+                take.push_back(t);
+                recs.push_back(&r);  // deque elements keep their address across push_back
+                sv->queue.pop_front();
+            }
+            if (!take.empty()) {
+                free_slot->busy = true;
+                free_slot->tickets = take;
+                ++sv->batches;
+                sv->batched_jobs += (int64_t)take.size();
+                if ((int32_t)take.size() > sv->max_seen_batch) sv->max_seen_batch = (int32_t)take.size();
+                lk.unlock();
+                const int rc = launch_batch(sv, *free_slot, recs);
+                const std::string m = rc ? mh_last_error() : "";
+                lk.lock();
+                if (rc) {
+                    // nothing of this batch may still run: wait for its stream
+                    lk.unlock();
+                    hipStreamSynchronize(free_slot->stream);
+                    lk.lock();
+                    for (int64_t t : take) {
+                        JobRec& r = sv->jobs[(size_t)t];
+                        r.state = J_FAILED;
+                        r.err = rc;
+                        r.msg = m;
+                        r.t_done = now_ms();
+                        if (r.holds_mem) sv->arena.release(r.off, r.spec.bytes);
+                        r.holds_mem = false;
+                        ++sv->completed;
+                    }
+                    free_slot->tickets.clear();
+                    free_slot->busy = false;
+                    sv->cv_done.notify_all();
+                }
+                progressed = true;
+            }
+        }
+        bool any_busy = false;
+        for (StreamSlot& sl : sv->slots) any_busy |= sl.busy;
+        if (sv->stop && sv->queue.empty() && !any_busy) break;
+        if (progressed) continue;
+        if (any_busy) {
+            // poll completions without holding the lock
+            lk.unlock();
+            std::this_thread::yield();
+            lk.lock();
+        } else {
+            sv->cv_work.wait(lk, [&] { return sv->stop.load() || !sv->queue.empty(); });
+        }
+    }
+}
+
+void server_free(mh_server* sv) {
+    hipSetDevice(sv->ctx->device);
+    for (StreamSlot& sl : sv->slots) {
+        if (sl.stream) hipStreamSynchronize(sl.stream);
+        if (sl.done) hipEventDestroy(sl.done);
+        if (sl.d_plan) hipFree(sl.d_plan);
+        if (sl.h_plan) hipHostFree(sl.h_plan);
+        if (sl.h_res) hipHostFree(sl.h_res);
+        if (sl.stream) hipStreamDestroy(sl.stream);
+    }
+    if (sv->d_arena) hipFree(sv->d_arena);
+    delete sv;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mh_server_create(mh_context* ctx, const mh_server_params* p, mh_server** out) {
+    if (!ctx || !out) return fail(MH_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    const int64_t budget = p && p->memory_budget > 0 ? p->memory_budget : (int64_t)64 << 30;
+    const int32_t max_batch = p && p->max_batch > 0 ? p->max_batch : 256;
+    const int32_t n_slots = p && p->streams > 0 ? p->streams : 4;
+    if (n_slots > 16 || max_batch > 65536)
+        return fail(MH_ERR_INVALID_ARGUMENT, "streams must be <= 16 and max_batch <= 65536");
+    CHK(set_device(ctx));
+    mh_server* sv = new mh_server();
+    sv->ctx = ctx;
+    sv->max_batch = max_batch;
+    sv->slots.resize((size_t)n_slots);
+    BatchPlan tmp;
+    const size_t plan = plan_bytes(max_batch, &tmp);
+    hipError_t e = hipMalloc(&sv->d_arena, (size_t)budget);
+    for (StreamSlot& sl : sv->slots) {
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipMalloc(&sl.d_plan, plan);
+        if (e == hipSuccess) e = hipHostMalloc(&sl.h_plan, plan, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipHostMalloc(&sl.h_res, plan - tmp.o_stats, hipHostMallocDefault);
+    }
+    if (e != hipSuccess) {
+        server_free(sv);
+        return fail(e == hipErrorOutOfMemory ? MH_ERR_OUT_OF_MEMORY : MH_ERR_HIP,
+                    "server resources (budget %lld B): %s", (long long)budget, hipGetErrorString(e));
+    }
+    sv->arena.reset((size_t)budget);
+    sv->worker = std::thread(worker_loop, sv);
+    *out = sv;
+    return MH_OK;
+}
+
+int mh_server_submit(mh_server* sv, const mh_job* job, int64_t* ticket) {
+    if (!sv || !job || !ticket) return fail(MH_ERR_INVALID_ARGUMENT, "NULL argument");
+    JobSpec s;
+    CHK(check_job(sv->ctx, *job, &s));
+    if (align_up(s.bytes) > sv->arena.size)
+        return fail(MH_ERR_OUT_OF_MEMORY, "job needs %zu B, more than the server budget %zu B", s.bytes,
+                    sv->arena.size);
+    std::lock_guard<std::mutex> g(sv->mu);
+    if (sv->stop) return fail(MH_ERR_INVALID_ARGUMENT, "server is shutting down");
+    JobRec r;
+    r.spec = s;
+    r.t_submit = now_ms();
+    sv->jobs.push_back(std::move(r));
+    *ticket = (int64_t)sv->jobs.size() - 1;
+    sv->queue.push_back(*ticket);
+    sv->cv_work.notify_one();
+    return MH_OK;
+}
+
+int mh_server_wait(mh_server* sv, int64_t ticket, mh_job_result* result, double* latency_ms) {
+    if (!sv) return fail(MH_ERR_INVALID_ARGUMENT, "server is NULL");
+    std::unique_lock<std::mutex> lk(sv->mu);
+    if (ticket < 0 || ticket >= (int64_t)sv->jobs.size())
+        return fail(MH_ERR_NOT_FOUND, "no job with ticket %lld", (long long)ticket);
+    sv->cv_done.wait(lk, [&] { return sv->jobs[(size_t)ticket].state >= J_DONE; });
+    const JobRec& r = sv->jobs[(size_t)ticket];
+    if (latency_ms) *latency_ms = r.t_done - r.t_submit;
+    if (r.state == J_FAILED) return fail(r.err, "%s", r.msg.c_str());
+    if (result) *result = r.result;
+    return MH_OK;
+}
+
+int mh_server_poll(mh_server* sv, int64_t ticket, int32_t* done) {
+    if (!sv || !done) return fail(MH_ERR_INVALID_ARGUMENT, "NULL argument");
+    std::lock_guard<std::mutex> g(sv->mu);
+    if (ticket < 0 || ticket >= (int64_t)sv->jobs.size())
+        return fail(MH_ERR_NOT_FOUND, "no job with ticket %lld", (long long)ticket);
+    *done = sv->jobs[(size_t)ticket].state >= J_DONE ? 1 : 0;
+    return MH_OK;
+}
+
+int mh_server_drain(mh_server* sv) {
+    if (!sv) return fail(MH_ERR_INVALID_ARGUMENT, "server is NULL");
+    std::unique_lock<std::mutex> lk(sv->mu);
+    sv->cv_done.wait(lk, [&] { return sv->completed == (int64_t)sv->jobs.size(); });
+    return MH_OK;
+}
+
+int mh_server_job_times(mh_server* sv, int64_t ticket, double times_ms[3]) {
+    if (!sv || !times_ms) return fail(MH_ERR_INVALID_ARGUMENT, "NULL argument");
+    std::lock_guard<std::mutex> g(sv->mu);
+    if (ticket < 0 || ticket >= (int64_t)sv->jobs.size())
+        return fail(MH_ERR_NOT_FOUND, "no job with ticket %lld", (long long)ticket);
+    const JobRec& r = sv->jobs[(size_t)ticket];
+    times_ms[0] = r.t_submit;
+    times_ms[1] = r.t_launch;
+    times_ms[2] = r.t_done;
+    return MH_OK;
+}
+
+int mh_server_get_stats(mh_server* sv, mh_server_stats* st) {
+    if (!sv || !st) return fail(MH_ERR_INVALID_ARGUMENT, "NULL argument");
+    std::lock_guard<std::mutex> g(sv->mu);
+    st->submitted = (int64_t)sv->jobs.size();
+    st->completed = sv->completed;
+    st->batches = sv->batches;
+    st->max_batch_jobs = sv->max_seen_batch;
+    st->memory_in_use = (int64_t)sv->arena.used;
+    st->memory_peak = (int64_t)sv->arena.peak;
+    st->memory_budget = (int64_t)sv->arena.size;
+    return MH_OK;
+}
+
+void mh_server_destroy(mh_server* sv) {
+    if (!sv) return;
+    {
+        std::lock_guard<std::mutex> g(sv->mu);
+        sv->stop = true;
+        sv->cv_work.notify_all();
+    }
+    if (sv->worker.joinable()) sv->worker.join();
+    server_free(sv);
+}
+
+}  // extern "C"

@@ -36,6 +36,22 @@ __device__ __forceinline__ void gstore(T* p, int64_t i, T v) {
     ((MH_GAS T*)p)[i] = v;
 }
 
+// Segment of a batched launch that owns block b: the j in [0, n_segs) with
+// starts[j] <= b < starts[j+1] (starts ascending, starts[0] = 0, empty
+// segments allowed).  Uniform per block, so the loads stay scalar.
+__device__ __forceinline__ int32_t find_segment(const int32_t* __restrict__ starts, int32_t n_segs,
+                                                int32_t b) {
+    int32_t lo = 0, hi = n_segs;
+    while (hi - lo > 1) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (starts[mid] <= b)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
 // ----------------------------------------------------------------- helpers
 __device__ __forceinline__ float clampf(float v, float lo, float hi) {
     // Rust f32::clamp: NaN passes through.

@@ -62,6 +62,42 @@ struct ResizeTaps {
     int32_t ksize;
 };
 
+// Cells of one image: cell c's pixel (px, py) is at
+//   rgb + (c / per_row) * block_row_stride + (c % per_row) * block_col_stride
+//       + py * row_stride + px * 3.
+struct CellSeg {
+    const uint8_t* rgb;
+    int64_t row_stride, block_row_stride, block_col_stride;
+    int32_t per_row, cw, ch, n_cells;
+    float* out;        // Lab means, n_cells x 3 (cell Lab kernels)
+    uint8_t* out_rgb;  // integer RGB means, n_cells x 3 (average-RGB kernels)
+};
+
+// K nearest materials of nq query Labs against one library (SoA Lab table).
+struct TopkSeg {
+    const float* mL;
+    const float* mA;
+    const float* mB;
+    const float* query;  // nq x 3
+    int32_t* out_idx;
+    float* out_d;        // optional
+    float4* out_lab;     // optional {l, a, b, sqdist}; rows padded with idx -1 up to out_stride
+    int32_t n, nq, k, out_stride;
+};
+
+// Finishing one placed grid: per-cell adjustment and compositing.
+struct CompSeg {
+    const int32_t* grid;
+    int32_t gw, gh, tw, th;
+    const uint8_t* tiles;       // n x th x tw x 3 resized materials
+    const uint8_t* tile_avg;    // n x 3
+    const uint8_t* target_avg;  // cells x 3
+    float strength;
+    int32_t adjust;             // 0: plain tiles (adj unused)
+    float4* adj;                // cells
+    uint8_t* out;               // (gw*tw) x (gh*th) x 3
+};
+
 void launch_cell_lab(const uint8_t* rgb, int64_t row_stride, int32_t per_row,
                      int64_t block_row_stride, int64_t block_col_stride, int32_t cw, int32_t ch,
                      int32_t n_cells, const float* lut, float* out, hipStream_t stream);
@@ -72,6 +108,23 @@ void launch_term_full(const float4* lab, int32_t n, float* out, hipStream_t stre
 void launch_topk(const float* mL, const float* mA, const float* mB, int32_t n, const float* query,
                  int32_t nq, int32_t k, int32_t* out_idx, float* out_d, float4* out_lab,
                  int32_t out_stride, hipStream_t stream);
+// Batched launches over segment tables: starts[j] is the first block of
+// segment j (starts[n_segs] = total).  *_blocks give each segment's count.
+int32_t cell_lab_blocks(int32_t n_cells);
+void launch_cell_lab_batch(const CellSeg* d_segs, const int32_t* d_starts, int32_t n_segs,
+                           int32_t total_blocks, const float* lut, hipStream_t stream);
+// a block per query; max_n decides the distance cache
+void launch_topk_batch(const TopkSeg* d_segs, const int32_t* d_starts, int32_t n_segs,
+                       int32_t total_queries, int32_t max_n, hipStream_t stream);
+// a block per cell
+void launch_avg_rgb_batch(const CellSeg* d_segs, const int32_t* d_starts, int32_t n_segs,
+                          int32_t total_cells, hipStream_t stream);
+int32_t cell_adjust_blocks(int32_t cells);
+int32_t composite_blocks(int64_t pixels);
+void launch_cell_adjust_batch(const CompSeg* d_segs, const int32_t* d_starts, int32_t n_segs,
+                              int32_t total_blocks, hipStream_t stream);
+void launch_composite_batch(const CompSeg* d_segs, const int32_t* d_starts, int32_t n_segs,
+                            int32_t total_blocks, hipStream_t stream);
 // General chain: any cell order, pre-filled grid and usage (state in LDS).
 size_t place_lds_bytes(int32_t cells, int32_t n);
 // Raster chain: generate_mosaic's raster order from an empty grid, where

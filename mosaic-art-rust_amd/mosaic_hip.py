@@ -33,7 +33,10 @@ EXPORTS = [
     "mh_average_lab", "mh_cell_lab_means", "mh_nearest_n", "mh_place", "mh_total_cost",
     "mh_penalties", "mh_swap_deltas", "mh_anneal", "mh_average_rgb", "mh_optimal_adjustments",
     "mh_adjust_pixels", "mh_resize", "mh_composite", "mh_generate", "mh_generate_device",
-    "mh_generate_device_result",
+    "mh_generate_device_result", "mh_job_footprint", "mh_batch_create", "mh_batch_set_graph",
+    "mh_batch_run", "mh_batch_results", "mh_batch_job_outputs", "mh_batch_destroy",
+    "mh_server_create", "mh_server_submit", "mh_server_wait", "mh_server_poll", "mh_server_drain",
+    "mh_server_job_times", "mh_server_get_stats", "mh_server_destroy",
 ]
 
 
@@ -85,6 +88,33 @@ class JobParams(ctypes.Structure):
 class JobResult(ctypes.Structure):
     _fields_ = [("place", PlaceStats), ("anneal", AnnealResult),
                 ("out_width", ctypes.c_int32), ("out_height", ctypes.c_int32)]
+
+
+class Job(ctypes.Structure):
+    _fields_ = [("lib", ctypes.c_void_p), ("d_target", ctypes.c_void_p),
+                ("target_pitch", ctypes.c_int64), ("width", ctypes.c_int32),
+                ("height", ctypes.c_int32), ("params", JobParams),
+                ("d_grid_out", ctypes.c_void_p), ("d_image_out", ctypes.c_void_p)]
+
+
+class ServerParams(ctypes.Structure):
+    _fields_ = [("memory_budget", ctypes.c_int64), ("max_batch", ctypes.c_int32),
+                ("streams", ctypes.c_int32)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_int64), ("completed", ctypes.c_int64),
+                ("batches", ctypes.c_int64), ("max_batch_jobs", ctypes.c_int32),
+                ("memory_in_use", ctypes.c_int64), ("memory_peak", ctypes.c_int64),
+                ("memory_budget", ctypes.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def _job_result_dict(res):
+    return {"place": res.place.as_dict(), "anneal": res.anneal.as_dict(),
+            "out_size": (res.out_width, res.out_height)}
 
 
 _lib = None
@@ -142,6 +172,22 @@ def load(path=LIB_PATH):
                                        ctypes.POINTER(JobResult)]),
         "mh_generate_device": (ctypes.c_int, [P, P, P, i32, i32, ctypes.POINTER(JobParams), P, P]),
         "mh_generate_device_result": (ctypes.c_int, [P, ctypes.POINTER(JobResult)]),
+        "mh_job_footprint": (i64, [P, ctypes.POINTER(Job)]),
+        "mh_batch_create": (ctypes.c_int, [P, P, i32, ctypes.POINTER(P)]),
+        "mh_batch_set_graph": (ctypes.c_int, [P, i32]),
+        "mh_batch_run": (ctypes.c_int, [P]),
+        "mh_batch_results": (ctypes.c_int, [P, P]),
+        "mh_batch_job_outputs": (ctypes.c_int, [P, i32, ctypes.POINTER(P), ctypes.POINTER(P)]),
+        "mh_batch_destroy": (None, [P]),
+        "mh_server_create": (ctypes.c_int, [P, ctypes.POINTER(ServerParams), ctypes.POINTER(P)]),
+        "mh_server_submit": (ctypes.c_int, [P, ctypes.POINTER(Job), ctypes.POINTER(i64)]),
+        "mh_server_wait": (ctypes.c_int, [P, i64, ctypes.POINTER(JobResult),
+                                          ctypes.POINTER(ctypes.c_double)]),
+        "mh_server_poll": (ctypes.c_int, [P, i64, ctypes.POINTER(i32)]),
+        "mh_server_drain": (ctypes.c_int, [P]),
+        "mh_server_job_times": (ctypes.c_int, [P, i64, P]),
+        "mh_server_get_stats": (ctypes.c_int, [P, ctypes.POINTER(ServerStats)]),
+        "mh_server_destroy": (None, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -217,6 +263,11 @@ class Context:
         ptr = self.device_alloc(a.nbytes)
         _check(self.lib.mh_copy_to_device(self.h, ctypes.c_void_p(ptr), _ptr(a), a.nbytes))
         return ptr
+
+    def copy_to_device(self, ptr, array):
+        """Copies a numpy array into an existing device buffer."""
+        a = np.ascontiguousarray(array)
+        _check(self.lib.mh_copy_to_device(self.h, ctypes.c_void_p(ptr), _ptr(a), a.nbytes))
 
     def copy_on_device(self, dst, src, nbytes):
         _check(self.lib.mh_copy_on_device(self.h, ctypes.c_void_p(dst), ctypes.c_void_p(src), nbytes))
@@ -383,8 +434,24 @@ class Context:
     def generate_device_result(self):
         res = JobResult()
         _check(self.lib.mh_generate_device_result(self.h, ctypes.byref(res)))
-        return {"place": res.place.as_dict(), "anneal": res.anneal.as_dict(),
-                "out_size": (res.out_width, res.out_height)}
+        return _job_result_dict(res)
+
+    # ------------------------------------------------- batches and serving
+    def job(self, lib, d_target, width, height, params, d_grid=None, d_image=None, pitch=0):
+        """An mh_job over device buffers (integer addresses)."""
+        return Job(lib.h.value, d_target, pitch, width, height, params, d_grid, d_image)
+
+    def job_footprint(self, job):
+        v = self.lib.mh_job_footprint(self.h, ctypes.byref(job))
+        if v < 0:
+            _check(int(v))
+        return int(v)
+
+    def batch(self, jobs):
+        return Batch(self, jobs)
+
+    def server(self, memory_budget=0, max_batch=0, streams=0):
+        return Server(self, memory_budget, max_batch, streams)
 
 
 class Library:
@@ -436,3 +503,99 @@ class Library:
             return None
         _check(rc)
         return v.value
+
+
+class Batch:
+    """A fixed set of jobs run as one fused launch per stage (mh_batch)."""
+
+    def __init__(self, ctx, jobs):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.n = len(jobs)
+        self._jobs = (Job * self.n)(*jobs)
+        h = ctypes.c_void_p()
+        _check(self.lib.mh_batch_create(ctx.h, ctypes.cast(self._jobs, ctypes.c_void_p), self.n,
+                                        ctypes.byref(h)))
+        self.h = h
+
+    def set_graph(self, enabled):
+        _check(self.lib.mh_batch_set_graph(self.h, 1 if enabled else 0))
+
+    def run(self):
+        _check(self.lib.mh_batch_run(self.h))
+
+    def results(self):
+        res = (JobResult * self.n)()
+        _check(self.lib.mh_batch_results(self.h, ctypes.cast(res, ctypes.c_void_p)))
+        return [_job_result_dict(r) for r in res]
+
+    def outputs(self, i):
+        g, im = ctypes.c_void_p(), ctypes.c_void_p()
+        _check(self.lib.mh_batch_job_outputs(self.h, i, ctypes.byref(g), ctypes.byref(im)))
+        return g.value, im.value
+
+    def close(self):
+        if self.h:
+            self.lib.mh_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching request loop (mh_server)."""
+
+    def __init__(self, ctx, memory_budget=0, max_batch=0, streams=0):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self._keep = {}
+        h = ctypes.c_void_p()
+        prm = ServerParams(memory_budget, max_batch, streams)
+        _check(self.lib.mh_server_create(ctx.h, ctypes.byref(prm), ctypes.byref(h)))
+        self.h = h
+
+    def submit(self, job):
+        t = ctypes.c_int64()
+        _check(self.lib.mh_server_submit(self.h, ctypes.byref(job), ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket):
+        res = JobResult()
+        lat = ctypes.c_double()
+        _check(self.lib.mh_server_wait(self.h, ticket, ctypes.byref(res), ctypes.byref(lat)))
+        out = _job_result_dict(res)
+        out["latency_ms"] = lat.value
+        return out
+
+    def poll(self, ticket):
+        d = ctypes.c_int32()
+        _check(self.lib.mh_server_poll(self.h, ticket, ctypes.byref(d)))
+        return bool(d.value)
+
+    def drain(self):
+        _check(self.lib.mh_server_drain(self.h))
+
+    def times(self, ticket):
+        t = np.zeros(3, np.float64)
+        _check(self.lib.mh_server_job_times(self.h, ticket, _ptr(t)))
+        return t
+
+    def stats(self):
+        st = ServerStats()
+        _check(self.lib.mh_server_get_stats(self.h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if self.h:
+            self.lib.mh_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

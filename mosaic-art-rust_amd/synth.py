@@ -13,6 +13,12 @@ Workloads by name (sizes as the benchmark configurations state them):
   batch   64 jobs: grid U{20..160} x U{15..120}, 16 px cells, library
           U{256..4096} materials 64x64 (uniform flat + noise 12), usage
           U{3..5}, 1,000 SA iterations each.
+  stream  5,000 jobs arriving as a Poisson process of rate lambda; cells
+          log-uniform in [300, 24000], aspect 4:3 or 16:9, 16 px cells;
+          each job uses one of 4 resident libraries (1,000 / 2,500 / 5,000 /
+          10,000 materials 64x64, mixture colours as in large); usage
+          U{3..5}, weight 0.25, colour 0.4, 1,000 SA iterations.  Targets are
+          crops of 4 canvases of 4000x2400 built like the large target.
 """
 import numpy as np
 
@@ -116,3 +122,38 @@ def make_batch(n_jobs=64, seed=0, cell=16, msize=64):
         mats = flat_materials(n, msize, rng)
         jobs.append((p, target, mats))
     return jobs
+
+
+def make_stream(n_jobs=5000, rate=2000.0, seed=0, libraries=(1000, 2500, 5000, 10000),
+                cells=(300, 24000), canvas=(4000, 2400), n_canvases=4, cell=16, msize=64,
+                iters=1000):
+    """The served job stream.  Returns a dict:
+      libs      list of material arrays (n x msize x msize x 3 u8)
+      canvases  list of canvas targets (H x W x 3 u8)
+      jobs      list of dicts: arrival (s), lib, canvas, ox, oy (crop origin in
+                pixels), grid_w, grid_h, width, height, max_usage, weight, color,
+                iters, seed
+    Arrival times are the partial sums of exponential gaps of mean 1/rate."""
+    rng = np.random.default_rng(seed)
+    libs = [flat_materials(n, msize, rng, colours=mixture_colours(n, rng)) for n in libraries]
+    cw, ch = canvas
+    canvases = [blob_target(cw, ch, 32, rng) for _ in range(n_canvases)]
+    lo, hi = np.log(cells[0]), np.log(cells[1])
+    t = 0.0
+    jobs = []
+    for j in range(n_jobs):
+        t += float(rng.exponential(1.0 / rate))
+        c = float(np.exp(rng.uniform(lo, hi)))
+        aspect = 16.0 / 9.0 if rng.integers(0, 2) else 4.0 / 3.0
+        gw = max(1, int(round(np.sqrt(c * aspect))))
+        gh = max(1, int(round(c / gw)))
+        gw = min(gw, cw // cell)
+        gh = min(gh, ch // cell)
+        w, h = gw * cell, gh * cell
+        jobs.append(dict(arrival=t, lib=int(rng.integers(0, len(libraries))),
+                         canvas=int(rng.integers(0, n_canvases)),
+                         ox=int(rng.integers(0, cw - w + 1)), oy=int(rng.integers(0, ch - h + 1)),
+                         grid_w=gw, grid_h=gh, width=w, height=h,
+                         max_usage=int(rng.integers(3, 6)), weight=0.25, color=0.4, iters=iters,
+                         seed=j))
+    return dict(libs=libs, canvases=canvases, jobs=jobs, canvas_size=canvas)

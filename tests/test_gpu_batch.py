@@ -1,0 +1,192 @@
+"""Batched (mh_batch) and served (mh_server) forms.
+
+Every job of a fused batch or of the request loop must give exactly what
+mh_generate_device gives for the same job alone (grid, composited image,
+placement counters, annealing result), which the parity tests pin to the
+oracle.  Jobs mix grid sizes, libraries (including one with fewer than K=100
+materials and an empty one), SA on/off, colour strength 0 and weight 0, and
+read their targets as crops of shared canvases (row pitch != width*3).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _stream(synth, n_jobs=12, seed=3):
+    return synth.make_stream(n_jobs=n_jobs, rate=1e9, seed=seed, libraries=(90, 300),
+                             cells=(20, 1200), canvas=(720, 560), n_canvases=2, iters=300)
+
+
+class Setup:
+    """Device-resident libraries + canvases and the job list of a stream."""
+
+    def __init__(self, ctx, mh, synth, n_jobs=12, seed=3):
+        self.ctx, self.mh = ctx, mh
+        st = _stream(synth, n_jobs, seed)
+        self.st = st
+        self.libs = [ctx.library_from_pixels(m) for m in st["libs"]]
+        self.libs.append(ctx.library_from_pixels(np.zeros((0, 64, 64, 3), np.uint8)))
+        self.W, self.H = st["canvas_size"]
+        self.canvases = [c.copy() for c in st["canvases"]]
+        self.d_can = [ctx.to_device(c) for c in self.canvases]
+        self.specs = []
+        for k, j in enumerate(st["jobs"]):
+            j = dict(j)
+            j["sa"] = 0 if k % 3 == 1 else 1
+            if k % 4 == 2:
+                j["color"] = 0.0
+            if k % 5 == 3:
+                j["weight"] = 0.0
+            self.specs.append(j)
+        # one job on the empty library
+        e = dict(self.specs[0])
+        e["lib"] = len(self.libs) - 1
+        self.specs.append(e)
+        self.allocs = []
+
+    def params(self, j):
+        return self.mh.JobParams(j["grid_w"], j["grid_h"], j["max_usage"], j["weight"], j["color"],
+                                 j["sa"], j["iters"], 100.0, 0.99995, j["seed"])
+
+    def alloc(self, nbytes):
+        p = self.ctx.device_alloc(nbytes)
+        self.allocs.append(p)
+        return p
+
+    def job(self, k, own_outputs):
+        j = self.specs[k]
+        ptr = self.d_can[j["canvas"]] + (j["oy"] * self.W + j["ox"]) * 3
+        d_g = d_i = None
+        if own_outputs:
+            d_g = self.alloc(j["grid_w"] * j["grid_h"] * 4)
+            d_i = self.alloc(j["width"] * j["height"] * 3)
+        return self.ctx.job(self.libs[j["lib"]], ptr, j["width"], j["height"], self.params(j),
+                            d_g, d_i, pitch=self.W * 3)
+
+    def alone(self, k):
+        """mh_generate_device on a contiguous copy of the job's crop."""
+        j = self.specs[k]
+        c = self.canvases[j["canvas"]][j["oy"]:j["oy"] + j["height"], j["ox"]:j["ox"] + j["width"]]
+        ctx = self.ctx
+        d_t = ctx.to_device(np.ascontiguousarray(c))
+        cells = j["grid_w"] * j["grid_h"]
+        d_g = ctx.device_alloc(cells * 4)
+        d_i = ctx.device_alloc(j["width"] * j["height"] * 3)
+        ctx.generate_device(self.libs[j["lib"]], d_t, j["width"], j["height"], self.params(j), d_g,
+                            d_i)
+        ctx.synchronize()
+        res = ctx.generate_device_result()
+        grid = ctx.to_host(d_g, cells, np.int32)
+        img = ctx.to_host(d_i, (j["height"], j["width"], 3), np.uint8)
+        for p in (d_t, d_g, d_i):
+            ctx.device_free(p)
+        return grid, img, res
+
+    def fetch(self, k, d_g, d_i):
+        j = self.specs[k]
+        grid = self.ctx.to_host(d_g, j["grid_w"] * j["grid_h"], np.int32)
+        img = self.ctx.to_host(d_i, (j["height"], j["width"], 3), np.uint8)
+        return grid, img
+
+    def close(self):
+        for p in self.allocs + self.d_can:
+            self.ctx.device_free(p)
+        for lib in self.libs:
+            lib.close()
+
+
+def _same(want, grid, img, res, sa):
+    wg, wi, wr = want
+    np.testing.assert_array_equal(grid, wg)
+    np.testing.assert_array_equal(img, wi)
+    assert res["place"] == wr["place"]
+    assert res["out_size"] == wr["out_size"]
+    if sa:
+        assert res["anneal"] == wr["anneal"]
+
+
+def test_batch_matches_jobs_alone(ctx, mh, synth):
+    s = Setup(ctx, mh, synth)
+    jobs = [s.job(k, own_outputs=(k % 2 == 1)) for k in range(len(s.specs))]
+    b = ctx.batch(jobs)
+    b.run()
+    res = b.results()
+    assert res[-1]["place"]["empty"] == s.specs[-1]["grid_w"] * s.specs[-1]["grid_h"]
+    for k in range(len(s.specs)):
+        print(f"batch job {k}: {s.specs[k]['grid_w']}x{s.specs[k]['grid_h']} lib {s.specs[k]['lib']}",
+              flush=True)
+        g, im = s.fetch(k, *b.outputs(k))
+        sa = s.specs[k]["sa"] and s.specs[k]["weight"] > 0
+        _same(s.alone(k), g, im, res[k], sa)
+    b.close()
+    s.close()
+
+
+def test_batch_graph_replays(ctx, mh, synth):
+    s = Setup(ctx, mh, synth, n_jobs=6, seed=5)
+    jobs = [s.job(k, own_outputs=True) for k in range(len(s.specs))]
+    b = ctx.batch(jobs)
+    b.set_graph(True)
+    want = [s.alone(k) for k in range(len(s.specs))]
+    for rep in range(3):
+        b.run()
+        res = b.results()
+        for k in range(len(s.specs)):
+            g, im = s.fetch(k, *b.outputs(k))
+            _same(want[k], g, im, res[k], s.specs[k]["sa"] and s.specs[k]["weight"] > 0)
+        # eager work on the same stream between replays
+        s.alone(0)
+    # replays re-read the inputs: change a canvas in place and compare again
+    s.canvases[0] = np.ascontiguousarray(255 - s.canvases[0])
+    ctx.copy_to_device(s.d_can[0], s.canvases[0])
+    b.run()
+    res = b.results()
+    for k in range(len(s.specs)):
+        g, im = s.fetch(k, *b.outputs(k))
+        _same(s.alone(k), g, im, res[k], s.specs[k]["sa"] and s.specs[k]["weight"] > 0)
+    b.close()
+    s.close()
+
+
+def test_server_results_equal_alone_under_memory_pressure(ctx, mh, synth):
+    s = Setup(ctx, mh, synth, n_jobs=16, seed=7)
+    jobs = [s.job(k, own_outputs=True) for k in range(len(s.specs))]
+    need = max(ctx.job_footprint(j) for j in jobs)
+    budget = int(need * 2.5)
+    srv = ctx.server(memory_budget=budget, max_batch=4, streams=2)
+    tickets = [srv.submit(j) for j in jobs]
+    got = [srv.wait(t) for t in tickets]
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == len(jobs) and st["submitted"] == len(jobs)
+    assert st["memory_peak"] <= budget and st["memory_in_use"] == 0
+    assert st["batches"] >= (len(jobs) + 3) // 4 and st["max_batch_jobs"] <= 4
+    for k in range(len(s.specs)):
+        print(f"served job {k}", flush=True)
+        assert got[k]["latency_ms"] >= 0.0
+        g, im = s.fetch(k, jobs[k].d_grid_out, jobs[k].d_image_out)
+        _same(s.alone(k), g, im, got[k], s.specs[k]["sa"] and s.specs[k]["weight"] > 0)
+    s.close()
+
+
+def test_server_rejects_bad_jobs(ctx, mh, synth):
+    s = Setup(ctx, mh, synth, n_jobs=2, seed=9)
+    j = s.job(0, own_outputs=False)
+    need = ctx.job_footprint(j)
+    srv = ctx.server(memory_budget=need // 2, max_batch=2, streams=1)
+    with pytest.raises(mh.MhError) as e:
+        srv.submit(j)
+    assert e.value.code == mh.MH_ERR_OUT_OF_MEMORY
+    srv.close()
+    srv = ctx.server(memory_budget=need * 4, max_batch=2, streams=1)
+    lab_only = ctx.library_from_lab(np.zeros((4, 3), np.float32))
+    bad = ctx.job(lab_only, j.d_target, j.width, j.height, j.params, pitch=j.target_pitch)
+    with pytest.raises(mh.MhError) as e:
+        srv.submit(bad)
+    assert e.value.code == mh.MH_ERR_INVALID_ARGUMENT
+    t = srv.submit(j)
+    assert srv.wait(t)["place"]["placed"] == j.params.grid_w * j.params.grid_h
+    srv.close()
+    lab_only.close()
+    s.close()
