@@ -88,6 +88,177 @@ def cpu_baseline(oracle, p, target, mats, budget_s):
             return jobs / el, jobs, el
 
 
+def _pct(xs, q):
+    return round(float(np.percentile(np.asarray(xs, np.float64), q)), 4) if len(xs) else None
+
+
+def run_batched(mh, synth, ctx, steps, warmup):
+    """64 independent jobs (own libraries, grids 20-160 x 15-120 at 16 px)
+    plain one at a time, then as one fused batch replayed from a hipGraph."""
+    jobs = synth.make_batch(64)
+    libs, d_t, d_g, d_i, d_g2, d_i2, prm = [], [], [], [], [], [], []
+    for k, (p, target, mats) in enumerate(jobs):
+        libs.append(ctx.library_from_pixels(mats))
+        d_t.append(ctx.to_device(target))
+        cells = p["grid_w"] * p["grid_h"]
+        img = p["width"] * p["height"] * 3
+        d_g.append(ctx.device_alloc(cells * 4))
+        d_i.append(ctx.device_alloc(img))
+        d_g2.append(ctx.device_alloc(cells * 4))
+        d_i2.append(ctx.device_alloc(img))
+        prm.append(mh.JobParams(p["grid_w"], p["grid_h"], p["max_usage"], p["weight"], p["color"], 1,
+                                p["iters"], 100.0, 0.99995, k))
+
+    def plain_pass(lat=None):
+        for k, (p, _, _) in enumerate(jobs):
+            t0 = time.perf_counter()
+            ctx.generate_device(libs[k], d_t[k], p["width"], p["height"], prm[k], d_g[k], d_i[k])
+            ctx.synchronize()
+            if lat is not None:
+                lat.append(1000.0 * (time.perf_counter() - t0))
+
+    plain_pass()  # warm: library caches (resized tiles, term matrices)
+    plain_lat = []
+    t0 = time.perf_counter()
+    plain_pass(plain_lat)
+    plain_s = time.perf_counter() - t0
+
+    b = ctx.batch([ctx.job(libs[k], d_t[k], p["width"], p["height"], prm[k], d_g2[k], d_i2[k])
+                   for k, (p, _, _) in enumerate(jobs)])
+    b.set_graph(True)
+    for _ in range(max(1, warmup)):
+        b.run()
+        ctx.synchronize()
+    step_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        t1 = time.perf_counter()
+        b.run()
+        ctx.synchronize()
+        step_ms.append(1000.0 * (time.perf_counter() - t1))
+    batch_s = time.perf_counter() - t0
+    same = True
+    for k, (p, _, _) in enumerate(jobs):
+        cells = p["grid_w"] * p["grid_h"]
+        same &= np.array_equal(ctx.to_host(d_g[k], cells, np.int32), ctx.to_host(d_g2[k], cells, np.int32))
+        n_img = p["width"] * p["height"] * 3
+        same &= np.array_equal(ctx.to_host(d_i[k], n_img, np.uint8), ctx.to_host(d_i2[k], n_img, np.uint8))
+    b.close()
+    cells = sum(p["grid_w"] * p["grid_h"] for p, _, _ in jobs)
+    for ptr in d_t + d_g + d_i + d_g2 + d_i2:
+        ctx.device_free(ptr)
+    for lib in libs:
+        lib.close()
+    n = len(jobs)
+    return {
+        "workload": f"{n} independent jobs, own libraries of 256-4,096 materials 64x64, grids "
+                    f"20-160 x 15-120 at 16 px ({cells} cells), usage 3-5, 1,000 SA proposals each",
+        "jobs_per_s": round(n * steps / batch_s, 2),
+        "step_ms": {"p50": _pct(step_ms, 50), "p99": _pct(step_ms, 99)},
+        "tiles_placed_per_s": round(cells * steps / batch_s, 1),
+        "plain_jobs_per_s": round(n / plain_s, 2),
+        "plain_job_ms": {"p50": _pct(plain_lat, 50), "p99": _pct(plain_lat, 99)},
+        "speedup_vs_plain": round((n * steps / batch_s) / (n / plain_s), 2),
+        "outputs_identical_to_plain": bool(same),
+        "graph": True,
+    }
+
+
+def run_served(mh, synth, ctx, n_jobs, rate, budget, streams, max_batch):
+    """The Poisson job stream through the server, and the same stream through
+    the plain path one job at a time (a job starts when it has arrived and the
+    previous one is done).  Latency = completion - arrival."""
+    st = synth.make_stream(n_jobs, rate)
+    jobs = st["jobs"]
+    W = st["canvas_size"][0]
+    libs = [ctx.library_from_pixels(m) for m in st["libs"]]
+    d_can = [ctx.to_device(c) for c in st["canvases"]]
+    d_tg = []
+    for j in jobs:  # each job's target: a contiguous crop of a canvas, on the device
+        ptr = ctx.device_alloc(j["width"] * j["height"] * 3)
+        ctx.copy_2d_on_device(ptr, j["width"] * 3, d_can[j["canvas"]] + (j["oy"] * W + j["ox"]) * 3,
+                              W * 3, j["width"] * 3, j["height"])
+        d_tg.append(ptr)
+    ctx.synchronize()
+    prm = [mh.JobParams(j["grid_w"], j["grid_h"], j["max_usage"], j["weight"], j["color"], 1,
+                        j["iters"], 100.0, 0.99995, j["seed"]) for j in jobs]
+    sample = list(range(0, n_jobs, max(1, n_jobs // 100)))
+    d_gs = {k: ctx.device_alloc(jobs[k]["grid_w"] * jobs[k]["grid_h"] * 4) for k in sample}
+    max_img = max(j["width"] * j["height"] * 3 for j in jobs)
+    max_cells = max(j["grid_w"] * j["grid_h"] for j in jobs)
+    d_img = ctx.device_alloc(max_img)
+    d_grid = ctx.device_alloc(max_cells * 4)
+    # warm the library caches (resized tiles, term matrices) once per library
+    for li in range(len(libs)):
+        k = next(i for i, j in enumerate(jobs) if j["lib"] == li)
+        ctx.generate_device(libs[li], d_tg[k], jobs[k]["width"], jobs[k]["height"], prm[k], d_grid, d_img)
+    ctx.synchronize()
+
+    # served
+    srv = ctx.server(memory_budget=budget, max_batch=max_batch, streams=streams)
+    mj = [ctx.job(libs[j["lib"]], d_tg[k], j["width"], j["height"], prm[k], d_gs.get(k), None)
+          for k, j in enumerate(jobs)]
+    tickets = []
+    t0 = time.perf_counter()
+    for k, j in enumerate(jobs):
+        due = t0 + j["arrival"]
+        while time.perf_counter() < due:
+            pass
+        tickets.append(srv.submit(mj[k]))
+    srv.drain()
+    times = np.array([srv.times(t) for t in tickets])
+    stats = srv.stats()
+    srv.close()
+    lat = times[:, 2] - times[:, 0]
+    span = (times[:, 2].max() - times[:, 0].min()) / 1000.0
+    served_grids = {k: ctx.to_host(d_gs[k], jobs[k]["grid_w"] * jobs[k]["grid_h"], np.int32)
+                    for k in sample}
+
+    # plain, same arrivals
+    plain_lat = []
+    same = True
+    t0 = time.perf_counter()
+    last = 0.0
+    for k, j in enumerate(jobs):
+        due = t0 + j["arrival"]
+        while time.perf_counter() < due:
+            pass
+        ctx.generate_device(libs[j["lib"]], d_tg[k], j["width"], j["height"], prm[k], d_grid, d_img)
+        ctx.synchronize()
+        last = time.perf_counter()
+        plain_lat.append(1000.0 * (last - due))
+        if k in d_gs:
+            cells = j["grid_w"] * j["grid_h"]
+            same &= np.array_equal(ctx.to_host(d_grid, cells, np.int32), served_grids[k])
+    plain_span = last - (t0 + jobs[0]["arrival"])
+    for ptr in d_tg + list(d_gs.values()) + d_can + [d_img, d_grid]:
+        ctx.device_free(ptr)
+    for lib in libs:
+        lib.close()
+    cells = sum(j["grid_w"] * j["grid_h"] for j in jobs)
+    served_jps = n_jobs / span
+    plain_jps = n_jobs / plain_span
+    return {
+        "workload": f"{n_jobs} Poisson jobs at {rate:g}/s, cells log-uniform 300-24,000 "
+                    f"({cells} total), 4:3 / 16:9 at 16 px, libraries 1k/2.5k/5k/10k of 64x64, "
+                    f"usage 3-5, 1,000 SA proposals each",
+        "jobs_per_s": round(served_jps, 2),
+        "latency_ms": {"p50": _pct(lat, 50), "p99": _pct(lat, 99)},
+        "plain_jobs_per_s": round(plain_jps, 2),
+        "plain_latency_ms": {"p50": _pct(plain_lat, 50), "p99": _pct(plain_lat, 99)},
+        "speedup_vs_plain": round(served_jps / plain_jps, 2),
+        "tiles_placed_per_s": round(cells / span, 1),
+        "placements_identical_to_plain": bool(same),
+        "checked_jobs": len(sample),
+        "batches": stats["batches"],
+        "mean_jobs_per_batch": round(stats["submitted"] / max(stats["batches"], 1), 2),
+        "max_jobs_per_batch": stats["max_batch_jobs"],
+        "memory_peak_gib": round(stats["memory_peak"] / 2**30, 3),
+        "memory_budget_gib": round(stats["memory_budget"] / 2**30, 3),
+        "streams": streams,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +269,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="also run a device-to-device copy of BYTES (counter calibration)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-job batched form")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served job stream")
+    ap.add_argument("--serve-jobs", type=int, default=5000)
+    ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrivals per second")
+    ap.add_argument("--serve-streams", type=int, default=4)
+    ap.add_argument("--serve-max-batch", type=int, default=256)
+    ap.add_argument("--serve-budget-gib", type=float, default=64.0)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -217,6 +395,13 @@ def main():
                       "anneal_rounds": res["anneal"]["rounds"]},
         "roofline": roof,
     }
+
+    if not args.no_batch:
+        out["batched"] = run_batched(mh, synth, ctx, max(3, args.steps // 4), 1)
+    if not args.no_serve:
+        out["served"] = run_served(mh, synth, ctx, args.serve_jobs, args.serve_rate,
+                                   int(args.serve_budget_gib * 2**30), args.serve_streams,
+                                   args.serve_max_batch)
 
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

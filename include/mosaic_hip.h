@@ -111,6 +111,10 @@ int mh_copy_to_host(mh_context *ctx, void *dst, const void *src, size_t bytes);
 /* Device-to-device copy by a plain streaming kernel, enqueued on the context
  * stream (also the known-size copy used to calibrate HBM byte counters). */
 int mh_copy_on_device(mh_context *ctx, void *dst, const void *src, size_t bytes);
+/* Device-to-device copy of rows x row_bytes between pitched buffers (e.g. a
+ * crop of a larger frame), enqueued on the context stream. */
+int mh_copy_2d_on_device(mh_context *ctx, void *dst, size_t dst_pitch, const void *src,
+                         size_t src_pitch, size_t row_bytes, size_t rows);
 
 /* --------------------------------------------------------------- library */
 /* SimilarityDatabase::new + add_tile x n (src/similarity.rs:51-66): the Lab

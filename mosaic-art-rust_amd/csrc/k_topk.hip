@@ -6,183 +6,182 @@
 // the bit pattern orders like the value and keys are unique; the K smallest
 // keys are the K nearest materials ascending with ties broken by index.
 //
-// One 256-thread block per query cell:
-//   1. squared distances for all N (VALU; cached in LDS when N <= 16384),
-//   2. radix select of the K-th smallest key over 11-bit digits (LDS
-//      histograms, usually 2-3 passes; early exit once a digit bin holds
-//      exactly the remaining count),
-//   3. gather the K keys <= threshold, bitonic sort in LDS, write ascending.
-// VALU-bound: 8 flops + 1 LDS atomic per material per pass.
-// The batched kernel serves many (library, query set) segments in one
-// launch: block b answers query b - starts[j] of the segment j holding b.
+// One wave (a 64-thread workgroup) answers a run of consecutive query cells.
+// Per cell it keeps a candidate buffer in LDS and a threshold tau such that
+// at least K keys are <= tau:
+//   * tau starts as the largest of this cell's keys to the previous cell's K
+//     answers (K materials have a key <= tau, so the K smallest do too);
+//     neighbouring cells have similar Lab means, so tau is tight;
+//   * one pass over the N materials appends every key <= tau (ballot +
+//     mbcnt compaction, 64 materials per step);
+//   * when the buffer would overflow it is sorted and cut to its K smallest,
+//     whose largest becomes the new (tighter) tau;
+//   * at the end the buffer is sorted (bitonic network in LDS) and the first
+//     K keys are the answer.
+// The first cell of a run starts from tau = +inf.  The cost is one distance
+// evaluation per material plus a few small sorts, with no histograms or
+// atomics.  The batched kernel serves many (library, query set) segments in
+// one launch: block b handles run b - starts[j] of the segment j holding b.
 #include "mh_device.hpp"
 #include "mh_kernels.hpp"
 
 namespace mh {
 
-constexpr int kTopkThreads = 256;
-constexpr int kDigitBits = 11;
-constexpr int kBins = 1 << kDigitBits;
-constexpr int kBinsPerThread = kBins / kTopkThreads;
-constexpr int32_t kDistCacheMax = 16384;
+constexpr int kBuf = 256;      // candidate buffer (keys) per wave
+constexpr int kMaxK = 128;     // largest K served
+constexpr int kMaxRun = 8;     // query cells per wave
 
 __device__ __forceinline__ uint64_t make_key(uint32_t dbits, int32_t i) {
     return ((uint64_t)dbits << 32) | (uint32_t)i;
 }
 
-// Exclusive prefix sum over the block (256 threads = 4 waves).
-__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* wsum) {
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    uint32_t inc = v;
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        uint32_t o = __shfl_up(inc, off, 64);
-        if (lane >= off) inc += o;
+    for (int off = 32; off >= 1; off >>= 1) {
+        const uint64_t o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
     }
-    if (lane == 63) wsum[wid] = inc;
-    __syncthreads();
-    uint32_t base = 0;
-    for (int w = 0; w < wid; ++w) base += wsum[w];
-    __syncthreads();
-    return base + inc - v;
+    return v;
 }
 
-struct TopkShared {
-    uint32_t hist[kBins];
-    uint64_t sel[128];
-    uint32_t wsum[4];
-    uint32_t digit, need, cnt, fill;
-};
+// Ascending bitonic sort of buf[0..P) by one wave (P = 128 or 256); entries
+// [count, P) are first filled with the largest key.  The network is fully
+// unrolled, so every index is a compile-time function of the lane.
+template <int P>
+__device__ __forceinline__ void wave_sort(uint64_t* buf, int32_t count) {
+    const int lane = threadIdx.x;
+#pragma unroll
+    for (int r = 0; r < P / 64; ++r)
+        if (r * 64 + lane >= count) buf[r * 64 + lane] = ~0ull;
+    __syncthreads();
+#pragma unroll
+    for (int size = 2; size <= P; size <<= 1) {
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+#pragma unroll
+            for (int h = 0; h < P / 128; ++h) {
+                const int t = h * 64 + lane;
+                const int i = ((t & ~(stride - 1)) << 1) | (t & (stride - 1));
+                const int j = i + stride;
+                const bool asc = (i & size) == 0;
+                const uint64_t a = buf[i], b = buf[j];
+                if ((a > b) == asc) {
+                    buf[i] = b;
+                    buf[j] = a;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
 
-// K nearest of query q of segment s (one 256-thread block).
-template <bool kCache>
-__device__ __forceinline__ void topk_block(const TopkSeg& s, int32_t q, TopkShared& sh,
-                                           uint32_t* dcache) {
-    const int tid = threadIdx.x;
+__device__ __forceinline__ void sort_buffer(uint64_t* buf, int32_t count) {
+    if (count <= 128)
+        wave_sort<128>(buf, count);
+    else
+        wave_sort<256>(buf, count);
+}
+
+// Answers query cells [q0, q0 + run) of segment s.
+__device__ __forceinline__ void topk_run(const TopkSeg& s, int32_t q0, int32_t run,
+                                         uint64_t* buf, int32_t* prev) {
+    const int lane = threadIdx.x;
     const float* __restrict__ mL = s.mL;
     const float* __restrict__ mA = s.mA;
     const float* __restrict__ mB = s.mB;
-    const int32_t n = s.n, k = s.k;
-    const float ql = s.query[3 * (int64_t)q], qa = s.query[3 * (int64_t)q + 1],
-                qb = s.query[3 * (int64_t)q + 2];
+    const int32_t n = s.n;
+    const int32_t k = s.k < n ? s.k : n;
+    const int32_t q_end = q0 + run < s.nq ? q0 + run : s.nq;
+    const uint64_t lanes_below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
 
-    auto dist_bits = [&](int32_t i) -> uint32_t {
-        if (kCache) return dcache[i];
-        return __float_as_uint(sq_distance(ql, qa, qb, mL[i], mA[i], mB[i]));
-    };
-
-    if (kCache) {
-        for (int32_t i = tid; i < n; i += kTopkThreads)
-            dcache[i] = __float_as_uint(sq_distance(ql, qa, qb, mL[i], mA[i], mB[i]));
-    }
-    if (tid < 128) sh.sel[tid] = ~0ull;
-    __syncthreads();
-
-    if (n <= 128) {
-        if (tid < n) sh.sel[tid] = make_key(dist_bits(tid), tid);
-        __syncthreads();
-    } else {
-        uint64_t prefix = 0;
-        uint32_t need = (uint32_t)k;
-        int lo = 0;
-        for (int hi = 62; hi >= 0; hi -= kDigitBits) {
-            const int lo_w = hi - kDigitBits + 1 > 0 ? hi - kDigitBits + 1 : 0;
-            const int width = hi - lo_w + 1;
-            const uint32_t mask = (1u << width) - 1u;
-            for (int b = tid; b < kBins; b += kTopkThreads) sh.hist[b] = 0;
-            __syncthreads();
-            for (int32_t i = tid; i < n; i += kTopkThreads) {
-                const uint64_t key = make_key(dist_bits(i), i);
-                if ((key >> (hi + 1)) == prefix) atomicAdd(&sh.hist[(uint32_t)(key >> lo_w) & mask], 1u);
+    for (int32_t q = q0; q < q_end; ++q) {
+        const float ql = gload(s.query, 3 * (int64_t)q), qa = gload(s.query, 3 * (int64_t)q + 1),
+                    qb = gload(s.query, 3 * (int64_t)q + 2);
+        auto dist_bits = [&](int32_t i) {
+            return __float_as_uint(sq_distance(ql, qa, qb, gload(mL, i), gload(mA, i), gload(mB, i)));
+        };
+        // threshold from the previous cell's answers
+        uint64_t tau = ~0ull;
+        if (q > q0) {
+            uint64_t m = 0;
+            for (int32_t r = lane; r < k; r += 64) {
+                const int32_t i = prev[r];
+                const uint64_t kk = make_key(dist_bits(i), i);
+                m = kk > m ? kk : m;
             }
-            __syncthreads();
-            uint32_t local[kBinsPerThread];
-            uint32_t sum = 0;
-#pragma unroll
-            for (int b = 0; b < kBinsPerThread; ++b) {
-                local[b] = sh.hist[tid * kBinsPerThread + b];
-                sum += local[b];
-            }
-            const uint32_t before = block_exclusive_scan(sum, sh.wsum);
-            if (before < need && need <= before + sum) {
-                uint32_t c = before;
-#pragma unroll
-                for (int b = 0; b < kBinsPerThread; ++b) {
-                    if (need > c && need <= c + local[b]) {
-                        sh.digit = tid * kBinsPerThread + b;
-                        sh.need = need - c;
-                        sh.cnt = local[b];
-                    }
-                    c += local[b];
-                }
-            }
-            __syncthreads();
-            prefix = (prefix << width) | sh.digit;
-            need = sh.need;
-            lo = lo_w;
-            const bool done = sh.cnt == need;
-            __syncthreads();
-            if (done) break;
+            tau = wave_max_u64(m);
         }
-        if (tid == 0) sh.fill = 0;
-        __syncthreads();
-        for (int32_t i = tid; i < n; i += kTopkThreads) {
-            const uint64_t key = make_key(dist_bits(i), i);
-            if ((key >> lo) <= prefix) {
-                const uint32_t slot = atomicAdd(&sh.fill, 1u);
-                if (slot < 128) sh.sel[slot] = key;
+        int32_t count = 0;
+        // appends the keys <= tau of one 64-material step to the buffer,
+        // first cutting the buffer to its k smallest when it would overflow
+        auto take = [&](uint64_t key, bool valid) {
+            bool pass = valid && key <= tau;
+            uint64_t mask = __ballot(pass);
+            if (mask == 0) return;
+            if (count + __popcll(mask) > kBuf) {
+                sort_buffer(buf, count);
+                count = k;
+                tau = buf[k - 1];
+                pass = pass && key <= tau;
+                mask = __ballot(pass);
+            }
+            if (pass) buf[count + __popcll(mask & lanes_below)] = key;
+            count += __popcll(mask);
+        };
+        // two steps per iteration; loads are clamped in bounds and issued
+        // together, the keys of lanes past n are masked out
+        for (int32_t base = 0; base < n; base += 128) {
+            const int32_t i0 = base + lane, i1 = i0 + 64;
+            const uint32_t d0 = dist_bits(i0 < n ? i0 : n - 1);
+            const uint32_t d1 = dist_bits(i1 < n ? i1 : n - 1);
+            take(make_key(d0, i0), i0 < n);
+            take(make_key(d1, i1), i1 < n);
+        }
+        sort_buffer(buf, count);
+        // answer: the first k keys, ascending
+        const int32_t out_stride = s.out_stride;
+        for (int32_t r = lane; r < out_stride; r += 64) {
+            const int64_t o = (int64_t)q * out_stride + r;
+            if (r < k) {
+                const uint64_t key = buf[r];
+                const int32_t m = (int32_t)(uint32_t)(key & 0xffffffffu);
+                const float d = __uint_as_float((uint32_t)(key >> 32));
+                prev[r] = m;
+                gstore(s.out_idx, o, m);
+                if (s.out_d) gstore(s.out_d, o, d);
+                if (s.out_lab)
+                    s.out_lab[o] = make_float4(gload(mL, m), gload(mA, m), gload(mB, m), d);
+            } else if (s.out_lab) {
+                gstore(s.out_idx, o, (int32_t)-1);  // padded (chain) rows: ranks beyond k
             }
         }
         __syncthreads();
-    }
-
-    // bitonic sort of 128 keys, ascending
-    for (int size = 2; size <= 128; size <<= 1) {
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            if (tid < 64) {
-                const int i = 2 * stride * (tid / stride) + (tid % stride);
-                const int j = i + stride;
-                const bool asc = (i & size) == 0;
-                const uint64_t a = sh.sel[i], b = sh.sel[j];
-                if ((a > b) == asc) {
-                    sh.sel[i] = b;
-                    sh.sel[j] = a;
-                }
-            }
-            __syncthreads();
-        }
-    }
-    const int32_t kk = k < n ? k : n;
-    const int32_t out_stride = s.out_stride;
-    // ranks beyond k hold the -1 sentinel in padded (chain) rows
-    if (s.out_lab && tid >= kk && tid < out_stride) s.out_idx[(int64_t)q * out_stride + tid] = -1;
-    if (tid < kk) {
-        const uint64_t key = sh.sel[tid];
-        const int32_t m = (int32_t)(uint32_t)(key & 0xffffffffu);
-        const float d = __uint_as_float((uint32_t)(key >> 32));
-        const int64_t o = (int64_t)q * out_stride + tid;
-        s.out_idx[o] = m;
-        if (s.out_d) s.out_d[o] = d;
-        if (s.out_lab) s.out_lab[o] = make_float4(mL[m], mA[m], mB[m], d);
     }
 }
 
-template <bool kCache>
-__global__ __launch_bounds__(kTopkThreads) void topk_kernel(TopkSeg s) {
-    extern __shared__ uint32_t dcache[];
-    __shared__ TopkShared sh;
-    topk_block<kCache>(s, blockIdx.x, sh, dcache);
+__global__ __launch_bounds__(64) void topk_kernel(TopkSeg s, int32_t run) {
+    __shared__ uint64_t buf[kBuf];
+    __shared__ int32_t prev[kMaxK];
+    topk_run(s, blockIdx.x * run, run, buf, prev);
 }
 
-template <bool kCache>
-__global__ __launch_bounds__(kTopkThreads) void topk_batch_kernel(const TopkSeg* __restrict__ segs,
-                                                                  const int32_t* __restrict__ starts,
-                                                                  int32_t n_segs) {
-    extern __shared__ uint32_t dcache[];
-    __shared__ TopkShared sh;
+__global__ __launch_bounds__(64) void topk_batch_kernel(const TopkSeg* __restrict__ segs,
+                                                        const int32_t* __restrict__ starts,
+                                                        int32_t n_segs, int32_t run) {
+    __shared__ uint64_t buf[kBuf];
+    __shared__ int32_t prev[kMaxK];
     const int32_t j = find_segment(starts, n_segs, blockIdx.x);
-    topk_block<kCache>(segs[j], blockIdx.x - starts[j], sh, dcache);
+    topk_run(segs[j], (blockIdx.x - starts[j]) * run, run, buf, prev);
 }
+
+// Query cells per wave: runs reuse the previous answer as a bound, but a
+// launch wants enough waves to fill the chip (~8 per CU).
+int32_t topk_run_length(int64_t total_queries) {
+    int64_t r = total_queries / (256 * 8);
+    return (int32_t)(r < 1 ? 1 : (r > kMaxRun ? kMaxRun : r));
+}
+
+int32_t topk_blocks(int32_t nq, int32_t run) { return nq > 0 ? (nq + run - 1) / run : 0; }
 
 void launch_topk(const float* mL, const float* mA, const float* mB, int32_t n, const float* query,
                  int32_t nq, int32_t k, int32_t* out_idx, float* out_d, float4* out_lab,
@@ -190,27 +189,15 @@ void launch_topk(const float* mL, const float* mA, const float* mB, int32_t n, c
     if (k > n) k = n;
     if (nq <= 0 || n <= 0 || k <= 0) return;
     const TopkSeg s{mL, mA, mB, query, out_idx, out_d, out_lab, n, nq, k, out_stride};
-    if (n <= kDistCacheMax) {
-        const size_t shmem = sizeof(uint32_t) * (size_t)n;
-        hipLaunchKernelGGL(topk_kernel<true>, dim3(nq), dim3(kTopkThreads), shmem, stream, s);
-    } else {
-        hipLaunchKernelGGL(topk_kernel<false>, dim3(nq), dim3(kTopkThreads), 0, stream, s);
-    }
+    const int32_t run = topk_run_length(nq);
+    hipLaunchKernelGGL(topk_kernel, dim3(topk_blocks(nq, run)), dim3(64), 0, stream, s, run);
 }
 
-// Segments with n > 0 only (a block per query); max_n decides the distance
-// cache for the whole launch.
 void launch_topk_batch(const TopkSeg* d_segs, const int32_t* d_starts, int32_t n_segs,
-                       int32_t total_queries, int32_t max_n, hipStream_t stream) {
-    if (n_segs <= 0 || total_queries <= 0) return;
-    if (max_n <= kDistCacheMax) {
-        const size_t shmem = sizeof(uint32_t) * (size_t)max_n;
-        hipLaunchKernelGGL(topk_batch_kernel<true>, dim3(total_queries), dim3(kTopkThreads), shmem,
-                           stream, d_segs, d_starts, n_segs);
-    } else {
-        hipLaunchKernelGGL(topk_batch_kernel<false>, dim3(total_queries), dim3(kTopkThreads), 0,
-                           stream, d_segs, d_starts, n_segs);
-    }
+                       int32_t total_blocks, int32_t run, hipStream_t stream) {
+    if (n_segs <= 0 || total_blocks <= 0) return;
+    hipLaunchKernelGGL(topk_batch_kernel, dim3(total_blocks), dim3(64), 0, stream, d_segs, d_starts,
+                       n_segs, run);
 }
 
 hipError_t raise_dynamic_lds(const void* kernel, int max_lds) {
@@ -222,15 +209,8 @@ hipError_t raise_dynamic_lds(const void* kernel, int max_lds) {
     return hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
 }
 
-hipError_t topk_configure(int max_lds) {
-    hipError_t e = raise_dynamic_lds(reinterpret_cast<const void*>(&topk_kernel<true>), max_lds);
-    if (e != hipSuccess) return e;
-    return raise_dynamic_lds(reinterpret_cast<const void*>(&topk_batch_kernel<true>), max_lds);
-}
-
 hipError_t configure_kernel_limits(int max_lds) {
-    hipError_t e = topk_configure(max_lds);
-    if (e == hipSuccess) e = place_configure(max_lds);
+    hipError_t e = place_configure(max_lds);
     if (e == hipSuccess) e = anneal_configure(max_lds);
     if (e == hipSuccess) e = resize_configure(max_lds);
     return e;

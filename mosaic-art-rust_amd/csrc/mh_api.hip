@@ -587,6 +587,19 @@ int mh_copy_on_device(mh_context* ctx, void* dst, const void* src, size_t bytes)
     return MH_OK;
 }
 
+int mh_copy_2d_on_device(mh_context* ctx, void* dst, size_t dst_pitch, const void* src,
+                         size_t src_pitch, size_t row_bytes, size_t rows) {
+    if (!ctx || (row_bytes && rows && (!dst || !src)))
+        return fail(MH_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (row_bytes > dst_pitch || row_bytes > src_pitch)
+        return fail(MH_ERR_INVALID_ARGUMENT, "row_bytes exceeds a pitch");
+    if (row_bytes == 0 || rows == 0) return MH_OK;
+    CHK(set_device(ctx));
+    HIPCHK(hipMemcpy2DAsync(dst, dst_pitch, src, src_pitch, row_bytes, rows, hipMemcpyDeviceToDevice,
+                            ctx->stream));
+    return MH_OK;
+}
+
 int mh_copy_to_host(mh_context* ctx, void* dst, const void* src, size_t bytes) {
     if (!ctx || (bytes && (!dst || !src))) return fail(MH_ERR_INVALID_ARGUMENT, "NULL argument");
     CHK(set_device(ctx));

@@ -131,7 +131,7 @@ struct BatchPlan {
     size_t o_avg = 0, o_avg_st = 0, o_comp = 0, o_adj_st = 0, o_comp_st = 0;
     size_t o_stats = 0, o_aout = 0, bytes = 0;
     int32_t lab_blocks = 0;
-    int32_t n_topk = 0, topk_blocks = 0, max_n = 0;
+    int32_t n_topk = 0, topk_blocks = 0, topk_run = 1;
     size_t place_lds = 0;
     int32_t n_anneal = 0;
     size_t anneal_lds = 0;
@@ -168,6 +168,8 @@ size_t plan_bytes(int32_t n, BatchPlan* p) {
     return o;
 }
 
+int32_t topk_blocks_of(int32_t cells, int32_t run) { return mh::topk_blocks(cells, run); }
+
 template <typename T>
 T* at(uint8_t* base, size_t off) {
     return reinterpret_cast<T*>(base + off);
@@ -197,7 +199,10 @@ int build_plan(const std::vector<JobSpec>& specs, const std::vector<uint8_t*>& j
 
     int64_t lab_blocks = 0, topk_blocks = 0, avg_blocks = 0, adj_blocks = 0, comp_blocks = 0;
     p->n_topk = p->n_anneal = p->n_avg = p->n_comp = 0;
-    p->max_n = 0;
+    int64_t topk_queries = 0;
+    for (const JobSpec& s : specs)
+        if (s.n > 0) topk_queries += s.cells;
+    p->topk_run = topk_run_length(topk_queries);
     p->place_lds = 0;
     p->anneal_lds = 0;
     p->narrow = true;
@@ -224,8 +229,7 @@ int build_plan(const std::vector<JobSpec>& specs, const std::vector<uint8_t*>& j
                                       cand_lab, s.n, s.cells, s.n < kMaxCand ? s.n : kMaxCand,
                                       kCandStride};
             ++p->n_topk;
-            topk_blocks += s.cells;
-            if (s.n > p->max_n) p->max_n = s.n;
+            topk_blocks += topk_blocks_of(s.cells, p->topk_run);
         }
         // a4-a9: placement chain
         PlaceJob pj{};
@@ -308,7 +312,7 @@ void enqueue_plan(const BatchPlan& p, uint8_t* d, const float* lut, hipStream_t 
     launch_cell_lab_batch(at<CellSeg>(d, p.o_lab), at<int32_t>(d, p.o_lab_st), p.n_jobs, p.lab_blocks,
                           lut, s);
     launch_topk_batch(at<TopkSeg>(d, p.o_topk), at<int32_t>(d, p.o_topk_st), p.n_topk, p.topk_blocks,
-                      p.max_n, s);
+                      p.topk_run, s);
     launch_place_batch(at<PlaceJob>(d, p.o_place), p.n_jobs, true, p.place_lds, s);
     if (p.n_anneal > 0)
         launch_anneal_batch(at<AnnealJob>(d, p.o_anneal), p.n_anneal, p.narrow, p.anneal_lds, s);

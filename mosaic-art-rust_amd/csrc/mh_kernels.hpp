@@ -113,9 +113,11 @@ void launch_topk(const float* mL, const float* mA, const float* mB, int32_t n, c
 int32_t cell_lab_blocks(int32_t n_cells);
 void launch_cell_lab_batch(const CellSeg* d_segs, const int32_t* d_starts, int32_t n_segs,
                            int32_t total_blocks, const float* lut, hipStream_t stream);
-// a block per query; max_n decides the distance cache
+// one wave per run of `run` consecutive queries of a segment
+int32_t topk_run_length(int64_t total_queries);
+int32_t topk_blocks(int32_t nq, int32_t run);
 void launch_topk_batch(const TopkSeg* d_segs, const int32_t* d_starts, int32_t n_segs,
-                       int32_t total_queries, int32_t max_n, hipStream_t stream);
+                       int32_t total_blocks, int32_t run, hipStream_t stream);
 // a block per cell
 void launch_avg_rgb_batch(const CellSeg* d_segs, const int32_t* d_starts, int32_t n_segs,
                           int32_t total_cells, hipStream_t stream);
@@ -170,7 +172,6 @@ hipError_t raise_dynamic_lds(const void* kernel, int max_lds);
 hipError_t configure_kernel_limits(int max_lds);
 hipError_t place_configure(int max_lds);
 hipError_t anneal_configure(int max_lds);
-hipError_t topk_configure(int max_lds);
 hipError_t resize_configure(int max_lds);
 
 }  // namespace mh

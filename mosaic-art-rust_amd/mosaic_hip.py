@@ -27,7 +27,8 @@ EXPORTS = [
     "mh_last_error", "mh_abi_version", "mh_context_create", "mh_context_destroy",
     "mh_context_synchronize", "mh_context_stream", "mh_context_set_timing",
     "mh_context_timing", "mh_phase_name", "mh_device_alloc", "mh_device_free",
-    "mh_copy_to_device", "mh_copy_to_host", "mh_copy_on_device", "mh_library_from_lab",
+    "mh_copy_to_device", "mh_copy_to_host", "mh_copy_on_device", "mh_copy_2d_on_device",
+    "mh_library_from_lab",
     "mh_library_from_pixels", "mh_library_destroy", "mh_library_size", "mh_library_lab",
     "mh_library_build_similarities", "mh_library_similarities", "mh_library_get_similarity",
     "mh_average_lab", "mh_cell_lab_means", "mh_nearest_n", "mh_place", "mh_total_cost",
@@ -145,6 +146,8 @@ def load(path=LIB_PATH):
         "mh_copy_to_device": (ctypes.c_int, [P, P, P, ctypes.c_size_t]),
         "mh_copy_to_host": (ctypes.c_int, [P, P, P, ctypes.c_size_t]),
         "mh_copy_on_device": (ctypes.c_int, [P, P, P, ctypes.c_size_t]),
+        "mh_copy_2d_on_device": (ctypes.c_int, [P, P, ctypes.c_size_t, P, ctypes.c_size_t,
+                                                ctypes.c_size_t, ctypes.c_size_t]),
         "mh_library_from_lab": (ctypes.c_int, [P, P, i32, ctypes.POINTER(P)]),
         "mh_library_from_pixels": (ctypes.c_int, [P, P, i32, i32, i32, ctypes.POINTER(P)]),
         "mh_library_destroy": (None, [P]),
@@ -271,6 +274,10 @@ class Context:
 
     def copy_on_device(self, dst, src, nbytes):
         _check(self.lib.mh_copy_on_device(self.h, ctypes.c_void_p(dst), ctypes.c_void_p(src), nbytes))
+
+    def copy_2d_on_device(self, dst, dst_pitch, src, src_pitch, row_bytes, rows):
+        _check(self.lib.mh_copy_2d_on_device(self.h, ctypes.c_void_p(dst), dst_pitch,
+                                             ctypes.c_void_p(src), src_pitch, row_bytes, rows))
 
     def to_host(self, ptr, shape, dtype):
         out = np.empty(shape, dtype)

@@ -36,6 +36,21 @@ def _u8(a):
     return np.clip(np.rint(a), 0, 255).astype(np.uint8)
 
 
+_NOISE_TABLES = {}
+
+
+def _noise_table(sigma):
+    """rint(N(0, sigma)) at the 65,536 mid-quantiles: indexing it with uniform
+    16-bit draws samples the rounded normal (inverse-CDF sampling)."""
+    t = _NOISE_TABLES.get(sigma)
+    if t is None:
+        from scipy.special import ndtri
+        u = (np.arange(65536, dtype=np.float64) + 0.5) / 65536.0
+        t = np.rint(ndtri(u) * sigma).astype(np.int16)
+        _NOISE_TABLES[sigma] = t
+    return t
+
+
 def ramp_target(width, height, rng):
     x = np.arange(width, dtype=np.float32)
     y = np.arange(height, dtype=np.float32)
@@ -71,15 +86,22 @@ def blob_target(width, height, n_blobs, rng, sigma=150.0, noise=8.0):
 
 
 def flat_materials(n, size, rng, colours=None, noise=12.0):
+    """n flat tiles: colour (rounded to integers) + rint(N(0, noise)) per
+    channel and pixel, clipped to [0, 255].  Since the colour is integral,
+    this equals rint(colour + N(0, noise)) in distribution."""
     if colours is None:
         colours = rng.integers(0, 256, (n, 3)).astype(np.float32)
+    col = np.rint(np.asarray(colours, np.float64)).astype(np.int16)
+    table = _noise_table(float(noise))
     out = np.empty((n, size, size, 3), np.uint8)
     chunk = max(1, (1 << 24) // (size * size * 3))
     for s in range(0, n, chunk):
         e = min(n, s + chunk)
-        v = colours[s:e, None, None, :] + rng.standard_normal((e - s, size, size, 3),
-                                                                dtype=np.float32) * noise
-        out[s:e] = _u8(v)
+        idx = rng.integers(0, 65536, (e - s, size, size, 3), dtype=np.uint16)
+        v = table[idx]
+        v += col[s:e, None, None, :]
+        np.clip(v, 0, 255, out=v)
+        out[s:e] = v
     return out
 
 
@@ -107,21 +129,36 @@ def make(name, seed=0):
     return p, target, mats
 
 
+def _parallel(fn, args):
+    """fn(*a) for each a on a thread pool (numpy releases the GIL); results in
+    order, so the output does not depend on the thread count."""
+    import concurrent.futures
+    import os
+    workers = max(1, min(16, os.cpu_count() or 1, len(args)))
+    with concurrent.futures.ThreadPoolExecutor(workers) as ex:
+        return list(ex.map(lambda a: fn(*a), args))
+
+
 def make_batch(n_jobs=64, seed=0, cell=16, msize=64):
-    """Independent jobs of the batched form: list of (params, target, materials)."""
+    """Independent jobs of the batched form: list of (params, target, materials).
+    Job j's pixels come from its own child stream of SeedSequence(seed)."""
     rng = np.random.default_rng(seed)
-    jobs = []
+    children = np.random.SeedSequence(seed).spawn(n_jobs)
+    specs = []
     for _ in range(n_jobs):
         gw = int(rng.integers(20, 161))
         gh = int(rng.integers(15, 121))
         n = int(rng.integers(256, 4097))
         usage = int(rng.integers(3, 6))
-        p = dict(width=gw * cell, height=gh * cell, n=n, msize=msize, grid_w=gw, grid_h=gh,
-                 max_usage=usage, weight=0.25, color=0.4, iters=1000)
-        target = blob_target(p["width"], p["height"], 8, rng, sigma=max(8.0, p["width"] / 12.0))
-        mats = flat_materials(n, msize, rng)
-        jobs.append((p, target, mats))
-    return jobs
+        specs.append(dict(width=gw * cell, height=gh * cell, n=n, msize=msize, grid_w=gw, grid_h=gh,
+                          max_usage=usage, weight=0.25, color=0.4, iters=1000))
+
+    def one(p, ss):
+        r = np.random.default_rng(ss)
+        target = blob_target(p["width"], p["height"], 8, r, sigma=max(8.0, p["width"] / 12.0))
+        return p, target, flat_materials(p["n"], msize, r)
+
+    return _parallel(one, list(zip(specs, children)))
 
 
 def make_stream(n_jobs=5000, rate=2000.0, seed=0, libraries=(1000, 2500, 5000, 10000),
@@ -135,9 +172,16 @@ def make_stream(n_jobs=5000, rate=2000.0, seed=0, libraries=(1000, 2500, 5000, 1
                 iters, seed
     Arrival times are the partial sums of exponential gaps of mean 1/rate."""
     rng = np.random.default_rng(seed)
-    libs = [flat_materials(n, msize, rng, colours=mixture_colours(n, rng)) for n in libraries]
+    kids = np.random.SeedSequence(seed).spawn(len(libraries) + n_canvases)
     cw, ch = canvas
-    canvases = [blob_target(cw, ch, 32, rng) for _ in range(n_canvases)]
+
+    def lib(n, ss):
+        r = np.random.default_rng(ss)
+        return flat_materials(n, msize, r, colours=mixture_colours(n, r))
+
+    libs = _parallel(lib, list(zip(libraries, kids[:len(libraries)])))
+    canvases = _parallel(lambda ss: blob_target(cw, ch, 32, np.random.default_rng(ss)),
+                         [(k,) for k in kids[len(libraries):]])
     lo, hi = np.log(cells[0]), np.log(cells[1])
     t = 0.0
     jobs = []
