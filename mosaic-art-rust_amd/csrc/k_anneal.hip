@@ -155,7 +155,7 @@ template <typename G>
 __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
                                                     const AnnealJob* __restrict__ jobs) {
     extern __shared__ __align__(16) uint32_t sm[];
-    const AnnealJob job = jobs ? jobs[blockIdx.x] : single;
+    const AnnealJob job = uniform_copy(jobs ? jobs[blockIdx.x] : single);
     const int lane = threadIdx.x;
     const int32_t gw = job.grid_w, gh = job.grid_h;
     const int32_t cells = gw * gh;
@@ -194,7 +194,7 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
             skip = (x1 == x2 && y1 == y2) || gget(grid_s, y1 * gw + x1) == kEmpty ||
                    gget(grid_s, y2 * gw + x2) == kEmpty;
         }
-        const uint64_t nonskip = __ballot(!skip);
+        const uint64_t nonskip = wave_ballot(!skip);
         float delta = 0.0f;
         bool acc = false;
         if (!skip) {
@@ -235,12 +235,12 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
                 if ((v >> 6) == tag && (int)(63 - (v & 63u)) < lane) conflict = true;
             }
         }
-        const uint64_t cmask = __ballot(conflict);
+        const uint64_t cmask = wave_ballot(conflict);
         int32_t first = cmask ? (__ffsll((unsigned long long)cmask) - 1) : 64;
         const int64_t remain = max_iter - s;
         if ((int64_t)first > remain) first = (int32_t)remain;
         const uint64_t commit = first >= 64 ? ~0ull : ((1ull << first) - 1ull);
-        const uint64_t accmask = __ballot(acc) & commit;
+        const uint64_t accmask = wave_ballot(acc) & commit;
         __syncthreads();
         if (lane < first) {
             if (acc) {

@@ -34,7 +34,7 @@ namespace mh {
 __global__ __launch_bounds__(64) void place_chain_kernel(PlaceJob single,
                                                          const PlaceJob* __restrict__ jobs) {
     extern __shared__ int32_t sm[];
-    const PlaceJob job = jobs ? jobs[blockIdx.x] : single;
+    const PlaceJob job = uniform_copy(jobs ? jobs[blockIdx.x] : single);
     const int lane = threadIdx.x;
     const int32_t gw = job.grid_w, gh = job.grid_h, n = job.n;
     const int32_t cells = gw * gh;
@@ -112,8 +112,8 @@ __global__ __launch_bounds__(64) void place_chain_kernel(PlaceJob single,
             // candidate passing the neighbour ban, else nearest_one.
             for (int32_t i = lane; i < n; i += 64) usage_s[i] = 0;
             ++st_resets;
-            const uint64_t b0 = __ballot(placeable[0]);
-            const uint64_t b1 = __ballot(placeable[1]);
+            const uint64_t b0 = wave_ballot(placeable[0]);
+            const uint64_t b1 = wave_ballot(placeable[1]);
             if (b0) {
                 rank = __ffsll((unsigned long long)b0) - 1;
                 stage = 1;
@@ -186,19 +186,28 @@ struct RasterState {
     int32_t x, y;          // current cell position
     int32_t left_m;        // previous winner (valid when x > 0)
     float4 left_lab;
-    uint32_t u0, u1;       // usage counts of the current cell's candidates
+    uint32_t u0;           // usage counts of the current cell's rank 0-63 candidates
     int32_t up_m;          // material above the current cell (valid when y > 0)
     float4 up_lab;
     uint32_t win_buf;      // winners of the current 64-cell block, one per lane
     int32_t st_main, st_ban, st_near, st_resets;
 };
 
-// One cell of the raster chain.  The next cell's usage counts and up
-// neighbour are read from LDS before this cell commits (so their latency
-// overlaps the decision) and then corrected in registers for this cell's
-// winner: +1 for its candidate that equals the winner, or counts of 0/1
-// after a usage reset; when the grid is one column wide the next cell's up
-// neighbour is this cell's winner.
+// One cell of the raster chain.  The next cell's usage counts (ranks 0-63)
+// and up neighbour are read from LDS before this cell commits (so their
+// latency overlaps the decision) and then corrected in registers for this
+// cell's winner: +1 for its candidate that equals the winner, or counts of
+// 0/1 after a usage reset; when the grid is one column wide the next cell's
+// up neighbour is this cell's winner.
+//
+// Fast path (ranks 0-63 only): candidates are in ascending (sqdist, idx)
+// order and the penalty is at most fl(2w) (two terms <= 1, times w), so with
+// f the first usable candidate, score_f <= fl(d_f + 2w) =: bound and any
+// candidate with d > bound scores strictly worse than f.  When f is the
+// only usable rank-0..63 candidate with d <= bound and rank 64 (the nearest
+// of ranks 64-127) is absent or beyond the bound, f wins without any
+// penalty.  Otherwise the exact full evaluation runs over both halves, with
+// the usage counts of ranks 64-127 read from LDS then.
 template <bool kOutputs>
 __device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn, int32_t c,
                                             RasterState& st, const PlaceJob& job,
@@ -211,7 +220,6 @@ __device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn,
     const int32_t yn = x + 1 == gw ? y + 1 : y;
     // pre-read for the next cell (sees every commit up to the previous cell)
     const uint32_t nu0 = usage_s[sn.m0 >= 0 ? sn.m0 : 0];
-    const uint32_t nu1 = usage_s[sn.m1 >= 0 ? sn.m1 : 0];
     int32_t nup_m = kEmpty;
     float4 nup_lab = make_float4(0, 0, 0, 0);
     if (yn > 0) {
@@ -222,76 +230,94 @@ __device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn,
     const int32_t up_m = y > 0 ? st.up_m : kEmpty;
     const int32_t left_m = x > 0 ? st.left_m : kEmpty;
     const bool pl0 = s.m0 >= 0 && s.m0 != up_m && s.m0 != left_m;
-    const bool pl1 = s.m1 >= 0 && s.m1 != up_m && s.m1 != left_m;
     const uint32_t max_usage = (uint32_t)job.max_usage;
     const bool ok0 = pl0 && st.u0 < max_usage;
-    const bool ok1 = pl1 && st.u1 < max_usage;
-    const uint64_t ob0 = __ballot(ok0), ob1 = __ballot(ok1);
+    const uint64_t ob0 = wave_ballot(ok0);
 
-    int32_t rank, stage;
-    uint32_t win_score = 0;
-    if (ob0 | ob1) {
-        // First usable candidate f (ranks are in ascending (sqdist, idx) order).
-        // The penalty is at most fl(2w) (two terms <= 1, times w), so
-        // score_f <= fl(d_f + 2w) =: bound, and a candidate with d > bound
-        // scores strictly worse than f.  Only candidates with d <= bound
-        // ("contenders") need their penalty terms.
-        const int32_t f = ob0 ? __ffsll((unsigned long long)ob0) - 1
-                              : 64 + __ffsll((unsigned long long)ob1) - 1;
-        const float d_f = f < 64 ? readlane_f(s.c0.w, f) : readlane_f(s.c1.w, f & 63);
-        rank = f;
+    int32_t rank = 0, stage = 0;
+    uint32_t win_score = 0, u1 = 0;
+    bool slow = true;
+    if (ob0) {
+        const int32_t f = __ffsll((unsigned long long)ob0) - 1;
+        const float d_f = readlane_f(s.c0.w, f);
         if (job.weight > 0.0f) {
             const float bound = d_f + 2.0f * job.weight;
-            const bool ct0 = ok0 && s.c0.w <= bound;
-            const bool ct1 = ok1 && s.c1.w <= bound;
-            const uint64_t cb0 = __ballot(ct0), cb1 = __ballot(ct1);
-            if (__popcll(cb0) + __popcll(cb1) > 1 || (kOutputs && job.score)) {
-                // calculate_penalty for the contenders: Up then Left (Down and
-                // Right are still empty in raster order)
-                uint32_t k0 = kNone, k1 = kNone;
-                if (cb0) {
-                    float p = 0.0f;
-                    if (y > 0) p += adj_term(s.c0, st.up_lab);
-                    if (x > 0) p += adj_term(s.c0, st.left_lab);
-                    const float sc = s.c0.w + p * job.weight;
-                    k0 = ct0 ? __float_as_uint(sc) : kNone;
-                }
-                if (cb1) {
-                    float p = 0.0f;
-                    if (y > 0) p += adj_term(s.c1, st.up_lab);
-                    if (x > 0) p += adj_term(s.c1, st.left_lab);
-                    const float sc = s.c1.w + p * job.weight;
-                    k1 = ct1 ? __float_as_uint(sc) : kNone;
-                }
-                const uint32_t kmin = wave_min_u32(k0 < k1 ? k0 : k1);
-                const uint64_t b0 = __ballot(k0 == kmin);
-                rank = b0 ? __ffsll((unsigned long long)b0) - 1
-                          : 64 + __ffsll((unsigned long long)__ballot(k1 == kmin)) - 1;
-                win_score = kmin;
+            const uint64_t far_half = wave_ballot(s.m1 >= 0 && s.c1.w <= bound);
+            const uint64_t cb0 = wave_ballot(ok0 && s.c0.w <= bound);
+            if (far_half == 0 && __popcll(cb0) == 1 && !(kOutputs && job.score)) {
+                rank = f;
+                slow = false;
             }
         } else {
+            rank = f;
             win_score = __float_as_uint(d_f);  // penalty 0: scores are the distances
+            slow = false;
         }
-        stage = 0;
-        ++st.st_main;
-    } else {
-        // fallback_tile_selection: reset every count, first placeable, else nearest_one
-        for (int32_t i = lane; i < n; i += 64) usage_s[i] = 0u;
-        ++st.st_resets;
-        const uint64_t b0 = __ballot(pl0);
-        const uint64_t b1 = __ballot(pl1);
-        if (b0) {
-            rank = __ffsll((unsigned long long)b0) - 1;
-            stage = 1;
-            ++st.st_ban;
-        } else if (b1) {
-            rank = 64 + __ffsll((unsigned long long)b1) - 1;
-            stage = 1;
-            ++st.st_ban;
+        if (!slow) ++st.st_main;
+    }
+    if (slow) {
+        u1 = usage_s[s.m1 >= 0 ? s.m1 : 0];
+        const bool pl1 = s.m1 >= 0 && s.m1 != up_m && s.m1 != left_m;
+        const bool ok1 = pl1 && u1 < max_usage;
+        const uint64_t ob1 = wave_ballot(ok1);
+        if (ob0 | ob1) {
+            const int32_t f = ob0 ? __ffsll((unsigned long long)ob0) - 1
+                                  : 64 + __ffsll((unsigned long long)ob1) - 1;
+            const float d_f = f < 64 ? readlane_f(s.c0.w, f) : readlane_f(s.c1.w, f & 63);
+            rank = f;
+            if (job.weight > 0.0f) {
+                const float bound = d_f + 2.0f * job.weight;
+                const bool ct0 = ok0 && s.c0.w <= bound;
+                const bool ct1 = ok1 && s.c1.w <= bound;
+                const uint64_t cb0 = wave_ballot(ct0), cb1 = wave_ballot(ct1);
+                if (__popcll(cb0) + __popcll(cb1) > 1 || (kOutputs && job.score)) {
+                    // calculate_penalty for the contenders: Up then Left (Down
+                    // and Right are still empty in raster order)
+                    uint32_t k0 = kNone, k1 = kNone;
+                    if (cb0) {
+                        float p = 0.0f;
+                        if (y > 0) p += adj_term(s.c0, st.up_lab);
+                        if (x > 0) p += adj_term(s.c0, st.left_lab);
+                        const float sc = s.c0.w + p * job.weight;
+                        k0 = ct0 ? __float_as_uint(sc) : kNone;
+                    }
+                    if (cb1) {
+                        float p = 0.0f;
+                        if (y > 0) p += adj_term(s.c1, st.up_lab);
+                        if (x > 0) p += adj_term(s.c1, st.left_lab);
+                        const float sc = s.c1.w + p * job.weight;
+                        k1 = ct1 ? __float_as_uint(sc) : kNone;
+                    }
+                    const uint32_t kmin = wave_min_u32(k0 < k1 ? k0 : k1);
+                    const uint64_t b0 = wave_ballot(k0 == kmin);
+                    rank = b0 ? __ffsll((unsigned long long)b0) - 1
+                              : 64 + __ffsll((unsigned long long)wave_ballot(k1 == kmin)) - 1;
+                    win_score = kmin;
+                }
+            } else {
+                win_score = __float_as_uint(d_f);
+            }
+            stage = 0;
+            ++st.st_main;
         } else {
-            rank = 0;
-            stage = 2;
-            ++st.st_near;
+            // fallback_tile_selection: reset every count, first placeable, else nearest_one
+            for (int32_t i = lane; i < n; i += 64) usage_s[i] = 0u;
+            ++st.st_resets;
+            const uint64_t b0 = wave_ballot(pl0);
+            const uint64_t b1 = wave_ballot(pl1);
+            if (b0) {
+                rank = __ffsll((unsigned long long)b0) - 1;
+                stage = 1;
+                ++st.st_ban;
+            } else if (b1) {
+                rank = 64 + __ffsll((unsigned long long)b1) - 1;
+                stage = 1;
+                ++st.st_ban;
+            } else {
+                rank = 0;
+                stage = 2;
+                ++st.st_near;
+            }
         }
     }
     const int wl = rank & 63;
@@ -305,7 +331,7 @@ __device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn,
     } else {
         w = __builtin_amdgcn_readlane(s.m1, wl);
         wlab = readlane4(s.c1, wl);
-        if (stage == 0) cnt = (uint32_t)__builtin_amdgcn_readlane((int)st.u1, wl);
+        if (stage == 0) cnt = (uint32_t)__builtin_amdgcn_readlane((int)u1, wl);
     }
     if (lane == 0) {
         usage_s[w] = cnt + 1u;
@@ -321,9 +347,8 @@ __device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn,
     if (lane == (c & 63)) st.win_buf = (uint32_t)w;
     if ((c & 63) == 63) gstore(job.grid, c - 63 + lane, (int32_t)st.win_buf);
     // carry the next cell's state, corrected for this commit
-    const uint32_t h0 = sn.m0 == w ? 1u : 0u, h1 = sn.m1 == w ? 1u : 0u;
+    const uint32_t h0 = sn.m0 == w ? 1u : 0u;
     st.u0 = stage == 0 ? nu0 + h0 : h0;
-    st.u1 = stage == 0 ? nu1 + h1 : h1;
     st.up_m = gw == 1 ? w : nup_m;
     st.up_lab = gw == 1 ? wlab : nup_lab;
     st.left_m = w;
@@ -336,7 +361,7 @@ template <bool kOutputs>
 __global__ __launch_bounds__(64) void place_raster_kernel(PlaceJob single,
                                                           const PlaceJob* __restrict__ jobs) {
     extern __shared__ __align__(16) uint32_t smr[];
-    const PlaceJob job = jobs ? jobs[blockIdx.x] : single;
+    const PlaceJob job = uniform_copy(jobs ? jobs[blockIdx.x] : single);
     const int lane = threadIdx.x;
     const int32_t gw = job.grid_w, n = job.n;
     const int32_t cells = gw * job.grid_h;
@@ -374,7 +399,6 @@ __global__ __launch_bounds__(64) void place_raster_kernel(PlaceJob single,
         CandSet h;
         load_cands(h, job, 0, cells, lane);
         st.u0 = usage_s[h.m0 >= 0 ? h.m0 : 0];
-        st.u1 = usage_s[h.m1 >= 0 ? h.m1 : 0];
     }
     int32_t c = 0;
     for (; c + kGroup <= cells; c += kGroup) {

@@ -116,14 +116,14 @@ __device__ __forceinline__ void topk_run(const TopkSeg& s, int32_t q0, int32_t r
         // first cutting the buffer to its k smallest when it would overflow
         auto take = [&](uint64_t key, bool valid) {
             bool pass = valid && key <= tau;
-            uint64_t mask = __ballot(pass);
+            uint64_t mask = wave_ballot(pass);
             if (mask == 0) return;
             if (count + __popcll(mask) > kBuf) {
                 sort_buffer(buf, count);
                 count = k;
                 tau = buf[k - 1];
                 pass = pass && key <= tau;
-                mask = __ballot(pass);
+                mask = wave_ballot(pass);
             }
             if (pass) buf[count + __popcll(mask & lanes_below)] = key;
             count += __popcll(mask);

@@ -36,6 +36,25 @@ __device__ __forceinline__ void gstore(T* p, int64_t i, T v) {
     ((MH_GAS T*)p)[i] = v;
 }
 
+// Lane mask of a predicate.  The builtin takes the compare's lane mask
+// directly (HIP's __ballot goes through an int select and a second compare).
+__device__ __forceinline__ uint64_t wave_ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
+// A job descriptor read from memory through a generic pointer lands in
+// VGPRs, and every branch on its fields then compiles as a divergent
+// (exec-mask) branch.  The descriptor is uniform across the wave: copying it
+// word by word through readfirstlane puts it in SGPRs.
+template <typename T>
+__device__ __forceinline__ T uniform_copy(const T& v) {
+    static_assert(sizeof(T) % 4 == 0, "descriptor size must be a multiple of 4");
+    T r;
+    const int32_t* src = reinterpret_cast<const int32_t*>(&v);
+    int32_t* dst = reinterpret_cast<int32_t*>(&r);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
+    return r;
+}
+
 // Segment of a batched launch that owns block b: the j in [0, n_segs) with
 // starts[j] <= b < starts[j+1] (starts ascending, starts[0] = 0, empty
 // segments allowed).  Uniform per block, so the loads stay scalar.
