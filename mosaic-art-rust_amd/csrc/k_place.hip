@@ -177,94 +177,107 @@ __device__ __forceinline__ void load_cands(CandSet& s, const PlaceJob& job, int3
     s.m1 = live ? s.m1 : -1;
 }
 
-__device__ __forceinline__ float4 readlane4(float4 v, int lane) {
-    return make_float4(readlane_f(v.x, lane), readlane_f(v.y, lane), readlane_f(v.z, lane),
-                       readlane_f(v.w, lane));
-}
+constexpr int32_t kWinRing = 64;  // winners awaiting their grid store (power of two)
+constexpr int32_t kNoMat = -2;    // "no previous winner" (never a material or the -1 sentinel)
 
 struct RasterState {
-    int32_t x, y;          // current cell position
-    int32_t left_m;        // previous winner (valid when x > 0)
-    float4 left_lab;
-    uint32_t u0;           // usage counts of the current cell's rank 0-63 candidates
-    int32_t up_m;          // material above the current cell (valid when y > 0)
-    float4 up_lab;
-    uint32_t win_buf;      // winners of the current 64-cell block, one per lane
-    int32_t st_main, st_ban, st_near, st_resets;
+    int32_t x, y;      // current cell position
+    int32_t left_m;    // previous winner (kEmpty at the start of a row)
+    int32_t up_m;      // material above the current cell (kEmpty in row 0)
+    uint32_t u0;       // usage counts of the current cell's rank 0-63 candidates
+    uint64_t ob0;      // rank 0-63 candidates of the current cell that are usable
+    // read one step ahead for the next cell: usage counts of its rank 0-63
+    // candidates and its up neighbour, both as of two commits ago
+    uint32_t pre_u;
+    int32_t pre_up;
+    int32_t prev_w;    // winner of the previous cell (kNoMat before the first)
+    int32_t prev_reset;  // the previous cell reset every usage count
+    int32_t st_ban, st_near, st_resets;
 };
 
-// One cell of the raster chain.  The next cell's usage counts (ranks 0-63)
-// and up neighbour are read from LDS before this cell commits (so their
-// latency overlaps the decision) and then corrected in registers for this
-// cell's winner: +1 for its candidate that equals the winner, or counts of
-// 0/1 after a usage reset; when the grid is one column wide the next cell's
-// up neighbour is this cell's winner.
+// One cell of the raster chain.
 //
-// Fast path (ranks 0-63 only): candidates are in ascending (sqdist, idx)
-// order and the penalty is at most fl(2w) (two terms <= 1, times w), so with
-// f the first usable candidate, score_f <= fl(d_f + 2w) =: bound and any
-// candidate with d > bound scores strictly worse than f.  When f is the
-// only usable rank-0..63 candidate with d <= bound and rank 64 (the nearest
-// of ranks 64-127) is absent or beyond the bound, f wins without any
-// penalty.  Otherwise the exact full evaluation runs over both halves, with
-// the usage counts of ranks 64-127 read from LDS then.
+// Speculation: the winner is usually the first usable candidate f.  Its
+// material w is read first and the next cell's usable mask is built from it
+// (w bans that candidate as the left neighbour and adds one use), so the
+// next decision does not wait for this cell's contender check.  The check
+// then confirms the speculation: candidates are in ascending (sqdist, idx)
+// order and the penalty is at most fl(2w_pen) (two terms <= 1, times the
+// weight), so score_f <= fl(d_f + 2 w_pen) =: bound and a candidate with
+// d > bound scores strictly worse than f.  When f is the only usable rank
+// 0-63 candidate within the bound and rank 64 (the nearest of ranks 64-127)
+// is absent or beyond it, f wins.  Otherwise the exact full evaluation runs
+// (usage counts of ranks 64-127 read from LDS, penalties for the contenders
+// with the neighbours' Labs, a DPP minimum, or the usage-reset fallback) and
+// the next cell's mask is rebuilt from the true winner.
+//
+// The cell after next has its usage counts and up neighbour read from LDS in
+// this step (so the reads have a whole step to land); when they are used one
+// step later they are corrected in registers for the two winners committed
+// in between (+1 each, or a base of 0 after a usage reset).  In grids one or
+// two columns wide the up neighbour is one of those two winners.  Lane 0
+// commits: one use of the winner (LDS atomic), the column's latest material,
+// and the winner into an LDS ring written to the grid once per group of cells
+// (a vector store inside the group would make the next use of a group-loaded
+// record wait for it).
 template <bool kOutputs>
-__device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn, int32_t c,
-                                            RasterState& st, const PlaceJob& job,
-                                            uint32_t* usage_s, int32_t* row_mat, float4* row_lab,
-                                            int32_t n, int lane) {
+__device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn,
+                                            const CandSet& snn, int32_t c, RasterState& st,
+                                            const PlaceJob& job, uint32_t* usage_s,
+                                            int32_t* row_mat, int32_t* win_s, int32_t n,
+                                            int lane) {
     const uint32_t kNone = 0xffffffffu;
     const int32_t gw = job.grid_w;
     const int32_t x = st.x, y = st.y;
     const int32_t xn = x + 1 == gw ? 0 : x + 1;
     const int32_t yn = x + 1 == gw ? y + 1 : y;
-    // pre-read for the next cell (sees every commit up to the previous cell)
-    const uint32_t nu0 = usage_s[sn.m0 >= 0 ? sn.m0 : 0];
-    int32_t nup_m = kEmpty;
-    float4 nup_lab = make_float4(0, 0, 0, 0);
-    if (yn > 0) {
-        nup_m = row_mat[xn];
-        nup_lab = row_lab[xn];
-    }
-
-    const int32_t up_m = y > 0 ? st.up_m : kEmpty;
-    const int32_t left_m = x > 0 ? st.left_m : kEmpty;
-    const bool pl0 = s.m0 >= 0 && s.m0 != up_m && s.m0 != left_m;
+    const int32_t x2 = xn + 1 == gw ? 0 : xn + 1;
+    const int32_t y2 = xn + 1 == gw ? yn + 1 : yn;
     const uint32_t max_usage = (uint32_t)job.max_usage;
-    const bool ok0 = pl0 && st.u0 < max_usage;
-    const uint64_t ob0 = wave_ballot(ok0);
+    // reads for the cell after next (they see every commit up to the previous cell)
+    const uint32_t nnu0 = usage_s[snn.m0 >= 0 ? snn.m0 : 0];
+    const int32_t nnup = y2 > 0 && gw > 2 ? row_mat[x2] : kEmpty;
 
-    int32_t rank = 0, stage = 0;
-    uint32_t win_score = 0, u1 = 0;
-    bool slow = true;
-    if (ob0) {
-        const int32_t f = __ffsll((unsigned long long)ob0) - 1;
+    // the next cell's counts before this cell's commit, and its up neighbour
+    const uint32_t base = (st.prev_reset ? 0u : st.pre_u) + (sn.m0 == st.prev_w ? 1u : 0u);
+    const int32_t up_fixed = yn == 0 ? kEmpty : (gw == 2 ? st.prev_w : st.pre_up);
+
+    // speculative winner and the next cell's usable mask
+    const uint64_t ob0 = st.ob0;
+    const int32_t f = ob0 ? (int32_t)__ffsll((unsigned long long)ob0) - 1 : 0;
+    int32_t w = __builtin_amdgcn_readlane(s.m0, f);
+    int32_t nup = gw == 1 ? w : up_fixed;
+    uint32_t nu = base + (sn.m0 == w ? 1u : 0u);
+    uint64_t nob = wave_ballot(sn.m0 >= 0 && sn.m0 != nup && (xn == 0 || sn.m0 != w) &&
+                               nu < max_usage);
+
+    // confirm it
+    bool fast = ob0 != 0;
+    if (fast && job.weight > 0.0f) {
         const float d_f = readlane_f(s.c0.w, f);
-        if (job.weight > 0.0f) {
-            const float bound = d_f + 2.0f * job.weight;
-            const uint64_t far_half = wave_ballot(s.m1 >= 0 && s.c1.w <= bound);
-            const uint64_t cb0 = wave_ballot(ok0 && s.c0.w <= bound);
-            if (far_half == 0 && __popcll(cb0) == 1 && !(kOutputs && job.score)) {
-                rank = f;
-                slow = false;
-            }
-        } else {
-            rank = f;
-            win_score = __float_as_uint(d_f);  // penalty 0: scores are the distances
-            slow = false;
-        }
-        if (!slow) ++st.st_main;
+        const float bound = d_f + 2.0f * job.weight;
+        const uint64_t far_half = wave_ballot(s.m1 >= 0 && s.c1.w <= bound);
+        const uint64_t cb0 = ob0 & wave_ballot(s.c0.w <= bound);
+        fast = far_half == 0 && __popcll(cb0) == 1;
     }
-    if (slow) {
-        u1 = usage_s[s.m1 >= 0 ? s.m1 : 0];
+    if (kOutputs && job.score) fast = false;
+
+    int32_t rank = f, stage = 0;
+    uint32_t win_score = 0;
+    if (!fast) {
+        const int32_t up_m = y > 0 ? st.up_m : kEmpty;
+        const int32_t left_m = x > 0 ? st.left_m : kEmpty;
+        const bool pl0 = s.m0 >= 0 && s.m0 != up_m && s.m0 != left_m;
+        const bool ok0 = ((ob0 >> lane) & 1ull) != 0;
+        const uint32_t u1 = usage_s[s.m1 >= 0 ? s.m1 : 0];
         const bool pl1 = s.m1 >= 0 && s.m1 != up_m && s.m1 != left_m;
         const bool ok1 = pl1 && u1 < max_usage;
         const uint64_t ob1 = wave_ballot(ok1);
         if (ob0 | ob1) {
-            const int32_t f = ob0 ? __ffsll((unsigned long long)ob0) - 1
-                                  : 64 + __ffsll((unsigned long long)ob1) - 1;
-            const float d_f = f < 64 ? readlane_f(s.c0.w, f) : readlane_f(s.c1.w, f & 63);
-            rank = f;
+            const int32_t f2 = ob0 ? __ffsll((unsigned long long)ob0) - 1
+                                   : 64 + __ffsll((unsigned long long)ob1) - 1;
+            const float d_f = f2 < 64 ? readlane_f(s.c0.w, f2) : readlane_f(s.c1.w, f2 & 63);
+            rank = f2;
             if (job.weight > 0.0f) {
                 const float bound = d_f + 2.0f * job.weight;
                 const bool ct0 = ok0 && s.c0.w <= bound;
@@ -273,18 +286,21 @@ __device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn,
                 if (__popcll(cb0) + __popcll(cb1) > 1 || (kOutputs && job.score)) {
                     // calculate_penalty for the contenders: Up then Left (Down
                     // and Right are still empty in raster order)
+                    const float4 up_lab = y > 0 ? gload4(job.lab4, up_m) : make_float4(0, 0, 0, 0);
+                    const float4 left_lab =
+                        x > 0 ? gload4(job.lab4, left_m) : make_float4(0, 0, 0, 0);
                     uint32_t k0 = kNone, k1 = kNone;
                     if (cb0) {
                         float p = 0.0f;
-                        if (y > 0) p += adj_term(s.c0, st.up_lab);
-                        if (x > 0) p += adj_term(s.c0, st.left_lab);
+                        if (y > 0) p += adj_term(s.c0, up_lab);
+                        if (x > 0) p += adj_term(s.c0, left_lab);
                         const float sc = s.c0.w + p * job.weight;
                         k0 = ct0 ? __float_as_uint(sc) : kNone;
                     }
                     if (cb1) {
                         float p = 0.0f;
-                        if (y > 0) p += adj_term(s.c1, st.up_lab);
-                        if (x > 0) p += adj_term(s.c1, st.left_lab);
+                        if (y > 0) p += adj_term(s.c1, up_lab);
+                        if (x > 0) p += adj_term(s.c1, left_lab);
                         const float sc = s.c1.w + p * job.weight;
                         k1 = ct1 ? __float_as_uint(sc) : kNone;
                     }
@@ -293,12 +309,12 @@ __device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn,
                     rank = b0 ? __ffsll((unsigned long long)b0) - 1
                               : 64 + __ffsll((unsigned long long)wave_ballot(k1 == kmin)) - 1;
                     win_score = kmin;
+                } else {
+                    win_score = __float_as_uint(d_f);  // sole contender: no penalty needed
                 }
             } else {
-                win_score = __float_as_uint(d_f);
+                win_score = __float_as_uint(d_f);  // penalty 0: scores are the distances
             }
-            stage = 0;
-            ++st.st_main;
         } else {
             // fallback_tile_selection: reset every count, first placeable, else nearest_one
             for (int32_t i = lane; i < n; i += 64) usage_s[i] = 0u;
@@ -319,40 +335,36 @@ __device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn,
                 ++st.st_near;
             }
         }
-    }
-    const int wl = rank & 63;
-    int32_t w;
-    float4 wlab;
-    uint32_t cnt = 0;
-    if (rank < 64) {
-        w = __builtin_amdgcn_readlane(s.m0, wl);
-        wlab = readlane4(s.c0, wl);
-        if (stage == 0) cnt = (uint32_t)__builtin_amdgcn_readlane((int)st.u0, wl);
-    } else {
-        w = __builtin_amdgcn_readlane(s.m1, wl);
-        wlab = readlane4(s.c1, wl);
-        if (stage == 0) cnt = (uint32_t)__builtin_amdgcn_readlane((int)u1, wl);
+        w = rank < 64 ? __builtin_amdgcn_readlane(s.m0, rank & 63)
+                      : __builtin_amdgcn_readlane(s.m1, rank & 63);
+        nup = gw == 1 ? w : up_fixed;
+        const uint32_t h = sn.m0 == w ? 1u : 0u;
+        nu = stage == 0 ? base + h : h;
+        nob = wave_ballot(sn.m0 >= 0 && sn.m0 != nup && (xn == 0 || sn.m0 != w) && nu < max_usage);
     }
     if (lane == 0) {
-        usage_s[w] = cnt + 1u;
+        atomicAdd(&usage_s[w], 1u);
         row_mat[x] = w;
-        row_lab[x] = wlab;
-        if (kOutputs) {
+        win_s[c & (kWinRing - 1)] = w;
+    }
+    if (kOutputs) {
+        const float dw = rank < 64 ? readlane_f(s.c0.w, rank & 63) : readlane_f(s.c1.w, rank & 63);
+        if (lane == 0) {
             if (job.stage) gstore(job.stage, c, (uint8_t)stage);
-            if (job.dist) gstore(job.dist, c, wlab.w);
-            if (job.score) gstore(job.score, c, stage == 0 ? __uint_as_float(win_score) : wlab.w);
+            if (job.dist) gstore(job.dist, c, dw);
+            if (job.score) gstore(job.score, c, stage == 0 ? __uint_as_float(win_score) : dw);
         }
     }
     __builtin_amdgcn_wave_barrier();
-    if (lane == (c & 63)) st.win_buf = (uint32_t)w;
-    if ((c & 63) == 63) gstore(job.grid, c - 63 + lane, (int32_t)st.win_buf);
-    // carry the next cell's state, corrected for this commit
-    const uint32_t h0 = sn.m0 == w ? 1u : 0u;
-    st.u0 = stage == 0 ? nu0 + h0 : h0;
-    st.up_m = gw == 1 ? w : nup_m;
-    st.up_lab = gw == 1 ? wlab : nup_lab;
+    // a reset here zeroed every count after the reads above
+    st.pre_u = nnu0;
+    st.pre_up = nnup;
+    st.prev_reset = stage != 0 ? 1 : 0;
+    st.prev_w = w;
+    st.up_m = nup;
     st.left_m = w;
-    st.left_lab = wlab;
+    st.u0 = nu;
+    st.ob0 = nob;
     st.x = xn;
     st.y = yn;
 }
@@ -365,8 +377,8 @@ __global__ __launch_bounds__(64) void place_raster_kernel(PlaceJob single,
     const int lane = threadIdx.x;
     const int32_t gw = job.grid_w, n = job.n;
     const int32_t cells = gw * job.grid_h;
-    float4* row_lab = reinterpret_cast<float4*>(smr);
-    int32_t* row_mat = reinterpret_cast<int32_t*>(row_lab + gw);
+    int32_t* win_s = reinterpret_cast<int32_t*>(smr);
+    int32_t* row_mat = win_s + kWinRing;
     uint32_t* usage_s = reinterpret_cast<uint32_t*>(row_mat + gw);
 
     if (n == 0) {  // src/main.rs:342-345: no tiles, every cell stays empty
@@ -380,49 +392,62 @@ __global__ __launch_bounds__(64) void place_raster_kernel(PlaceJob single,
     for (int32_t i = lane; i < n; i += 64) usage_s[i] = job.zero_usage ? 0u : gload(job.usage, i);
     __syncthreads();
 
-    // Candidate records for kGroup cells are loaded at the top of each group
-    // and consumed in the same loop iteration, so the compiler's wait counts
-    // stay exact (a load consumed after the loop's back edge would make it
-    // wait for every load in flight); one memory latency per group.
+    // Candidate records for kGroup cells (and the two after them) are loaded at
+    // the top of each group and consumed in the same loop iteration, so the
+    // compiler's wait counts stay exact (a load consumed after the loop's back
+    // edge would make it wait for every load in flight); one memory latency
+    // per group.  The previous group's winners are stored just before the loads.
     constexpr int kGroup = 12;
-    CandSet q[kGroup + 1];
+    static_assert(kGroup <= kWinRing, "a group must fit the winner ring");
+    CandSet q[kGroup + 2];
     RasterState st;
     st.x = 0;
     st.y = 0;
     st.left_m = kEmpty;
-    st.left_lab = make_float4(0, 0, 0, 0);
     st.up_m = kEmpty;
-    st.up_lab = make_float4(0, 0, 0, 0);
-    st.win_buf = 0;
-    st.st_main = st.st_ban = st.st_near = st.st_resets = 0;
+    st.prev_w = kNoMat;
+    st.prev_reset = 0;
+    st.st_ban = st.st_near = st.st_resets = 0;
     {
-        CandSet h;
-        load_cands(h, job, 0, cells, lane);
-        st.u0 = usage_s[h.m0 >= 0 ? h.m0 : 0];
+        CandSet h0, h1;
+        load_cands(h0, job, 0, cells, lane);
+        load_cands(h1, job, 1, cells, lane);
+        st.u0 = usage_s[h0.m0 >= 0 ? h0.m0 : 0];
+        st.ob0 = wave_ballot(h0.m0 >= 0 && st.u0 < (uint32_t)job.max_usage);
+        st.pre_u = usage_s[h1.m0 >= 0 ? h1.m0 : 0];
+        st.pre_up = kEmpty;  // cell 1 is in row 0 unless the grid is one column wide
     }
     int32_t c = 0;
     for (; c + kGroup <= cells; c += kGroup) {
+        if (c > 0 && lane < kGroup) {
+            const int32_t p = c - kGroup + lane;
+            gstore(job.grid, p, win_s[p & (kWinRing - 1)]);
+        }
 #pragma unroll
-        for (int i = 0; i <= kGroup; ++i) load_cands(q[i], job, c + i, cells, lane);
+        for (int i = 0; i < kGroup + 2; ++i) load_cands(q[i], job, c + i, cells, lane);
 #pragma unroll
         for (int i = 0; i < kGroup; ++i)
-            raster_step<kOutputs>(q[i], q[i + 1], c + i, st, job, usage_s, row_mat, row_lab, n,
-                                  lane);
+            raster_step<kOutputs>(q[i], q[i + 1], q[i + 2], c + i, st, job, usage_s, row_mat,
+                                  win_s, n, lane);
     }
+    if (c > 0 && lane < kGroup) {  // the last full group's winners
+        const int32_t p = c - kGroup + lane;
+        gstore(job.grid, p, win_s[p & (kWinRing - 1)]);
+    }
+    const int32_t tail0 = c;
     for (; c < cells; ++c) {  // tail: fewer than kGroup cells
-        CandSet a, b;
-        load_cands(a, job, c, cells, lane);
-        load_cands(b, job, c + 1, cells, lane);
-        raster_step<kOutputs>(a, b, c, st, job, usage_s, row_mat, row_lab, n, lane);
+        CandSet a0, a1, a2;
+        load_cands(a0, job, c, cells, lane);
+        load_cands(a1, job, c + 1, cells, lane);
+        load_cands(a2, job, c + 2, cells, lane);
+        raster_step<kOutputs>(a0, a1, a2, c, st, job, usage_s, row_mat, win_s, n, lane);
     }
-    // flush the partial winner block
-    const int32_t tail = cells & 63;
-    if (tail != 0 && lane < tail) gstore(job.grid, cells - tail + lane, (int32_t)st.win_buf);
     __syncthreads();
+    if (tail0 + lane < cells) gstore(job.grid, tail0 + lane, win_s[(tail0 + lane) & (kWinRing - 1)]);
     for (int32_t i = lane; i < n; i += 64) gstore(job.usage, i, usage_s[i]);
     if (lane == 0 && job.stats) {
-        gstore(job.stats, 0, st.st_main + st.st_ban + st.st_near);
-        gstore(job.stats, 1, st.st_main);
+        gstore(job.stats, 0, cells);
+        gstore(job.stats, 1, cells - st.st_ban - st.st_near);
         gstore(job.stats, 2, st.st_ban);
         gstore(job.stats, 3, st.st_near);
         gstore(job.stats, 4, st.st_resets);
@@ -435,7 +460,7 @@ size_t place_lds_bytes(int32_t cells, int32_t n) {
 }
 
 size_t place_raster_lds_bytes(int32_t grid_w, int32_t n) {
-    return sizeof(float4) * (size_t)grid_w + sizeof(int32_t) * (size_t)grid_w +
+    return sizeof(int32_t) * ((size_t)kWinRing + (size_t)grid_w) +
            sizeof(uint32_t) * (size_t)(n > 0 ? n : 1);
 }
 
