@@ -1,4 +1,5 @@
-"""Builds libmosaic_hip.so (gfx950) in-tree from csrc/*.hip.
+"""Builds libmosaic_hip.so (gfx950) in-tree from csrc/*.hip, then the C++
+host API (host/mosaic.cpp) with its test program (test_mosaic).
 
 Flags: -ffp-contract=off keeps the reference's unfused f32 arithmetic;
 packed FP32 instructions are disabled (-packed-fp32-ops target feature).
@@ -51,7 +52,26 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_host(verbose)
     return LIB
+
+
+HOST = os.path.join(HERE, "host")
+HOST_TESTS = os.path.join(HERE, "test_mosaic")
+
+
+def build_host(verbose=False):
+    """The C++ host API and its tests, linked against libmosaic_hip.so."""
+    include = os.path.join(os.path.dirname(HERE), "include")
+    cmd = [os.environ.get("CXX", "g++"), "-std=c++17", "-O2", "-Wall", "-Wextra", f"-I{include}",
+           os.path.join(HOST, "mosaic.cpp"), os.path.join(HOST, "test_mosaic.cpp"),
+           f"-L{HERE}", "-lmosaic_hip", "-Wl,-rpath,$ORIGIN", "-o", HOST_TESTS + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"host build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(HOST_TESTS + ".tmp", HOST_TESTS)
+    if verbose:
+        print(f"built {HOST_TESTS}")
 
 
 if __name__ == "__main__":
