@@ -73,6 +73,8 @@ typedef struct mh_library mh_library;
 
 /* ---------------------------------------------------------------- errors */
 const char *mh_last_error(void);
+/* Version of the structure layouts below (2: score_mode in the placement and
+ * job parameters). */
 int32_t mh_abi_version(void);
 
 /* --------------------------------------------------------------- context */
@@ -156,10 +158,19 @@ int mh_nearest_n(mh_context *ctx, const mh_library *lib, const float *query, int
                  int32_t *idx, float *sqdist);
 
 /* ------------------------------------------------------ a4-a9: placement */
+/* Candidate score of the placement scan.  MH_SCORE_SQDIST is the CLI's
+ * squared Lab distance + penalty (src/main.rs:385-398); MH_SCORE_DISTANCE is
+ * the GUI generator's Euclidean distance + penalty (sqrt of the same sum,
+ * src/gui/app_full.rs:1436-1448).  Candidates, bans, usage limits and
+ * fallbacks are the same in both. */
+#define MH_SCORE_SQDIST 0
+#define MH_SCORE_DISTANCE 1
+
 typedef struct {
     int32_t grid_w, grid_h;
     int32_t max_usage;   /* UsageTracker::new(max_usage_per_image)          */
     float weight;        /* adjacency_penalty_weight; <= 0 disables penalty */
+    int32_t score_mode;  /* MH_SCORE_SQDIST or MH_SCORE_DISTANCE             */
 } mh_place_params;
 
 typedef struct {
@@ -172,7 +183,7 @@ typedef struct {
  * state; a fresh mosaic passes an all-MH_EMPTY grid and zero usage.
  * Optional outputs per cell: stage (0 scored scan, 1 fallback
  * first-placeable, 2 nearest_one, 255 untouched), dist (chosen candidate's
- * squared distance), score (its total score). */
+ * colour distance as the score mode measures it), score (its total score). */
 int mh_place(mh_context *ctx, const mh_library *lib, const float *cell_lab,
              const mh_place_params *params, const int32_t *order, int32_t n_order, int32_t *grid,
              uint32_t *usage, uint8_t *stage, float *dist, float *score, mh_place_stats *stats);
@@ -243,6 +254,7 @@ typedef struct {
     int64_t optimization_iterations;
     float initial_temperature, temperature_decay;
     uint64_t seed;
+    int32_t score_mode;             /* MH_SCORE_SQDIST or MH_SCORE_DISTANCE  */
 } mh_job_params;
 
 typedef struct {

@@ -12,6 +12,15 @@
 // matrix), score = sqdist + penalty; the reference's first strict-< winner
 // is the minimum of (score, rank).  Many jobs run as many independent waves.
 //
+// The GUI generator's variant (src/gui/app_full.rs:1396-1563) scores
+// sqrt(sqdist) + penalty with everything else unchanged; with score_sqrt set
+// the candidate distances are replaced by their correctly rounded square
+// roots as they are loaded.  sqrt is monotone, so the candidates stay in
+// ascending distance order and the contender bound below holds unchanged.
+// The raster chain converts a cell's records one step before they are
+// scored (when their loads have landed anyway), in a variant compiled for
+// it, so the squared-distance chain carries no extra work.
+//
 // Two variants:
 //  * place_raster_kernel -- what generate_mosaic runs: raster order from an
 //    empty grid, so only Up and Left can be filled.  The left neighbour is
@@ -82,7 +91,8 @@ __global__ __launch_bounds__(64) void place_chain_kernel(PlaceJob single,
             placeable[s] = false;
             if (r < K) {
                 const int32_t m = gload(cidx, r);
-                const float4 lm = gload4(clab, r);
+                float4 lm = gload4(clab, r);
+                if (job.score_sqrt) lm.w = sqrtf(lm.w);
                 const bool pl = m != nb_up && m != nb_down && m != nb_left && m != nb_right;
                 placeable[s] = pl;
                 if (pl && usage_s[m] < max_usage) {
@@ -129,7 +139,8 @@ __global__ __launch_bounds__(64) void place_chain_kernel(PlaceJob single,
             }
         }
         const int32_t w = gload(cidx, rank);
-        const float wd = gload4(clab, rank).w;
+        const float wd_raw = gload4(clab, rank).w;
+        const float wd = job.score_sqrt ? sqrtf(wd_raw) : wd_raw;
         __syncthreads();
         if (lane == 0) {
             usage_s[w] = usage_s[w] + 1u;
@@ -157,7 +168,7 @@ __global__ __launch_bounds__(64) void place_chain_kernel(PlaceJob single,
 // ------------------------------------------------------------- raster chain
 struct CandSet {
     int32_t m0, m1;   // material of rank lane / lane+64 (-1: no candidate)
-    float4 c0, c1;    // {l, a, b, sqdist}
+    float4 c0, c1;    // {l, a, b, d}: d = sqdist, or sqrt(sqdist) for the GUI score
 };
 
 // Loads one cell's candidate records.  Every load is issued unconditionally
@@ -175,6 +186,12 @@ __device__ __forceinline__ void load_cands(CandSet& s, const PlaceJob& job, int3
     const bool live = cell < cells;
     s.m0 = live ? s.m0 : -1;
     s.m1 = live ? s.m1 : -1;
+}
+
+// GUI score: the Euclidean distance replaces the squared one.
+__device__ __forceinline__ void to_distance(CandSet& s) {
+    s.c0.w = sqrtf(s.c0.w);
+    s.c1.w = sqrtf(s.c1.w);
 }
 
 constexpr int32_t kWinRing = 64;  // winners awaiting their grid store (power of two)
@@ -369,11 +386,14 @@ __device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn,
     st.y = yn;
 }
 
-template <bool kOutputs>
+// kScore: kScoreSq / kScoreDist fix the score for every job of the launch,
+// kScoreMixed reads each job's score_sqrt.
+template <bool kOutputs, int kScore>
 __global__ __launch_bounds__(64) void place_raster_kernel(PlaceJob single,
                                                           const PlaceJob* __restrict__ jobs) {
     extern __shared__ __align__(16) uint32_t smr[];
     const PlaceJob job = uniform_copy(jobs ? jobs[blockIdx.x] : single);
+    const bool sq = kScore == kScoreMixed ? job.score_sqrt != 0 : kScore == kScoreDist;
     const int lane = threadIdx.x;
     const int32_t gw = job.grid_w, n = job.n;
     const int32_t cells = gw * job.grid_h;
@@ -425,10 +445,14 @@ __global__ __launch_bounds__(64) void place_raster_kernel(PlaceJob single,
         }
 #pragma unroll
         for (int i = 0; i < kGroup + 2; ++i) load_cands(q[i], job, c + i, cells, lane);
+        if (sq) to_distance(q[0]);
 #pragma unroll
-        for (int i = 0; i < kGroup; ++i)
+        for (int i = 0; i < kGroup; ++i) {
             raster_step<kOutputs>(q[i], q[i + 1], q[i + 2], c + i, st, job, usage_s, row_mat,
                                   win_s, n, lane);
+            // the step waited for the records after q[i + 1]'s, so these are in
+            if (sq && i + 1 < kGroup) to_distance(q[i + 1]);
+        }
     }
     if (c > 0 && lane < kGroup) {  // the last full group's winners
         const int32_t p = c - kGroup + lane;
@@ -440,6 +464,7 @@ __global__ __launch_bounds__(64) void place_raster_kernel(PlaceJob single,
         load_cands(a0, job, c, cells, lane);
         load_cands(a1, job, c + 1, cells, lane);
         load_cands(a2, job, c + 2, cells, lane);
+        if (sq) to_distance(a0);
         raster_step<kOutputs>(a0, a1, a2, c, st, job, usage_s, row_mat, win_s, n, lane);
     }
     __syncthreads();
@@ -466,36 +491,56 @@ size_t place_raster_lds_bytes(int32_t grid_w, int32_t n) {
 
 void launch_place_single(const PlaceJob& job, bool raster, size_t lds_bytes, hipStream_t stream) {
     const bool outputs = job.stage || job.dist || job.score;
-    if (raster && outputs)
-        hipLaunchKernelGGL(place_raster_kernel<true>, dim3(1), dim3(64), lds_bytes, stream, job,
-                           (const PlaceJob*)nullptr);
+    const PlaceJob* none = nullptr;
+    if (raster && outputs && job.score_sqrt)
+        hipLaunchKernelGGL((place_raster_kernel<true, kScoreDist>), dim3(1), dim3(64), lds_bytes,
+                           stream, job, none);
+    else if (raster && outputs)
+        hipLaunchKernelGGL((place_raster_kernel<true, kScoreSq>), dim3(1), dim3(64), lds_bytes,
+                           stream, job, none);
+    else if (raster && job.score_sqrt)
+        hipLaunchKernelGGL((place_raster_kernel<false, kScoreDist>), dim3(1), dim3(64), lds_bytes,
+                           stream, job, none);
     else if (raster)
-        hipLaunchKernelGGL(place_raster_kernel<false>, dim3(1), dim3(64), lds_bytes, stream, job,
-                           (const PlaceJob*)nullptr);
+        hipLaunchKernelGGL((place_raster_kernel<false, kScoreSq>), dim3(1), dim3(64), lds_bytes,
+                           stream, job, none);
     else
         hipLaunchKernelGGL(place_chain_kernel, dim3(1), dim3(64), lds_bytes, stream, job,
                            (const PlaceJob*)nullptr);
 }
 
-void launch_place_batch(const PlaceJob* d_jobs, int32_t n_jobs, bool raster, size_t lds_bytes,
-                        hipStream_t stream) {
+void launch_place_batch(const PlaceJob* d_jobs, int32_t n_jobs, bool raster, int32_t score_kind,
+                        size_t lds_bytes, hipStream_t stream) {
     if (n_jobs <= 0) return;
     PlaceJob unused{};
-    if (raster)
-        hipLaunchKernelGGL(place_raster_kernel<false>, dim3(n_jobs), dim3(64), lds_bytes, stream,
-                           unused, d_jobs);
+    if (raster && score_kind == kScoreMixed)
+        hipLaunchKernelGGL((place_raster_kernel<false, kScoreMixed>), dim3(n_jobs), dim3(64),
+                           lds_bytes, stream, unused, d_jobs);
+    else if (raster && score_kind == kScoreDist)
+        hipLaunchKernelGGL((place_raster_kernel<false, kScoreDist>), dim3(n_jobs), dim3(64),
+                           lds_bytes, stream, unused, d_jobs);
+    else if (raster)
+        hipLaunchKernelGGL((place_raster_kernel<false, kScoreSq>), dim3(n_jobs), dim3(64),
+                           lds_bytes, stream, unused, d_jobs);
     else
         hipLaunchKernelGGL(place_chain_kernel, dim3(n_jobs), dim3(64), lds_bytes, stream, unused,
                            d_jobs);
 }
 
 hipError_t place_configure(int max_lds) {
-    hipError_t e = raise_dynamic_lds(reinterpret_cast<const void*>(&place_chain_kernel), max_lds);
-    if (e == hipSuccess)
-        e = raise_dynamic_lds(reinterpret_cast<const void*>(&place_raster_kernel<true>), max_lds);
-    if (e == hipSuccess)
-        e = raise_dynamic_lds(reinterpret_cast<const void*>(&place_raster_kernel<false>), max_lds);
-    return e;
+    const void* kernels[] = {
+        reinterpret_cast<const void*>(&place_chain_kernel),
+        reinterpret_cast<const void*>(&place_raster_kernel<true, kScoreSq>),
+        reinterpret_cast<const void*>(&place_raster_kernel<true, kScoreDist>),
+        reinterpret_cast<const void*>(&place_raster_kernel<false, kScoreSq>),
+        reinterpret_cast<const void*>(&place_raster_kernel<false, kScoreDist>),
+        reinterpret_cast<const void*>(&place_raster_kernel<false, kScoreMixed>),
+    };
+    for (const void* k : kernels) {
+        const hipError_t e = raise_dynamic_lds(k, max_lds);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 }  // namespace mh

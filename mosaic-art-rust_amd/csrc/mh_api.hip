@@ -185,7 +185,8 @@ void fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s) {
 // the whole grid in raster order from an empty grid (generate_mosaic's case,
 // the fast chain); otherwise `order` (or raster) over the given grid/usage.
 int run_place(mh_context* ctx, const mh_library* lib, const float* d_cell_lab, int32_t gw,
-              int32_t gh, int32_t max_usage, float weight, bool raster, const int32_t* d_order,
+              int32_t gh, int32_t max_usage, float weight, int32_t score_mode, bool raster,
+              const int32_t* d_order,
               int32_t n_order, int32_t* d_grid, uint32_t* d_usage, bool zero_usage,
               uint8_t* d_stage, float* d_dist, float* d_score, int32_t* d_stats) {
     const int32_t cells = gw * gh;
@@ -218,6 +219,7 @@ int run_place(mh_context* ctx, const mh_library* lib, const float* d_cell_lab, i
     job.grid_h = gh;
     job.max_usage = max_usage;
     job.weight = weight;
+    job.score_sqrt = score_mode == MH_SCORE_DISTANCE ? 1 : 0;
     job.stage = d_stage;
     job.dist = d_dist;
     job.score = d_score;
@@ -398,6 +400,13 @@ int check_geometry(int32_t gw, int32_t gh) {
     return MH_OK;
 }
 
+int check_score_mode(int32_t mode) {
+    if (mode != MH_SCORE_SQDIST && mode != MH_SCORE_DISTANCE)
+        return fail(MH_ERR_INVALID_ARGUMENT, "score_mode %d is neither MH_SCORE_SQDIST nor "
+                    "MH_SCORE_DISTANCE", mode);
+    return MH_OK;
+}
+
 void to_result(const AnnealOut& o, mh_anneal_result* r) {
     r->initial_cost = o.initial_cost;
     r->final_cost = o.final_cost;
@@ -417,7 +426,7 @@ extern "C" {
 
 const char* mh_last_error(void) { return g_err.c_str(); }
 
-int32_t mh_abi_version(void) { return 1; }
+int32_t mh_abi_version(void) { return 2; }
 
 int mh_context_create(int32_t device, mh_context** out) {
     if (!out) return fail(MH_ERR_INVALID_ARGUMENT, "out is NULL");
@@ -848,6 +857,7 @@ int mh_place(mh_context* ctx, const mh_library* lib, const float* cell_lab,
     if (!isfinite(p->weight)) return fail(MH_ERR_INVALID_ARGUMENT, "weight must be finite");
     CHK(check_geometry(p->grid_w, p->grid_h));
     if (p->max_usage < 0) return fail(MH_ERR_INVALID_ARGUMENT, "max_usage must be >= 0");
+    CHK(check_score_mode(p->score_mode));
     const int32_t cells = p->grid_w * p->grid_h;
     if (!finite3(cell_lab, (int64_t)cells * 3))
         return fail(MH_ERR_INVALID_ARGUMENT, "cell Lab values must be finite");
@@ -877,7 +887,8 @@ int mh_place(mh_context* ctx, const mh_library* lib, const float* cell_lab,
     bool empty_grid = true;
     for (int32_t i = 0; i < cells && empty_grid; ++i) empty_grid = grid[i] == MH_EMPTY;
     const bool raster = !order && empty_grid;
-    CHK(run_place(ctx, lib, d_lab, p->grid_w, p->grid_h, p->max_usage, p->weight, raster, d_order,
+    CHK(run_place(ctx, lib, d_lab, p->grid_w, p->grid_h, p->max_usage, p->weight, p->score_mode,
+                  raster, d_order,
                   order ? n_order : cells, d_grid, d_usage, false, d_stage, d_dist, d_score,
                   d_stats));
     int32_t st[6];
@@ -1112,6 +1123,7 @@ int mh_generate_device(mh_context* ctx, const mh_library* lib, const uint8_t* d_
     if (p->optimization_iterations < 0)
         return fail(MH_ERR_INVALID_ARGUMENT, "optimization_iterations < 0");
     if (!isfinite(p->weight)) return fail(MH_ERR_INVALID_ARGUMENT, "weight must be finite");
+    CHK(check_score_mode(p->score_mode));
     const int32_t gw = p->grid_w, gh = p->grid_h, cells = gw * gh;
     const int32_t tw = width / gw, th = height / gh;
     if (tw <= 0 || th <= 0)
@@ -1136,7 +1148,8 @@ int mh_generate_device(mh_context* ctx, const mh_library* lib, const uint8_t* d_
                     th, cells, ctx->d_lut, d_lab, ctx->stream);
     phase_end(ctx, MH_PHASE_CELL_LAB);
     // a3-a9: candidates + placement chain (raster order, empty grid, zero usage)
-    CHK(run_place(ctx, lib, d_lab, gw, gh, p->max_usage, p->weight, true, nullptr, cells, d_grid,
+    CHK(run_place(ctx, lib, d_lab, gw, gh, p->max_usage, p->weight, p->score_mode, true, nullptr,
+                  cells, d_grid,
                   d_usage, true, nullptr, nullptr, nullptr, d_stats));
     // a10-a12: SA when enabled and weight > 0 (src/main.rs:608)
     const bool sa = p->enable_optimization && p->weight > 0.0f;

@@ -63,6 +63,7 @@ int check_job(const mh_context* ctx, const mh_job& j, JobSpec* s) {
     if (p.optimization_iterations < 0)
         return fail(MH_ERR_INVALID_ARGUMENT, "optimization_iterations < 0");
     if (!isfinite(p.weight)) return fail(MH_ERR_INVALID_ARGUMENT, "weight must be finite");
+    CHK(check_score_mode(p.score_mode));
     if (j.width <= 0 || j.height <= 0) return fail(MH_ERR_INVALID_ARGUMENT, "bad image size");
     const mh_library* lib = j.lib;
     if (lib->ctx != ctx) return fail(MH_ERR_INVALID_ARGUMENT, "library belongs to another context");
@@ -133,6 +134,7 @@ struct BatchPlan {
     int32_t lab_blocks = 0;
     int32_t n_topk = 0, topk_blocks = 0, topk_run = 1;
     size_t place_lds = 0;
+    int32_t place_score = kScoreSq;  // kScoreSq / kScoreDist / kScoreMixed over the jobs
     int32_t n_anneal = 0;
     size_t anneal_lds = 0;
     bool narrow = true;
@@ -204,6 +206,7 @@ int build_plan(const std::vector<JobSpec>& specs, const std::vector<uint8_t*>& j
         if (s.n > 0) topk_queries += s.cells;
     p->topk_run = topk_run_length(topk_queries);
     p->place_lds = 0;
+    p->place_score = kScoreSq;
     p->anneal_lds = 0;
     p->narrow = true;
     p->anneal_job.clear();
@@ -246,10 +249,13 @@ int build_plan(const std::vector<JobSpec>& specs, const std::vector<uint8_t*>& j
         pj.grid_h = s.gh;
         pj.max_usage = j.params.max_usage;
         pj.weight = j.params.weight;
+        pj.score_sqrt = j.params.score_mode == MH_SCORE_DISTANCE ? 1 : 0;
         pj.stats = d_stats + (size_t)i * kStatsInts;
         place[i] = pj;
         const size_t pl = place_raster_lds_bytes(s.gw, s.n);
         if (pl > p->place_lds) p->place_lds = pl;
+        const int32_t kind = pj.score_sqrt ? kScoreDist : kScoreSq;
+        p->place_score = i == 0 || p->place_score == kind ? kind : kScoreMixed;
         // a10-a12: annealing chain
         if (s.sa) {
             AnnealJob aj{};
@@ -313,7 +319,7 @@ void enqueue_plan(const BatchPlan& p, uint8_t* d, const float* lut, hipStream_t 
                           lut, s);
     launch_topk_batch(at<TopkSeg>(d, p.o_topk), at<int32_t>(d, p.o_topk_st), p.n_topk, p.topk_blocks,
                       p.topk_run, s);
-    launch_place_batch(at<PlaceJob>(d, p.o_place), p.n_jobs, true, p.place_lds, s);
+    launch_place_batch(at<PlaceJob>(d, p.o_place), p.n_jobs, true, p.place_score, p.place_lds, s);
     if (p.n_anneal > 0)
         launch_anneal_batch(at<AnnealJob>(d, p.o_anneal), p.n_anneal, p.narrow, p.anneal_lds, s);
     if (p.n_avg > 0) {

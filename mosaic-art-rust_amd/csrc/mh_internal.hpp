@@ -140,6 +140,7 @@ int library_tiles(const mh_library* lib, int32_t tw, int32_t th, hipStream_t str
                   const uint8_t** tiles, const uint8_t** avg);
 int sync(mh_context* ctx);
 int check_geometry(int32_t gw, int32_t gh);
+int check_score_mode(int32_t mode);
 void to_result(const AnnealOut& o, mh_anneal_result* r);
 
 template <typename T>

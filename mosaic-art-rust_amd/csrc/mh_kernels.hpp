@@ -27,6 +27,7 @@ struct PlaceJob {
     int32_t grid_w, grid_h;
     int32_t max_usage;
     float weight;
+    int32_t score_sqrt;       // 1: candidates score sqrt(sqdist) + penalty (GUI variant)
     uint8_t* stage;           // optional per-cell outputs
     float* dist;
     float* score;
@@ -135,8 +136,11 @@ size_t place_raster_lds_bytes(int32_t grid_w, int32_t n);
 // One job passed by value (kernel argument), or n_jobs jobs from a device
 // array (one wave each).
 void launch_place_single(const PlaceJob& job, bool raster, size_t lds_bytes, hipStream_t stream);
-void launch_place_batch(const PlaceJob* d_jobs, int32_t n_jobs, bool raster, size_t lds_bytes,
-                        hipStream_t stream);
+// score_kind: kScoreSq or kScoreDist when every job scores alike, else
+// kScoreMixed (each job's score_sqrt is read).
+enum : int32_t { kScoreSq = 0, kScoreDist = 1, kScoreMixed = 2 };
+void launch_place_batch(const PlaceJob* d_jobs, int32_t n_jobs, bool raster, int32_t score_kind,
+                        size_t lds_bytes, hipStream_t stream);
 size_t anneal_lds_bytes(int32_t cells, bool narrow);
 void launch_anneal_single(const AnnealJob& job, bool narrow, size_t lds_bytes, hipStream_t stream);
 void launch_anneal_batch(const AnnealJob* d_jobs, int32_t n_jobs, bool narrow, size_t lds_bytes,

@@ -757,6 +757,7 @@ MosaicOutput MaterialLibrary::generate_mosaic(const RgbImage& target,
     p.initial_temperature = s.optimization.initial_temperature;
     p.temperature_decay = s.optimization.temperature_decay;
     p.seed = s.optimization.seed;
+    p.score_mode = (int32_t)s.score_mode;
     if (s.grid_w <= 0 || s.grid_h <= 0) throw Error(MH_ERR_INVALID_ARGUMENT, "grid must be positive");
     const uint32_t tw = target.width / (uint32_t)s.grid_w, th = target.height / (uint32_t)s.grid_h;
     MosaicOutput out;

@@ -217,6 +217,11 @@ struct MosaicGeneratorImpl {
 // generate_mosaic minus file I/O: materials are equally sized RGB images
 // (material i is tile i of the result grid's indices), the target is
 // resized by the caller to the output size (src/main.rs:491-505).
+// Candidate score of the placement scan: the CLI's squared Lab distance or
+// the GUI generator's Euclidean distance (src/gui/app_full.rs:1436-1448),
+// plus the adjacency penalty in both.
+enum class ScoreMode : int32_t { SquaredDistance = MH_SCORE_SQDIST, Distance = MH_SCORE_DISTANCE };
+
 struct MosaicSettings {
     int32_t grid_w = 50, grid_h = 28;
     int32_t max_usage_per_image = 3;
@@ -225,6 +230,7 @@ struct MosaicSettings {
     bool enable_optimization = true;
     size_t optimization_iterations = 1000;
     OptimizationConfig optimization;
+    ScoreMode score_mode = ScoreMode::SquaredDistance;
 };
 
 struct MosaicOutput {

@@ -20,6 +20,10 @@ MH_ERR_OUT_OF_MEMORY = -4
 MH_ERR_NOT_FOUND = -5
 MH_ERR_UNSUPPORTED = -6
 MH_EMPTY = -1
+# Placement score: CLI squared distance + penalty, or the GUI generator's
+# Euclidean distance + penalty (src/gui/app_full.rs:1436-1448).
+SCORE_SQDIST = 0
+SCORE_DISTANCE = 1
 MH_PHASES = 8
 
 # Every symbol include/mosaic_hip.h declares.
@@ -49,7 +53,8 @@ class MhError(RuntimeError):
 
 class PlaceParams(ctypes.Structure):
     _fields_ = [("grid_w", ctypes.c_int32), ("grid_h", ctypes.c_int32),
-                ("max_usage", ctypes.c_int32), ("weight", ctypes.c_float)]
+                ("max_usage", ctypes.c_int32), ("weight", ctypes.c_float),
+                ("score_mode", ctypes.c_int32)]
 
 
 class PlaceStats(ctypes.Structure):
@@ -83,7 +88,7 @@ class JobParams(ctypes.Structure):
                 ("color_strength", ctypes.c_float), ("enable_optimization", ctypes.c_int32),
                 ("optimization_iterations", ctypes.c_int64),
                 ("initial_temperature", ctypes.c_float), ("temperature_decay", ctypes.c_float),
-                ("seed", ctypes.c_uint64)]
+                ("seed", ctypes.c_uint64), ("score_mode", ctypes.c_int32)]
 
 
 class JobResult(ctypes.Structure):
@@ -317,7 +322,7 @@ class Context:
 
     # ------------------------------------------------------------ a4-a9
     def place(self, lib, cell_lab, grid_w, grid_h, max_usage, weight, order=None, grid=None,
-              usage=None):
+              usage=None, score_mode=SCORE_SQDIST):
         cells = grid_w * grid_h
         cl = _c(cell_lab, np.float32).reshape(cells, 3)
         g = np.full(cells, MH_EMPTY, np.int32) if grid is None else _c(grid, np.int32).copy()
@@ -327,7 +332,7 @@ class Context:
         dist = np.full(cells, np.nan, np.float32)
         score = np.full(cells, np.nan, np.float32)
         st = PlaceStats()
-        prm = PlaceParams(grid_w, grid_h, max_usage, weight)
+        prm = PlaceParams(grid_w, grid_h, max_usage, weight, score_mode)
         _check(self.lib.mh_place(self.h, lib.h, _ptr(cl), ctypes.byref(prm), _ptr(o),
                                  0 if o is None else len(o), _ptr(g), _ptr(u), _ptr(stage),
                                  _ptr(dist), _ptr(score), ctypes.byref(st)))
@@ -416,7 +421,8 @@ class Context:
     # ---------------------------------------------------------- whole job
     def generate(self, lib, target, grid_w, grid_h, max_usage, weight, color_strength,
                  enable_optimization=True, optimization_iterations=1000,
-                 initial_temperature=100.0, temperature_decay=0.99995, seed=0, image=True):
+                 initial_temperature=100.0, temperature_decay=0.99995, seed=0, image=True,
+                 score_mode=SCORE_SQDIST):
         t = _c(target, np.uint8)
         h, w = t.shape[:2]
         tw, th = w // grid_w, h // grid_h
@@ -424,7 +430,7 @@ class Context:
         img = np.zeros((grid_h * th, grid_w * tw, 3), np.uint8) if image else None
         prm = JobParams(grid_w, grid_h, max_usage, weight, color_strength,
                         1 if enable_optimization else 0, optimization_iterations,
-                        initial_temperature, temperature_decay, seed)
+                        initial_temperature, temperature_decay, seed, score_mode)
         res = JobResult()
         _check(self.lib.mh_generate(self.h, lib.h, _ptr(t), w, h, ctypes.byref(prm), _ptr(grid),
                                     _ptr(img), ctypes.byref(res)))

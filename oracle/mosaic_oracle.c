@@ -324,12 +324,16 @@ static float calc_penalty(const float *lab, const int32_t *grid, int32_t gw, int
  * processing order (NULL: raster order over the whole grid).  grid/usage
  * are in/out state (UsageTracker keyed by material index, src/lib.rs:57-89).
  * stage[cell]: 0 scored scan, 1 fallback first-placeable, 2 nearest_one,
- * 255 untouched.  dist/score: chosen candidate's squared distance and total
- * score (the fallbacks add no penalty). */
+ * 255 untouched.  dist/score: chosen candidate's colour distance and total
+ * score (the fallbacks add no penalty).
+ * score_mode 0 scores sqdist + penalty (src/main.rs:385-398); score_mode 1 is
+ * the GUI generator's find_best_tile_primary (src/gui/app_full.rs:1396-1475):
+ * sqrt(sqdist) + penalty, same candidates, bans, usage rule and fallbacks
+ * (:1477-1563).  dist holds the distance the mode scores with. */
 void orc_place(const float *lab, int32_t n, const float *cell_lab, int32_t gw, int32_t gh,
-               int32_t max_usage, float weight, const int32_t *order, int32_t n_order,
-               int32_t *grid, uint32_t *usage, uint8_t *stage, float *dist, float *score,
-               orc_place_stats *st) {
+               int32_t max_usage, float weight, int32_t score_mode, const int32_t *order,
+               int32_t n_order, int32_t *grid, uint32_t *usage, uint8_t *stage, float *dist,
+               float *score, orc_place_stats *st) {
     orc_place_stats s;
     memset(&s, 0, sizeof(s));
     int32_t cells = order ? n_order : gw * gh;
@@ -344,6 +348,8 @@ void orc_place(const float *lab, int32_t n, const float *cell_lab, int32_t gw, i
             continue;
         }
         orc_nearest_n(lab, n, cell_lab + 3 * (int64_t)cell, K, cidx, cd);
+        if (score_mode == 1) /* app_full.rs:1436-1439: sqrt of the same f32 sum */
+            for (int32_t r = 0; r < K; ++r) cd[r] = sqrtf(cd[r]);
         int32_t best = -1;
         float best_score = 0.0f;
         for (int32_t r = 0; r < K; ++r) {

@@ -81,9 +81,9 @@ typedef struct {
 } orc_place_stats;
 
 void orc_place(const float *lab, int32_t n, const float *cell_lab, int32_t grid_w, int32_t grid_h,
-               int32_t max_usage, float weight, const int32_t *order, int32_t n_order,
-               int32_t *grid, uint32_t *usage, uint8_t *stage, float *dist, float *score,
-               orc_place_stats *stats);
+               int32_t max_usage, float weight, int32_t score_mode, const int32_t *order,
+               int32_t n_order, int32_t *grid, uint32_t *usage, uint8_t *stage, float *dist,
+               float *score, orc_place_stats *stats);
 
 /* ---- a10-a12: cost, swap delta, annealing ---- */
 float orc_total_cost(const float *lab, int32_t n, const int32_t *grid, int32_t grid_w,

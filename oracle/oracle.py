@@ -65,7 +65,7 @@ def load():
         "orc_sim_full": (None, [P, i32, P]),
         "orc_sim_packed": (None, [P, i32, P]),
         "orc_nearest_n": (None, [P, i32, P, i32, P, P]),
-        "orc_place": (None, [P, i32, P, i32, i32, i32, f32, P, i32, P, P, P, P, P,
+        "orc_place": (None, [P, i32, P, i32, i32, i32, f32, i32, P, i32, P, P, P, P, P,
                              ctypes.POINTER(PlaceStats)]),
         "orc_total_cost": (f32, [P, i32, P, i32, i32]),
         "orc_swap_delta": (f32, [P, i32, P, i32, i32, i32, i32, i32, i32]),
@@ -177,7 +177,10 @@ def nearest_n(lab, query, k):
     return idx, d
 
 
-def place(lab, cell_lab, grid_w, grid_h, max_usage, weight, order=None, grid=None, usage=None):
+def place(lab, cell_lab, grid_w, grid_h, max_usage, weight, order=None, grid=None, usage=None,
+          score_mode=0):
+    """Placement chain; score_mode 0 = CLI (sqdist + penalty, src/main.rs:385-398),
+    1 = GUI (sqrt(sqdist) + penalty, src/gui/app_full.rs:1436-1448)."""
     lab = _c(lab, np.float32).reshape(-1, 3)
     n = lab.shape[0]
     cells = grid_w * grid_h
@@ -189,7 +192,7 @@ def place(lab, cell_lab, grid_w, grid_h, max_usage, weight, order=None, grid=Non
     dist = np.full(cells, np.nan, np.float32)
     score = np.full(cells, np.nan, np.float32)
     st = PlaceStats()
-    load().orc_place(_p(lab), n, _p(cl), grid_w, grid_h, max_usage, weight, _p(o),
+    load().orc_place(_p(lab), n, _p(cl), grid_w, grid_h, max_usage, weight, score_mode, _p(o),
                      0 if o is None else len(o), _p(g), _p(u), _p(stage), _p(dist), _p(score),
                      ctypes.byref(st))
     return {"grid": g, "usage": u[:n], "stage": stage, "dist": dist, "score": score,
@@ -302,11 +305,12 @@ def composite(target, materials, grid, grid_w, grid_h, strength):
 
 def generate(target, materials, grid_w, grid_h, max_usage, weight, color_strength,
              enable_optimization=True, optimization_iterations=1000, initial_temperature=100.0,
-             temperature_decay=0.99995, seed=0, image=True):
-    """generate_mosaic (src/main.rs:477-685) without file I/O."""
+             temperature_decay=0.99995, seed=0, image=True, score_mode=0):
+    """generate_mosaic (src/main.rs:477-685) without file I/O; score_mode 1 places
+    with the GUI generator's scoring (src/gui/app_full.rs:1396-1563)."""
     lab = stack_lab_means(materials)
     cell_lab = cell_lab_means(target, grid_w, grid_h)
-    pl = place(lab, cell_lab, grid_w, grid_h, max_usage, weight)
+    pl = place(lab, cell_lab, grid_w, grid_h, max_usage, weight, score_mode=score_mode)
     grid = pl["grid"]
     ann = None
     if enable_optimization and weight > 0.0:

@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol(mh):
 
 def test_version_and_error_string(mh):
     lib = mh.load()
-    assert lib.mh_abi_version() == 1
+    assert lib.mh_abi_version() == 2
     assert isinstance(lib.mh_last_error(), bytes)
 
 

@@ -5,7 +5,8 @@ mh_generate_device gives for the same job alone (grid, composited image,
 placement counters, annealing result), which the parity tests pin to the
 oracle.  Jobs mix grid sizes, libraries (including one with fewer than K=100
 materials and an empty one), SA on/off, colour strength 0 and weight 0, and
-read their targets as crops of shared canvases (row pitch != width*3).
+read their targets as crops of shared canvases (row pitch != width*3);
+every third job places with the GUI generator's Euclidean score.
 """
 import numpy as np
 import pytest
@@ -38,6 +39,7 @@ class Setup:
                 j["color"] = 0.0
             if k % 5 == 3:
                 j["weight"] = 0.0
+            j["score_mode"] = 1 if k % 3 == 2 else 0  # GUI scoring on every third job
             self.specs.append(j)
         # one job on the empty library
         e = dict(self.specs[0])
@@ -47,7 +49,8 @@ class Setup:
 
     def params(self, j):
         return self.mh.JobParams(j["grid_w"], j["grid_h"], j["max_usage"], j["weight"], j["color"],
-                                 j["sa"], j["iters"], 100.0, 0.99995, j["seed"])
+                                 j["sa"], j["iters"], 100.0, 0.99995, j["seed"],
+                                 j.get("score_mode", 0))
 
     def alloc(self, nbytes):
         p = self.ctx.device_alloc(nbytes)

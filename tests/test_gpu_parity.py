@@ -146,14 +146,17 @@ def _check_place(got, want):
     assert got["stats"] == want["stats"]
 
 
+@pytest.mark.parametrize("score_mode", [0, 1])
 @pytest.mark.parametrize("name", ["small", "medium"])
-def test_place_workloads(ctx, orc, synth, name):
+def test_place_workloads(ctx, orc, synth, name, score_mode):
     p, target, mats = synth.make(name)
     lab = orc.stack_lab_means(mats)
     cl = orc.cell_lab_means(target, p["grid_w"], p["grid_h"])
     lib = ctx.library_from_lab(lab)
-    got = ctx.place(lib, cl, p["grid_w"], p["grid_h"], p["max_usage"], p["weight"])
-    want = orc.place(lab, cl, p["grid_w"], p["grid_h"], p["max_usage"], p["weight"])
+    got = ctx.place(lib, cl, p["grid_w"], p["grid_h"], p["max_usage"], p["weight"],
+                    score_mode=score_mode)
+    want = orc.place(lab, cl, p["grid_w"], p["grid_h"], p["max_usage"], p["weight"],
+                     score_mode=score_mode)
     _check_place(got, want)
     if name == "small":  # 200 x 3 = 600 < 1200 cells: the reset path must run
         assert want["stats"]["resets"] >= 1
@@ -168,14 +171,17 @@ def test_place_workloads(ctx, orc, synth, name):
     (3, 7, 7, 2, 0.0),     # weight 0: no penalty
     (150, 13, 11, 1, -1.0),  # negative weight behaves like 0
     (500, 20, 20, 3, 2.0),
+    (300, 1, 40, 2, 4.0),   # one column: the up neighbour is the previous winner
+    (300, 2, 30, 2, 4.0),   # two columns
 ])
-def test_place_edge_cases(ctx, orc, n, gw, gh, usage, weight):
+@pytest.mark.parametrize("score_mode", [0, 1])
+def test_place_edge_cases(ctx, orc, n, gw, gh, usage, weight, score_mode):
     rng = _rng(n * 131 + gw)
     lab = (rng.random((n, 3)) * [100, 60, 60] - [0, 30, 30]).astype(np.float32)
     cl = (rng.random((gw * gh, 3)) * [100, 60, 60] - [0, 30, 30]).astype(np.float32)
     lib = ctx.library_from_lab(lab)
-    got = ctx.place(lib, cl, gw, gh, usage, weight)
-    want = orc.place(lab, cl, gw, gh, usage, weight)
+    got = ctx.place(lib, cl, gw, gh, usage, weight, score_mode=score_mode)
+    want = orc.place(lab, cl, gw, gh, usage, weight, score_mode=score_mode)
     _check_place(got, want)
     lib.close()
 
@@ -189,7 +195,8 @@ def test_place_empty_library(ctx, orc):
     lib.close()
 
 
-def test_place_order_and_prefilled_state(ctx, orc):
+@pytest.mark.parametrize("score_mode", [0, 1])
+def test_place_order_and_prefilled_state(ctx, orc, score_mode):
     # find_and_use_best_tile_with_position at arbitrary cells with a
     # pre-filled grid and usage (src/main.rs:1250-1283): all 4 neighbours count
     rng = _rng(11)
@@ -201,9 +208,34 @@ def test_place_order_and_prefilled_state(ctx, orc):
     usage = rng.integers(0, 3, n).astype(np.uint32)
     order = rng.permutation(np.nonzero(grid < 0)[0]).astype(np.int32)
     lib = ctx.library_from_lab(lab)
-    got = ctx.place(lib, cl, gw, gh, 3, 0.5, order=order, grid=grid, usage=usage)
-    want = orc.place(lab, cl, gw, gh, 3, 0.5, order=order, grid=grid, usage=usage)
+    got = ctx.place(lib, cl, gw, gh, 3, 0.5, order=order, grid=grid, usage=usage,
+                    score_mode=score_mode)
+    want = orc.place(lab, cl, gw, gh, 3, 0.5, order=order, grid=grid, usage=usage,
+                     score_mode=score_mode)
     _check_place(got, want)
+    lib.close()
+
+
+@pytest.mark.parametrize("raster", [True, False])
+def test_place_gui_score_changes_the_winner(ctx, raster):
+    # material 1 is nearer (sqdist 1 vs 4) but identical to the left
+    # neighbour 0, so its penalty is 4*1 against material 2's 4*(1/2):
+    # squared scores 1+4 < 4+2 pick 1, Euclidean scores 1+4 > 2+2 pick 2
+    # (src/main.rs:385-398 vs src/gui/app_full.rs:1436-1448)
+    lab = np.array([[1, 0, 0], [1, 0, 0], [2, 0, 0]], np.float32)
+    lib = ctx.library_from_lab(lab)
+    for mode, pick, dist, score in [(0, 1, 1.0, 5.0), (1, 2, 2.0, 4.0)]:
+        if raster:
+            # raster chain: cell 0 (target = material 0's Lab) takes material 0
+            cl = np.array([[1, 0, 0], [0, 0, 0]], np.float32)
+            got = ctx.place(lib, cl, 2, 1, 1, 4.0, score_mode=mode)
+            assert got["grid"][0] == 0
+        else:
+            cl = np.array([[9, 9, 9], [0, 0, 0]], np.float32)
+            got = ctx.place(lib, cl, 2, 1, 3, 4.0, order=np.array([1], np.int32),
+                            grid=np.array([0, -1], np.int32), score_mode=mode)
+        assert got["grid"][1] == pick and got["stage"][1] == 0
+        assert got["dist"][1] == np.float32(dist) and got["score"][1] == np.float32(score)
     lib.close()
 
 
@@ -383,14 +415,15 @@ def test_composite(ctx, orc, synth, strength):
 
 
 # ------------------------------------------------------------ whole job
-@pytest.mark.parametrize("name", ["small", "medium"])
-def test_generate_matches_oracle(ctx, orc, synth, name):
+@pytest.mark.parametrize("name,score_mode", [("small", 0), ("medium", 0), ("small", 1),
+                                             ("medium", 1)])
+def test_generate_matches_oracle(ctx, orc, synth, name, score_mode):
     p, target, mats = synth.make(name)
     lib = ctx.library_from_pixels(mats)
     got = ctx.generate(lib, target, p["grid_w"], p["grid_h"], p["max_usage"], p["weight"],
-                       p["color"], True, p["iters"], seed=0)
+                       p["color"], True, p["iters"], seed=0, score_mode=score_mode)
     want = orc.generate(target, mats, p["grid_w"], p["grid_h"], p["max_usage"], p["weight"],
-                        p["color"], True, p["iters"], seed=0)
+                        p["color"], True, p["iters"], seed=0, score_mode=score_mode)
     np.testing.assert_array_equal(got["grid"], want["grid"])
     assert got["place"] == want["place"]["stats"]
     for key in ["initial_cost", "final_cost", "best_cost"]:

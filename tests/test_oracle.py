@@ -221,9 +221,11 @@ def test_place_penalty_prefers_dissimilar_neighbours(orc):
     assert r2["score"][1] == score
 
 
-def _place_bruteforce(lab, cell_lab, gw, gh, max_usage, weight):
+def _place_bruteforce(lab, cell_lab, gw, gh, max_usage, weight, score_mode=0):
     """Independent restatement for tiny grids: full candidate scoring with
-    numpy, ranks by lexsort, reset on failure (src/main.rs:335-475)."""
+    numpy, ranks by lexsort, reset on failure (src/main.rs:335-475).
+    score_mode 1 scores the Euclidean distance as the GUI generator does
+    (src/gui/app_full.rs:1436-1448)."""
     n = lab.shape[0]
     K = min(n, 100)
     grid = np.full(gw * gh, -1)
@@ -246,7 +248,8 @@ def _place_bruteforce(lab, cell_lab, gw, gh, max_usage, weight):
                     s = f32(np.sqrt(f32((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2])))
                     p = f32(p + f32(f32(1.0) / f32(s + f32(1.0))))
                 p = f32(p * f32(weight))
-            score = f32(sq[m] + p)
+            d_m = f32(np.sqrt(sq[m])) if score_mode == 1 else sq[m]
+            score = f32(d_m + p)
             if best is None or score < best[0]:
                 best = (score, m)
         if best is None:
@@ -259,16 +262,81 @@ def _place_bruteforce(lab, cell_lab, gw, gh, max_usage, weight):
     return grid
 
 
+@pytest.mark.parametrize("score_mode", [0, 1])
 @pytest.mark.parametrize("seed", range(6))
-def test_place_matches_bruteforce_on_tiny_grids(orc, seed):
+def test_place_matches_bruteforce_on_tiny_grids(orc, seed, score_mode):
     rng = _rng(100 + seed)
     n = int(rng.integers(1, 30))
     gw, gh = int(rng.integers(1, 7)), int(rng.integers(1, 6))
     lab = np.round(rng.uniform(-20, 20, (n, 3))).astype(np.float32)
     cl = np.round(rng.uniform(-20, 20, (gw * gh, 3))).astype(np.float32)
     usage, w = int(rng.integers(0, 4)), float(rng.choice([0.0, 0.25, 3.0]))
-    got = orc.place(lab, cl, gw, gh, usage, w)["grid"]
-    assert np.array_equal(got, _place_bruteforce(lab, cl, gw, gh, usage, w))
+    got = orc.place(lab, cl, gw, gh, usage, w, score_mode=score_mode)["grid"]
+    assert np.array_equal(got, _place_bruteforce(lab, cl, gw, gh, usage, w, score_mode))
+
+
+def test_gui_and_cli_scores_pick_different_winners(orc):
+    # material 1 is nearer (sqdist 1 vs 4) but identical to the left neighbour
+    # 0 (penalty 4*1 vs 4*1/2): squared 1+4 < 4+2, Euclidean 1+4 > 2+2
+    lab = np.array([[1, 0, 0], [1, 0, 0], [2, 0, 0]], np.float32)
+    cl = np.array([[9, 9, 9], [0, 0, 0]], np.float32)
+    for mode, pick, dist, score in [(0, 1, 1.0, 5.0), (1, 2, 2.0, 4.0)]:
+        r = orc.place(lab, cl, 2, 1, 3, 4.0, order=np.array([1], np.int32),
+                      grid=np.array([0, -1], np.int32), score_mode=mode)
+        assert r["grid"][1] == pick and r["stage"][1] == 0
+        assert r["dist"][1] == dist and r["score"][1] == score
+
+
+def _gui_tiles():
+    # the two tiles of the GUI generator tests (src/gui/app_full.rs:2813-2824)
+    return np.array([[50, 0, 0], [75, 10, 5]], np.float32)
+
+
+def _gui_place(orc, lab, gw, gh, max_usage, weight, cells, target=(50.0, 0.0, 0.0)):
+    """find_best_tile_for_position at `cells` in turn with one target Lab
+    (src/gui/app_full.rs:1347-1394); the reference tests use an empty
+    similarity database, i.e. zero penalties, so weight 0 matches them."""
+    cl = np.tile(np.array(target, np.float32), (gw * gh, 1))
+    grid = np.full(gw * gh, -1, np.int32)
+    usage = np.zeros(max(len(lab), 1), np.uint32)
+    stages = []
+    for c in cells:
+        r = orc.place(lab, cl, gw, gh, max_usage, weight, order=np.array([c], np.int32),
+                      grid=grid, usage=usage, score_mode=1)
+        grid, usage = r["grid"], np.resize(r["usage"], max(len(lab), 1)).astype(np.uint32)
+        stages.append(int(r["stage"][c]))
+    return grid, stages
+
+
+@pytest.mark.parametrize("weight", [0.0, 0.8])
+def test_gui_comprehensive_fallback_scenario(orc, weight):
+    # src/gui/app_full.rs:2804-2856: 2x2, usage limit 1, every cell filled;
+    # (0,1) needs the usage reset (both tiles used, tile 0 is above it)
+    grid, stages = _gui_place(orc, _gui_tiles(), 2, 2, 1, weight, [0, 1, 2, 3])
+    assert grid.tolist() == [0, 1, 1, 0]
+    assert stages == [0, 0, 1, 0]
+
+
+def test_gui_usage_tracker_reset_in_fallback(orc):
+    # src/gui/app_full.rs:3132-3165: one tile, limit 1, (0,0) then (1,1)
+    grid, stages = _gui_place(orc, _gui_tiles()[:1], 2, 2, 1, 0.3, [0, 3])
+    assert grid.tolist() == [0, -1, -1, 0] and stages == [0, 1]
+
+
+def test_gui_fallback_with_adjacency_constraints(orc):
+    # src/gui/app_full.rs:2682-2723: limit 1, (0,0) then (1,0)
+    grid, stages = _gui_place(orc, _gui_tiles(), 2, 2, 1, 0.3, [0, 1])
+    assert grid.tolist() == [0, 1, -1, -1] and stages == [0, 0]
+
+
+def test_gui_final_fallback_and_empty_library(orc):
+    # src/gui/app_full.rs:2725-2753: the only tile is banned by its
+    # neighbour, so nearest_one places it anyway; :2640-2654: no tiles -> None
+    grid, stages = _gui_place(orc, _gui_tiles()[:1], 2, 2, 3, 0.3, [0, 1])
+    assert grid.tolist() == [0, 0, -1, -1] and stages == [0, 2]
+    r = orc.place(np.zeros((0, 3), np.float32), np.zeros((4, 3), np.float32), 2, 2, 3, 0.3,
+                  score_mode=1)
+    assert r["grid"].tolist() == [-1] * 4 and r["stats"]["empty"] == 4
 
 
 # ------------------------------------------------ a10-a12 cost, delta, SA
