@@ -92,6 +92,51 @@ def _pct(xs, q):
     return round(float(np.percentile(np.asarray(xs, np.float64), q)), 4) if len(xs) else None
 
 
+def run_ingest(mh, synth, ctx, reps, oracle=None):
+    """Material ingest: 10,000 materials of 64x64 (one size), then 10,000 of
+    four sizes (48x48, 64x64, 96x64, 128x96) interleaved, into resident
+    libraries (pixels + Lab means).  Device time is the Lab kernel's; the
+    end-to-end rate includes host packing and the PCIe upload."""
+    rng = np.random.default_rng(11)
+    uniform = list(synth.flat_materials(10000, 64, rng))
+    sizes = [(48, 48), (64, 64), (96, 64), (128, 96)]
+    by_size = [synth.flat_materials(2500, 1, rng) for _ in sizes]  # colours only
+    mixed = []
+    for i in range(10000):
+        w, h = sizes[i % 4]
+        c = by_size[i % 4][i // 4, 0, 0].astype(np.int16)
+        noise = rng.integers(-12, 13, (h, w, 3), dtype=np.int16)
+        mixed.append(np.clip(c + noise, 0, 255).astype(np.uint8))
+    out = {}
+    for name, mats in (("uniform_64", uniform), ("mixed_4_sizes", mixed)):
+        px = sum(m.shape[0] * m.shape[1] for m in mats)
+        ctx.library_from_images(mats[:64]).close()  # warm
+        ctx.set_timing(True)
+        ctx.timing()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.library_from_images(mats).close()
+        el = (time.perf_counter() - t0) / reps
+        ph = ctx.timing()["cell_lab"]
+        ctx.set_timing(False)
+        k_ms = ph[0] / max(ph[1], 1)
+        out[name] = {"materials": len(mats), "pixels": px,
+                     "lab_kernel_ms": round(k_ms, 4),
+                     "lab_pixels_per_s": round(px / (k_ms * 1e-3), 1),
+                     "lab_hbm_gbs": round(3.0 * px / (k_ms * 1e-3) / 1e9, 2),
+                     "end_to_end_ms": round(1000 * el, 2),
+                     "materials_per_s_end_to_end": round(len(mats) / el, 1)}
+    if oracle is not None:  # the C restatement of calculate_average_lab, one core
+        sample = uniform[:300]
+        t0 = time.perf_counter()
+        for m in sample:
+            oracle.average_lab(m)
+        el = time.perf_counter() - t0
+        out["cpu_port_materials_per_s"] = round(len(sample) / el, 1)
+        out["cpu_sample"] = "300 materials of 64x64, one core"
+    return out
+
+
 def run_batched(mh, synth, ctx, steps, warmup):
     """64 independent jobs (own libraries, grids 20-160 x 15-120 at 16 px)
     plain one at a time, then as one fused batch replayed from a hipGraph."""
@@ -285,6 +330,7 @@ def main():
                     help="also run a device-to-device copy of BYTES (counter calibration)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-job batched form")
     ap.add_argument("--no-serve", action="store_true", help="skip the served job stream")
+    ap.add_argument("--no-ingest", action="store_true", help="skip the material-ingest leg")
     ap.add_argument("--serve-jobs", type=int, default=5000)
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrivals per second")
     ap.add_argument("--serve-streams", type=int, default=4)
@@ -417,10 +463,16 @@ def main():
                                    int(args.serve_budget_gib * 2**30), args.serve_streams,
                                    args.serve_max_batch)
 
+    orc = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle
-        oracle.load()
+        import oracle as orc
+        orc.load()
+    if not args.no_ingest:
+        out["ingest"] = run_ingest(mh, synth, ctx, 3, orc)
+
+    if not args.no_cpu:
+        oracle = orc
         rate, jobs, el = cpu_baseline(oracle, p, target, mats, args.cpu_seconds)
         out["cpu_baseline"] = {"value": round(rate, 4), "unit": "jobs/s", "cores": 1,
                                "kind": "port",

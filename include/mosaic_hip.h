@@ -127,6 +127,19 @@ int mh_library_from_lab(mh_context *ctx, const float *lab, int32_t n, mh_library
  * on the GPU.  The pixels stay resident for compositing. */
 int mh_library_from_pixels(mh_context *ctx, const uint8_t *rgb, int32_t n, int32_t w, int32_t h,
                            mh_library **out);
+/* Material ingest of images of different sizes (load_tiles' per-file
+ * process_tile, src/main.rs:166-301): image i is widths[i] x heights[i]
+ * RGB8, row-major, at rgb + offsets[i].  Each dimension is >= 1 and an image
+ * holds at most 2^28 pixels.  mh_average_lab_many writes
+ * calculate_average_lab of each image (out = n x 3) from one batched
+ * launch; mh_library_from_images keeps the pixels resident and resizes each
+ * material from its own size when tiles are composited
+ * (resize_image, src/main.rs:710-738). */
+int mh_average_lab_many(mh_context *ctx, const uint8_t *rgb, const int64_t *offsets,
+                        const int32_t *widths, const int32_t *heights, int32_t n, float *out);
+int mh_library_from_images(mh_context *ctx, const uint8_t *rgb, const int64_t *offsets,
+                           const int32_t *widths, const int32_t *heights, int32_t n,
+                           mh_library **out);
 void mh_library_destroy(mh_library *lib);
 int32_t mh_library_size(const mh_library *lib);
 /* SimilarityDatabase::get_lab_color for every material (n x 3 f32). */

@@ -49,11 +49,13 @@ __global__ void resize_h_kernel(const uint8_t* __restrict__ src, int32_t n, int3
     d[2] = (uint8_t)clip8(a2);
 }
 
-// Vertical pass: tmp [n][sh][dw][3] -> dst [n][dh][dw][3].
+// Vertical pass: tmp [n][sh][dw][3] -> dst [slot(m)][dh][dw][3], slot(m) =
+// dst_index[m] (or m without an index).
 __global__ void resize_v_kernel(const uint8_t* __restrict__ tmp, int32_t n, int32_t sh, int32_t dw,
                                 int32_t dh, const int32_t* __restrict__ lo,
                                 const int32_t* __restrict__ cnt, const int32_t* __restrict__ kk,
-                                int32_t ks, uint8_t* __restrict__ dst) {
+                                int32_t ks, const int32_t* __restrict__ dst_index,
+                                uint8_t* __restrict__ dst) {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t total = (int64_t)n * dh * dw;
     if (idx >= total) return;
@@ -72,7 +74,8 @@ __global__ void resize_v_kernel(const uint8_t* __restrict__ tmp, int32_t n, int3
         a1 += (int32_t)s[t * step + 1] * w;
         a2 += (int32_t)s[t * step + 2] * w;
     }
-    uint8_t* d = dst + idx * 3;
+    const int64_t slot = dst_index ? (int64_t)dst_index[m] : m;
+    uint8_t* d = dst + ((slot * dh + o) * dw + x) * 3;
     d[0] = (uint8_t)clip8(a0);
     d[1] = (uint8_t)clip8(a1);
     d[2] = (uint8_t)clip8(a2);
@@ -89,7 +92,7 @@ __global__ __launch_bounds__(256) void resize_fused_kernel(
     const int32_t* __restrict__ hlo, const int32_t* __restrict__ hcnt,
     const int32_t* __restrict__ hk, int32_t hks, const int32_t* __restrict__ vlo,
     const int32_t* __restrict__ vcnt, const int32_t* __restrict__ vk, int32_t vks,
-    uint8_t* __restrict__ dst) {
+    const int32_t* __restrict__ dst_index, uint8_t* __restrict__ dst) {
     extern __shared__ __align__(16) uint8_t rsm[];
     const int32_t m = blockIdx.x;
     const int32_t src_bytes = sw * sh * 3;
@@ -124,7 +127,7 @@ __global__ __launch_bounds__(256) void resize_fused_kernel(
         d[2] = (uint8_t)clip8(a2);
     }
     __syncthreads();
-    uint8_t* out = dst + (int64_t)m * dh * dw * 3;
+    uint8_t* out = dst + (int64_t)(dst_index ? dst_index[m] : m) * dh * dw * 3;
     for (int32_t idx = tid; idx < dh * dw; idx += 256) {
         const int32_t o = idx / dw, x = idx - o * dw;
         const uint8_t* col = s_tmp + (vlo[o] * dw + x) * 3;
@@ -150,12 +153,12 @@ size_t resize_fused_lds_bytes(int32_t sw, int32_t sh, int32_t dw) {
 
 void launch_resize(const uint8_t* src, int32_t n, int32_t sw, int32_t sh, int32_t dw, int32_t dh,
                    const ResizeTaps& h, const ResizeTaps& v, uint8_t* tmp, uint8_t* dst,
-                   hipStream_t stream) {
+                   hipStream_t stream, const int32_t* dst_index) {
     if (n <= 0 || dw <= 0 || dh <= 0) return;
     const size_t lds = resize_fused_lds_bytes(sw, sh, dw);
     if (lds <= (size_t)kFusedResizeMaxLds) {
         hipLaunchKernelGGL(resize_fused_kernel, dim3(n), dim3(256), lds, stream, src, sw, sh, dw, dh,
-                           h.lo, h.cnt, h.k, h.ksize, v.lo, v.cnt, v.k, v.ksize, dst);
+                           h.lo, h.cnt, h.k, h.ksize, v.lo, v.cnt, v.k, v.ksize, dst_index, dst);
         return;
     }
     const int64_t th = (int64_t)n * sh * dw;
@@ -163,7 +166,7 @@ void launch_resize(const uint8_t* src, int32_t n, int32_t sw, int32_t sh, int32_
                        n, sw, sh, dw, h.lo, h.cnt, h.k, h.ksize, tmp);
     const int64_t tv = (int64_t)n * dh * dw;
     hipLaunchKernelGGL(resize_v_kernel, dim3((unsigned)((tv + 255) / 256)), dim3(256), 0, stream, tmp,
-                       n, sh, dw, dh, v.lo, v.cnt, v.k, v.ksize, dst);
+                       n, sh, dw, dh, v.lo, v.cnt, v.k, v.ksize, dst_index, dst);
 }
 
 // Integer average RGB of cells (calculate_average_rgb): one 256-thread

@@ -262,19 +262,32 @@ int library_tiles(const mh_library* clib, int32_t tw, int32_t th, hipStream_t st
     auto key = std::make_pair(tw, th);
     auto it = lib->tiles.find(key);
     if (it == lib->tiles.end()) {
-        ResizeTaps h, v;
-        CHK(get_taps(ctx, lib->pw, tw, &h));
-        CHK(get_taps(ctx, lib->ph, th, &v));
+        // Lanczos taps per source size; the two-pass fallback's intermediate
+        // is sized for the largest group
+        std::vector<std::pair<ResizeTaps, ResizeTaps>> taps(lib->groups.size());
+        size_t tmp_bytes = 16;
+        for (size_t g = 0; g < lib->groups.size(); ++g) {
+            const PixelGroup& pg = lib->groups[g];
+            CHK(get_taps(ctx, pg.w, tw, &taps[g].first));
+            CHK(get_taps(ctx, pg.h, th, &taps[g].second));
+            tmp_bytes = std::max(tmp_bytes, (size_t)pg.count * pg.h * tw * 3);
+        }
         const size_t n = (size_t)(lib->n > 0 ? lib->n : 1);
         TileSet ts;
         uint8_t* tmp = nullptr;
         hipError_t e = hipMalloc(&ts.tiles, n * th * tw * 3);
         if (e == hipSuccess) e = hipMalloc(&ts.avg, n * 3);
-        if (e == hipSuccess) e = hipMalloc(&tmp, n * lib->ph * tw * 3);
+        if (e == hipSuccess) e = hipMalloc(&tmp, tmp_bytes);
         if (e == hipSuccess && lib->n > 0) {
             const bool timed = stream == ctx->stream;
             if (timed) phase_begin(ctx, MH_PHASE_RESIZE);
-            launch_resize(lib->d_pixels, lib->n, lib->pw, lib->ph, tw, th, h, v, tmp, ts.tiles, stream);
+            // one launch per source size into disjoint tile slots (the
+            // two-pass fallback's intermediate is reused in stream order)
+            for (size_t g = 0; g < lib->groups.size(); ++g) {
+                const PixelGroup& pg = lib->groups[g];
+                launch_resize(lib->d_pixels + pg.offset, pg.count, pg.w, pg.h, tw, th,
+                              taps[g].first, taps[g].second, tmp, ts.tiles, stream, pg.d_index);
+            }
             if (timed) phase_end(ctx, MH_PHASE_RESIZE);
             launch_avg_rgb(ts.tiles, (int64_t)tw * 3, 1, (int64_t)tw * th * 3, 0, tw, th, lib->n, ts.avg,
                            stream);
@@ -680,8 +693,7 @@ int mh_library_from_pixels(mh_context* ctx, const uint8_t* rgb, int32_t n, int32
     mh_library* lib = nullptr;
     CHK(library_alloc(ctx, n, &lib));
     const size_t bytes = (size_t)n * w * h * 3;
-    lib->pw = w;
-    lib->ph = h;
+    if (n > 0) lib->groups.push_back(PixelGroup{w, h, n, 0, nullptr});
     int rc = MH_OK;
     hipError_t e = hipMalloc(&lib->d_pixels, bytes ? bytes : 16);
     if (e == hipSuccess && bytes) e = hipMemcpy(lib->d_pixels, rgb, bytes, hipMemcpyHostToDevice);
@@ -710,6 +722,158 @@ int mh_library_from_pixels(mh_context* ctx, const uint8_t* rgb, int32_t n, int32
     return MH_OK;
 }
 
+// ------------------------------------------- material ingest, mixed sizes
+// Images of different sizes are grouped by size (in order of first
+// appearance) and packed group after group, so one Lab segment and one
+// resize launch serve a whole group.
+constexpr int64_t kMaxImagePixels = (int64_t)1 << 28;
+
+struct RaggedPlan {
+    std::vector<PixelGroup> groups;    // offsets into the packed buffer
+    std::vector<int32_t> group_first;  // first packed slot of each group
+    std::vector<int32_t> slot_image;   // packed slot -> image index
+    size_t bytes = 0;
+};
+
+static int ragged_plan(const uint8_t* rgb, const int64_t* offsets, const int32_t* widths,
+                       const int32_t* heights, int32_t n, RaggedPlan* p) {
+    if (n < 0 || (n > 0 && (!rgb || !offsets || !widths || !heights)))
+        return fail(MH_ERR_INVALID_ARGUMENT, "images: NULL argument or n < 0");
+    std::map<std::pair<int32_t, int32_t>, int32_t> group_of;
+    std::vector<std::vector<int32_t>> members;
+    for (int32_t i = 0; i < n; ++i) {
+        const int32_t w = widths[i], h = heights[i];
+        if (w <= 0 || h <= 0 || (int64_t)w * h > kMaxImagePixels)
+            return fail(MH_ERR_INVALID_ARGUMENT, "image %d: size %d x %d outside 1..2^28 pixels", i,
+                        w, h);
+        if (offsets[i] < 0) return fail(MH_ERR_INVALID_ARGUMENT, "image %d: negative offset", i);
+        auto it = group_of.find(std::make_pair(w, h));
+        if (it == group_of.end()) {
+            it = group_of.emplace(std::make_pair(w, h), (int32_t)members.size()).first;
+            members.emplace_back();
+        }
+        members[it->second].push_back(i);
+    }
+    size_t off = 0;
+    for (const std::vector<int32_t>& m : members) {
+        PixelGroup g;
+        g.w = widths[m[0]];
+        g.h = heights[m[0]];
+        g.count = (int32_t)m.size();
+        g.offset = (int64_t)off;
+        p->group_first.push_back((int32_t)p->slot_image.size());
+        p->slot_image.insert(p->slot_image.end(), m.begin(), m.end());
+        off += (size_t)g.count * g.w * g.h * 3;
+        p->groups.push_back(g);
+    }
+    p->bytes = off;
+    return MH_OK;
+}
+
+// Packs the images in slot order and copies them to d_dst (p.bytes bytes).
+static int ragged_upload(mh_context* ctx, const uint8_t* rgb, const int64_t* offsets,
+                         const RaggedPlan& p, uint8_t* d_dst) {
+    if (p.bytes == 0) return MH_OK;
+    std::vector<uint8_t> pack(p.bytes);
+    for (size_t g = 0; g < p.groups.size(); ++g) {
+        const PixelGroup& pg = p.groups[g];
+        const size_t img = (size_t)pg.w * pg.h * 3;
+        for (int32_t k = 0; k < pg.count; ++k)
+            memcpy(pack.data() + pg.offset + k * img, rgb + offsets[p.slot_image[p.group_first[g] + k]],
+                   img);
+    }
+    HIPCHK(hipMemcpyAsync(d_dst, pack.data(), p.bytes, hipMemcpyHostToDevice, ctx->stream));
+    return sync(ctx);  // `pack` is pageable and local
+}
+
+// calculate_average_lab of every packed image (one batched launch, one
+// segment per size group); lab_out = n x 3 in image order.
+static int ragged_lab(mh_context* ctx, const uint8_t* d_pixels, const RaggedPlan& p,
+                      float* lab_out) {
+    const int32_t n = (int32_t)p.slot_image.size();
+    if (n == 0) return MH_OK;
+    const int32_t n_segs = (int32_t)p.groups.size();
+    std::vector<CellSeg> segs((size_t)n_segs);
+    std::vector<int32_t> starts((size_t)n_segs + 1);
+    float* d_lab;
+    CHK(slot(ctx, S_OUT0, (size_t)n * 3, &d_lab));
+    int32_t blocks = 0;
+    for (int32_t g = 0; g < n_segs; ++g) {
+        const PixelGroup& pg = p.groups[g];
+        segs[g] = CellSeg{d_pixels + pg.offset, (int64_t)pg.w * 3, (int64_t)pg.w * pg.h * 3, 0, 1,
+                          pg.w, pg.h, pg.count, d_lab + 3 * (int64_t)p.group_first[g], nullptr};
+        starts[g] = blocks;
+        blocks += cell_lab_blocks(pg.count);
+    }
+    starts[n_segs] = blocks;
+    CellSeg* d_segs;
+    int32_t* d_starts;
+    CHK(upload(ctx, S_IN1, segs.data(), segs.size(), &d_segs));
+    CHK(upload(ctx, S_IN2, starts.data(), starts.size(), &d_starts));
+    phase_begin(ctx, MH_PHASE_CELL_LAB);
+    launch_cell_lab_batch(d_segs, d_starts, n_segs, blocks, ctx->d_lut, ctx->stream);
+    phase_end(ctx, MH_PHASE_CELL_LAB);
+    HIPCHK(hipGetLastError());
+    std::vector<float> packed((size_t)n * 3);
+    CHK(download(ctx, packed.data(), d_lab, packed.size()));
+    CHK(sync(ctx));
+    for (int32_t k = 0; k < n; ++k)
+        memcpy(lab_out + 3 * (size_t)p.slot_image[k], packed.data() + 3 * (size_t)k, 3 * sizeof(float));
+    return MH_OK;
+}
+
+int mh_average_lab_many(mh_context* ctx, const uint8_t* rgb, const int64_t* offsets,
+                        const int32_t* widths, const int32_t* heights, int32_t n, float* out) {
+    if (!ctx || (n > 0 && !out)) return fail(MH_ERR_INVALID_ARGUMENT, "mh_average_lab_many: NULL argument");
+    RaggedPlan p;
+    CHK(ragged_plan(rgb, offsets, widths, heights, n, &p));
+    if (n == 0) return MH_OK;
+    CHK(set_device(ctx));
+    uint8_t* d_in;
+    CHK(slot(ctx, S_IN0, p.bytes, &d_in));
+    CHK(ragged_upload(ctx, rgb, offsets, p, d_in));
+    return ragged_lab(ctx, d_in, p, out);
+}
+
+int mh_library_from_images(mh_context* ctx, const uint8_t* rgb, const int64_t* offsets,
+                           const int32_t* widths, const int32_t* heights, int32_t n,
+                           mh_library** out) {
+    if (!ctx || !out) return fail(MH_ERR_INVALID_ARGUMENT, "mh_library_from_images: NULL argument");
+    RaggedPlan p;
+    CHK(ragged_plan(rgb, offsets, widths, heights, n, &p));
+    CHK(set_device(ctx));
+    mh_library* lib = nullptr;
+    CHK(library_alloc(ctx, n, &lib));
+    int rc = MH_OK;
+    hipError_t e = hipMalloc(&lib->d_pixels, p.bytes ? p.bytes : 16);
+    if (e != hipSuccess)
+        rc = fail(e == hipErrorOutOfMemory ? MH_ERR_OUT_OF_MEMORY : MH_ERR_HIP, "library pixels: %s",
+                  hipGetErrorString(e));
+    for (size_t g = 0; g < p.groups.size() && !rc; ++g) {
+        PixelGroup pg = p.groups[g];
+        const int32_t* idx = p.slot_image.data() + p.group_first[g];
+        bool identity = p.group_first[g] == 0;
+        for (int32_t k = 0; k < pg.count && identity; ++k) identity = idx[k] == k;
+        if (!identity) {
+            e = hipMalloc(&pg.d_index, sizeof(int32_t) * pg.count);
+            if (e == hipSuccess)
+                e = hipMemcpy(pg.d_index, idx, sizeof(int32_t) * pg.count, hipMemcpyHostToDevice);
+            if (e != hipSuccess)
+                rc = fail(MH_ERR_HIP, "library index: %s", hipGetErrorString(e));
+        }
+        lib->groups.push_back(pg);  // owned by the library even on failure
+    }
+    if (!rc) rc = ragged_upload(ctx, rgb, offsets, p, lib->d_pixels);
+    if (!rc) rc = ragged_lab(ctx, lib->d_pixels, p, lib->h_lab.data());
+    if (!rc) rc = library_upload_lab(lib);
+    if (rc) {
+        mh_library_destroy(lib);
+        return rc;
+    }
+    *out = lib;
+    return MH_OK;
+}
+
 void mh_library_destroy(mh_library* lib) {
     if (!lib) return;
     if (lib->ctx) {
@@ -723,6 +887,8 @@ void mh_library_destroy(mh_library* lib) {
     if (lib->d_sim) hipFree(lib->d_sim);
     if (lib->d_term) hipFree(lib->d_term);
     if (lib->d_pixels) hipFree(lib->d_pixels);
+    for (PixelGroup& g : lib->groups)
+        if (g.d_index) hipFree(g.d_index);
     for (auto& kv : lib->tiles) {
         if (kv.second.tiles) hipFree(kv.second.tiles);
         if (kv.second.avg) hipFree(kv.second.avg);

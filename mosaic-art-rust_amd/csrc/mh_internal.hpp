@@ -95,6 +95,15 @@ struct mh_context {
     int64_t last_iters = 0;
 };
 
+// Material pixels of one size: `count` RGB8 images of w x h stored back to
+// back from byte `offset` of mh_library::d_pixels; image k of the group is
+// material d_index[k] (d_index == nullptr: the group is materials 0..count-1).
+struct PixelGroup {
+    int32_t w = 0, h = 0, count = 0;
+    int64_t offset = 0;
+    int32_t* d_index = nullptr;
+};
+
 struct mh_library {
     mh_context* ctx = nullptr;
     int32_t n = 0;
@@ -106,7 +115,7 @@ struct mh_library {
     float* d_sim = nullptr;    // n x n after build_similarities
     float* d_term = nullptr;   // n x n 1/(sim+1), built on first use by the cost kernels
     uint8_t* d_pixels = nullptr;
-    int32_t pw = 0, ph = 0;
+    std::vector<PixelGroup> groups;  // materials grouped by size (empty: Labs only)
     // lazily built device caches (term matrix, resized tiles per cell size)
     std::mutex cache_mu;
     std::map<std::pair<int32_t, int32_t>, mh::host::TileSet> tiles;

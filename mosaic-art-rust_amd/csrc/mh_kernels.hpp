@@ -153,9 +153,11 @@ void launch_swap_deltas(const int32_t* grid, int32_t gw, int32_t gh, const float
                         hipStream_t stream);
 void launch_penalties(const int32_t* grid, int32_t gw, int32_t gh, const float4* lab4, float weight,
                       const int32_t* queries, int32_t nq, float* out, hipStream_t stream);
+// n images of sw x sh (contiguous) -> dw x dh tiles; image m lands in tile
+// slot dst_index[m] (nullptr: slot m).
 void launch_resize(const uint8_t* src, int32_t n, int32_t sw, int32_t sh, int32_t dw, int32_t dh,
                    const ResizeTaps& h, const ResizeTaps& v, uint8_t* tmp, uint8_t* dst,
-                   hipStream_t stream);
+                   hipStream_t stream, const int32_t* dst_index = nullptr);
 void launch_avg_rgb(const uint8_t* rgb, int64_t row_stride, int32_t per_row,
                     int64_t block_row_stride, int64_t block_col_stride, int32_t cw, int32_t ch,
                     int32_t n_cells, uint8_t* out, hipStream_t stream);

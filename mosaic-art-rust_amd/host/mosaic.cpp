@@ -7,7 +7,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <filesystem>
 #include <fstream>
+#include <iostream>
 #include <mutex>
 #include <sstream>
 
@@ -722,20 +724,126 @@ bool MosaicGeneratorImpl::is_aspect_ratio_match(float img_aspect, float target_a
     return std::fabs(img_aspect - target_aspect) <= tolerance;
 }
 
+// ------------------------------------------------------- material ingest
+namespace {
+
+// The images as one buffer with per-image offsets and sizes (the layout of
+// mh_average_lab_many / mh_library_from_images).
+struct PackedImages {
+    std::vector<uint8_t> bytes;
+    std::vector<int64_t> offsets;
+    std::vector<int32_t> widths, heights;
+};
+
+PackedImages pack_images(const std::vector<const RgbImage*>& images) {
+    PackedImages p;
+    size_t total = 0;
+    for (const RgbImage* im : images) {
+        if (im->data.size() != (size_t)im->width * im->height * 3)
+            throw Error(MH_ERR_INVALID_ARGUMENT, "image data does not match its size");
+        total += im->data.size();
+    }
+    p.bytes.reserve(total);
+    for (const RgbImage* im : images) {
+        p.offsets.push_back((int64_t)p.bytes.size());
+        p.widths.push_back((int32_t)im->width);
+        p.heights.push_back((int32_t)im->height);
+        p.bytes.insert(p.bytes.end(), im->data.begin(), im->data.end());
+    }
+    return p;
+}
+
+}  // namespace
+
+bool is_material_file(const Path& path) {
+    const size_t slash = path.find_last_of('/');
+    const std::string name = slash == Path::npos ? path : path.substr(slash + 1);
+    const size_t dot = name.find_last_of('.');
+    // Path::extension is None for a name that only starts with a dot (".png")
+    if (dot == std::string::npos || dot == 0) return false;
+    std::string ext = name.substr(dot + 1);
+    for (char& c : ext) c = (char)std::tolower((unsigned char)c);
+    return ext == "png" || ext == "jpg" || ext == "jpeg" || ext == "webp";
+}
+
+MaterialSelection select_materials(
+    const std::vector<std::optional<std::pair<uint32_t, uint32_t>>>& sizes, float target_aspect,
+    float aspect_tolerance, size_t max_materials) {
+    MaterialSelection sel;
+    for (size_t i = 0; i < sizes.size(); ++i) {  // process_tile, :263-285
+        if (!sizes[i]) continue;
+        const float aspect = (float)sizes[i]->first / (float)sizes[i]->second;
+        if (MosaicGeneratorImpl::is_aspect_ratio_match(aspect, target_aspect, aspect_tolerance))
+            sel.picked.push_back(i);
+    }
+    if (sel.picked.empty()) {  // :214-249: unfiltered sample of the first 2*max entries
+        sel.fallback = true;
+        const size_t take = std::min(sizes.size(), max_materials * 2);
+        for (size_t i = 0; i < take; ++i)
+            if (sizes[i]) sel.picked.push_back(i);
+    }
+    if (sel.picked.size() > max_materials) sel.picked.resize(max_materials);  // :252-254
+    return sel;
+}
+
+std::vector<Tile> load_tiles(const std::vector<MaterialSource>& entries, float target_aspect,
+                             float aspect_tolerance, size_t max_materials,
+                             std::shared_ptr<Context> ctx) {
+    std::vector<std::optional<std::pair<uint32_t, uint32_t>>> sizes(entries.size());
+    for (size_t i = 0; i < entries.size(); ++i)
+        if (entries[i].image && entries[i].image->width > 0 && entries[i].image->height > 0)
+            sizes[i] = std::make_pair(entries[i].image->width, entries[i].image->height);
+    const MaterialSelection sel = select_materials(sizes, target_aspect, aspect_tolerance,
+                                                   max_materials);
+    std::vector<Tile> tiles;
+    if (sel.picked.empty()) return tiles;
+    std::vector<const RgbImage*> images;
+    for (size_t i : sel.picked) images.push_back(&*entries[i].image);
+    const PackedImages p = pack_images(images);
+    std::vector<float> lab(images.size() * 3);
+    check(mh_average_lab_many(or_default(std::move(ctx))->get(), p.bytes.data(), p.offsets.data(),
+                              p.widths.data(), p.heights.data(), (int32_t)images.size(),
+                              lab.data()));
+    for (size_t k = 0; k < sel.picked.size(); ++k) {
+        const RgbImage& im = *images[k];
+        Tile t;
+        t.path = entries[sel.picked[k]].path;
+        t.lab_color = Lab{lab[3 * k], lab[3 * k + 1], lab[3 * k + 2]};
+        t.aspect_ratio = (float)im.width / (float)im.height;
+        t.source_index = sel.picked[k];
+        tiles.push_back(std::move(t));
+    }
+    return tiles;
+}
+
+SimilarityDatabase prepare_similarity_db(const std::vector<Tile>& tiles, const Path& db_path,
+                                         bool rebuild_similarity) {
+    SimilarityDatabase db;
+    if (rebuild_similarity || !std::filesystem::exists(db_path)) {
+        for (const Tile& t : tiles) db.add_tile(t.path, t.lab_color);
+        db.build_similarities();
+        try {
+            db.save_to_file(db_path);
+        } catch (const std::exception& e) {
+            std::cerr << "Warning: Failed to save similarity database: " << e.what() << "\n";
+        }
+    } else {
+        db = SimilarityDatabase::load_or_new(db_path);
+    }
+    for (const Tile& t : tiles)
+        if (!db.get_lab_color(t.path)) db.add_tile(t.path, t.lab_color);
+    db.build_similarities();
+    return db;
+}
+
 // ----------------------------------------------------------- whole job
 MaterialLibrary::MaterialLibrary(const std::vector<RgbImage>& materials, std::shared_ptr<Context> ctx)
     : ctx_(or_default(std::move(ctx))) {
-    const uint32_t w = materials.empty() ? 1 : materials[0].width;
-    const uint32_t h = materials.empty() ? 1 : materials[0].height;
-    std::vector<uint8_t> px;
-    px.reserve((size_t)materials.size() * w * h * 3);
-    for (const RgbImage& m : materials) {
-        if (m.width != w || m.height != h)
-            throw Error(MH_ERR_INVALID_ARGUMENT, "materials must share one size (resize them first)");
-        px.insert(px.end(), m.data.begin(), m.data.end());
-    }
-    check(mh_library_from_pixels(ctx_->get(), px.data(), (int32_t)materials.size(), (int32_t)w,
-                                 (int32_t)h, &lib_));
+    std::vector<const RgbImage*> images;
+    for (const RgbImage& m : materials) images.push_back(&m);
+    const PackedImages p = pack_images(images);
+    check(mh_library_from_images(ctx_->get(), p.bytes.data(), p.offsets.data(), p.widths.data(),
+                                 p.heights.data(), (int32_t)materials.size(), &lib_));
 }
 
 MaterialLibrary::~MaterialLibrary() {

@@ -11,6 +11,8 @@
 //   calculate_optimal_adjustment  src/color_adjustment.rs:99-154
 //   UsageTracker                  src/lib.rs:57-89
 //   MosaicGeneratorImpl           src/lib.rs:91-124
+//   Tile, load_tiles, process_tile src/lib.rs:44-50, src/main.rs:166-301
+//   similarity-DB preparation     src/main.rs:119-143 (MosaicGenerator::new)
 // Differences a caller sees:
 //   * paths are std::string, grids are Grid (grid[y][x], std::nullopt = None);
 //   * Option<T> is std::optional<T>; anyhow::Result errors are thrown as
@@ -213,10 +215,58 @@ struct MosaicGeneratorImpl {
     static bool is_aspect_ratio_match(float img_aspect, float target_aspect, float tolerance);
 };
 
+// ------------------------------------------------------- material ingest
+struct Tile {  // src/lib.rs:44-50
+    Path path;
+    Lab lab_color;
+    float aspect_ratio = 0.0f;  // width / height in f32
+    size_t source_index = 0;    // position of the image in load_tiles' entries
+};
+
+// One directory entry handed to load_tiles: the path and the decoded RGB8
+// image, or std::nullopt when the file did not decode (the reference logs
+// such files and skips them, src/main.rs:200-206).  Decoding and directory
+// scanning stay with the caller.
+struct MaterialSource {
+    Path path;
+    std::optional<RgbImage> image;
+};
+
+// The extension filter of the directory scan (src/main.rs:172-184): png,
+// jpg, jpeg or webp in any letter case.
+bool is_material_file(const Path& path);
+
+// The entries load_tiles keeps, in entry order (src/main.rs:166-261): those
+// whose aspect ratio matches; when none does, the decodable ones among the
+// first min(entries, 2 * max_materials); then at most max_materials of them.
+// sizes[i] = {width, height}, or std::nullopt for an entry that failed.
+struct MaterialSelection {
+    std::vector<size_t> picked;
+    bool fallback = false;  // no aspect match: the unfiltered sample was taken
+};
+MaterialSelection select_materials(
+    const std::vector<std::optional<std::pair<uint32_t, uint32_t>>>& sizes, float target_aspect,
+    float aspect_tolerance, size_t max_materials);
+
+// load_tiles over decoded entries: select_materials, then the average Lab of
+// every picked image from one batched GPU launch (mh_average_lab_many).
+std::vector<Tile> load_tiles(const std::vector<MaterialSource>& entries, float target_aspect,
+                             float aspect_tolerance, size_t max_materials,
+                             std::shared_ptr<Context> ctx = nullptr);
+
+// MosaicGenerator::new's database step (src/main.rs:119-143): a fresh
+// database of the tiles, built and saved, when rebuild_similarity is set or
+// db_path does not exist (a failed save only warns on stderr); otherwise
+// load_or_new(db_path).  Tiles missing from it are then added and the
+// similarities rebuilt.
+SimilarityDatabase prepare_similarity_db(const std::vector<Tile>& tiles, const Path& db_path,
+                                         bool rebuild_similarity);
+
 // ----------------------------------------------------------- whole job
-// generate_mosaic minus file I/O: materials are equally sized RGB images
-// (material i is tile i of the result grid's indices), the target is
-// resized by the caller to the output size (src/main.rs:491-505).
+// generate_mosaic minus file I/O: materials are RGB images of any sizes
+// (material i is tile i of the result grid's indices; each is resized from
+// its own size to the cell size), the target is resized by the caller to the
+// output size (src/main.rs:491-505).
 // Candidate score of the placement scan: the CLI's squared Lab distance or
 // the GUI generator's Euclidean distance (src/gui/app_full.rs:1436-1448),
 // plus the adjacency penalty in both.

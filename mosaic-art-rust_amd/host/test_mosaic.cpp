@@ -745,6 +745,165 @@ TEST(test_generate_mosaic_whole_job) {
     for (int32_t m : out.grid) CHECK(m >= 0 && m < 12);
 }
 
+// ================================================= main.rs: material ingest
+using Sizes = std::vector<std::optional<std::pair<uint32_t, uint32_t>>>;
+
+Sizes squares(size_t n, uint32_t side = 100) {
+    return Sizes(n, std::make_pair(side, side));
+}
+
+std::vector<size_t> iota_n(size_t n) {
+    std::vector<size_t> v(n);
+    for (size_t i = 0; i < n; ++i) v[i] = i;
+    return v;
+}
+
+// src/main.rs:1697-1727
+HOST_TEST(test_load_tiles_file_extension_filtering) {
+    for (const char* p : {"red.png", "red.jpg", "red.jpeg", "dir/RED.PNG", "x.WebP", "a.b.jpg"})
+        CHECK(is_material_file(p));
+    for (const char* p : {"text.txt", "data.dat", "png", ".png", "dir.png/file", "file.", "x.gif"})
+        CHECK(!is_material_file(p));
+}
+
+// src/main.rs:1060-1078, :1093-1106, :1108-1121
+HOST_TEST(test_load_tiles_selection_basic_limit_and_fallback) {
+    MaterialSelection s = select_materials(squares(3), 1.0f, 0.1f, 10);
+    CHECK(s.picked == iota_n(3) && !s.fallback);
+    s = select_materials(squares(3), 1.0f, 0.1f, 2);
+    CHECK(s.picked == iota_n(2));
+    s = select_materials(squares(3), 10.0f, 0.1f, 10);
+    CHECK(s.picked == iota_n(3) && s.fallback);
+}
+
+// src/main.rs:1730-1753, :1861-1886: undecodable entries are skipped
+HOST_TEST(test_load_tiles_corrupted_image_handling) {
+    Sizes sz = squares(2);
+    sz[0] = std::nullopt;
+    CHECK(select_materials(sz, 1.0f, 0.1f, 10).picked == std::vector<size_t>{1});
+    CHECK(select_materials(sz, 7.0f, 0.1f, 10).picked == std::vector<size_t>{1});
+}
+
+// src/main.rs:1755-1777: the unfiltered sample is the first 2*max entries
+HOST_TEST(test_load_tiles_aspect_ratio_fallback_sampling) {
+    MaterialSelection s = select_materials(squares(10), 3.0f, 0.1f, 5);
+    CHECK(s.fallback && s.picked == iota_n(5));
+    Sizes sz = squares(10);
+    sz[1] = sz[3] = std::nullopt;  // the sample (entries 0-5) holds 4 decodable ones
+    s = select_materials(sz, 3.0f, 0.1f, 3);
+    CHECK((s.picked == std::vector<size_t>{0, 2, 4}));
+    s = select_materials(sz, 3.0f, 0.1f, 4);
+    CHECK((s.picked == std::vector<size_t>{0, 2, 4, 5}));
+}
+
+// src/main.rs:1779-1796, :1825-1838, :2013-2022
+HOST_TEST(test_load_tiles_max_materials_and_empty) {
+    CHECK(select_materials(squares(20), 1.0f, 0.1f, 10).picked == iota_n(10));
+    CHECK(select_materials(Sizes{}, 1.0f, 0.1f, 10).picked.empty());
+    CHECK(select_materials(squares(3), 1.0f, 0.1f, 0).picked.empty());
+    CHECK(select_materials(squares(3), 5.0f, 0.1f, 0).picked.empty());
+}
+
+// src/main.rs:1987-2011: the aspect test of process_tile; aspect = w/h in f32
+HOST_TEST(test_load_tiles_aspect_tolerance_edges) {
+    Sizes sz = {std::make_pair(100u, 100u), std::make_pair(200u, 100u), std::make_pair(177u, 100u)};
+    CHECK(select_materials(sz, 1.0f, -0.1f, 10).fallback);  // nothing matches a negative tolerance
+    CHECK(select_materials(sz, 1.0f, 1000.0f, 10).picked == iota_n(3));
+    CHECK((select_materials(sz, 16.0f / 9.0f, 0.1f, 10).picked == std::vector<size_t>{2}));
+    CHECK((select_materials(sz, 2.0f, 0.0f, 10).picked == std::vector<size_t>{1}));
+}
+
+// src/main.rs:808-827, :1798-1823, :2291-2305
+TEST(test_load_tiles_lab_and_aspect) {
+    std::vector<MaterialSource> e;
+    e.push_back({"red.png", RgbImage(100, 100, Rgb{255, 0, 0})});
+    e.push_back({"broken.png", std::nullopt});
+    for (int i = 0; i < 5; ++i)
+        e.push_back({"img_" + std::to_string(i) + ".png",
+                     RgbImage(100, 100, Rgb{(uint8_t)(i * 50 + 50), 0, 0})});
+    e.push_back({"tiny.png", RgbImage(1, 1, Rgb{255, 255, 255})});
+    e.push_back({"wide.png", RgbImage(300, 100, Rgb{0, 0, 255})});
+    std::vector<Tile> t = load_tiles(e, 1.0f, 0.1f, 10);
+    CHECK_EQ(t.size(), 7u);  // broken and wide are out
+    CHECK_EQ(t[0].path, "red.png");
+    CHECK(std::fabs(t[0].lab_color.l - 53.0f) < 5.0f);
+    CHECK(std::fabs(t[0].lab_color.a - 80.0f) < 5.0f);
+    CHECK(std::fabs(t[0].lab_color.b - 67.0f) < 5.0f);
+    for (const Tile& x : t) {
+        CHECK_EQ(x.aspect_ratio, 1.0f);
+        CHECK(x.lab_color.l > 0.0f && x.lab_color.l <= 100.0f);
+        CHECK_EQ(e[x.source_index].path, x.path);
+    }
+    CHECK_EQ(t[6].path, "tiny.png");
+}
+
+// the batched ingest equals calculate_average_lab image by image, for
+// images of several sizes in mixed order
+TEST(test_load_tiles_mixed_sizes_match_single_image_lab) {
+    std::vector<MaterialSource> e;
+    const uint32_t dims[][2] = {{64, 64}, {3, 7}, {100, 37}, {64, 64}, {1, 1}, {5, 300}, {3, 7}};
+    uint32_t seed = 7;
+    for (auto& d : dims) {
+        RgbImage im(d[0], d[1]);
+        for (uint8_t& b : im.data) b = (uint8_t)((seed = seed * 1103515245u + 12345u) >> 24);
+        e.push_back({"m" + std::to_string(e.size()) + ".png", im});
+    }
+    std::vector<Tile> t = load_tiles(e, 1.0f, 1000.0f, 100);
+    CHECK_EQ(t.size(), e.size());
+    for (size_t i = 0; i < t.size(); ++i) {
+        const Lab one = MosaicGeneratorImpl::calculate_average_lab(*e[i].image);
+        CHECK(std::memcmp(&one, &t[i].lab_color, sizeof(Lab)) == 0);
+    }
+}
+
+// src/main.rs:1561-1585, :1587-1624, :1656-1695
+TEST(test_new_with_similarity_database_rebuild_load_and_missing_tiles) {
+    std::vector<MaterialSource> e;
+    e.push_back({"red.png", RgbImage(100, 100, Rgb{255, 0, 0})});
+    e.push_back({"green.png", RgbImage(100, 100, Rgb{0, 255, 0})});
+    e.push_back({"blue.png", RgbImage(100, 100, Rgb{0, 0, 255})});
+    const std::string path = temp_path("prepare");
+    std::remove(path.c_str());
+    std::vector<Tile> t = load_tiles(e, 1.0f, 0.1f, 10);
+    SimilarityDatabase db = prepare_similarity_db(t, path, true);
+    CHECK_EQ(db.size(), 3u);
+    CHECK(std::ifstream(path).good());
+    // an existing file is loaded, not rebuilt
+    SimilarityDatabase db2 = prepare_similarity_db(t, path, false);
+    CHECK_EQ(db2.size(), 3u);
+    CHECK(db2.get_similarity("red.png", "blue.png") == db.get_similarity("red.png", "blue.png"));
+    // a new tile is added to the loaded database
+    e.push_back({"yellow.png", RgbImage(100, 100, Rgb{255, 255, 0})});
+    t = load_tiles(e, 1.0f, 0.1f, 10);
+    CHECK_EQ(t.size(), 4u);
+    SimilarityDatabase db3 = prepare_similarity_db(t, path, false);
+    CHECK_EQ(db3.size(), 4u);
+    CHECK(db3.get_similarity("yellow.png", "red.png").has_value());
+    std::remove(path.c_str());
+}
+
+// materials of different sizes in one library: each is resized from its own
+// size, so a solid material composites to its colour whatever its size
+TEST(test_generate_mosaic_mixed_size_materials) {
+    std::vector<RgbImage> mats = {RgbImage(40, 40, Rgb{255, 0, 0}), RgbImage(7, 13, Rgb{0, 255, 0}),
+                                  RgbImage(64, 48, Rgb{0, 0, 255}), RgbImage(7, 13, Rgb{250, 250, 0})};
+    MaterialLibrary lib(mats);
+    RgbImage target(64, 16);
+    for (uint32_t y = 0; y < 16; ++y)
+        for (uint32_t x = 0; x < 64; ++x) {
+            const Rgb c[4] = {{255, 0, 0}, {0, 255, 0}, {0, 0, 255}, {250, 250, 0}};
+            target.put_pixel(x, y, c[x / 16]);
+        }
+    MosaicSettings s;
+    s.grid_w = 4;
+    s.grid_h = 1;
+    s.color_adjustment_strength = 0.0f;
+    s.enable_optimization = false;
+    MosaicOutput out = lib.generate_mosaic(target, s);
+    CHECK((out.grid == std::vector<int32_t>{0, 1, 2, 3}));
+    for (uint32_t x = 0; x < 64; ++x) CHECK(out.image.get_pixel(x, 5) == target.get_pixel(x, 5));
+}
+
 int main(int argc, char** argv) {
     bool host_only = false, list = false;
     for (int i = 1; i < argc; ++i) {

@@ -33,7 +33,8 @@ EXPORTS = [
     "mh_context_timing", "mh_phase_name", "mh_device_alloc", "mh_device_free",
     "mh_copy_to_device", "mh_copy_to_host", "mh_copy_on_device", "mh_copy_2d_on_device",
     "mh_library_from_lab",
-    "mh_library_from_pixels", "mh_library_destroy", "mh_library_size", "mh_library_lab",
+    "mh_library_from_pixels", "mh_average_lab_many", "mh_library_from_images",
+    "mh_library_destroy", "mh_library_size", "mh_library_lab",
     "mh_library_build_similarities", "mh_library_similarities", "mh_library_get_similarity",
     "mh_average_lab", "mh_cell_lab_means", "mh_nearest_n", "mh_place", "mh_total_cost",
     "mh_penalties", "mh_swap_deltas", "mh_anneal", "mh_average_rgb", "mh_optimal_adjustments",
@@ -155,6 +156,8 @@ def load(path=LIB_PATH):
                                                 ctypes.c_size_t, ctypes.c_size_t]),
         "mh_library_from_lab": (ctypes.c_int, [P, P, i32, ctypes.POINTER(P)]),
         "mh_library_from_pixels": (ctypes.c_int, [P, P, i32, i32, i32, ctypes.POINTER(P)]),
+        "mh_average_lab_many": (ctypes.c_int, [P, P, P, P, P, i32, P]),
+        "mh_library_from_images": (ctypes.c_int, [P, P, P, P, P, i32, ctypes.POINTER(P)]),
         "mh_library_destroy": (None, [P]),
         "mh_library_size": (i32, [P]),
         "mh_library_lab": (ctypes.c_int, [P, P]),
@@ -310,6 +313,18 @@ class Context:
 
     def library_from_pixels(self, rgb):
         return Library(self, pixels=rgb)
+
+    def library_from_images(self, images):
+        """Library from RGB8 images of different sizes (list of h x w x 3)."""
+        return Library(self, images=images)
+
+    def average_lab_many(self, images):
+        """calculate_average_lab of each RGB8 image (list of h x w x 3)."""
+        buf, off, w, h = _pack_images(images)
+        out = np.zeros((len(images), 3), np.float32)
+        _check(self.lib.mh_average_lab_many(self.h, _ptr(buf), _ptr(off), _ptr(w), _ptr(h),
+                                            len(images), _ptr(out)))
+        return out
 
     # --------------------------------------------------------------- a3
     def nearest_n(self, lib, query, k):
@@ -467,14 +482,31 @@ class Context:
         return Server(self, memory_budget, max_batch, streams)
 
 
+def _pack_images(images):
+    """Concatenates RGB8 images into one buffer: (bytes, offsets, widths, heights)."""
+    arrs = [_c(im, np.uint8) for im in images]
+    off = np.zeros(len(arrs), np.int64)
+    w = np.array([a.shape[1] for a in arrs], np.int32)
+    h = np.array([a.shape[0] for a in arrs], np.int32)
+    sizes = np.array([a.size for a in arrs], np.int64)
+    if len(arrs):
+        off[1:] = np.cumsum(sizes)[:-1]
+    buf = np.concatenate([a.reshape(-1) for a in arrs]) if arrs else np.zeros(1, np.uint8)
+    return buf, off, w, h
+
+
 class Library:
     """Device-resident material library (SimilarityDatabase + Lab table)."""
 
-    def __init__(self, ctx, lab=None, pixels=None):
+    def __init__(self, ctx, lab=None, pixels=None, images=None):
         self.ctx = ctx
         self.lib = ctx.lib
         h = ctypes.c_void_p()
-        if pixels is not None:
+        if images is not None:
+            buf, off, w, hh = _pack_images(images)
+            _check(self.lib.mh_library_from_images(ctx.h, _ptr(buf), _ptr(off), _ptr(w), _ptr(hh),
+                                                   len(images), ctypes.byref(h)))
+        elif pixels is not None:
             px = _c(pixels, np.uint8)
             n, mh, mw = px.shape[:3]
             _check(self.lib.mh_library_from_pixels(ctx.h, _ptr(px), n, mw, mh, ctypes.byref(h)))

@@ -414,6 +414,82 @@ def test_composite(ctx, orc, synth, strength):
     lib.close()
 
 
+# ------------------------------------------- material ingest, mixed sizes
+_MIXED = [(64, 64), (3, 7), (100, 37), (64, 64), (1, 1), (5, 300), (3, 7), (48, 48), (64, 64),
+          (200, 150)]
+
+
+def _mixed_materials(seed, sizes=_MIXED):
+    rng = _rng(seed)
+    out = []
+    for w, h in sizes:
+        base = rng.integers(0, 256, 3)
+        out.append(np.clip(base + rng.integers(-40, 41, (h, w, 3)), 0, 255).astype(np.uint8))
+    return out
+
+
+def test_average_lab_many(ctx, orc):
+    # load_tiles' per-file calculate_average_lab (src/main.rs:263-301) for
+    # images of several sizes in mixed order, from one launch
+    mats = _mixed_materials(31)
+    got = ctx.average_lab_many(mats)
+    assert_bits_equal(got, np.stack([orc.average_lab(m) for m in mats]))
+    assert ctx.average_lab_many([]).shape == (0, 3)
+
+
+def test_library_from_images(ctx, orc):
+    mats = _mixed_materials(32)
+    lib = ctx.library_from_images(mats)
+    assert lib.n == len(mats)
+    assert_bits_equal(lib.lab(), np.stack([orc.average_lab(m) for m in mats]))
+    lib.close()
+    # one size: the same library as mh_library_from_pixels
+    uni = _mixed_materials(33, [(30, 20)] * 6)
+    a, b = ctx.library_from_images(uni), ctx.library_from_pixels(np.stack(uni))
+    assert_bits_equal(a.lab(), b.lab())
+    target = _rng(34).integers(0, 256, (3 * 10, 4 * 12, 3)).astype(np.uint8)
+    grid = _rng(35).integers(0, 6, 12).astype(np.int32)
+    np.testing.assert_array_equal(ctx.composite(a, target, grid, 4, 3, 0.5),
+                                  ctx.composite(b, target, grid, 4, 3, 0.5))
+    a.close()
+    b.close()
+    empty = ctx.library_from_images([])
+    assert empty.n == 0
+    empty.close()
+
+
+@pytest.mark.parametrize("strength", [0.0, 0.4])
+def test_composite_mixed_size_materials(ctx, orc, strength):
+    # each material is resized from its own size (resize_image,
+    # src/main.rs:710-738); the oracle composites the tiles it resized one by
+    # one, which its identity resize leaves unchanged
+    mats = _mixed_materials(36)
+    gw, gh, tw, th = 10, 8, 16, 15
+    target = _rng(37).integers(0, 256, (gh * th, gw * tw, 3)).astype(np.uint8)
+    grid = _rng(38).integers(0, len(mats), gw * gh).astype(np.int32)
+    grid[3] = -1
+    lib = ctx.library_from_images(mats)
+    got = ctx.composite(lib, target, grid, gw, gh, strength)
+    tiles = np.stack([orc.resize(m, tw, th) for m in mats])
+    np.testing.assert_array_equal(got, orc.composite(target, tiles, grid, gw, gh, strength))
+    lib.close()
+
+
+def test_generate_mixed_size_materials(ctx, orc, synth):
+    mats = _mixed_materials(39, _MIXED * 4)
+    gw, gh, tw, th = 12, 9, 16, 15
+    target = synth.blob_target(gw * tw, gh * th, 6, _rng(40))
+    lib = ctx.library_from_images(mats)
+    got = ctx.generate(lib, target, gw, gh, 2, 0.3, 0.4, True, 300, seed=3)
+    lab = np.stack([orc.average_lab(m) for m in mats])
+    grid = orc.place(lab, orc.cell_lab_means(target, gw, gh), gw, gh, 2, 0.3)["grid"]
+    grid = orc.anneal(lab, grid, gw, gh, 300, 100.0, 0.99995, seed=3)["grid"]
+    tiles = np.stack([orc.resize(m, tw, th) for m in mats])
+    np.testing.assert_array_equal(got["grid"], grid)
+    np.testing.assert_array_equal(got["image"], orc.composite(target, tiles, grid, gw, gh, 0.4))
+    lib.close()
+
+
 # ------------------------------------------------------------ whole job
 @pytest.mark.parametrize("name,score_mode", [("small", 0), ("medium", 0), ("small", 1),
                                              ("medium", 1)])
