@@ -88,6 +88,28 @@ def cpu_baseline(oracle, p, target, mats, budget_s):
             return jobs / el, jobs, el
 
 
+def cpu_baseline_all_cores(oracle, p, target, mats, budget_s, threads):
+    """The same whole jobs on `threads` host threads, one job per thread at a
+    time (jobs are independent; each is the single-threaded restatement, whose
+    C calls release the GIL).  Returns (jobs/s, jobs, seconds)."""
+    import concurrent.futures
+    deadline = time.perf_counter() + budget_s
+    t0 = time.perf_counter()
+
+    def worker(w):
+        done = 0
+        while time.perf_counter() < deadline:
+            oracle.generate(target, mats, p["grid_w"], p["grid_h"], p["max_usage"], p["weight"],
+                            p["color"], True, p["iters"], seed=1000 * w + done)
+            done += 1
+        return done
+
+    with concurrent.futures.ThreadPoolExecutor(threads) as ex:
+        jobs = sum(ex.map(worker, range(threads)))
+    el = time.perf_counter() - t0
+    return jobs / el, jobs, el
+
+
 def _pct(xs, q):
     return round(float(np.percentile(np.asarray(xs, np.float64), q)), 4) if len(xs) else None
 
@@ -405,8 +427,9 @@ def main():
             continue
         amount, unit, bound = algorithmic_work(name, p, n)
         rate = amount / (ms * 1e-3)
+        peak = HBM_PEAK_GBS * 1e9 if unit == "B" else FP32_PEAK_TFLOPS * 1e12
         kernels[name] = {"avg_ms": round(ms, 5), "algorithmic": amount, "unit": unit,
-                         "rate": rate, "bound": bound}
+                         "rate": rate, "bound": bound, "roofline_frac": round(rate / peak, 6)}
     dominant = max(kernels, key=lambda k: kernels[k]["avg_ms"])
     d_amount, d_unit, d_bound = algorithmic_work(dominant, p, n)
     d_ms = kernels[dominant]["avg_ms"]
@@ -479,6 +502,13 @@ def main():
                                "sample": f"{jobs} whole {args.workload} jobs ({el:.1f} s) of the C "
                                          f"restatement on one host core "
                                          f"({platform.processor() or platform.machine()})"}
+        threads = max(1, min(16, len(os.sched_getaffinity(0))))
+        rate_all, jobs_all, el_all = cpu_baseline_all_cores(orc, p, target, mats, args.cpu_seconds,
+                                                            threads)
+        out["cpu_baseline_all_cores"] = {
+            "value": round(rate_all, 4), "unit": "jobs/s", "cores": threads, "kind": "port",
+            "sample": f"{jobs_all} whole {args.workload} jobs ({el_all:.1f} s), one job per "
+                      f"thread on {threads} host threads"}
     else:
         out["cpu_baseline"] = None
 
