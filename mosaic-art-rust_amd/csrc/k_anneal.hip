@@ -33,12 +33,19 @@ __device__ __forceinline__ int32_t gget(const G* g, int32_t i) {
 
 // One adjacency term 1/(sim(a,b)+1): a gather from the library's term
 // matrix when it is resident, else computed from the Lab table (same bits).
-struct TermSource {
-    const float4* lab4;
+// The source is a type rather than a run-time test inside every read, so a
+// caller's gathers are straight-line code that the compiler issues together
+// (behind a test per read, each gather was waited for on its own).
+struct TermMatrix {
     const float* term;
     int32_t n;
     __device__ __forceinline__ float operator()(int32_t a, int32_t b) const {
-        if (term) return gload(term, (int64_t)a * n + b);
+        return gload(term, (int64_t)a * n + b);
+    }
+};
+struct TermLab {
+    const float4* lab4;
+    __device__ __forceinline__ float operator()(int32_t a, int32_t b) const {
         return adj_term(gload4(lab4, a), gload4(lab4, b));
     }
 };
@@ -46,46 +53,54 @@ struct TermSource {
 // calculate_swap_delta (src/adjacency.rs:133-214) on a grid in any address
 // space.  Term order: neighbours of p1 (Up, Down, Left, Right, skipping p2),
 // then of p2 (skipping p1), then the mutual term when adjacent.
-template <typename G>
-__device__ float swap_delta_dev(const G* grid, int32_t gw, int32_t gh, const TermSource& T,
+//
+// Branch-free form: the eight neighbour slots are read first, then every
+// term of both rows (m1 and m2) is requested at once -- an absent slot reads
+// a valid dummy entry -- so the gathers cost one memory latency instead of
+// one each.  Absent terms are added as +0.0: the sums start at +0 and only
+// grow, and x + (+0) == x bit for bit, so this equals skipping them.
+template <typename G, typename TS>
+__device__ float swap_delta_dev(const G* grid, int32_t gw, int32_t gh, const TS& T,
                                 int32_t x1, int32_t y1, int32_t x2, int32_t y2) {
-    const int32_t m1 = gget(grid, y1 * gw + x1), m2 = gget(grid, y2 * gw + x2);
+    const int32_t p1 = y1 * gw + x1, p2 = y2 * gw + x2;
+    const int32_t m1 = gget(grid, p1), m2 = gget(grid, p2);
     if (m1 == kEmpty || m2 == kEmpty) return 0.0f;
     if (m1 == m2) return 0.0f;
-    float old_cost = 0.0f, new_cost = 0.0f;
+    const bool in[8] = {y1 > 0, y1 < gh - 1, x1 > 0, x1 < gw - 1,
+                        y2 > 0, y2 < gh - 1, x2 > 0, x2 < gw - 1};
+    const int32_t pos[8] = {p1 - gw, p1 + gw, p1 - 1, p1 + 1, p2 - gw, p2 + gw, p2 - 1, p2 + 1};
+    int32_t a[8];
+    bool ok[8];
     bool is_adj = false;
-    int32_t ax[4], ay[4], cnt = 0;
-    if (y1 > 0) { ax[cnt] = x1; ay[cnt] = y1 - 1; ++cnt; }
-    if (y1 < gh - 1) { ax[cnt] = x1; ay[cnt] = y1 + 1; ++cnt; }
-    if (x1 > 0) { ax[cnt] = x1 - 1; ay[cnt] = y1; ++cnt; }
-    if (x1 < gw - 1) { ax[cnt] = x1 + 1; ay[cnt] = y1; ++cnt; }
-    for (int32_t i = 0; i < cnt; ++i) {
-        if (ax[i] == x2 && ay[i] == y2) {
-            is_adj = true;
-            continue;
-        }
-        const int32_t a = gget(grid, ay[i] * gw + ax[i]);
-        if (a == kEmpty) continue;
-        old_cost += T(m1, a);
-        new_cost += T(m2, a);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int32_t other = i < 4 ? p2 : p1;
+        const bool self = in[i] && pos[i] == other;  // the swap partner
+        if (i < 4) is_adj = is_adj || self;
+        const int32_t g = in[i] ? gget(grid, pos[i]) : kEmpty;
+        ok[i] = in[i] && !self && g != kEmpty;
+        a[i] = ok[i] ? g : 0;
     }
-    cnt = 0;
-    if (y2 > 0) { ax[cnt] = x2; ay[cnt] = y2 - 1; ++cnt; }
-    if (y2 < gh - 1) { ax[cnt] = x2; ay[cnt] = y2 + 1; ++cnt; }
-    if (x2 > 0) { ax[cnt] = x2 - 1; ay[cnt] = y2; ++cnt; }
-    if (x2 < gw - 1) { ax[cnt] = x2 + 1; ay[cnt] = y2; ++cnt; }
-    for (int32_t i = 0; i < cnt; ++i) {
-        if (ax[i] == x1 && ay[i] == y1) continue;
-        const int32_t a = gget(grid, ay[i] * gw + ax[i]);
-        if (a == kEmpty) continue;
-        old_cost += T(m2, a);
-        new_cost += T(m1, a);
+    float r1[8], r2[8];  // T(m1, a_i), T(m2, a_i)
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        r1[i] = T(m1, a[i]);
+        r2[i] = T(m2, a[i]);
     }
-    if (is_adj) {
-        const float t = T(m1, m2);
-        old_cost += t;
-        new_cost += t;
+    const float tm = T(m1, m2);
+    float old_cost = 0.0f, new_cost = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {  // p1's neighbours: old with m1, new with m2
+        old_cost += ok[i] ? r1[i] : 0.0f;
+        new_cost += ok[i] ? r2[i] : 0.0f;
     }
+#pragma unroll
+    for (int i = 4; i < 8; ++i) {  // p2's neighbours: old with m2, new with m1
+        old_cost += ok[i] ? r2[i] : 0.0f;
+        new_cost += ok[i] ? r1[i] : 0.0f;
+    }
+    old_cost += is_adj ? tm : 0.0f;
+    new_cost += is_adj ? tm : 0.0f;
     return new_cost - old_cost;
 }
 
@@ -97,8 +112,8 @@ __device__ float swap_delta_dev(const G* grid, int32_t gw, int32_t gh, const Ter
 // it leaves the sum exactly as the reference's skipped addition does.
 // scratch holds 2*chunk + 2 floats and is 16-byte aligned.
 // The result is broadcast to every lane.
-template <typename G>
-__device__ float total_cost_wave(const G* grid, int32_t gw, int32_t gh, const TermSource& T,
+template <typename G, typename TS>
+__device__ float total_cost_wave(const G* grid, int32_t gw, int32_t gh, const TS& T,
                                  float* scratch, int32_t chunk) {
     const int lane = threadIdx.x;
     const int32_t cells = gw * gh;
@@ -167,9 +182,12 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
     __syncthreads();
 
     // the stamp words double as the cost scratch (2 terms per cell + padding)
-    const TermSource T{job.lab4, job.term, job.n};
-    float cost = total_cost_wave(grid_s, gw, gh, T, reinterpret_cast<float*>(stamp),
-                                 (stamps / 2 - 1) & ~1);
+    const TermMatrix TM{job.term, job.n};
+    const TermLab TL{job.lab4};
+    float* cost_scratch = reinterpret_cast<float*>(stamp);
+    const int32_t chunk = (stamps / 2 - 1) & ~1;
+    float cost = job.term ? total_cost_wave(grid_s, gw, gh, TM, cost_scratch, chunk)
+                          : total_cost_wave(grid_s, gw, gh, TL, cost_scratch, chunk);
     for (int32_t i = lane; i < cells; i += 64) stamp[i] = 0u;
     __syncthreads();
 
@@ -198,15 +216,21 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
         float delta = 0.0f;
         bool acc = false;
         if (!skip) {
-            delta = swap_delta_dev(grid_s, gw, gh, T, x1, y1, x2, y2);
+            // the temperature and the uniform draw are requested before the
+            // delta, so their latency overlaps the term gathers
+            float temp = 1.0f, u = 0.0f;
+            if (!job.greedy) {
+                temp = gload(job.temps, kbase + __popcll(nonskip & lanes_below));
+                u = draw_unit(job.seed, (uint64_t)it);
+            }
+            delta = job.term ? swap_delta_dev(grid_s, gw, gh, TM, x1, y1, x2, y2)
+                             : swap_delta_dev(grid_s, gw, gh, TL, x1, y1, x2, y2);
             if (job.greedy) {
                 acc = delta < 0.0f;
             } else if (delta < 0.0f) {
                 acc = true;
             } else {
-                const int64_t k = kbase + __popcll(nonskip & lanes_below);
-                const float prob = exp_f32(-delta / gload(job.temps, k));
-                acc = draw_unit(job.seed, (uint64_t)it) < prob;
+                acc = u < exp_f32(-delta / temp);
             }
         }
         const int32_t p1 = y1 * gw + x1, p2 = y2 * gw + x2;
@@ -337,8 +361,8 @@ __global__ __launch_bounds__(64) void total_cost_kernel(const int32_t* __restric
                                                         const float* __restrict__ term, int32_t n,
                                                         float* __restrict__ scratch, int32_t chunk,
                                                         float* __restrict__ out) {
-    const TermSource T{lab4, term, n};
-    const float t = total_cost_wave(grid, gw, gh, T, scratch, chunk);
+    const float t = term ? total_cost_wave(grid, gw, gh, TermMatrix{term, n}, scratch, chunk)
+                         : total_cost_wave(grid, gw, gh, TermLab{lab4}, scratch, chunk);
     if (threadIdx.x == 0) *out = t;
 }
 
@@ -355,9 +379,9 @@ __global__ void swap_delta_kernel(const int32_t* __restrict__ grid, int32_t gw, 
                                   float* __restrict__ out) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= p) return;
-    const TermSource T{lab4, term, n};
     const int32_t* q = pairs + 4 * (int64_t)i;
-    out[i] = swap_delta_dev(grid, gw, gh, T, q[0], q[1], q[2], q[3]);
+    out[i] = term ? swap_delta_dev(grid, gw, gh, TermMatrix{term, n}, q[0], q[1], q[2], q[3])
+                  : swap_delta_dev(grid, gw, gh, TermLab{lab4}, q[0], q[1], q[2], q[3]);
 }
 
 void launch_swap_deltas(const int32_t* grid, int32_t gw, int32_t gh, const float4* lab4,

@@ -171,10 +171,11 @@ struct CandSet {
     float4 c0, c1;    // {l, a, b, d}: d = sqdist, or sqrt(sqdist) for the GUI score
 };
 
-// Loads one cell's candidate records.  Every load is issued unconditionally
-// (rows beyond the grid read the last row, then are masked; ranks >= K hold
-// the -1 sentinel written by the candidate kernel) so the compiler's memory
-// wait counts stay exact across the unrolled group.
+// Loads one cell's candidate records (ranks >= K hold the -1 sentinel
+// written by the candidate kernel).  Every load is issued unconditionally so
+// the compiler's memory wait counts stay exact across the unrolled group.
+// Cells past the grid read the last row: the chain only looks at them as the
+// next cells of the last one, whose results are never used.
 __device__ __forceinline__ void load_cands(CandSet& s, const PlaceJob& job, int32_t cell,
                                            int32_t cells, int lane) {
     const int32_t row = cell < cells ? cell : cells - 1;
@@ -183,9 +184,6 @@ __device__ __forceinline__ void load_cands(CandSet& s, const PlaceJob& job, int3
     s.c0 = gload4(job.cand_lab, base + lane);
     s.m1 = gload(job.cand_idx, base + lane + 64);
     s.c1 = gload4(job.cand_lab, base + lane + 64);
-    const bool live = cell < cells;
-    s.m0 = live ? s.m0 : -1;
-    s.m1 = live ? s.m1 : -1;
 }
 
 // GUI score: the Euclidean distance replaces the squared one.
@@ -201,12 +199,12 @@ struct RasterState {
     int32_t x, y;      // current cell position
     int32_t left_m;    // previous winner (kEmpty at the start of a row)
     int32_t up_m;      // material above the current cell (kEmpty in row 0)
-    uint32_t u0;       // usage counts of the current cell's rank 0-63 candidates
     uint64_t ob0;      // rank 0-63 candidates of the current cell that are usable
     // read one step ahead for the next cell: usage counts of its rank 0-63
     // candidates and its up neighbour, both as of two commits ago
     uint32_t pre_u;
     int32_t pre_up;
+    int32_t lane_zero;  // 0 in every lane, unknown to the compiler (see raster_step)
     int32_t prev_w;    // winner of the previous cell (kNoMat before the first)
     int32_t prev_reset;  // the previous cell reset every usage count
     int32_t st_ban, st_near, st_resets;
@@ -253,29 +251,43 @@ __device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn,
     const uint32_t max_usage = (uint32_t)job.max_usage;
     // reads for the cell after next (they see every commit up to the previous cell)
     const uint32_t nnu0 = usage_s[snn.m0 >= 0 ? snn.m0 : 0];
-    const int32_t nnup = y2 > 0 && gw > 2 ? row_mat[x2] : kEmpty;
+    // The up neighbour is the same in every lane; a lane-dependent address
+    // keeps it in a vector register, so the read is waited for only where the
+    // next step compares against it (a uniform value would be moved to a
+    // scalar register right after the read, stalling on it here).
+    const int32_t nnup = y2 > 0 && gw > 2 ? row_mat[x2 + st.lane_zero] : kEmpty;
 
-    // the next cell's counts before this cell's commit, and its up neighbour
+    // The next cell's usable mask, built from lane masks.  Its counts before
+    // this cell's commit are `base`; this cell's winner w adds one use to the
+    // lanes holding w, bans them as the left neighbour unless the next cell
+    // starts a row, and is the up neighbour in a one-column grid:
+    //   usable = valid & up-ban & base < max & (lanes != w | (row start & base + 1 < max))
+    // Everything but the w compare is known before w.
     const uint32_t base = (st.prev_reset ? 0u : st.pre_u) + (sn.m0 == st.prev_w ? 1u : 0u);
     const int32_t up_fixed = yn == 0 ? kEmpty : (gw == 2 ? st.prev_w : st.pre_up);
+    const uint64_t lt0 = lanes_ult(base, max_usage);
+    // base + 1 < max for max >= 1; with max == 0, lt0 is empty and masks it
+    const uint64_t lt1 = lanes_ult(base, max_usage - 1u);
+    const uint64_t ne_up = gw == 1 ? ~0ull : lanes_ne(sn.m0, up_fixed);
+    const uint64_t m_base = lanes_nonneg(sn.m0) & ne_up & lt0;
+    const uint64_t keep_w = xn != 0 || gw == 1 ? 0ull : lt1;
 
     // speculative winner and the next cell's usable mask
     const uint64_t ob0 = st.ob0;
     const int32_t f = ob0 ? (int32_t)__ffsll((unsigned long long)ob0) - 1 : 0;
     int32_t w = __builtin_amdgcn_readlane(s.m0, f);
     int32_t nup = gw == 1 ? w : up_fixed;
-    uint32_t nu = base + (sn.m0 == w ? 1u : 0u);
-    uint64_t nob = wave_ballot(sn.m0 >= 0 && sn.m0 != nup && (xn == 0 || sn.m0 != w) &&
-                               nu < max_usage);
+    uint64_t nob = m_base & (~lanes_eq(sn.m0, w) | keep_w);
 
-    // confirm it
+    // confirm it: every other usable rank 0-63 candidate and rank 64 (the
+    // nearest of ranks 64-127) lie beyond the bound
     bool fast = ob0 != 0;
     if (fast && job.weight > 0.0f) {
         const float d_f = readlane_f(s.c0.w, f);
         const float bound = d_f + 2.0f * job.weight;
-        const uint64_t far_half = wave_ballot(s.m1 >= 0 && s.c1.w <= bound);
-        const uint64_t cb0 = ob0 & wave_ballot(s.c0.w <= bound);
-        fast = far_half == 0 && __popcll(cb0) == 1;
+        const uint64_t far_half = lanes_nonneg(s.m1) & lanes_fle(s.c1.w, bound);
+        const uint64_t others = (ob0 & (ob0 - 1ull)) & lanes_fle(s.c0.w, bound);
+        fast = (far_half | others) == 0;
     }
     if (kOutputs && job.score) fast = false;
 
@@ -356,11 +368,14 @@ __device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn,
                       : __builtin_amdgcn_readlane(s.m1, rank & 63);
         nup = gw == 1 ? w : up_fixed;
         const uint32_t h = sn.m0 == w ? 1u : 0u;
-        nu = stage == 0 ? base + h : h;
+        const uint32_t nu = stage == 0 ? base + h : h;  // a reset zeroed every count
         nob = wave_ballot(sn.m0 >= 0 && sn.m0 != nup && (xn == 0 || sn.m0 != w) && nu < max_usage);
     }
     if (lane == 0) {
-        atomicAdd(&usage_s[w], 1u);
+        // the lane-dependent (zero) offset keeps this one lane's add a plain
+        // ds_add: with a uniform address the compiler wraps it in code that
+        // combines the adds of all active lanes
+        atomicAdd(&usage_s[w + st.lane_zero], 1u);
         row_mat[x] = w;
         win_s[c & (kWinRing - 1)] = w;
     }
@@ -380,7 +395,6 @@ __device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn,
     st.prev_w = w;
     st.up_m = nup;
     st.left_m = w;
-    st.u0 = nu;
     st.ob0 = nob;
     st.x = xn;
     st.y = yn;
@@ -425,6 +439,7 @@ __global__ __launch_bounds__(64) void place_raster_kernel(PlaceJob single,
     st.y = 0;
     st.left_m = kEmpty;
     st.up_m = kEmpty;
+    st.lane_zero = job.n < 0 ? lane : 0;  // n > 0 here
     st.prev_w = kNoMat;
     st.prev_reset = 0;
     st.st_ban = st.st_near = st.st_resets = 0;
@@ -432,8 +447,8 @@ __global__ __launch_bounds__(64) void place_raster_kernel(PlaceJob single,
         CandSet h0, h1;
         load_cands(h0, job, 0, cells, lane);
         load_cands(h1, job, 1, cells, lane);
-        st.u0 = usage_s[h0.m0 >= 0 ? h0.m0 : 0];
-        st.ob0 = wave_ballot(h0.m0 >= 0 && st.u0 < (uint32_t)job.max_usage);
+        const uint32_t u0 = usage_s[h0.m0 >= 0 ? h0.m0 : 0];
+        st.ob0 = wave_ballot(h0.m0 >= 0 && u0 < (uint32_t)job.max_usage);
         st.pre_u = usage_s[h1.m0 >= 0 ? h1.m0 : 0];
         st.pre_up = kEmpty;  // cell 1 is in row 0 unless the grid is one column wide
     }

@@ -40,6 +40,26 @@ __device__ __forceinline__ void gstore(T* p, int64_t i, T v) {
 // directly (HIP's __ballot goes through an int select and a second compare).
 __device__ __forceinline__ uint64_t wave_ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
+// Lane masks of one compare each (bit l set when the compare holds in lane l;
+// inactive lanes read 0).  Each is a single v_cmp into a scalar register
+// pair, so conditions combine with scalar mask logic instead of being
+// rebuilt per lane for a ballot.  Predicates are LLVM's icmp/fcmp codes.
+__device__ __forceinline__ uint64_t lanes_eq(int32_t a, int32_t b) {
+    return __builtin_amdgcn_uicmp((uint32_t)a, (uint32_t)b, 32);
+}
+__device__ __forceinline__ uint64_t lanes_ne(int32_t a, int32_t b) {
+    return __builtin_amdgcn_uicmp((uint32_t)a, (uint32_t)b, 33);
+}
+__device__ __forceinline__ uint64_t lanes_ult(uint32_t a, uint32_t b) {
+    return __builtin_amdgcn_uicmp(a, b, 36);
+}
+__device__ __forceinline__ uint64_t lanes_nonneg(int32_t a) {
+    return __builtin_amdgcn_sicmp(a, -1, 38);  // a > -1
+}
+__device__ __forceinline__ uint64_t lanes_fle(float a, float b) {
+    return __builtin_amdgcn_fcmpf(a, b, 5);  // ordered <=: false for NaN, as in C
+}
+
 // A job descriptor read from memory through a generic pointer lands in
 // VGPRs, and every branch on its fields then compiles as a divergent
 // (exec-mask) branch.  The descriptor is uniform across the wave: copying it
