@@ -249,13 +249,17 @@ __device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn,
     const int32_t x2 = xn + 1 == gw ? 0 : xn + 1;
     const int32_t y2 = xn + 1 == gw ? yn + 1 : yn;
     const uint32_t max_usage = (uint32_t)job.max_usage;
-    // reads for the cell after next (they see every commit up to the previous cell)
-    const uint32_t nnu0 = usage_s[snn.m0 >= 0 ? snn.m0 : 0];
-    // The up neighbour is the same in every lane; a lane-dependent address
-    // keeps it in a vector register, so the read is waited for only where the
-    // next step compares against it (a uniform value would be moved to a
-    // scalar register right after the read, stalling on it here).
-    const int32_t nnup = y2 > 0 && gw > 2 ? row_mat[x2 + st.lane_zero] : kEmpty;
+    // Reads for the cell after next (they see every commit up to the previous
+    // cell).  A -1 sentinel reads the word before the counts (the last column
+    // entry), which its masked lane ignores.
+    const uint32_t nnu0 = usage_s[snn.m0];
+    // The up neighbour: row_mat starts all kEmpty, so row 0 needs no test, and
+    // in grids of one or two columns the value is not used.  It is the same in
+    // every lane; a lane-dependent address keeps it in a vector register, so
+    // the read is waited for only where the next step compares against it (a
+    // uniform value would be moved to a scalar register right after the read,
+    // stalling on it here).
+    const int32_t nnup = row_mat[x2 + st.lane_zero];
 
     // The next cell's usable mask, built from lane masks.  Its counts before
     // this cell's commit are `base`; this cell's winner w adds one use to the
@@ -264,7 +268,7 @@ __device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn,
     //   usable = valid & up-ban & base < max & (lanes != w | (row start & base + 1 < max))
     // Everything but the w compare is known before w.
     const uint32_t base = (st.prev_reset ? 0u : st.pre_u) + (sn.m0 == st.prev_w ? 1u : 0u);
-    const int32_t up_fixed = yn == 0 ? kEmpty : (gw == 2 ? st.prev_w : st.pre_up);
+    const int32_t up_fixed = gw == 2 ? (yn == 0 ? kEmpty : st.prev_w) : st.pre_up;
     const uint64_t lt0 = lanes_ult(base, max_usage);
     // base + 1 < max for max >= 1; with max == 0, lt0 is empty and masks it
     const uint64_t lt1 = lanes_ult(base, max_usage - 1u);
@@ -285,7 +289,8 @@ __device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn,
     if (fast && job.weight > 0.0f) {
         const float d_f = readlane_f(s.c0.w, f);
         const float bound = d_f + 2.0f * job.weight;
-        const uint64_t far_half = lanes_nonneg(s.m1) & lanes_fle(s.c1.w, bound);
+        // sentinel ranks carry an infinite distance
+        const uint64_t far_half = lanes_fle(s.c1.w, bound);
         const uint64_t others = (ob0 & (ob0 - 1ull)) & lanes_fle(s.c0.w, bound);
         fast = (far_half | others) == 0;
     }
@@ -298,7 +303,7 @@ __device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn,
         const int32_t left_m = x > 0 ? st.left_m : kEmpty;
         const bool pl0 = s.m0 >= 0 && s.m0 != up_m && s.m0 != left_m;
         const bool ok0 = ((ob0 >> lane) & 1ull) != 0;
-        const uint32_t u1 = usage_s[s.m1 >= 0 ? s.m1 : 0];
+        const uint32_t u1 = usage_s[s.m1];  // a -1 sentinel reads a word it ignores
         const bool pl1 = s.m1 >= 0 && s.m1 != up_m && s.m1 != left_m;
         const bool ok1 = pl1 && u1 < max_usage;
         const uint64_t ob1 = wave_ballot(ok1);
@@ -424,6 +429,7 @@ __global__ __launch_bounds__(64) void place_raster_kernel(PlaceJob single,
         return;
     }
     for (int32_t i = lane; i < n; i += 64) usage_s[i] = job.zero_usage ? 0u : gload(job.usage, i);
+    for (int32_t i = lane; i < gw; i += 64) row_mat[i] = kEmpty;
     __syncthreads();
 
     // Candidate records for kGroup cells (and the two after them) are loaded at

@@ -152,7 +152,9 @@ __device__ __forceinline__ void topk_run(const TopkSeg& s, int32_t q0, int32_t r
                 if (s.out_lab)
                     s.out_lab[o] = make_float4(gload(mL, m), gload(mA, m), gload(mB, m), d);
             } else if (s.out_lab) {
-                gstore(s.out_idx, o, (int32_t)-1);  // padded (chain) rows: ranks beyond k
+                // padded (chain) rows: ranks beyond k, -1 with an infinite distance
+                gstore(s.out_idx, o, (int32_t)-1);
+                s.out_lab[o] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(0x7f800000));
             }
         }
         __syncthreads();

@@ -19,6 +19,7 @@ struct PlaceJob {
     int32_t n;
     const int32_t* cand_idx;  // cells x kCandStride, ascending (sqdist, idx)
     const float4* cand_lab;   // same layout: {l, a, b, sqdist} of each candidate
+                              // (ranks >= K: idx -1, sqdist +inf)
     const int32_t* order;     // n_order cell indices, or nullptr = raster
     int32_t n_order;
     int32_t* grid;            // cells, in/out
