@@ -242,21 +242,23 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
         __syncthreads();
         bool conflict = false;
         if (!skip) {
-            int32_t q[10];
-            int32_t nq = 0;
-            q[nq++] = p1;
-            q[nq++] = p2;
-            if (y1 > 0) q[nq++] = p1 - gw;
-            if (y1 < gh - 1) q[nq++] = p1 + gw;
-            if (x1 > 0) q[nq++] = p1 - 1;
-            if (x1 < gw - 1) q[nq++] = p1 + 1;
-            if (y2 > 0) q[nq++] = p2 - gw;
-            if (y2 < gh - 1) q[nq++] = p2 + gw;
-            if (x2 > 0) q[nq++] = p2 - 1;
-            if (x2 < gw - 1) q[nq++] = p2 + 1;
-            for (int32_t i = 0; i < nq; ++i) {
+            // read set: both cells and their in-grid neighbours, as ten fixed
+            // slots (an out-of-grid neighbour re-reads its own cell, which is
+            // checked anyway), so the reads are issued together
+            const int32_t q[10] = {p1,
+                                   p2,
+                                   y1 > 0 ? p1 - gw : p1,
+                                   y1 < gh - 1 ? p1 + gw : p1,
+                                   x1 > 0 ? p1 - 1 : p1,
+                                   x1 < gw - 1 ? p1 + 1 : p1,
+                                   y2 > 0 ? p2 - gw : p2,
+                                   y2 < gh - 1 ? p2 + gw : p2,
+                                   x2 > 0 ? p2 - 1 : p2,
+                                   x2 < gw - 1 ? p2 + 1 : p2};
+#pragma unroll
+            for (int i = 0; i < 10; ++i) {
                 const uint32_t v = stamp[q[i]];
-                if ((v >> 6) == tag && (int)(63 - (v & 63u)) < lane) conflict = true;
+                conflict |= (v >> 6) == tag && (int)(63 - (v & 63u)) < lane;
             }
         }
         const uint64_t cmask = wave_ballot(conflict);
@@ -276,21 +278,28 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
             if (job.trace_accept) gstore(job.trace_accept, it, (uint8_t)(acc ? 1 : 0));
         }
         // running cost in proposal order; every lane runs the same uniform
-        // loop, so cost/best/improved stay identical across the wave
-        {
+        // loop, so cost/best/improved stay identical across the wave.  The
+        // loop body has no branch: a new best is a compare and two selects.
+        if (job.greedy) {  // optimize_greedy counts every accepted swap
             uint64_t m = accmask;
             while (m) {
                 const int j = __ffsll((unsigned long long)m) - 1;
                 m &= m - 1;
-                const float dj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(delta), j));
-                cost = cost + dj;
-                if (job.greedy) {
-                    ++improved;
-                } else if (cost < best) {
-                    best = cost;
-                    ++improved;
-                }
+                cost = cost + __int_as_float(__builtin_amdgcn_readlane(__float_as_int(delta), j));
             }
+            improved += __popcll(accmask);
+        } else {
+            uint32_t better_n = 0;
+            uint64_t m = accmask;
+            while (m) {
+                const int j = __ffsll((unsigned long long)m) - 1;
+                m &= m - 1;
+                cost = cost + __int_as_float(__builtin_amdgcn_readlane(__float_as_int(delta), j));
+                const bool better = cost < best;
+                best = better ? cost : best;
+                better_n += better ? 1u : 0u;
+            }
+            improved += better_n;
         }
         accepted += __popcll(accmask);
         const int64_t ev = __popcll(nonskip & commit);
