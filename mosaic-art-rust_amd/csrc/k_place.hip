@@ -239,8 +239,8 @@ template <bool kOutputs>
 __device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn,
                                             const CandSet& snn, int32_t c, RasterState& st,
                                             const PlaceJob& job, uint32_t* usage_s,
-                                            int32_t* row_mat, int32_t* win_s, int32_t n,
-                                            int lane) {
+                                            int32_t* row_mat, int32_t* win_s,
+                                            int32_t* lane_scratch, int32_t n, int lane) {
     const uint32_t kNone = 0xffffffffu;
     const int32_t gw = job.grid_w;
     const int32_t x = st.x, y = st.y;
@@ -376,13 +376,15 @@ __device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn,
         const uint32_t nu = stage == 0 ? base + h : h;  // a reset zeroed every count
         nob = wave_ballot(sn.m0 >= 0 && sn.m0 != nup && (xn == 0 || sn.m0 != w) && nu < max_usage);
     }
-    if (lane == 0) {
-        // the lane-dependent (zero) offset keeps this one lane's add a plain
-        // ds_add: with a uniform address the compiler wraps it in code that
-        // combines the adds of all active lanes
-        atomicAdd(&usage_s[w + st.lane_zero], 1u);
-        row_mat[x] = w;
-        win_s[c & (kWinRing - 1)] = w;
+    {
+        // Lane 0 commits; every other lane writes its own scratch word
+        // instead, so the commit is straight-line code with no branch on the
+        // lane (the step stays one block for the scheduler).  A per-lane
+        // address also keeps the add a plain ds_add.
+        const bool l0 = lane == 0;
+        atomicAdd(l0 ? &usage_s[w] : reinterpret_cast<uint32_t*>(lane_scratch), 1u);
+        *(l0 ? &row_mat[x] : lane_scratch) = w;
+        *(l0 ? &win_s[c & (kWinRing - 1)] : lane_scratch) = w;
     }
     if (kOutputs) {
         const float dw = rank < 64 ? readlane_f(s.c0.w, rank & 63) : readlane_f(s.c1.w, rank & 63);
@@ -417,7 +419,8 @@ __global__ __launch_bounds__(64) void place_raster_kernel(PlaceJob single,
     const int32_t gw = job.grid_w, n = job.n;
     const int32_t cells = gw * job.grid_h;
     int32_t* win_s = reinterpret_cast<int32_t*>(smr);
-    int32_t* row_mat = win_s + kWinRing;
+    int32_t* lane_scratch = win_s + kWinRing + lane;  // 64 words, one per lane
+    int32_t* row_mat = win_s + kWinRing + 64;
     uint32_t* usage_s = reinterpret_cast<uint32_t*>(row_mat + gw);
 
     if (n == 0) {  // src/main.rs:342-345: no tiles, every cell stays empty
@@ -476,7 +479,7 @@ __global__ __launch_bounds__(64) void place_raster_kernel(PlaceJob single,
 #pragma unroll
         for (int i = 0; i < kGroup; ++i) {
             raster_step<kOutputs>(q[i], q[i + 1], q[i + 2], c + i, st, job, usage_s, row_mat,
-                                  win_s, n, lane);
+                                  win_s, lane_scratch, n, lane);
             // the step waited for the records after q[i + 1]'s, so these are in
             if (sq && i + 1 < kGroup) to_distance(q[i + 1]);
         }
@@ -494,7 +497,7 @@ __global__ __launch_bounds__(64) void place_raster_kernel(PlaceJob single,
         for (int i = 0; i < kGroup; ++i) {
             if (i < rem) {
                 raster_step<kOutputs>(q[i], q[i + 1], q[i + 2], c + i, st, job, usage_s,
-                                      row_mat, win_s, n, lane);
+                                      row_mat, win_s, lane_scratch, n, lane);
                 if (sq && i + 1 < kGroup) to_distance(q[i + 1]);
             }
         }
@@ -518,7 +521,7 @@ size_t place_lds_bytes(int32_t cells, int32_t n) {
 }
 
 size_t place_raster_lds_bytes(int32_t grid_w, int32_t n) {
-    return sizeof(int32_t) * ((size_t)kWinRing + (size_t)grid_w) +
+    return sizeof(int32_t) * ((size_t)kWinRing + 64 + (size_t)grid_w) +
            sizeof(uint32_t) * (size_t)(n > 0 ? n : 1);
 }
 
