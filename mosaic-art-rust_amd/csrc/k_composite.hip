@@ -208,13 +208,6 @@ __device__ __forceinline__ void avg_rgb_block(const CellSeg& s, int32_t cell) {
 
 __global__ __launch_bounds__(256) void avg_rgb_kernel(CellSeg s) { avg_rgb_block(s, blockIdx.x); }
 
-__global__ __launch_bounds__(256) void avg_rgb_batch_kernel(const CellSeg* __restrict__ segs,
-                                                            const int32_t* __restrict__ starts,
-                                                            int32_t n_segs) {
-    const int32_t j = find_segment(starts, n_segs, blockIdx.x);
-    avg_rgb_block(segs[j], blockIdx.x - starts[j]);
-}
-
 void launch_avg_rgb(const uint8_t* rgb, int64_t row_stride, int32_t per_row,
                     int64_t block_row_stride, int64_t block_col_stride, int32_t cw, int32_t ch,
                     int32_t n_cells, uint8_t* out, hipStream_t stream) {
@@ -222,14 +215,6 @@ void launch_avg_rgb(const uint8_t* rgb, int64_t row_stride, int32_t per_row,
     const CellSeg s{rgb, row_stride, block_row_stride, block_col_stride, per_row, cw, ch, n_cells,
                     nullptr, out};
     hipLaunchKernelGGL(avg_rgb_kernel, dim3(n_cells), dim3(256), 0, stream, s);
-}
-
-// One block per cell; segments with n_cells > 0 only.
-void launch_avg_rgb_batch(const CellSeg* d_segs, const int32_t* d_starts, int32_t n_segs,
-                          int32_t total_cells, hipStream_t stream) {
-    if (n_segs <= 0 || total_cells <= 0) return;
-    hipLaunchKernelGGL(avg_rgb_batch_kernel, dim3(total_cells), dim3(256), 0, stream, d_segs,
-                       d_starts, n_segs);
 }
 
 // Per-cell adjustment: calculate_optimal_adjustment(tile_avg[grid[c]],

@@ -41,12 +41,21 @@ __device__ __forceinline__ void cell_lab_block(const CellSeg& s, int32_t blk,
     const int sum_cell = tid / 3, sum_ch = tid % 3;
     const bool summer = tid < 3 * kCellsPerBlock && cell0 + sum_cell < n_cells;
     float acc = 0.0f;
+    // integer RGB sums of the pixels this thread converts (calculate_average_rgb,
+    // src/main.rs:687-708, when out_rgb is requested): thread t sees pixel t of
+    // every chunk of each of the block's cells
+    const bool want_rgb = s.out_rgb != nullptr;
+    uint32_t rsum[kCellsPerBlock][3];
+#pragma unroll
+    for (int g = 0; g < kCellsPerBlock; ++g) rsum[g][0] = rsum[g][1] = rsum[g][2] = 0u;
 
     __syncthreads();
     for (int32_t base = 0; base < npx; base += kChunk) {
         const int32_t len = min(kChunk, npx - base);
-        for (int i = tid; i < kCellsPerBlock * kChunk; i += kLabThreads) {
-            const int g = i / kChunk, p = i % kChunk;
+        static_assert(kChunk == kLabThreads, "one pixel per thread and cell per chunk");
+#pragma unroll
+        for (int g = 0; g < kCellsPerBlock; ++g) {
+            const int p = tid;
             const int32_t cell = cell0 + g;
             if (cell < n_cells && p < len) {
                 const int32_t pix = base + p;
@@ -54,11 +63,17 @@ __device__ __forceinline__ void cell_lab_block(const CellSeg& s, int32_t blk,
                 const uint8_t* src = s.rgb + (int64_t)(cell / per_row) * s.block_row_stride +
                                      (int64_t)(cell % per_row) * s.block_col_stride +
                                      (int64_t)py * s.row_stride + (int64_t)px * 3;
+                const uint8_t r = src[0], gg = src[1], b = src[2];
                 float L, A, B;
-                rgb8_to_lab(sh.lut, src[0], src[1], src[2], L, A, B);
+                rgb8_to_lab(sh.lut, r, gg, b, L, A, B);
                 sh.lab[0][g][p] = L;
                 sh.lab[1][g][p] = A;
                 sh.lab[2][g][p] = B;
+                if (want_rgb) {
+                    rsum[g][0] += r;
+                    rsum[g][1] += gg;
+                    rsum[g][2] += b;
+                }
             }
         }
         __syncthreads();
@@ -71,6 +86,25 @@ __device__ __forceinline__ void cell_lab_block(const CellSeg& s, int32_t blk,
     if (summer) {
         const float total = (float)((uint32_t)cw * (uint32_t)ch);
         s.out[(int64_t)(cell0 + sum_cell) * 3 + sum_ch] = acc / total;
+    }
+    if (want_rgb) {
+        // per-thread partial sums through LDS (the Lab staging area is free
+        // now), then one lane per (cell, channel) adds the 256 partials;
+        // integer sums, so the order is free
+        uint32_t* part = reinterpret_cast<uint32_t*>(&sh.lab[0][0][0]);
+        static_assert(sizeof(sh.lab) >= 3 * kCellsPerBlock * (kLabThreads + 1) * sizeof(uint32_t),
+                      "partials fit the staging area");
+#pragma unroll
+        for (int g = 0; g < kCellsPerBlock; ++g)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) part[(g * 3 + c) * (kLabThreads + 1) + tid] = rsum[g][c];
+        __syncthreads();
+        if (summer) {
+            const uint32_t* v = part + (sum_cell * 3 + sum_ch) * (kLabThreads + 1);
+            uint64_t t = 0;
+            for (int i = 0; i < kLabThreads; ++i) t += v[i];
+            s.out_rgb[(int64_t)(cell0 + sum_cell) * 3 + sum_ch] = (uint8_t)(t / (uint64_t)npx);
+        }
     }
 }
 
@@ -94,10 +128,11 @@ int32_t cell_lab_blocks(int32_t n_cells) {
 
 void launch_cell_lab(const uint8_t* rgb, int64_t row_stride, int32_t per_row,
                      int64_t block_row_stride, int64_t block_col_stride, int32_t cw, int32_t ch,
-                     int32_t n_cells, const float* lut, float* out, hipStream_t stream) {
+                     int32_t n_cells, const float* lut, float* out, hipStream_t stream,
+                     uint8_t* out_rgb) {
     if (n_cells <= 0) return;
     const CellSeg s{rgb, row_stride, block_row_stride, block_col_stride, per_row, cw, ch, n_cells,
-                    out, nullptr};
+                    out, out_rgb};
     hipLaunchKernelGGL(cell_lab_kernel, dim3(cell_lab_blocks(n_cells)), dim3(kLabThreads), 0,
                        stream, s, lut);
 }

@@ -348,9 +348,11 @@ int run_anneal(mh_context* ctx, const mh_library* lib, int32_t* d_grid, int32_t 
 }
 
 
+// tavg_ready: the target cells' average RGB, already computed (by the cell
+// Lab pass), or nullptr to compute it here.
 int run_composite(mh_context* ctx, const mh_library* lib, const uint8_t* d_target, int32_t width,
                   int32_t height, const int32_t* d_grid, int32_t gw, int32_t gh, float strength,
-                  uint8_t* d_out) {
+                  uint8_t* d_out, const uint8_t* tavg_ready) {
     const int32_t tw = width / gw, th = height / gh;
     if (tw <= 0 || th <= 0) return MH_OK;
     const uint8_t *tiles, *tile_avg;
@@ -361,11 +363,15 @@ int run_composite(mh_context* ctx, const mh_library* lib, const uint8_t* d_targe
     float4* adj = nullptr;
     const int32_t adjust = strength > 0.0f ? 1 : 0;
     if (adjust) {
-        uint8_t* tavg;
-        if ((rc = slot(ctx, S_TARGET_AVG, (size_t)cells * 3, &tavg))) return rc;
+        const uint8_t* tavg = tavg_ready;
+        if (!tavg) {
+            uint8_t* t;
+            if ((rc = slot(ctx, S_TARGET_AVG, (size_t)cells * 3, &t))) return rc;
+            launch_avg_rgb(d_target, (int64_t)width * 3, gw, (int64_t)th * width * 3,
+                           (int64_t)tw * 3, tw, th, cells, t, ctx->stream);
+            tavg = t;
+        }
         if ((rc = slot(ctx, S_ADJ, (size_t)cells, &adj))) return rc;
-        launch_avg_rgb(d_target, (int64_t)width * 3, gw, (int64_t)th * width * 3, (int64_t)tw * 3, tw,
-                       th, cells, tavg, ctx->stream);
         launch_cell_adjust(d_grid, cells, tile_avg, tavg, strength, adj, ctx->stream);
     }
     phase_end(ctx, MH_PHASE_ADJUST);
@@ -1274,7 +1280,7 @@ int mh_composite(mh_context* ctx, const mh_library* lib, const uint8_t* target, 
     CHK(upload(ctx, S_IN0, target, (size_t)width * height * 3, &d_t));
     CHK(upload(ctx, S_GRID, grid, (size_t)cells, &d_grid));
     CHK(slot(ctx, S_OUT0, out_bytes, &d_out));
-    CHK(run_composite(ctx, lib, d_t, width, height, d_grid, gw, gh, strength, d_out));
+    CHK(run_composite(ctx, lib, d_t, width, height, d_grid, gw, gh, strength, d_out, nullptr));
     CHK(download(ctx, out, d_out, out_bytes));
     return sync(ctx);
 }
@@ -1308,10 +1314,15 @@ int mh_generate_device(mh_context* ctx, const mh_library* lib, const uint8_t* d_
     CHK(slot(ctx, S_USAGE, (size_t)(lib->n > 0 ? lib->n : 1), &d_usage));
     CHK(slot(ctx, S_STATS, 8, &d_stats));
     CHK(slot(ctx, S_ANNEAL_OUT, 1, &d_aout));
-    // a1: target cell Lab means (src/main.rs:536-541)
+    // a1: target cell Lab means (src/main.rs:536-541); the same pixel pass
+    // gives the cells' average RGB for the colour adjustment (a13)
+    float strength = p->color_strength;
+    strength = strength < 0.0f ? 0.0f : (strength > 1.0f ? 1.0f : strength);  // src/main.rs:162
+    uint8_t* tavg = nullptr;
+    if (d_image_out && strength > 0.0f) CHK(slot(ctx, S_TARGET_AVG, (size_t)cells * 3, &tavg));
     phase_begin(ctx, MH_PHASE_CELL_LAB);
     launch_cell_lab(d_target, (int64_t)width * 3, gw, (int64_t)th * width * 3, (int64_t)tw * 3, tw,
-                    th, cells, ctx->d_lut, d_lab, ctx->stream);
+                    th, cells, ctx->d_lut, d_lab, ctx->stream, tavg);
     phase_end(ctx, MH_PHASE_CELL_LAB);
     // a3-a9: candidates + placement chain (raster order, empty grid, zero usage)
     CHK(run_place(ctx, lib, d_lab, gw, gh, p->max_usage, p->weight, p->score_mode, true, nullptr,
@@ -1330,11 +1341,10 @@ int mh_generate_device(mh_context* ctx, const mh_library* lib, const uint8_t* d_
         ap.greedy = 0;
         CHK(run_anneal(ctx, lib, d_grid, gw, gh, &ap, nullptr, nullptr, d_aout));
     }
-    // a13-a16: composite (strength clamped, src/main.rs:162)
-    float strength = p->color_strength;
-    strength = strength < 0.0f ? 0.0f : (strength > 1.0f ? 1.0f : strength);
+    // a13-a16: composite
     if (d_image_out)
-        CHK(run_composite(ctx, lib, d_target, width, height, d_grid, gw, gh, strength, d_image_out));
+        CHK(run_composite(ctx, lib, d_target, width, height, d_grid, gw, gh, strength, d_image_out,
+                          tavg));
     ctx->last_out_w = gw * tw;
     ctx->last_out_h = gh * th;
     HIPCHK(hipGetLastError());

@@ -129,7 +129,7 @@ int prepare_job(mh_context* ctx, JobSpec* s, hipStream_t stream) {
 struct BatchPlan {
     int32_t n_jobs = 0;
     size_t o_lab = 0, o_lab_st = 0, o_topk = 0, o_topk_st = 0, o_place = 0, o_anneal = 0;
-    size_t o_avg = 0, o_avg_st = 0, o_comp = 0, o_adj_st = 0, o_comp_st = 0;
+    size_t o_comp = 0, o_adj_st = 0, o_comp_st = 0;
     size_t o_stats = 0, o_aout = 0, bytes = 0;
     int32_t lab_blocks = 0;
     int32_t n_topk = 0, topk_blocks = 0, topk_run = 1;
@@ -138,7 +138,7 @@ struct BatchPlan {
     int32_t n_anneal = 0;
     size_t anneal_lds = 0;
     bool narrow = true;
-    int32_t n_avg = 0, avg_blocks = 0, adj_blocks = 0;
+    int32_t adj_blocks = 0;
     int32_t n_comp = 0, comp_blocks = 0;
     std::vector<int32_t> anneal_job;  // job index of each anneal entry
 };
@@ -159,8 +159,6 @@ size_t plan_bytes(int32_t n, BatchPlan* p) {
     p->o_topk_st = take((nn + 1) * sizeof(int32_t));
     p->o_place = take(nn * sizeof(PlaceJob));
     p->o_anneal = take(nn * sizeof(AnnealJob));
-    p->o_avg = take(nn * sizeof(CellSeg));
-    p->o_avg_st = take((nn + 1) * sizeof(int32_t));
     p->o_comp = take(nn * sizeof(CompSeg));
     p->o_adj_st = take((nn + 1) * sizeof(int32_t));
     p->o_comp_st = take((nn + 1) * sizeof(int32_t));
@@ -191,16 +189,14 @@ int build_plan(const std::vector<JobSpec>& specs, const std::vector<uint8_t*>& j
     int32_t* topk_st = at<int32_t>(h, p->o_topk_st);
     PlaceJob* place = at<PlaceJob>(h, p->o_place);
     AnnealJob* anneal = at<AnnealJob>(h, p->o_anneal);
-    CellSeg* avg = at<CellSeg>(h, p->o_avg);
-    int32_t* avg_st = at<int32_t>(h, p->o_avg_st);
     CompSeg* comp = at<CompSeg>(h, p->o_comp);
     int32_t* adj_st = at<int32_t>(h, p->o_adj_st);
     int32_t* comp_st = at<int32_t>(h, p->o_comp_st);
     int32_t* d_stats = at<int32_t>(d, p->o_stats);
     AnnealOut* d_aout = at<AnnealOut>(d, p->o_aout);
 
-    int64_t lab_blocks = 0, topk_blocks = 0, avg_blocks = 0, adj_blocks = 0, comp_blocks = 0;
-    p->n_topk = p->n_anneal = p->n_avg = p->n_comp = 0;
+    int64_t lab_blocks = 0, topk_blocks = 0, adj_blocks = 0, comp_blocks = 0;
+    p->n_topk = p->n_anneal = p->n_comp = 0;
     int64_t topk_queries = 0;
     for (const JobSpec& s : specs)
         if (s.n > 0) topk_queries += s.cells;
@@ -222,8 +218,11 @@ int build_plan(const std::vector<JobSpec>& specs, const std::vector<uint8_t*>& j
         uint8_t* image = s.o_image == SIZE_MAX ? j.d_image_out : at<uint8_t>(m, s.o_image);
         // a1: cell Lab means of the target
         lab_st[i] = (int32_t)lab_blocks;
+        // (with the colour adjustment, the same pixel pass gives the cells'
+        // average RGB, a13)
+        uint8_t* tavg = s.adjust ? at<uint8_t>(m, s.o_tavg) : nullptr;
         lab[i] = CellSeg{j.d_target, s.pitch, (int64_t)s.th * s.pitch, (int64_t)s.tw * 3, s.gw, s.tw,
-                         s.th, s.cells, cell_lab, nullptr};
+                         s.th, s.cells, cell_lab, tavg};
         lab_blocks += cell_lab_blocks(s.cells);
         // a3: K nearest (libraries with materials only)
         if (s.n > 0) {
@@ -275,16 +274,8 @@ int build_plan(const std::vector<JobSpec>& specs, const std::vector<uint8_t*>& j
             ++p->n_anneal;
             if (s.n >= 32768) p->narrow = false;
         }
-        // a13-a16: cell averages, adjustments, composite
-        uint8_t* tavg = s.adjust ? at<uint8_t>(m, s.o_tavg) : nullptr;
+        // a13-a16: adjustments, composite
         float4* adj = s.adjust ? at<float4>(m, s.o_adj) : nullptr;
-        if (s.adjust) {
-            avg_st[p->n_avg] = (int32_t)avg_blocks;
-            avg[p->n_avg] = CellSeg{j.d_target, s.pitch, (int64_t)s.th * s.pitch, (int64_t)s.tw * 3,
-                                    s.gw, s.tw, s.th, s.cells, nullptr, tavg};
-            ++p->n_avg;
-            avg_blocks += s.cells;
-        }
         adj_st[p->n_comp] = (int32_t)adj_blocks;
         comp_st[p->n_comp] = (int32_t)comp_blocks;
         comp[p->n_comp] = CompSeg{grid, s.gw, s.gh, s.tw, s.th, s.tiles, s.tile_avg, tavg, s.strength,
@@ -298,17 +289,14 @@ int build_plan(const std::vector<JobSpec>& specs, const std::vector<uint8_t*>& j
         const size_t al = anneal_lds_bytes(s.cells, p->narrow);
         if (al > p->anneal_lds) p->anneal_lds = al;
     }
-    if (lab_blocks > INT32_MAX || topk_blocks > INT32_MAX || avg_blocks > INT32_MAX ||
-        comp_blocks > INT32_MAX)
+    if (lab_blocks > INT32_MAX || topk_blocks > INT32_MAX || comp_blocks > INT32_MAX)
         return fail(MH_ERR_UNSUPPORTED, "batch too large for one launch");
     lab_st[n] = (int32_t)lab_blocks;
     topk_st[p->n_topk] = (int32_t)topk_blocks;
-    avg_st[p->n_avg] = (int32_t)avg_blocks;
     adj_st[p->n_comp] = (int32_t)adj_blocks;
     comp_st[p->n_comp] = (int32_t)comp_blocks;
     p->lab_blocks = (int32_t)lab_blocks;
     p->topk_blocks = (int32_t)topk_blocks;
-    p->avg_blocks = (int32_t)avg_blocks;
     p->adj_blocks = (int32_t)adj_blocks;
     p->comp_blocks = (int32_t)comp_blocks;
     return MH_OK;
@@ -322,12 +310,9 @@ void enqueue_plan(const BatchPlan& p, uint8_t* d, const float* lut, hipStream_t 
     launch_place_batch(at<PlaceJob>(d, p.o_place), p.n_jobs, true, p.place_score, p.place_lds, s);
     if (p.n_anneal > 0)
         launch_anneal_batch(at<AnnealJob>(d, p.o_anneal), p.n_anneal, p.narrow, p.anneal_lds, s);
-    if (p.n_avg > 0) {
-        launch_avg_rgb_batch(at<CellSeg>(d, p.o_avg), at<int32_t>(d, p.o_avg_st), p.n_avg,
-                             p.avg_blocks, s);
+    if (p.adj_blocks > 0)  // the target averages came from the cell Lab pass
         launch_cell_adjust_batch(at<CompSeg>(d, p.o_comp), at<int32_t>(d, p.o_adj_st), p.n_comp,
                                  p.adj_blocks, s);
-    }
     launch_composite_batch(at<CompSeg>(d, p.o_comp), at<int32_t>(d, p.o_comp_st), p.n_comp,
                            p.comp_blocks, s);
 }

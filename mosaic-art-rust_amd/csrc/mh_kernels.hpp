@@ -100,9 +100,12 @@ struct CompSeg {
     uint8_t* out;               // (gw*tw) x (gh*th) x 3
 };
 
+// Cell Lab means; with out_rgb, also the cells' integer average RGB
+// (calculate_average_rgb) from the same pixel pass.
 void launch_cell_lab(const uint8_t* rgb, int64_t row_stride, int32_t per_row,
                      int64_t block_row_stride, int64_t block_col_stride, int32_t cw, int32_t ch,
-                     int32_t n_cells, const float* lut, float* out, hipStream_t stream);
+                     int32_t n_cells, const float* lut, float* out, hipStream_t stream,
+                     uint8_t* out_rgb = nullptr);
 void launch_sim_full(const float4* lab, int32_t n, float* out, hipStream_t stream);
 void launch_sim_packed(const float4* lab, int32_t n, float* out, hipStream_t stream);
 void launch_term_full(const float4* lab, int32_t n, float* out, hipStream_t stream);
@@ -120,9 +123,6 @@ int32_t topk_run_length(int64_t total_queries);
 int32_t topk_blocks(int32_t nq, int32_t run);
 void launch_topk_batch(const TopkSeg* d_segs, const int32_t* d_starts, int32_t n_segs,
                        int32_t total_blocks, int32_t run, hipStream_t stream);
-// a block per cell
-void launch_avg_rgb_batch(const CellSeg* d_segs, const int32_t* d_starts, int32_t n_segs,
-                          int32_t total_cells, hipStream_t stream);
 int32_t cell_adjust_blocks(int32_t cells);
 int32_t composite_blocks(int64_t pixels);
 void launch_cell_adjust_batch(const CompSeg* d_segs, const int32_t* d_starts, int32_t n_segs,
