@@ -302,11 +302,16 @@ int build_plan(const std::vector<JobSpec>& specs, const std::vector<uint8_t*>& j
     return MH_OK;
 }
 
-void enqueue_plan(const BatchPlan& p, uint8_t* d, const float* lut, hipStream_t s) {
+// The stages up to the candidate lists (cell Lab means, nearest materials).
+void enqueue_candidates(const BatchPlan& p, uint8_t* d, const float* lut, hipStream_t s) {
     launch_cell_lab_batch(at<CellSeg>(d, p.o_lab), at<int32_t>(d, p.o_lab_st), p.n_jobs, p.lab_blocks,
                           lut, s);
     launch_topk_batch(at<TopkSeg>(d, p.o_topk), at<int32_t>(d, p.o_topk_st), p.n_topk, p.topk_blocks,
                       p.topk_run, s);
+}
+
+// The stages from placement on (chains, adjustment, composite).
+void enqueue_chains(const BatchPlan& p, uint8_t* d, hipStream_t s) {
     launch_place_batch(at<PlaceJob>(d, p.o_place), p.n_jobs, true, p.place_score, p.place_lds, s);
     if (p.n_anneal > 0)
         launch_anneal_batch(at<AnnealJob>(d, p.o_anneal), p.n_anneal, p.narrow, p.anneal_lds, s);
@@ -315,6 +320,11 @@ void enqueue_plan(const BatchPlan& p, uint8_t* d, const float* lut, hipStream_t 
                                  p.adj_blocks, s);
     launch_composite_batch(at<CompSeg>(d, p.o_comp), at<int32_t>(d, p.o_comp_st), p.n_comp,
                            p.comp_blocks, s);
+}
+
+void enqueue_plan(const BatchPlan& p, uint8_t* d, const float* lut, hipStream_t s) {
+    enqueue_candidates(p, d, lut, s);
+    enqueue_chains(p, d, s);
 }
 
 void fill_result(const JobSpec& s, const int32_t* st, const AnnealOut* ao, mh_job_result* r) {
@@ -333,18 +343,51 @@ void fill_result(const JobSpec& s, const int32_t* st, const AnnealOut* ao, mh_jo
 }  // namespace
 
 // ================================================================= batch
+// A batch runs as two groups of jobs on two streams: the jobs whose grids
+// have at least half the cells of the largest one ("long": their placement
+// chains set the batch time) on the context stream, the others on a side
+// stream.  The short jobs' wide stages and chains then overlap the long
+// chains instead of running before and after them.  With one group (similar
+// sizes, or a single job) the side stream is unused.
 struct mh_batch {
     mh_context* ctx = nullptr;
     std::vector<JobSpec> specs;
     std::vector<uint8_t*> job_mem;
     uint8_t* d_jobs = nullptr;   // all jobs' working memory
-    uint8_t* d_plan = nullptr;   // descriptor tables + results
+    uint8_t* d_plan = nullptr;   // both groups' descriptor tables + results
     std::vector<uint8_t> h_plan;
-    BatchPlan plan;
+    int32_t n_groups = 0;
+    std::vector<int32_t> members[2];  // job indices of each group
+    BatchPlan plan[2];
+    size_t plan_off[2] = {0, 0};      // each group's offset in d_plan
+    std::vector<int32_t> group_of, index_in;  // job -> (group, position)
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     bool use_graph = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
+
+namespace {
+
+void enqueue_batch(mh_batch* b) {
+    mh_context* ctx = b->ctx;
+    if (b->n_groups == 1) {
+        enqueue_plan(b->plan[0], b->d_plan + b->plan_off[0], ctx->d_lut, ctx->stream);
+        return;
+    }
+    // the long jobs' candidates first on the whole device, then their chains
+    // start while the short jobs run beside them
+    enqueue_candidates(b->plan[0], b->d_plan + b->plan_off[0], ctx->d_lut, ctx->stream);
+    hipEventRecord(b->fork, ctx->stream);
+    hipStreamWaitEvent(b->side, b->fork, 0);
+    enqueue_chains(b->plan[0], b->d_plan + b->plan_off[0], ctx->stream);
+    enqueue_plan(b->plan[1], b->d_plan + b->plan_off[1], ctx->d_lut, b->side);
+    hipEventRecord(b->join, b->side);
+    hipStreamWaitEvent(ctx->stream, b->join, 0);
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -359,10 +402,14 @@ void mh_batch_destroy(mh_batch* b) {
     if (!b) return;
     hipSetDevice(b->ctx->device);
     hipStreamSynchronize(b->ctx->stream);
+    if (b->side) hipStreamSynchronize(b->side);
     if (b->exec) hipGraphExecDestroy(b->exec);
     if (b->graph) hipGraphDestroy(b->graph);
     if (b->d_jobs) hipFree(b->d_jobs);
     if (b->d_plan) hipFree(b->d_plan);
+    if (b->fork) hipEventDestroy(b->fork);
+    if (b->join) hipEventDestroy(b->join);
+    if (b->side) hipStreamDestroy(b->side);
     delete b;
 }
 
@@ -388,10 +435,36 @@ int mh_batch_create(mh_context* ctx, const mh_job* jobs, int32_t n_jobs, mh_batc
         offs[i] = total;
         total = align_up(total + b->specs[i].bytes);
     }
+    if (!rc) {  // groups: long chains on the context stream, the rest beside them
+        int32_t max_cells = 0;
+        for (const JobSpec& sp : b->specs) max_cells = std::max(max_cells, sp.cells);
+        b->group_of.resize((size_t)n_jobs);
+        b->index_in.resize((size_t)n_jobs);
+        for (int32_t i = 0; i < n_jobs; ++i) {
+            const int g = 2 * (int64_t)b->specs[i].cells >= max_cells ? 0 : 1;
+            b->group_of[i] = g;
+            b->index_in[i] = (int32_t)b->members[g].size();
+            b->members[g].push_back(i);
+        }
+        if (b->members[1].empty()) {
+            b->n_groups = 1;
+        } else {
+            b->n_groups = 2;
+            hipError_t e = hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&b->join, hipEventDisableTiming);
+            if (e != hipSuccess) rc = fail(MH_ERR_HIP, "batch stream: %s", hipGetErrorString(e));
+        }
+    }
     hipError_t e = hipSuccess;
+    size_t plan_total = 0;
     if (!rc) {
+        for (int g = 0; g < b->n_groups; ++g) {
+            b->plan_off[g] = plan_total;
+            plan_total = align_up(plan_total + plan_bytes((int32_t)b->members[g].size(), &b->plan[g]));
+        }
         e = hipMalloc(&b->d_jobs, total ? total : kAlign);
-        if (e == hipSuccess) e = hipMalloc(&b->d_plan, plan_bytes(n_jobs, &b->plan));
+        if (e == hipSuccess) e = hipMalloc(&b->d_plan, plan_total);
         if (e != hipSuccess)
             rc = fail(e == hipErrorOutOfMemory ? MH_ERR_OUT_OF_MEMORY : MH_ERR_HIP, "batch memory: %s",
                       hipGetErrorString(e));
@@ -399,11 +472,20 @@ int mh_batch_create(mh_context* ctx, const mh_job* jobs, int32_t n_jobs, mh_batc
     if (!rc) {
         b->job_mem.resize((size_t)n_jobs);
         for (int32_t i = 0; i < n_jobs; ++i) b->job_mem[i] = b->d_jobs + offs[i];
-        b->h_plan.assign(b->plan.bytes, 0);
-        rc = build_plan(b->specs, b->job_mem, b->h_plan.data(), b->d_plan, &b->plan);
+        b->h_plan.assign(plan_total, 0);
+        for (int g = 0; g < b->n_groups && !rc; ++g) {
+            std::vector<JobSpec> sp;
+            std::vector<uint8_t*> mem;
+            for (int32_t i : b->members[g]) {
+                sp.push_back(b->specs[i]);
+                mem.push_back(b->job_mem[i]);
+            }
+            rc = build_plan(sp, mem, b->h_plan.data() + b->plan_off[g], b->d_plan + b->plan_off[g],
+                            &b->plan[g]);
+        }
     }
     if (!rc) {
-        e = hipMemcpy(b->d_plan, b->h_plan.data(), b->plan.bytes, hipMemcpyHostToDevice);
+        e = hipMemcpy(b->d_plan, b->h_plan.data(), plan_total, hipMemcpyHostToDevice);
         if (e != hipSuccess) rc = fail(MH_ERR_HIP, "batch tables: %s", hipGetErrorString(e));
     }
     if (rc) {
@@ -425,13 +507,14 @@ int mh_batch_run(mh_batch* b) {
     mh_context* ctx = b->ctx;
     CHK(set_device(ctx));
     if (!b->use_graph) {
-        enqueue_plan(b->plan, b->d_plan, ctx->d_lut, ctx->stream);
+        enqueue_batch(b);
         HIPCHK(hipGetLastError());
         return MH_OK;
     }
     if (!b->exec) {
+        // the side stream joins the capture through the fork event
         HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        enqueue_plan(b->plan, b->d_plan, ctx->d_lut, ctx->stream);
+        enqueue_batch(b);
         hipError_t le = hipGetLastError();
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(ctx->stream, &g);
@@ -451,21 +534,27 @@ int mh_batch_results(mh_batch* b, mh_job_result* results) {
     if (!b || !results) return fail(MH_ERR_INVALID_ARGUMENT, "NULL argument");
     mh_context* ctx = b->ctx;
     CHK(set_device(ctx));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    const int32_t n = b->plan.n_jobs;
-    std::vector<int32_t> st((size_t)n * kStatsInts);
-    std::vector<AnnealOut> ao((size_t)n);
-    HIPCHK(hipMemcpy(st.data(), b->d_plan + b->plan.o_stats, st.size() * sizeof(int32_t),
-                     hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(ao.data(), b->d_plan + b->plan.o_aout, ao.size() * sizeof(AnnealOut),
-                     hipMemcpyDeviceToHost));
-    for (int32_t i = 0; i < n; ++i)
-        fill_result(b->specs[i], &st[(size_t)i * kStatsInts], &ao[i], &results[i]);
+    HIPCHK(hipStreamSynchronize(ctx->stream));  // the side stream joined it
+    for (int g = 0; g < b->n_groups; ++g) {
+        const BatchPlan& p = b->plan[g];
+        const uint8_t* d = b->d_plan + b->plan_off[g];
+        std::vector<int32_t> st((size_t)p.n_jobs * kStatsInts);
+        std::vector<AnnealOut> ao((size_t)p.n_jobs);
+        HIPCHK(hipMemcpy(st.data(), d + p.o_stats, st.size() * sizeof(int32_t),
+                         hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(ao.data(), d + p.o_aout, ao.size() * sizeof(AnnealOut),
+                         hipMemcpyDeviceToHost));
+        for (int32_t k = 0; k < p.n_jobs; ++k) {
+            const int32_t i = b->members[g][k];
+            fill_result(b->specs[i], &st[(size_t)k * kStatsInts], &ao[k], &results[i]);
+        }
+    }
     return MH_OK;
 }
 
 int mh_batch_job_outputs(mh_batch* b, int32_t i, int32_t** d_grid, uint8_t** d_image) {
-    if (!b || i < 0 || i >= b->plan.n_jobs) return fail(MH_ERR_INVALID_ARGUMENT, "bad job index");
+    if (!b || i < 0 || i >= (int32_t)b->specs.size())
+        return fail(MH_ERR_INVALID_ARGUMENT, "bad job index");
     const JobSpec& s = b->specs[i];
     if (d_grid)
         *d_grid = s.o_grid == SIZE_MAX ? s.job.d_grid_out : at<int32_t>(b->job_mem[i], s.o_grid);
