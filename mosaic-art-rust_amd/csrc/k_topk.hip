@@ -44,42 +44,104 @@ __device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
     return v;
 }
 
-// Ascending bitonic sort of buf[0..P) by one wave (P = 128 or 256); entries
-// [count, P) are first filled with the largest key.  The network is fully
-// unrolled, so every index is a compile-time function of the lane.
-template <int P>
-__device__ __forceinline__ void wave_sort(uint64_t* buf, int32_t count) {
-    const int lane = threadIdx.x;
-#pragma unroll
-    for (int r = 0; r < P / 64; ++r)
-        if (r * 64 + lane >= count) buf[r * 64 + lane] = ~0ull;
-    __syncthreads();
+// The value of lane ^ m (m a power of two below 64) for every lane: DPP
+// lane swizzles within rows of 16 for m = 1, 2, 8, the swizzle unit for
+// m = 4, 16 (within halves of 32), a lane permute for m = 32.  m is a
+// compile-time constant after unrolling, so only one case remains.
+__device__ __forceinline__ uint32_t lane_xor(uint32_t v, int m) {
+    switch (m) {
+        case 1: return __builtin_amdgcn_mov_dpp(v, 0xb1, 0xf, 0xf, false);   // quad_perm [1,0,3,2]
+        case 2: return __builtin_amdgcn_mov_dpp(v, 0x4e, 0xf, 0xf, false);   // quad_perm [2,3,0,1]
+        case 8: return __builtin_amdgcn_mov_dpp(v, 0x128, 0xf, 0xf, false);  // row_ror:8
+        case 4: return __builtin_amdgcn_ds_swizzle(v, (4 << 10) | 0x1f);      // xor 4
+        case 16: return __builtin_amdgcn_ds_swizzle(v, (16 << 10) | 0x1f);    // xor 16
+        default: return __shfl_xor(v, 32, 64);
+    }
+}
+
+__device__ __forceinline__ uint64_t lane_xor64(uint64_t v, int m) {
+    return ((uint64_t)lane_xor((uint32_t)(v >> 32), m) << 32) | lane_xor((uint32_t)v, m);
+}
+
+// Lanes whose index has bit b set (b a power of two below 64).
+__device__ __forceinline__ constexpr uint64_t lane_bit_mask(int b) {
+    return b == 1 ? 0xAAAAAAAAAAAAAAAAull
+         : b == 2 ? 0xCCCCCCCCCCCCCCCCull
+         : b == 4 ? 0xF0F0F0F0F0F0F0F0ull
+         : b == 8 ? 0xFF00FF00FF00FF00ull
+         : b == 16 ? 0xFFFF0000FFFF0000ull
+                   : 0xFFFFFFFF00000000ull;
+}
+
+// Ascending bitonic sort of the S*64 keys held in registers, element
+// s*64 + lane in key[s].  Pairs 64 or more apart are in one lane
+// (compare-exchange in registers); closer pairs are in partner lanes, which
+// exchange through the lane swizzles above -- no LDS round trip per stage.
+// Whether a lane keeps the smaller key of its pair is a compile-time lane
+// mask for each stage.
+template <int S>
+__device__ __forceinline__ void reg_sort(uint64_t (&key)[S]) {
+    constexpr int P = S * 64;
 #pragma unroll
     for (int size = 2; size <= P; size <<= 1) {
 #pragma unroll
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            if (stride >= 64) {
+                const int sd = stride / 64;
 #pragma unroll
-            for (int h = 0; h < P / 128; ++h) {
-                const int t = h * 64 + lane;
-                const int i = ((t & ~(stride - 1)) << 1) | (t & (stride - 1));
-                const int j = i + stride;
-                const bool asc = (i & size) == 0;
-                const uint64_t a = buf[i], b = buf[j];
-                if ((a > b) == asc) {
-                    buf[i] = b;
-                    buf[j] = a;
+                for (int t = 0; t < S; ++t) {
+                    if ((t & sd) != 0) continue;
+                    const bool asc = ((t * 64) & size) == 0;
+                    const uint64_t a = key[t], b = key[t | sd];
+                    const bool swap = (a > b) == asc;
+                    key[t] = swap ? b : a;
+                    key[t | sd] = swap ? a : b;
+                }
+            } else {
+#pragma unroll
+                for (int t = 0; t < S; ++t) {
+                    // this lane is the pair's lower element where its stride
+                    // bit is clear; the pair ascends where bit `size` of the
+                    // element index is clear; the lower element of an
+                    // ascending pair (and the upper of a descending one) keeps
+                    // the minimum
+                    const uint64_t upper = lane_bit_mask(stride);
+                    const uint64_t desc = size < 64 ? lane_bit_mask(size)
+                                        : (((t * 64) & size) != 0 ? ~0ull : 0ull);
+                    const uint64_t keep_min = ~(upper ^ desc);
+                    const uint64_t mine = key[t];
+                    const uint64_t other = lane_xor64(mine, stride);
+                    const uint64_t other_less = __builtin_amdgcn_uicmpl(other, mine, 36);
+                    // take the partner's key where it is the one to keep (keys
+                    // are unique but for the padding, where either is the same)
+                    const uint64_t take = ~(other_less ^ keep_min);
+                    key[t] = __builtin_amdgcn_inverse_ballot_w64(take) ? other : mine;
                 }
             }
-            __syncthreads();
         }
     }
 }
 
+// Sorts buf[0..count) ascending in place (count <= 256); the sorted keys
+// are also left in key[] (slot-major, padded with the largest key).
 __device__ __forceinline__ void sort_buffer(uint64_t* buf, int32_t count) {
-    if (count <= 128)
-        wave_sort<128>(buf, count);
-    else
-        wave_sort<256>(buf, count);
+    const int lane = threadIdx.x;
+    if (count <= 128) {
+        uint64_t key[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) key[t] = t * 64 + lane < count ? buf[t * 64 + lane] : ~0ull;
+        reg_sort<2>(key);
+#pragma unroll
+        for (int t = 0; t < 2; ++t) buf[t * 64 + lane] = key[t];
+    } else {
+        uint64_t key[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) key[t] = t * 64 + lane < count ? buf[t * 64 + lane] : ~0ull;
+        reg_sort<4>(key);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) buf[t * 64 + lane] = key[t];
+    }
+    __builtin_amdgcn_wave_barrier();
 }
 
 // Answers query cells [q0, q0 + run) of segment s.
@@ -100,7 +162,34 @@ __device__ __forceinline__ void topk_run(const TopkSeg& s, int32_t q0, int32_t r
         auto dist_bits = [&](int32_t i) {
             return __float_as_uint(sq_distance(ql, qa, qb, gload(mL, i), gload(mA, i), gload(mB, i)));
         };
-        // threshold from the previous cell's answers
+        // Four 64-material steps per load phase: all twelve loads are issued
+        // before the first distance is needed (one memory latency per four
+        // steps).  Loads past n are clamped in bounds; f(d, i, valid) sees
+        // each step's distance bits, material index and in-range flag.
+        constexpr int kSteps = 4;
+        auto scan = [&](auto&& f) {
+            for (int32_t base = 0; base < n; base += 64 * kSteps) {
+                float l[kSteps], a[kSteps], b[kSteps];
+#pragma unroll
+                for (int u = 0; u < kSteps; ++u) {
+                    const int32_t i = base + 64 * u + lane;
+                    const int32_t ic = i < n ? i : n - 1;
+                    l[u] = gload(mL, ic);
+                    a[u] = gload(mA, ic);
+                    b[u] = gload(mB, ic);
+                }
+                uint32_t d[kSteps];
+#pragma unroll
+                for (int u = 0; u < kSteps; ++u)
+                    d[u] = __float_as_uint(sq_distance(ql, qa, qb, l[u], a[u], b[u]));
+#pragma unroll
+                for (int u = 0; u < kSteps; ++u) {
+                    const int32_t i = base + 64 * u + lane;
+                    f(d[u], i);
+                }
+            }
+        };
+        // threshold: the previous cell's answers
         uint64_t tau = ~0ull;
         if (q > q0) {
             uint64_t m = 0;
@@ -110,33 +199,48 @@ __device__ __forceinline__ void topk_run(const TopkSeg& s, int32_t q0, int32_t r
                 m = kk > m ? kk : m;
             }
             tau = wave_max_u64(m);
+        } else if (n > kBuf) {
+            // The first cell of a run: one pass keeps every lane's two
+            // smallest keys; k of those 128 are real keys no larger than
+            // their k-th smallest, so it bounds the answer (and the scan below
+            // rarely overflows the buffer).
+            uint64_t two[2] = {~0ull, ~0ull};
+            scan([&](uint32_t d, int32_t i) {
+                const uint64_t key = i < n ? make_key(d, i) : ~0ull;
+                const bool lt0 = key < two[0], lt1 = key < two[1];
+                two[1] = lt0 ? two[0] : (lt1 ? key : two[1]);
+                two[0] = lt0 ? key : two[0];
+            });
+            reg_sort<2>(two);
+            const uint64_t v = k - 1 < 64 ? two[0] : two[1];
+            const int src = (k - 1) & 63;
+            tau = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int32_t)(v >> 32), src) << 32) |
+                  (uint32_t)__builtin_amdgcn_readlane((int32_t)(uint32_t)v, src);
         }
         int32_t count = 0;
-        // appends the keys <= tau of one 64-material step to the buffer,
+        // Lanes whose key (d, i) is <= tau, compared as two 32-bit halves
+        // with single-compare lane masks (the distance bits are unsigned-
+        // ordered like the distances).
+        auto passing = [&](uint32_t d, int32_t i) {
+            const uint32_t th = (uint32_t)(tau >> 32), tl = (uint32_t)tau;
+            return lanes_ult(d, th) | (lanes_eq((int32_t)d, (int32_t)th) & lanes_ule((uint32_t)i, tl));
+        };
+        // appends the passing lanes of one 64-material step to the buffer,
         // first cutting the buffer to its k smallest when it would overflow
-        auto take = [&](uint64_t key, bool valid) {
-            bool pass = valid && key <= tau;
-            uint64_t mask = wave_ballot(pass);
-            if (mask == 0) return;
+        auto take = [&](uint32_t d, int32_t i, uint64_t mask) {
             if (count + __popcll(mask) > kBuf) {
                 sort_buffer(buf, count);
                 count = k;
                 tau = buf[k - 1];
-                pass = pass && key <= tau;
-                mask = wave_ballot(pass);
+                mask &= passing(d, i);
             }
-            if (pass) buf[count + __popcll(mask & lanes_below)] = key;
+            if ((mask >> lane) & 1ull) buf[count + __popcll(mask & lanes_below)] = make_key(d, i);
             count += __popcll(mask);
         };
-        // two steps per iteration; loads are clamped in bounds and issued
-        // together, the keys of lanes past n are masked out
-        for (int32_t base = 0; base < n; base += 128) {
-            const int32_t i0 = base + lane, i1 = i0 + 64;
-            const uint32_t d0 = dist_bits(i0 < n ? i0 : n - 1);
-            const uint32_t d1 = dist_bits(i1 < n ? i1 : n - 1);
-            take(make_key(d0, i0), i0 < n);
-            take(make_key(d1, i1), i1 < n);
-        }
+        scan([&](uint32_t d, int32_t i) {
+            const uint64_t mask = passing(d, i) & lanes_slt(i, n);
+            if (mask) take(d, i, mask);
+        });
         sort_buffer(buf, count);
         // answer: the first k keys, ascending
         const int32_t out_stride = s.out_stride;

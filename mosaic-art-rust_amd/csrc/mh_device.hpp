@@ -53,6 +53,12 @@ __device__ __forceinline__ uint64_t lanes_ne(int32_t a, int32_t b) {
 __device__ __forceinline__ uint64_t lanes_ult(uint32_t a, uint32_t b) {
     return __builtin_amdgcn_uicmp(a, b, 36);
 }
+__device__ __forceinline__ uint64_t lanes_ule(uint32_t a, uint32_t b) {
+    return __builtin_amdgcn_uicmp(a, b, 37);
+}
+__device__ __forceinline__ uint64_t lanes_slt(int32_t a, int32_t b) {
+    return __builtin_amdgcn_sicmp(a, b, 40);
+}
 __device__ __forceinline__ uint64_t lanes_nonneg(int32_t a) {
     return __builtin_amdgcn_sicmp(a, -1, 38);  // a > -1
 }
