@@ -173,10 +173,10 @@ __device__ __forceinline__ void topk_run(const TopkSeg& s, int32_t q0, int32_t r
 #pragma unroll
                 for (int u = 0; u < kSteps; ++u) {
                     const int32_t i = base + 64 * u + lane;
-                    const int32_t ic = i < n ? i : n - 1;
-                    l[u] = gload(mL, ic);
-                    a[u] = gload(mA, ic);
-                    b[u] = gload(mB, ic);
+                    const uint32_t off = 4u * (uint32_t)(i < n ? i : n - 1);  // n < 2^30
+                    l[u] = gload_at(mL, off);
+                    a[u] = gload_at(mA, off);
+                    b[u] = gload_at(mB, off);
                 }
                 uint32_t d[kSteps];
 #pragma unroll

@@ -31,6 +31,13 @@ template <typename T>
 __device__ __forceinline__ T gload(const T* p, int64_t i) {
     return ((const MH_GAS T*)p)[i];
 }
+// A load at a 32-bit byte offset from a uniform base: one address register
+// for any number of arrays indexed alike (the base stays scalar).
+template <typename T>
+__device__ __forceinline__ T gload_at(const T* p, uint32_t byte_off) {
+    const MH_GAS char* base = (const MH_GAS char*)p;
+    return *(const MH_GAS T*)(base + byte_off);
+}
 template <typename T>
 __device__ __forceinline__ void gstore(T* p, int64_t i, T v) {
     ((MH_GAS T*)p)[i] = v;
