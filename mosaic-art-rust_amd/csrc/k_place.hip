@@ -26,11 +26,11 @@
 //    empty grid, so only Up and Left can be filled.  The left neighbour is
 //    the previous winner (registers), the up neighbour comes from a per-column
 //    LDS row (material + Lab of the last placement in that column), usage
-//    counts live in LDS, candidate records are loaded 12 cells at a time into
+//    counts live in LDS, candidate records are loaded 24 cells at a time into
 //    registers, the min is a DPP reduction, and the winner's id / Lab / count
-//    are read from its lane.  Winners are buffered 64 at a time in one
-//    register and stored coalesced.  Latency-bound: a few hundred cycles per
-//    cell.
+//    are read from its lane.  Winners go to a 64-entry LDS ring that is
+//    stored to the grid, coalesced, once per group of 24 cells.
+//    Latency-bound: a few hundred cycles per cell.
 //  * place_chain_kernel -- any cell order over a pre-filled grid and usage
 //    (find_and_use_best_tile_with_position at arbitrary cells); grid and usage
 //    in LDS, all four neighbours read per cell.

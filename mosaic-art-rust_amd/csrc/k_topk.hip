@@ -16,11 +16,15 @@
 //     mbcnt compaction, 64 materials per step);
 //   * when the buffer would overflow it is sorted and cut to its K smallest,
 //     whose largest becomes the new (tighter) tau;
-//   * at the end the buffer is sorted (bitonic network in LDS) and the first
-//     K keys are the answer.
-// The first cell of a run starts from tau = +inf.  The cost is one distance
-// evaluation per material plus a few small sorts, with no histograms or
-// atomics.  The batched kernel serves many (library, query set) segments in
+//   * at the end the buffer is sorted and the first K keys are the answer.
+//     Sorts are bitonic networks over 2 or 4 keys per lane held in
+//     registers; partners are exchanged by DPP / swizzle / permute lane
+//     moves, so no sort step touches LDS.
+// The first cell of a run (no previous answers) gets its tau from one extra
+// pass that keeps each lane's two smallest keys: the K-th smallest of those
+// 128 real keys bounds the K-th smallest key overall.  The cost is one
+// distance evaluation per material plus a few register sorts, with no
+// histograms or atomics.  The batched kernel serves many (library, query set) segments in
 // one launch: block b handles run b - starts[j] of the segment j holding b.
 #include "mh_device.hpp"
 #include "mh_kernels.hpp"
