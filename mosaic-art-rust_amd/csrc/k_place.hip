@@ -196,7 +196,8 @@ constexpr int32_t kWinRing = 64;  // winners awaiting their grid store (power of
 constexpr int32_t kNoMat = -2;    // "no previous winner" (never a material or the -1 sentinel)
 
 struct RasterState {
-    int32_t x, y;      // current cell position
+    int32_t x;         // current cell's column
+    int32_t xn, x2;    // columns of the next cell and of the one after it
     int32_t left_m;    // previous winner (kEmpty at the start of a row)
     int32_t up_m;      // material above the current cell (kEmpty in row 0)
     uint64_t ob0;      // rank 0-63 candidates of the current cell that are usable
@@ -210,7 +211,25 @@ struct RasterState {
     int32_t st_ban, st_near, st_resets;
 };
 
-// One cell of the raster chain.
+// Lane 0's LDS write targets, with every other lane sent to its own scratch
+// word: a target is base + index * mul bytes, where lane 0 holds the array
+// and mul = 4 and the other lanes hold their scratch word and mul = 0, so one
+// 24-bit multiply-add forms each address (indices are below 2^24).
+struct CommitPorts {
+    uint32_t mul;                  // 4 in lane 0, else 0
+    uint32_t* usage;               // usage_s (lane 0) or the lane's scratch word
+    int32_t* row;                  // row_mat (lane 0) or the lane's scratch word
+    int32_t* ring;                 // win_s (lane 0) or the lane's scratch word
+};
+
+template <typename T>
+__device__ __forceinline__ T* port_at(T* base, int32_t index, uint32_t mul) {
+    return reinterpret_cast<T*>(reinterpret_cast<char*>(base) + __umul24((uint32_t)index, mul));
+}
+
+// One cell of the raster chain, in grids at least three columns wide (the
+// up neighbour of the cell after next was committed at least two cells
+// before it, so it can be read a step early).
 //
 // Speculation: the winner is usually the first usable candidate f.  Its
 // material w is read first and the next cell's usable mask is built from it
@@ -229,8 +248,7 @@ struct RasterState {
 // The cell after next has its usage counts and up neighbour read from LDS in
 // this step (so the reads have a whole step to land); when they are used one
 // step later they are corrected in registers for the two winners committed
-// in between (+1 each, or a base of 0 after a usage reset).  In grids one or
-// two columns wide the up neighbour is one of those two winners.  Lane 0
+// in between (+1 each, or a base of 0 after a usage reset).  Lane 0
 // commits: one use of the winner (LDS atomic), the column's latest material,
 // and the winner into an LDS ring written to the grid once per group of cells
 // (a vector store inside the group would make the next use of a group-loaded
@@ -239,48 +257,42 @@ template <bool kOutputs>
 __device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn,
                                             const CandSet& snn, int32_t c, RasterState& st,
                                             const PlaceJob& job, uint32_t* usage_s,
-                                            int32_t* row_mat, int32_t* win_s,
-                                            int32_t* lane_scratch, int32_t n, int lane) {
+                                            const int32_t* row_mat, const CommitPorts& port,
+                                            int32_t n, int lane) {
     const uint32_t kNone = 0xffffffffu;
     const int32_t gw = job.grid_w;
-    const int32_t x = st.x, y = st.y;
-    const int32_t xn = x + 1 == gw ? 0 : x + 1;
-    const int32_t yn = x + 1 == gw ? y + 1 : y;
-    const int32_t x2 = xn + 1 == gw ? 0 : xn + 1;
-    const int32_t y2 = xn + 1 == gw ? yn + 1 : yn;
+    const int32_t x = st.x, xn = st.xn, x2 = st.x2;
+    const int32_t x3 = x2 + 1 == gw ? 0 : x2 + 1;
     const uint32_t max_usage = (uint32_t)job.max_usage;
     // Reads for the cell after next (they see every commit up to the previous
     // cell).  A -1 sentinel reads the word before the counts (the last column
     // entry), which its masked lane ignores.
     const uint32_t nnu0 = usage_s[snn.m0];
-    // The up neighbour: row_mat starts all kEmpty, so row 0 needs no test, and
-    // in grids of one or two columns the value is not used.  It is the same in
-    // every lane; a lane-dependent address keeps it in a vector register, so
-    // the read is waited for only where the next step compares against it (a
-    // uniform value would be moved to a scalar register right after the read,
-    // stalling on it here).
+    // The up neighbour: row_mat starts all kEmpty, so row 0 needs no test.  It
+    // is the same in every lane; a lane-dependent address keeps it in a
+    // vector register, so the read is waited for only where the next step
+    // compares against it (a uniform value would be moved to a scalar
+    // register right after the read, stalling on it here).
     const int32_t nnup = row_mat[x2 + st.lane_zero];
 
     // The next cell's usable mask, built from lane masks.  Its counts before
     // this cell's commit are `base`; this cell's winner w adds one use to the
-    // lanes holding w, bans them as the left neighbour unless the next cell
-    // starts a row, and is the up neighbour in a one-column grid:
+    // lanes holding w and bans them as the left neighbour unless the next
+    // cell starts a row:
     //   usable = valid & up-ban & base < max & (lanes != w | (row start & base + 1 < max))
     // Everything but the w compare is known before w.
     const uint32_t base = (st.prev_reset ? 0u : st.pre_u) + (sn.m0 == st.prev_w ? 1u : 0u);
-    const int32_t up_fixed = gw == 2 ? (yn == 0 ? kEmpty : st.prev_w) : st.pre_up;
+    const int32_t up_next = st.pre_up;
     const uint64_t lt0 = lanes_ult(base, max_usage);
     // base + 1 < max for max >= 1; with max == 0, lt0 is empty and masks it
     const uint64_t lt1 = lanes_ult(base, max_usage - 1u);
-    const uint64_t ne_up = gw == 1 ? ~0ull : lanes_ne(sn.m0, up_fixed);
-    const uint64_t m_base = lanes_nonneg(sn.m0) & ne_up & lt0;
-    const uint64_t keep_w = xn != 0 || gw == 1 ? 0ull : lt1;
+    const uint64_t m_base = lanes_nonneg(sn.m0) & lanes_ne(sn.m0, up_next) & lt0;
+    const uint64_t keep_w = xn == 0 ? lt1 : 0ull;
 
     // speculative winner and the next cell's usable mask
     const uint64_t ob0 = st.ob0;
     const int32_t f = ob0 ? (int32_t)__ffsll((unsigned long long)ob0) - 1 : 0;
     int32_t w = __builtin_amdgcn_readlane(s.m0, f);
-    int32_t nup = gw == 1 ? w : up_fixed;
     uint64_t nob = m_base & (~lanes_eq(sn.m0, w) | keep_w);
 
     // confirm it: every other usable rank 0-63 candidate and rank 64 (the
@@ -299,7 +311,8 @@ __device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn,
     int32_t rank = f, stage = 0;
     uint32_t win_score = 0;
     if (!fast) {
-        const int32_t up_m = y > 0 ? st.up_m : kEmpty;
+        // every cell gets a tile, so a filled up neighbour means row > 0
+        const int32_t up_m = st.up_m;
         const int32_t left_m = x > 0 ? st.left_m : kEmpty;
         const bool pl0 = s.m0 >= 0 && s.m0 != up_m && s.m0 != left_m;
         const bool ok0 = ((ob0 >> lane) & 1ull) != 0;
@@ -320,20 +333,21 @@ __device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn,
                 if (__popcll(cb0) + __popcll(cb1) > 1 || (kOutputs && job.score)) {
                     // calculate_penalty for the contenders: Up then Left (Down
                     // and Right are still empty in raster order)
-                    const float4 up_lab = y > 0 ? gload4(job.lab4, up_m) : make_float4(0, 0, 0, 0);
+                    const float4 up_lab =
+                        up_m >= 0 ? gload4(job.lab4, up_m) : make_float4(0, 0, 0, 0);
                     const float4 left_lab =
                         x > 0 ? gload4(job.lab4, left_m) : make_float4(0, 0, 0, 0);
                     uint32_t k0 = kNone, k1 = kNone;
                     if (cb0) {
                         float p = 0.0f;
-                        if (y > 0) p += adj_term(s.c0, up_lab);
+                        if (up_m >= 0) p += adj_term(s.c0, up_lab);
                         if (x > 0) p += adj_term(s.c0, left_lab);
                         const float sc = s.c0.w + p * job.weight;
                         k0 = ct0 ? __float_as_uint(sc) : kNone;
                     }
                     if (cb1) {
                         float p = 0.0f;
-                        if (y > 0) p += adj_term(s.c1, up_lab);
+                        if (up_m >= 0) p += adj_term(s.c1, up_lab);
                         if (x > 0) p += adj_term(s.c1, left_lab);
                         const float sc = s.c1.w + p * job.weight;
                         k1 = ct1 ? __float_as_uint(sc) : kNone;
@@ -371,21 +385,18 @@ __device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn,
         }
         w = rank < 64 ? __builtin_amdgcn_readlane(s.m0, rank & 63)
                       : __builtin_amdgcn_readlane(s.m1, rank & 63);
-        nup = gw == 1 ? w : up_fixed;
         const uint32_t h = sn.m0 == w ? 1u : 0u;
         const uint32_t nu = stage == 0 ? base + h : h;  // a reset zeroed every count
-        nob = wave_ballot(sn.m0 >= 0 && sn.m0 != nup && (xn == 0 || sn.m0 != w) && nu < max_usage);
+        nob = wave_ballot(sn.m0 >= 0 && sn.m0 != up_next && (xn == 0 || sn.m0 != w) &&
+                          nu < max_usage);
     }
-    {
-        // Lane 0 commits; every other lane writes its own scratch word
-        // instead, so the commit is straight-line code with no branch on the
-        // lane (the step stays one block for the scheduler).  A per-lane
-        // address also keeps the add a plain ds_add.
-        const bool l0 = lane == 0;
-        atomicAdd(l0 ? &usage_s[w] : reinterpret_cast<uint32_t*>(lane_scratch), 1u);
-        *(l0 ? &row_mat[x] : lane_scratch) = w;
-        *(l0 ? &win_s[c & (kWinRing - 1)] : lane_scratch) = w;
-    }
+    // Lane 0 commits; every other lane writes its own scratch word instead,
+    // so the commit is straight-line code with no branch on the lane (the
+    // step stays one block for the scheduler).  A per-lane address also
+    // keeps the add a plain ds_add.
+    atomicAdd(port_at(port.usage, w, port.mul), 1u);
+    *port_at(port.row, x, port.mul) = w;
+    *port_at(port.ring, c & (kWinRing - 1), port.mul) = w;
     if (kOutputs) {
         const float dw = rank < 64 ? readlane_f(s.c0.w, rank & 63) : readlane_f(s.c1.w, rank & 63);
         if (lane == 0) {
@@ -400,11 +411,99 @@ __device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn,
     st.pre_up = nnup;
     st.prev_reset = stage != 0 ? 1 : 0;
     st.prev_w = w;
-    st.up_m = nup;
+    st.up_m = up_next;
     st.left_m = w;
     st.ob0 = nob;
     st.x = xn;
-    st.y = yn;
+    st.xn = x2;
+    st.x2 = x3;
+}
+
+// Raster order in grids one or two columns wide: the up neighbour is the
+// winner one (one column) or two (two columns) cells back and the left
+// neighbour the previous winner in the second column.  One cell at a time
+// with every usable candidate scored -- the same decisions as raster_step,
+// without its read-ahead of the up neighbour, which needs three columns.
+template <bool kOutputs>
+__device__ void raster_narrow(const PlaceJob& job, bool sq, uint32_t* usage_s, RasterState& st,
+                              int lane) {
+    const int32_t gw = job.grid_w, cells = gw * job.grid_h;
+    const uint32_t max_usage = (uint32_t)job.max_usage;
+    const float weight = job.weight;
+    int32_t w1 = kEmpty, w2 = kEmpty;  // winners one and two cells back
+    for (int32_t c = 0; c < cells; ++c) {
+        CandSet s;
+        load_cands(s, job, c, cells, lane);
+        if (sq) to_distance(s);
+        const int32_t up_m = c >= gw ? (gw == 1 ? w1 : w2) : kEmpty;
+        const int32_t left_m = gw == 2 && (c & 1) ? w1 : kEmpty;
+        const bool pl0 = s.m0 >= 0 && s.m0 != up_m && s.m0 != left_m;
+        const bool pl1 = s.m1 >= 0 && s.m1 != up_m && s.m1 != left_m;
+        const bool ok0 = pl0 && usage_s[s.m0 >= 0 ? s.m0 : 0] < max_usage;
+        const bool ok1 = pl1 && usage_s[s.m1 >= 0 ? s.m1 : 0] < max_usage;
+        float sc0 = s.c0.w, sc1 = s.c1.w;
+        if (weight > 0.0f) {
+            // calculate_penalty: Up then Left (Down and Right are still empty)
+            float p0 = 0.0f, p1 = 0.0f;
+            if (up_m >= 0) {
+                const float4 u = gload4(job.lab4, up_m);
+                p0 += adj_term(s.c0, u);
+                p1 += adj_term(s.c1, u);
+            }
+            if (left_m >= 0) {
+                const float4 l = gload4(job.lab4, left_m);
+                p0 += adj_term(s.c0, l);
+                p1 += adj_term(s.c1, l);
+            }
+            sc0 = s.c0.w + p0 * weight;
+            sc1 = s.c1.w + p1 * weight;
+        }
+        // first strict-< winner: the minimum of (score, rank)
+        const uint64_t k0 = ok0 ? ((uint64_t)__float_as_uint(sc0) << 32) | (uint32_t)lane : ~0ull;
+        const uint64_t k1 =
+            ok1 ? ((uint64_t)__float_as_uint(sc1) << 32) | (uint32_t)(lane + 64) : ~0ull;
+        const uint64_t best = wave_min_u64(k0 < k1 ? k0 : k1);
+        int32_t rank, stage = 0;
+        if (best != ~0ull) {
+            rank = (int32_t)(uint32_t)best;
+        } else {
+            // fallback_tile_selection: reset every count, first placeable, else nearest_one
+            for (int32_t i = lane; i < job.n; i += 64) usage_s[i] = 0u;
+            ++st.st_resets;
+            const uint64_t b0 = wave_ballot(pl0);
+            const uint64_t b1 = wave_ballot(pl1);
+            if (b0) {
+                rank = __ffsll((unsigned long long)b0) - 1;
+                stage = 1;
+                ++st.st_ban;
+            } else if (b1) {
+                rank = 64 + __ffsll((unsigned long long)b1) - 1;
+                stage = 1;
+                ++st.st_ban;
+            } else {
+                rank = 0;
+                stage = 2;
+                ++st.st_near;
+            }
+        }
+        const int32_t w = rank < 64 ? __builtin_amdgcn_readlane(s.m0, rank & 63)
+                                    : __builtin_amdgcn_readlane(s.m1, rank & 63);
+        const float dw = rank < 64 ? readlane_f(s.c0.w, rank & 63) : readlane_f(s.c1.w, rank & 63);
+        __syncthreads();
+        if (lane == 0) {
+            usage_s[w] += 1u;
+            gstore(job.grid, c, w);
+            if (kOutputs) {
+                if (job.stage) gstore(job.stage, c, (uint8_t)stage);
+                if (job.dist) gstore(job.dist, c, dw);
+                if (job.score)
+                    gstore(job.score, c, stage == 0 ? __uint_as_float((uint32_t)(best >> 32)) : dw);
+            }
+        }
+        __syncthreads();
+        w2 = w1;
+        w1 = w;
+    }
 }
 
 // kScore: kScoreSq / kScoreDist fix the score for every job of the launch,
@@ -435,76 +534,88 @@ __global__ __launch_bounds__(64) void place_raster_kernel(PlaceJob single,
     for (int32_t i = lane; i < gw; i += 64) row_mat[i] = kEmpty;
     __syncthreads();
 
-    // Candidate records for kGroup cells (and the two after them) are loaded at
-    // the top of each group and consumed in the same loop iteration, so the
-    // compiler's wait counts stay exact (a load consumed after the loop's back
-    // edge would make it wait for every load in flight); one memory latency
-    // per group.  The previous group's winners are stored just before the loads.
-    // 24 cells per group: one load latency per 24 cells for 256 VGPRs (a few
-    // spilled to AGPRs, one wave per SIMD); 12 cells cost 6% more chain time.
-    constexpr int kGroup = 24;
-    static_assert(kGroup <= kWinRing / 2, "a group and the one before it must fit the winner ring");
-    CandSet q[kGroup + 2];
     RasterState st;
-    st.x = 0;
-    st.y = 0;
-    st.left_m = kEmpty;
-    st.up_m = kEmpty;
-    st.lane_zero = job.n < 0 ? lane : 0;  // n > 0 here
-    st.prev_w = kNoMat;
-    st.prev_reset = 0;
     st.st_ban = st.st_near = st.st_resets = 0;
-    {
-        CandSet h0, h1;
-        load_cands(h0, job, 0, cells, lane);
-        load_cands(h1, job, 1, cells, lane);
-        const uint32_t u0 = usage_s[h0.m0 >= 0 ? h0.m0 : 0];
-        st.ob0 = wave_ballot(h0.m0 >= 0 && u0 < (uint32_t)job.max_usage);
-        st.pre_u = usage_s[h1.m0 >= 0 ? h1.m0 : 0];
-        st.pre_up = kEmpty;  // cell 1 is in row 0 unless the grid is one column wide
-    }
-    // winners of the group starting at g0 (at most kGroup cells) to the grid
-    auto flush = [&](int32_t g0) {
-        if (lane < kGroup && g0 + lane < cells) {
-            const int32_t p = g0 + lane;
-            gstore(job.grid, p, win_s[p & (kWinRing - 1)]);
+    if (gw < 3) {
+        raster_narrow<kOutputs>(job, sq, usage_s, st, lane);
+    } else {
+        CommitPorts port;
+        port.mul = lane == 0 ? 4u : 0u;
+        port.usage = lane == 0 ? usage_s : reinterpret_cast<uint32_t*>(lane_scratch);
+        port.row = lane == 0 ? row_mat : lane_scratch;
+        port.ring = lane == 0 ? win_s : lane_scratch;
+        // Candidate records for kGroup cells (and the two after them) are loaded
+        // at the top of each group and consumed in the same loop iteration, so
+        // the compiler's wait counts stay exact (a load consumed after the
+        // loop's back edge would make it wait for every load in flight); one
+        // memory latency per group.  The previous group's winners are stored
+        // just before the loads.  24 cells per group: one load latency per 24
+        // cells for 256 VGPRs (a few spilled to AGPRs, one wave per SIMD); 12
+        // cells cost 6% more chain time.
+        constexpr int kGroup = 24;
+        static_assert(kGroup <= kWinRing / 2,
+                      "a group and the one before it must fit the winner ring");
+        CandSet q[kGroup + 2];
+        st.x = 0;
+        st.xn = 1;
+        st.x2 = 2;
+        st.left_m = kEmpty;
+        st.up_m = kEmpty;
+        st.lane_zero = job.n < 0 ? lane : 0;  // n > 0 here
+        st.prev_w = kNoMat;
+        st.prev_reset = 0;
+        {
+            CandSet h0, h1;
+            load_cands(h0, job, 0, cells, lane);
+            load_cands(h1, job, 1, cells, lane);
+            const uint32_t u0 = usage_s[h0.m0 >= 0 ? h0.m0 : 0];
+            st.ob0 = wave_ballot(h0.m0 >= 0 && u0 < (uint32_t)job.max_usage);
+            st.pre_u = usage_s[h1.m0 >= 0 ? h1.m0 : 0];
+            st.pre_up = kEmpty;  // cell 1 is in row 0
         }
-    };
-    int32_t c = 0;
-    for (; c + kGroup <= cells; c += kGroup) {
-        if (c > 0) flush(c - kGroup);
+        // winners of the group starting at g0 (at most kGroup cells) to the grid
+        auto flush = [&](int32_t g0) {
+            if (lane < kGroup && g0 + lane < cells) {
+                const int32_t p = g0 + lane;
+                gstore(job.grid, p, win_s[p & (kWinRing - 1)]);
+            }
+        };
+        int32_t c = 0;
+        for (; c + kGroup <= cells; c += kGroup) {
+            if (c > 0) flush(c - kGroup);
 #pragma unroll
-        for (int i = 0; i < kGroup + 2; ++i) load_cands(q[i], job, c + i, cells, lane);
-        if (sq) to_distance(q[0]);
+            for (int i = 0; i < kGroup + 2; ++i) load_cands(q[i], job, c + i, cells, lane);
+            if (sq) to_distance(q[0]);
 #pragma unroll
-        for (int i = 0; i < kGroup; ++i) {
-            raster_step<kOutputs>(q[i], q[i + 1], q[i + 2], c + i, st, job, usage_s, row_mat,
-                                  win_s, lane_scratch, n, lane);
-            // the step waited for the records after q[i + 1]'s, so these are in
-            if (sq && i + 1 < kGroup) to_distance(q[i + 1]);
-        }
-    }
-    int32_t last = c - kGroup;  // start of the last group run (negative: none)
-    if (c < cells) {
-        // the last, partial group: the same loads (one memory latency), with
-        // the steps past the grid skipped
-        if (c > 0) flush(c - kGroup);
-        const int32_t rem = cells - c;
-#pragma unroll
-        for (int i = 0; i < kGroup + 2; ++i) load_cands(q[i], job, c + i, cells, lane);
-        if (sq) to_distance(q[0]);
-#pragma unroll
-        for (int i = 0; i < kGroup; ++i) {
-            if (i < rem) {
+            for (int i = 0; i < kGroup; ++i) {
                 raster_step<kOutputs>(q[i], q[i + 1], q[i + 2], c + i, st, job, usage_s,
-                                      row_mat, win_s, lane_scratch, n, lane);
+                                      row_mat, port, n, lane);
+                // the step waited for the records after q[i + 1]'s, so these are in
                 if (sq && i + 1 < kGroup) to_distance(q[i + 1]);
             }
         }
-        last = c;
+        int32_t last = c - kGroup;  // start of the last group run (negative: none)
+        if (c < cells) {
+            // the last, partial group: the same loads (one memory latency), with
+            // the steps past the grid skipped
+            if (c > 0) flush(c - kGroup);
+            const int32_t rem = cells - c;
+#pragma unroll
+            for (int i = 0; i < kGroup + 2; ++i) load_cands(q[i], job, c + i, cells, lane);
+            if (sq) to_distance(q[0]);
+#pragma unroll
+            for (int i = 0; i < kGroup; ++i) {
+                if (i < rem) {
+                    raster_step<kOutputs>(q[i], q[i + 1], q[i + 2], c + i, st, job, usage_s,
+                                          row_mat, port, n, lane);
+                    if (sq && i + 1 < kGroup) to_distance(q[i + 1]);
+                }
+            }
+            last = c;
+        }
+        __syncthreads();
+        if (last >= 0) flush(last);
     }
-    __syncthreads();
-    if (last >= 0) flush(last);
     for (int32_t i = lane; i < n; i += 64) gstore(job.usage, i, usage_s[i]);
     if (lane == 0 && job.stats) {
         gstore(job.stats, 0, cells);

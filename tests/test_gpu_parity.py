@@ -173,6 +173,7 @@ def test_place_workloads(ctx, orc, synth, name, score_mode):
     (500, 20, 20, 3, 2.0),
     (300, 1, 40, 2, 4.0),   # one column: the up neighbour is the previous winner
     (300, 2, 30, 2, 4.0),   # two columns
+    (300, 3, 25, 2, 4.0),   # three columns: the narrowest grid with the read-ahead chain
 ])
 @pytest.mark.parametrize("score_mode", [0, 1])
 def test_place_edge_cases(ctx, orc, n, gw, gh, usage, weight, score_mode):
