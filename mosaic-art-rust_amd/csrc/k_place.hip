@@ -28,8 +28,8 @@
 //    LDS row (material + Lab of the last placement in that column), usage
 //    counts live in LDS, candidate records are loaded 24 cells at a time into
 //    registers, the min is a DPP reduction, and the winner's id / Lab / count
-//    are read from its lane.  Winners go to a 64-entry LDS ring that is
-//    stored to the grid, coalesced, once per group of 24 cells.
+//    are read from its lane.  Winners wait one per lane in a register that
+//    is stored to the grid, coalesced, once per group of 24 cells.
 //    Latency-bound: a few hundred cycles per cell.
 //  * place_chain_kernel -- any cell order over a pre-filled grid and usage
 //    (find_and_use_best_tile_with_position at arbitrary cells); grid and usage
@@ -192,12 +192,14 @@ __device__ __forceinline__ void to_distance(CandSet& s) {
     s.c1.w = sqrtf(s.c1.w);
 }
 
-constexpr int32_t kWinRing = 64;  // winners awaiting their grid store (power of two)
 constexpr int32_t kNoMat = -2;    // "no previous winner" (never a material or the -1 sentinel)
+// The count a -1 sentinel candidate reads (the guard word before the usage
+// counts): above every usage limit, and one more use does not wrap it.
+constexpr uint32_t kGuard = 0x7fffffffu;
 
 struct RasterState {
     int32_t x;         // current cell's column
-    int32_t xn, x2;    // columns of the next cell and of the one after it
+    float pen2;        // 2 * weight (the penalty bound), -inf for weight <= 0
     int32_t left_m;    // previous winner (kEmpty at the start of a row)
     int32_t up_m;      // material above the current cell (kEmpty in row 0)
     uint64_t ob0;      // rank 0-63 candidates of the current cell that are usable
@@ -207,7 +209,6 @@ struct RasterState {
     int32_t pre_up;
     int32_t lane_zero;  // 0 in every lane, unknown to the compiler (see raster_step)
     int32_t prev_w;    // winner of the previous cell (kNoMat before the first)
-    int32_t prev_reset;  // the previous cell reset every usage count
     int32_t st_ban, st_near, st_resets;
 };
 
@@ -219,8 +220,47 @@ struct CommitPorts {
     uint32_t mul;                  // 4 in lane 0, else 0
     uint32_t* usage;               // usage_s (lane 0) or the lane's scratch word
     int32_t* row;                  // row_mat (lane 0) or the lane's scratch word
-    int32_t* ring;                 // win_s (lane 0) or the lane's scratch word
 };
+
+// v with lane `slot` replaced by the uniform value x.  Written out because a
+// per-lane select (lane == slot ? x : v) has its lane masks hoisted out of
+// the unrolled group, one 64-bit mask per slot, more than the scalar
+// registers hold.
+template <int kSlot>
+__device__ __forceinline__ int32_t write_lane(int32_t v, int32_t x) {
+    // (x is uniform; the read-first-lane only pins it to a scalar register)
+    asm("v_writelane_b32 %0, %1, %2"
+        : "+v"(v)
+        : "s"(__builtin_amdgcn_readfirstlane(x)), "n"(kSlot));
+    return v;
+}
+
+// f(Slot<0>{}), ..., f(Slot<N - 1>{}) in order, each with its index as a
+// compile-time constant.
+template <int I>
+struct Slot {
+    static constexpr int value = I;
+};
+template <int I, int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (I < N) {
+        f(Slot<I>{});
+        static_for<I + 1, N>(f);
+    }
+}
+
+// Index of the lowest set bit of a uniform mask, -1 when it is empty: the
+// scalar find-first-one as is (the generic form adds a separate zero test).
+__device__ __forceinline__ int32_t first_lane(uint64_t m) {
+    // (m is uniform; the read-first-lanes only pin it to scalar registers)
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(m >> 32));
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(uint32_t)m);
+    const uint64_t ms = ((uint64_t)hi << 32) | lo;
+    int32_t r;
+    asm("s_ff1_i32_b64 %0, %1" : "=s"(r) : "s"(ms));
+    // an asm result counts as per-lane; re-reading it marks it uniform
+    return __builtin_amdgcn_readfirstlane(r);
+}
 
 template <typename T>
 __device__ __forceinline__ T* port_at(T* base, int32_t index, uint32_t mul) {
@@ -249,25 +289,28 @@ __device__ __forceinline__ T* port_at(T* base, int32_t index, uint32_t mul) {
 // this step (so the reads have a whole step to land); when they are used one
 // step later they are corrected in registers for the two winners committed
 // in between (+1 each, or a base of 0 after a usage reset).  Lane 0
-// commits: one use of the winner (LDS atomic), the column's latest material,
-// and the winner into an LDS ring written to the grid once per group of cells
-// (a vector store inside the group would make the next use of a group-loaded
-// record wait for it).
-template <bool kOutputs>
+// commits one use of the winner (LDS atomic) and the column's latest
+// material; the winner itself is written into lane kSlot of a register that
+// is stored to the grid once per group of cells (a vector store inside the
+// group would make the next use of a group-loaded record wait for it).
+template <bool kOutputs, int kSlot>
 __device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn,
-                                            const CandSet& snn, int32_t c, RasterState& st,
-                                            const PlaceJob& job, uint32_t* usage_s,
-                                            const int32_t* row_mat, const CommitPorts& port,
-                                            int32_t n, int lane) {
+                                            const CandSet& snn, int32_t c,
+                                            RasterState& st, const PlaceJob& job,
+                                            uint32_t* usage_s, const int32_t* row_mat,
+                                            const CommitPorts& port, int32_t& wins, int32_t n,
+                                            int lane) {
     const uint32_t kNone = 0xffffffffu;
     const int32_t gw = job.grid_w;
-    const int32_t x = st.x, xn = st.xn, x2 = st.x2;
-    const int32_t x3 = x2 + 1 == gw ? 0 : x2 + 1;
+    const int32_t x = st.x;
+    const bool wrap_n = x + 1 == gw;
+    const int32_t xn = wrap_n ? 0 : x + 1;
+    const int32_t x2 = xn + 1 == gw ? 0 : xn + 1;
     const uint32_t max_usage = (uint32_t)job.max_usage;
     // Reads for the cell after next (they see every commit up to the previous
-    // cell).  A -1 sentinel reads the word before the counts (the last column
-    // entry), which its masked lane ignores.
-    const uint32_t nnu0 = usage_s[snn.m0];
+    // cell).  A -1 sentinel reads the guard word before the counts, which is
+    // above every usage limit, so the lane is never usable.
+    uint32_t nnu0 = usage_s[snn.m0];
     // The up neighbour: row_mat starts all kEmpty, so row 0 needs no test.  It
     // is the same in every lane; a lane-dependent address keeps it in a
     // vector register, so the read is waited for only where the next step
@@ -281,42 +324,43 @@ __device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn,
     // cell starts a row:
     //   usable = valid & up-ban & base < max & (lanes != w | (row start & base + 1 < max))
     // Everything but the w compare is known before w.
-    const uint32_t base = (st.prev_reset ? 0u : st.pre_u) + (sn.m0 == st.prev_w ? 1u : 0u);
+    const uint32_t base = st.pre_u + (sn.m0 == st.prev_w ? 1u : 0u);
     const int32_t up_next = st.pre_up;
     const uint64_t lt0 = lanes_ult(base, max_usage);
-    // base + 1 < max for max >= 1; with max == 0, lt0 is empty and masks it
-    const uint64_t lt1 = lanes_ult(base, max_usage - 1u);
-    const uint64_t m_base = lanes_nonneg(sn.m0) & lanes_ne(sn.m0, up_next) & lt0;
-    const uint64_t keep_w = xn == 0 ? lt1 : 0ull;
+    // keep_w: base + 1 < max for max >= 1 (with max == 0, lt0 is empty and
+    // masks it) ...
+    // (sentinel lanes read the guard count, so lt0 excludes them)
+    const uint64_t m_base = lanes_ne(sn.m0, up_next) & lt0;
+    // ... and lt1 only where the next cell starts a row (base < 0 never holds)
+    const uint64_t keep_w = lanes_ult(base, wrap_n ? max_usage - 1u : 0u);
 
-    // speculative winner and the next cell's usable mask
+    // speculative winner and the next cell's usable mask; with no usable
+    // rank 0-63 candidate f is lane 63, and the check below fails
     const uint64_t ob0 = st.ob0;
-    const int32_t f = ob0 ? (int32_t)__ffsll((unsigned long long)ob0) - 1 : 0;
+    const int32_t fi = first_lane(ob0);  // -1 when ob0 is empty
+    const int32_t f = fi & 63;
     int32_t w = __builtin_amdgcn_readlane(s.m0, f);
     uint64_t nob = m_base & (~lanes_eq(sn.m0, w) | keep_w);
 
     // confirm it: every other usable rank 0-63 candidate and rank 64 (the
-    // nearest of ranks 64-127) lie beyond the bound
-    bool fast = ob0 != 0;
-    if (fast && job.weight > 0.0f) {
-        const float d_f = readlane_f(s.c0.w, f);
-        const float bound = d_f + 2.0f * job.weight;
-        // sentinel ranks carry an infinite distance
-        const uint64_t far_half = lanes_fle(s.c1.w, bound);
-        const uint64_t others = (ob0 & (ob0 - 1ull)) & lanes_fle(s.c0.w, bound);
-        fast = (far_half | others) == 0;
-    }
-    if (kOutputs && job.score) fast = false;
+    // nearest of ranks 64-127) lie beyond the bound.  With a weight <= 0 the
+    // bound is -inf and f wins whenever it exists; sentinel ranks carry an
+    // infinite distance.
+    const float bound = readlane_f(s.c0.w, f) + st.pen2;
+    const uint64_t viol = ((ob0 & (ob0 - 1ull)) & lanes_fle(s.c0.w, bound)) |
+                          lanes_fle(s.c1.w, bound);
+    uint32_t slow = (uint32_t)(viol >> 32) | (uint32_t)viol | ((uint32_t)fi >> 31);
+    if (kOutputs && job.score) slow = 1;
 
     int32_t rank = f, stage = 0;
     uint32_t win_score = 0;
-    if (!fast) {
+    if (__builtin_expect(slow != 0, 0)) {
         // every cell gets a tile, so a filled up neighbour means row > 0
         const int32_t up_m = st.up_m;
         const int32_t left_m = x > 0 ? st.left_m : kEmpty;
         const bool pl0 = s.m0 >= 0 && s.m0 != up_m && s.m0 != left_m;
         const bool ok0 = ((ob0 >> lane) & 1ull) != 0;
-        const uint32_t u1 = usage_s[s.m1];  // a -1 sentinel reads a word it ignores
+        const uint32_t u1 = usage_s[s.m1];  // a -1 sentinel reads the guard word
         const bool pl1 = s.m1 >= 0 && s.m1 != up_m && s.m1 != left_m;
         const bool ok1 = pl1 && u1 < max_usage;
         const uint64_t ob1 = wave_ballot(ok1);
@@ -367,6 +411,9 @@ __device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn,
             // fallback_tile_selection: reset every count, first placeable, else nearest_one
             for (int32_t i = lane; i < n; i += 64) usage_s[i] = 0u;
             ++st.st_resets;
+            // the counts read ahead for the cell after next predate the reset
+            // (sentinel lanes keep the guard)
+            nnu0 = nnu0 == kGuard ? kGuard : 0u;
             const uint64_t b0 = wave_ballot(pl0);
             const uint64_t b1 = wave_ballot(pl1);
             if (b0) {
@@ -389,6 +436,11 @@ __device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn,
         const uint32_t nu = stage == 0 ? base + h : h;  // a reset zeroed every count
         nob = wave_ballot(sn.m0 >= 0 && sn.m0 != up_next && (xn == 0 || sn.m0 != w) &&
                           nu < max_usage);
+        // Wait here for this path's neighbour-Lab gathers (vmcnt 0; expcnt and
+        // lgkmcnt untouched).  Left pending, they would make every later step
+        // wait for all memory reads in flight, since the fast path cannot tell
+        // them from the group's candidate loads.
+        __builtin_amdgcn_s_waitcnt(0x0f70);
     }
     // Lane 0 commits; every other lane writes its own scratch word instead,
     // so the commit is straight-line code with no branch on the lane (the
@@ -396,7 +448,7 @@ __device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn,
     // keeps the add a plain ds_add.
     atomicAdd(port_at(port.usage, w, port.mul), 1u);
     *port_at(port.row, x, port.mul) = w;
-    *port_at(port.ring, c & (kWinRing - 1), port.mul) = w;
+    wins = write_lane<kSlot>(wins, w);  // the winner waits in lane kSlot
     if (kOutputs) {
         const float dw = rank < 64 ? readlane_f(s.c0.w, rank & 63) : readlane_f(s.c1.w, rank & 63);
         if (lane == 0) {
@@ -406,17 +458,13 @@ __device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn,
         }
     }
     __builtin_amdgcn_wave_barrier();
-    // a reset here zeroed every count after the reads above
     st.pre_u = nnu0;
     st.pre_up = nnup;
-    st.prev_reset = stage != 0 ? 1 : 0;
     st.prev_w = w;
     st.up_m = up_next;
     st.left_m = w;
     st.ob0 = nob;
     st.x = xn;
-    st.xn = x2;
-    st.x2 = x3;
 }
 
 // Raster order in grids one or two columns wide: the up neighbour is the
@@ -424,9 +472,14 @@ __device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn,
 // neighbour the previous winner in the second column.  One cell at a time
 // with every usable candidate scored -- the same decisions as raster_step,
 // without its read-ahead of the up neighbour, which needs three columns.
+struct NarrowStats {
+    int32_t ban, near, resets;
+};
+
 template <bool kOutputs>
-__device__ void raster_narrow(const PlaceJob& job, bool sq, uint32_t* usage_s, RasterState& st,
-                              int lane) {
+__device__ __forceinline__ NarrowStats raster_narrow(const PlaceJob& job, bool sq,
+                                                     uint32_t* usage_s, int lane) {
+    NarrowStats st{0, 0, 0};
     const int32_t gw = job.grid_w, cells = gw * job.grid_h;
     const uint32_t max_usage = (uint32_t)job.max_usage;
     const float weight = job.weight;
@@ -469,21 +522,21 @@ __device__ void raster_narrow(const PlaceJob& job, bool sq, uint32_t* usage_s, R
         } else {
             // fallback_tile_selection: reset every count, first placeable, else nearest_one
             for (int32_t i = lane; i < job.n; i += 64) usage_s[i] = 0u;
-            ++st.st_resets;
+            ++st.resets;
             const uint64_t b0 = wave_ballot(pl0);
             const uint64_t b1 = wave_ballot(pl1);
             if (b0) {
                 rank = __ffsll((unsigned long long)b0) - 1;
                 stage = 1;
-                ++st.st_ban;
+                ++st.ban;
             } else if (b1) {
                 rank = 64 + __ffsll((unsigned long long)b1) - 1;
                 stage = 1;
-                ++st.st_ban;
+                ++st.ban;
             } else {
                 rank = 0;
                 stage = 2;
-                ++st.st_near;
+                ++st.near;
             }
         }
         const int32_t w = rank < 64 ? __builtin_amdgcn_readlane(s.m0, rank & 63)
@@ -504,6 +557,7 @@ __device__ void raster_narrow(const PlaceJob& job, bool sq, uint32_t* usage_s, R
         w2 = w1;
         w1 = w;
     }
+    return st;
 }
 
 // kScore: kScoreSq / kScoreDist fix the score for every job of the launch,
@@ -517,10 +571,11 @@ __global__ __launch_bounds__(64) void place_raster_kernel(PlaceJob single,
     const int lane = threadIdx.x;
     const int32_t gw = job.grid_w, n = job.n;
     const int32_t cells = gw * job.grid_h;
-    int32_t* win_s = reinterpret_cast<int32_t*>(smr);
-    int32_t* lane_scratch = win_s + kWinRing + lane;  // 64 words, one per lane
-    int32_t* row_mat = win_s + kWinRing + 64;
-    uint32_t* usage_s = reinterpret_cast<uint32_t*>(row_mat + gw);
+    int32_t* lane_scratch = reinterpret_cast<int32_t*>(smr) + lane;  // 64 words, one per lane
+    int32_t* row_mat = reinterpret_cast<int32_t*>(smr) + 64;
+    // one guard word between the column row and the counts: a -1 sentinel
+    // candidate reads it as its count
+    uint32_t* usage_s = reinterpret_cast<uint32_t*>(row_mat + gw + 1);
 
     if (n == 0) {  // src/main.rs:342-345: no tiles, every cell stays empty
         for (int32_t i = lane; i < cells; i += 64) gstore(job.grid, i, kEmpty);
@@ -532,18 +587,21 @@ __global__ __launch_bounds__(64) void place_raster_kernel(PlaceJob single,
     }
     for (int32_t i = lane; i < n; i += 64) usage_s[i] = job.zero_usage ? 0u : gload(job.usage, i);
     for (int32_t i = lane; i < gw; i += 64) row_mat[i] = kEmpty;
+    if (lane == 0) usage_s[-1] = kGuard;
     __syncthreads();
 
     RasterState st;
     st.st_ban = st.st_near = st.st_resets = 0;
     if (gw < 3) {
-        raster_narrow<kOutputs>(job, sq, usage_s, st, lane);
+        const NarrowStats ns = raster_narrow<kOutputs>(job, sq, usage_s, lane);
+        st.st_ban = ns.ban;
+        st.st_near = ns.near;
+        st.st_resets = ns.resets;
     } else {
         CommitPorts port;
         port.mul = lane == 0 ? 4u : 0u;
         port.usage = lane == 0 ? usage_s : reinterpret_cast<uint32_t*>(lane_scratch);
         port.row = lane == 0 ? row_mat : lane_scratch;
-        port.ring = lane == 0 ? win_s : lane_scratch;
         // Candidate records for kGroup cells (and the two after them) are loaded
         // at the top of each group and consumed in the same loop iteration, so
         // the compiler's wait counts stay exact (a load consumed after the
@@ -553,32 +611,28 @@ __global__ __launch_bounds__(64) void place_raster_kernel(PlaceJob single,
         // cells for 256 VGPRs (a few spilled to AGPRs, one wave per SIMD); 12
         // cells cost 6% more chain time.
         constexpr int kGroup = 24;
-        static_assert(kGroup <= kWinRing / 2,
-                      "a group and the one before it must fit the winner ring");
+        static_assert(kGroup <= 64, "a group's winners wait in one lane each");
         CandSet q[kGroup + 2];
         st.x = 0;
-        st.xn = 1;
-        st.x2 = 2;
+        st.pen2 = job.weight > 0.0f ? 2.0f * job.weight : -__builtin_inff();
         st.left_m = kEmpty;
         st.up_m = kEmpty;
         st.lane_zero = job.n < 0 ? lane : 0;  // n > 0 here
         st.prev_w = kNoMat;
-        st.prev_reset = 0;
         {
             CandSet h0, h1;
             load_cands(h0, job, 0, cells, lane);
             load_cands(h1, job, 1, cells, lane);
             const uint32_t u0 = usage_s[h0.m0 >= 0 ? h0.m0 : 0];
             st.ob0 = wave_ballot(h0.m0 >= 0 && u0 < (uint32_t)job.max_usage);
-            st.pre_u = usage_s[h1.m0 >= 0 ? h1.m0 : 0];
+            st.pre_u = usage_s[h1.m0];  // a sentinel reads the guard
             st.pre_up = kEmpty;  // cell 1 is in row 0
         }
-        // winners of the group starting at g0 (at most kGroup cells) to the grid
+        // Winners of the group starting at g0 (at most kGroup cells) to the
+        // grid: cell g0 + i's winner waits in lane i of `wins`.
+        int32_t wins = kEmpty;
         auto flush = [&](int32_t g0) {
-            if (lane < kGroup && g0 + lane < cells) {
-                const int32_t p = g0 + lane;
-                gstore(job.grid, p, win_s[p & (kWinRing - 1)]);
-            }
+            if (lane < kGroup && g0 + lane < cells) gstore(job.grid, g0 + lane, wins);
         };
         int32_t c = 0;
         for (; c + kGroup <= cells; c += kGroup) {
@@ -586,13 +640,13 @@ __global__ __launch_bounds__(64) void place_raster_kernel(PlaceJob single,
 #pragma unroll
             for (int i = 0; i < kGroup + 2; ++i) load_cands(q[i], job, c + i, cells, lane);
             if (sq) to_distance(q[0]);
-#pragma unroll
-            for (int i = 0; i < kGroup; ++i) {
-                raster_step<kOutputs>(q[i], q[i + 1], q[i + 2], c + i, st, job, usage_s,
-                                      row_mat, port, n, lane);
+            static_for<0, kGroup>([&](auto slot) __attribute__((always_inline)) {
+                constexpr int i = decltype(slot)::value;
+                raster_step<kOutputs, i>(q[i], q[i + 1], q[i + 2], c + i, st, job, usage_s,
+                                         row_mat, port, wins, n, lane);
                 // the step waited for the records after q[i + 1]'s, so these are in
                 if (sq && i + 1 < kGroup) to_distance(q[i + 1]);
-            }
+            });
         }
         int32_t last = c - kGroup;  // start of the last group run (negative: none)
         if (c < cells) {
@@ -603,17 +657,16 @@ __global__ __launch_bounds__(64) void place_raster_kernel(PlaceJob single,
 #pragma unroll
             for (int i = 0; i < kGroup + 2; ++i) load_cands(q[i], job, c + i, cells, lane);
             if (sq) to_distance(q[0]);
-#pragma unroll
-            for (int i = 0; i < kGroup; ++i) {
+            static_for<0, kGroup>([&](auto slot) __attribute__((always_inline)) {
+                constexpr int i = decltype(slot)::value;
                 if (i < rem) {
-                    raster_step<kOutputs>(q[i], q[i + 1], q[i + 2], c + i, st, job, usage_s,
-                                          row_mat, port, n, lane);
+                    raster_step<kOutputs, i>(q[i], q[i + 1], q[i + 2], c + i, st, job,
+                                             usage_s, row_mat, port, wins, n, lane);
                     if (sq && i + 1 < kGroup) to_distance(q[i + 1]);
                 }
-            }
+            });
             last = c;
         }
-        __syncthreads();
         if (last >= 0) flush(last);
     }
     for (int32_t i = lane; i < n; i += 64) gstore(job.usage, i, usage_s[i]);
@@ -632,7 +685,7 @@ size_t place_lds_bytes(int32_t cells, int32_t n) {
 }
 
 size_t place_raster_lds_bytes(int32_t grid_w, int32_t n) {
-    return sizeof(int32_t) * ((size_t)kWinRing + 64 + (size_t)grid_w) +
+    return sizeof(int32_t) * (64 + (size_t)grid_w + 1) +
            sizeof(uint32_t) * (size_t)(n > 0 ? n : 1);
 }
 
