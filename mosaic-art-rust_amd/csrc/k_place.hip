@@ -337,24 +337,24 @@ __device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn,
     // speculative winner and the next cell's usable mask; with no usable
     // rank 0-63 candidate f is lane 63, and the check below fails
     const uint64_t ob0 = st.ob0;
-    const int32_t fi = first_lane(ob0);  // -1 when ob0 is empty
-    const int32_t f = fi & 63;
+    const int32_t f = first_lane(ob0) & 63;
     int32_t w = __builtin_amdgcn_readlane(s.m0, f);
     uint64_t nob = m_base & (~lanes_eq(sn.m0, w) | keep_w);
 
     // confirm it: every other usable rank 0-63 candidate and rank 64 (the
     // nearest of ranks 64-127) lie beyond the bound.  With a weight <= 0 the
     // bound is -inf and f wins whenever it exists; sentinel ranks carry an
-    // infinite distance.
-    const float bound = readlane_f(s.c0.w, f) + st.pen2;
-    const uint64_t viol = ((ob0 & (ob0 - 1ull)) & lanes_fle(s.c0.w, bound)) |
-                          lanes_fle(s.c1.w, bound);
-    uint32_t slow = (uint32_t)(viol >> 32) | (uint32_t)viol | ((uint32_t)fi >> 31);
-    if (kOutputs && job.score) slow = 1;
+    // infinite distance.  With no usable rank 0-63 candidate the bound is
+    // +inf, which every rank 64-127 lane meets, so the check fails.
+    const uint64_t rest = ob0 & (ob0 - 1ull);  // usable ranks after f
+    const float pen = ob0 ? st.pen2 : __builtin_inff();
+    const float bound = readlane_f(s.c0.w, f) + pen;
+    uint64_t viol = (rest & lanes_fle(s.c0.w, bound)) | lanes_fle(s.c1.w, bound);
+    if (kOutputs && job.score) viol = 1;
 
     int32_t rank = f, stage = 0;
     uint32_t win_score = 0;
-    if (__builtin_expect(slow != 0, 0)) {
+    if (__builtin_expect(viol != 0, 0)) {
         // every cell gets a tile, so a filled up neighbour means row > 0
         const int32_t up_m = st.up_m;
         const int32_t left_m = x > 0 ? st.left_m : kEmpty;
