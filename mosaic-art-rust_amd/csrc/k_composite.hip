@@ -6,8 +6,8 @@
 //
 // The resize depends only on (material, cell size), so it runs once per
 // material per job instead of once per cell: a horizontal Lanczos3 pass to a
-// u8 intermediate, then a vertical pass, both integer fixed-point with taps
-// the host computes.  Compositing is one coalesced per-pixel pass: output
+// u8 intermediate, then a vertical pass, both integer fixed-point with 16-bit
+// taps and a per-axis precision that the host computes.  Compositing is one coalesced per-pixel pass: output
 // pixel -> cell -> material -> resized pixel -> per-cell adjustment -> u8.
 // HBM-bound: ~9 B per output pixel (3 B target read for the cell average, 3 B
 // tile read, 3 B written).
@@ -16,10 +16,9 @@
 
 namespace mh {
 
-constexpr int kPrec = 22;
-
-__device__ __forceinline__ uint32_t clip8(int32_t acc) {
-    const int32_t v = acc >> kPrec;
+// A pass's sum has `prec` fraction bits: keep the integer part, clamped.
+__device__ __forceinline__ uint32_t clip8(int32_t acc, int32_t prec) {
+    const int32_t v = acc >> prec;
     return (uint32_t)(v < 0 ? 0 : (v > 255 ? 255 : v));
 }
 
@@ -27,7 +26,7 @@ __device__ __forceinline__ uint32_t clip8(int32_t acc) {
 __global__ void resize_h_kernel(const uint8_t* __restrict__ src, int32_t n, int32_t sw, int32_t sh,
                                 int32_t dw, const int32_t* __restrict__ lo,
                                 const int32_t* __restrict__ cnt, const int32_t* __restrict__ kk,
-                                int32_t ks, uint8_t* __restrict__ tmp) {
+                                int32_t ks, int32_t prec, uint8_t* __restrict__ tmp) {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t total = (int64_t)n * sh * dw;
     if (idx >= total) return;
@@ -35,7 +34,7 @@ __global__ void resize_h_kernel(const uint8_t* __restrict__ src, int32_t n, int3
     const int64_t row = idx / dw;  // m*sh + y
     const uint8_t* s = src + (row * sw + lo[o]) * 3;
     const int32_t* k = kk + (int64_t)o * ks;
-    int32_t a0 = 1 << (kPrec - 1), a1 = a0, a2 = a0;
+    int32_t a0 = 1 << (prec - 1), a1 = a0, a2 = a0;
     const int32_t c = cnt[o];
     for (int32_t t = 0; t < c; ++t) {
         const int32_t w = k[t];
@@ -44,9 +43,9 @@ __global__ void resize_h_kernel(const uint8_t* __restrict__ src, int32_t n, int3
         a2 += (int32_t)s[3 * t + 2] * w;
     }
     uint8_t* d = tmp + idx * 3;
-    d[0] = (uint8_t)clip8(a0);
-    d[1] = (uint8_t)clip8(a1);
-    d[2] = (uint8_t)clip8(a2);
+    d[0] = (uint8_t)clip8(a0, prec);
+    d[1] = (uint8_t)clip8(a1, prec);
+    d[2] = (uint8_t)clip8(a2, prec);
 }
 
 // Vertical pass: tmp [n][sh][dw][3] -> dst [slot(m)][dh][dw][3], slot(m) =
@@ -54,7 +53,7 @@ __global__ void resize_h_kernel(const uint8_t* __restrict__ src, int32_t n, int3
 __global__ void resize_v_kernel(const uint8_t* __restrict__ tmp, int32_t n, int32_t sh, int32_t dw,
                                 int32_t dh, const int32_t* __restrict__ lo,
                                 const int32_t* __restrict__ cnt, const int32_t* __restrict__ kk,
-                                int32_t ks, const int32_t* __restrict__ dst_index,
+                                int32_t ks, int32_t prec, const int32_t* __restrict__ dst_index,
                                 uint8_t* __restrict__ dst) {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t total = (int64_t)n * dh * dw;
@@ -66,7 +65,7 @@ __global__ void resize_v_kernel(const uint8_t* __restrict__ tmp, int32_t n, int3
     const uint8_t* s = tmp + ((m * sh + lo[o]) * dw + x) * 3;
     const int32_t* k = kk + (int64_t)o * ks;
     const int64_t step = (int64_t)dw * 3;
-    int32_t a0 = 1 << (kPrec - 1), a1 = a0, a2 = a0;
+    int32_t a0 = 1 << (prec - 1), a1 = a0, a2 = a0;
     const int32_t c = cnt[o];
     for (int32_t t = 0; t < c; ++t) {
         const int32_t w = k[t];
@@ -76,9 +75,9 @@ __global__ void resize_v_kernel(const uint8_t* __restrict__ tmp, int32_t n, int3
     }
     const int64_t slot = dst_index ? (int64_t)dst_index[m] : m;
     uint8_t* d = dst + ((slot * dh + o) * dw + x) * 3;
-    d[0] = (uint8_t)clip8(a0);
-    d[1] = (uint8_t)clip8(a1);
-    d[2] = (uint8_t)clip8(a2);
+    d[0] = (uint8_t)clip8(a0, prec);
+    d[1] = (uint8_t)clip8(a1, prec);
+    d[2] = (uint8_t)clip8(a2, prec);
 }
 
 // Both passes for one material per 256-thread block, staged in LDS: the
@@ -90,8 +89,8 @@ constexpr int kFusedResizeMaxLds = 96 * 1024;
 __global__ __launch_bounds__(256) void resize_fused_kernel(
     const uint8_t* __restrict__ src, int32_t sw, int32_t sh, int32_t dw, int32_t dh,
     const int32_t* __restrict__ hlo, const int32_t* __restrict__ hcnt,
-    const int32_t* __restrict__ hk, int32_t hks, const int32_t* __restrict__ vlo,
-    const int32_t* __restrict__ vcnt, const int32_t* __restrict__ vk, int32_t vks,
+    const int32_t* __restrict__ hk, int32_t hks, int32_t hprec, const int32_t* __restrict__ vlo,
+    const int32_t* __restrict__ vcnt, const int32_t* __restrict__ vk, int32_t vks, int32_t vprec,
     const int32_t* __restrict__ dst_index, uint8_t* __restrict__ dst) {
     extern __shared__ __align__(16) uint8_t rsm[];
     const int32_t m = blockIdx.x;
@@ -113,7 +112,7 @@ __global__ __launch_bounds__(256) void resize_fused_kernel(
         const int32_t y = idx / dw, o = idx - y * dw;
         const uint8_t* row = s_src + (y * sw + hlo[o]) * 3;
         const int32_t* k = hk + (int64_t)o * hks;
-        int32_t a0 = 1 << (kPrec - 1), a1 = a0, a2 = a0;
+        int32_t a0 = 1 << (hprec - 1), a1 = a0, a2 = a0;
         const int32_t c = hcnt[o];
         for (int32_t t = 0; t < c; ++t) {
             const int32_t w = k[t];
@@ -122,9 +121,9 @@ __global__ __launch_bounds__(256) void resize_fused_kernel(
             a2 += (int32_t)row[3 * t + 2] * w;
         }
         uint8_t* d = s_tmp + idx * 3;
-        d[0] = (uint8_t)clip8(a0);
-        d[1] = (uint8_t)clip8(a1);
-        d[2] = (uint8_t)clip8(a2);
+        d[0] = (uint8_t)clip8(a0, hprec);
+        d[1] = (uint8_t)clip8(a1, hprec);
+        d[2] = (uint8_t)clip8(a2, hprec);
     }
     __syncthreads();
     uint8_t* out = dst + (int64_t)(dst_index ? dst_index[m] : m) * dh * dw * 3;
@@ -133,7 +132,7 @@ __global__ __launch_bounds__(256) void resize_fused_kernel(
         const uint8_t* col = s_tmp + (vlo[o] * dw + x) * 3;
         const int32_t* k = vk + (int64_t)o * vks;
         const int32_t step = dw * 3;
-        int32_t a0 = 1 << (kPrec - 1), a1 = a0, a2 = a0;
+        int32_t a0 = 1 << (vprec - 1), a1 = a0, a2 = a0;
         const int32_t c = vcnt[o];
         for (int32_t t = 0; t < c; ++t) {
             const int32_t w = k[t];
@@ -141,9 +140,9 @@ __global__ __launch_bounds__(256) void resize_fused_kernel(
             a1 += (int32_t)col[t * step + 1] * w;
             a2 += (int32_t)col[t * step + 2] * w;
         }
-        out[idx * 3] = (uint8_t)clip8(a0);
-        out[idx * 3 + 1] = (uint8_t)clip8(a1);
-        out[idx * 3 + 2] = (uint8_t)clip8(a2);
+        out[idx * 3] = (uint8_t)clip8(a0, vprec);
+        out[idx * 3 + 1] = (uint8_t)clip8(a1, vprec);
+        out[idx * 3 + 2] = (uint8_t)clip8(a2, vprec);
     }
 }
 
@@ -158,15 +157,16 @@ void launch_resize(const uint8_t* src, int32_t n, int32_t sw, int32_t sh, int32_
     const size_t lds = resize_fused_lds_bytes(sw, sh, dw);
     if (lds <= (size_t)kFusedResizeMaxLds) {
         hipLaunchKernelGGL(resize_fused_kernel, dim3(n), dim3(256), lds, stream, src, sw, sh, dw, dh,
-                           h.lo, h.cnt, h.k, h.ksize, v.lo, v.cnt, v.k, v.ksize, dst_index, dst);
+                           h.lo, h.cnt, h.k, h.ksize, h.prec, v.lo, v.cnt, v.k, v.ksize, v.prec,
+                           dst_index, dst);
         return;
     }
     const int64_t th = (int64_t)n * sh * dw;
     hipLaunchKernelGGL(resize_h_kernel, dim3((unsigned)((th + 255) / 256)), dim3(256), 0, stream, src,
-                       n, sw, sh, dw, h.lo, h.cnt, h.k, h.ksize, tmp);
+                       n, sw, sh, dw, h.lo, h.cnt, h.k, h.ksize, h.prec, tmp);
     const int64_t tv = (int64_t)n * dh * dw;
     hipLaunchKernelGGL(resize_v_kernel, dim3((unsigned)((tv + 255) / 256)), dim3(256), 0, stream, tmp,
-                       n, sh, dw, dh, v.lo, v.cnt, v.k, v.ksize, dst_index, dst);
+                       n, sh, dw, dh, v.lo, v.cnt, v.k, v.ksize, v.prec, dst_index, dst);
 }
 
 // Integer average RGB of cells (calculate_average_rgb): one 256-thread

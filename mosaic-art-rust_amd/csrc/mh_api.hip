@@ -77,9 +77,19 @@ bool finite3(const float* v, int64_t count) {
     return true;
 }
 
-// Lanczos3 fixed-point taps for one axis (same construction as the
-// kernels assume): support 3*max(scale,1), weights normalised in double and
-// rounded half away from zero to Q22.
+// Lanczos3 fixed-point taps for one axis: the convolution coefficients of
+// fast_image_resize 5.1.4's default Convolution(Lanczos3) for u8 pixels
+// (src/main.rs:710-738 calls it; the crate follows Pillow-SIMD):
+//  * scale = in/out, filter scale fs = max(scale, 1), radius r = 3 fs;
+//  * output o centres on c = (o + 0.5) scale and reads source pixels
+//    [max(floor(c - r), 0), min(ceil(c + r), in)), pixel x weighted
+//    lanczos3((x - (c - 0.5)) * (1 / fs)), the weights divided by their sum;
+//  * one precision p per axis: the largest p < 22 for which every weight
+//    times 2^(p + 1), rounded, would still be below 2^15 -- or the first p
+//    for which it no longer is; weights are rounded half away from zero to
+//    p fraction bits (16-bit integers).
+// The passes add (1 << (p - 1)) + sum(pixel * weight) in 32 bits and keep
+// (sum >> p) clamped to 0..255.
 double sinc(double x) {
     if (x == 0.0) return 1.0;
     x = x * M_PI;
@@ -98,32 +108,40 @@ int get_taps(mh_context* ctx, int32_t in_size, int32_t out_size, ResizeTaps* t) 
     if (it == ctx->taps.end()) {
         const double scale = (double)in_size / (double)out_size;
         const double fs = scale < 1.0 ? 1.0 : scale;
-        const double support = 3.0 * fs;
-        const int32_t ks = (int32_t)ceil(support) * 2 + 1;
+        const double radius = 3.0 * fs;
+        const double recip = 1.0 / fs;
+        const int32_t ks = (int32_t)ceil(radius) * 2 + 1;
         std::vector<int32_t> lo(out_size), cnt(out_size), k((size_t)out_size * ks, 0);
-        std::vector<double> w(ks);
+        std::vector<double> w((size_t)out_size * ks, 0.0);
+        double max_w = 0.0;
         for (int32_t o = 0; o < out_size; ++o) {
-            const double center = ((double)o + 0.5) * scale;
-            int32_t xmin = (int32_t)(center - support + 0.5);
-            if (xmin < 0) xmin = 0;
-            int32_t xmax = (int32_t)(center + support + 0.5);
-            if (xmax > in_size) xmax = in_size;
-            xmax -= xmin;
+            const double in_center = ((double)o + 0.5) * scale;
+            const double fmin = floor(in_center - radius), fmax = ceil(in_center + radius);
+            const int32_t xmin = fmin > 0.0 ? (int32_t)fmin : 0;
+            const int32_t xmax = fmax < (double)in_size ? (int32_t)fmax : in_size;
+            const double center = in_center - 0.5;
+            double* wo = &w[(size_t)o * ks];
             double ww = 0.0;
-            for (int32_t x = 0; x < xmax; ++x) {
-                w[x] = lanczos3(((double)(x + xmin) - center + 0.5) / fs);
-                ww += w[x];
+            for (int32_t x = xmin; x < xmax; ++x) {
+                wo[x - xmin] = lanczos3(((double)x - center) * recip);
+                ww += wo[x - xmin];
             }
-            for (int32_t x = 0; x < xmax; ++x) {
-                const double v = ww != 0.0 ? w[x] / ww : 0.0;
-                const double f = v * (double)(1 << 22);
-                k[(size_t)o * ks + x] = (int32_t)(f < 0.0 ? f - 0.5 : f + 0.5);
-            }
+            if (ww != 0.0)
+                for (int32_t t = 0; t < xmax - xmin; ++t) wo[t] /= ww;
+            for (int32_t t = 0; t < xmax - xmin; ++t) max_w = wo[t] > max_w ? wo[t] : max_w;
             lo[o] = xmin;
-            cnt[o] = xmax;
+            cnt[o] = xmax - xmin;
         }
+        int32_t prec = 0;
+        for (int32_t p = 0; p < 22; ++p) {
+            prec = p;
+            if (std::round(max_w * (double)(1 << (p + 1))) >= 32768.0) break;
+        }
+        for (size_t i = 0; i < k.size(); ++i)
+            k[i] = (int32_t)std::round(w[i] * (double)(1 << prec));  // half away from zero
         TapsBuf tb;
         tb.ksize = ks;
+        tb.prec = prec;
         int rc;
         if ((rc = ensure(ctx, tb.lo, sizeof(int32_t) * out_size))) return rc;
         if ((rc = ensure(ctx, tb.cnt, sizeof(int32_t) * out_size))) return rc;
@@ -137,6 +155,7 @@ int get_taps(mh_context* ctx, int32_t in_size, int32_t out_size, ResizeTaps* t) 
     t->cnt = (const int32_t*)it->second.cnt.p;
     t->k = (const int32_t*)it->second.k.p;
     t->ksize = it->second.ksize;
+    t->prec = it->second.prec;
     return MH_OK;
 }
 

@@ -57,6 +57,7 @@ struct TileSet {
 struct TapsBuf {
     DevBuf lo, cnt, k;
     int32_t ksize = 0;
+    int32_t prec = 0;
 };
 
 struct PhaseMark {

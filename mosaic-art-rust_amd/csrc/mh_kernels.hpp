@@ -56,12 +56,15 @@ struct AnnealJob {
     AnnealOut* result;
 };
 
-// Fixed-point Lanczos3 taps for one axis (device pointers).
+// Fixed-point Lanczos3 taps for one axis (device pointers): output o sums
+// cnt[o] source pixels from lo[o] with the 16-bit weights k[o * ksize + t],
+// which carry `prec` fraction bits.
 struct ResizeTaps {
     const int32_t* lo;
     const int32_t* cnt;
     const int32_t* k;
     int32_t ksize;
+    int32_t prec;
 };
 
 // Cells of one image: cell c's pixel (px, py) is at

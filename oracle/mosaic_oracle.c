@@ -702,12 +702,18 @@ void orc_apply_adjustment(const float adj[4], const uint8_t *in, int64_t npx, ui
     for (int64_t i = 0; i < npx; ++i) orc_adjust_pixel(adj, in + 3 * i, out + 3 * i);
 }
 
-/* Lanczos3 resampling with fixed-point taps (the algorithm family of
- * fast_image_resize's default Convolution(Lanczos3), src/main.rs:710-738):
- * support 3*max(scale,1), taps normalised in double then rounded to Q22,
- * horizontal pass to u8, then vertical pass to u8 (round, clamp). */
-#define ORC_PREC 22
-
+/* Lanczos3 resampling, fast_image_resize 5.1.4's default Convolution(Lanczos3)
+ * for u8 pixels (src/main.rs:710-738; the crate follows Pillow-SIMD):
+ *  - scale = in/out, fs = max(scale, 1), radius r = 3 fs;
+ *  - output o centres on c = (o + 0.5) scale and reads source pixels
+ *    [max(floor(c - r), 0), min(ceil(c + r), in)); pixel x has the weight
+ *    lanczos3((x - (c - 0.5)) * (1 / fs)), and the weights are divided by
+ *    their sum;
+ *  - one precision p per axis: stepping p = 0, 1, ..., 21, stop at the first p
+ *    for which round(max weight * 2^(p + 1)) >= 2^15; weights are rounded
+ *    half away from zero to p fraction bits (16-bit integers);
+ *  - a pass sums (1 << (p - 1)) + pixel * weight in 32 bits and keeps
+ *    (sum >> p) clamped to 0..255; horizontal pass to u8, then vertical. */
 static double sinc(double x) {
     if (x == 0.0) return 1.0;
     x = x * M_PI;
@@ -719,40 +725,50 @@ static double lanczos3(double x) {
     return 0.0;
 }
 
-static int32_t lz_taps(int32_t in_size, int32_t out_size, int32_t *lo, int32_t *cnt, int32_t **kk) {
+static int32_t lz_taps(int32_t in_size, int32_t out_size, int32_t *lo, int32_t *cnt, int32_t **kk,
+                       int32_t *prec_out) {
     double scale = (double)in_size / (double)out_size;
     double fs = scale < 1.0 ? 1.0 : scale;
-    double support = 3.0 * fs;
-    int32_t ksize = (int32_t)ceil(support) * 2 + 1;
+    double radius = 3.0 * fs;
+    double recip = 1.0 / fs;
+    int32_t ksize = (int32_t)ceil(radius) * 2 + 1;
     int32_t *k = (int32_t *)calloc((size_t)out_size * (size_t)ksize, sizeof(int32_t));
-    double *w = (double *)malloc(sizeof(double) * (size_t)ksize);
+    double *w = (double *)calloc((size_t)out_size * (size_t)ksize, sizeof(double));
+    double max_w = 0.0;
     for (int32_t o = 0; o < out_size; ++o) {
-        double center = ((double)o + 0.5) * scale;
-        int32_t xmin = (int32_t)(center - support + 0.5);
-        if (xmin < 0) xmin = 0;
-        int32_t xmax = (int32_t)(center + support + 0.5);
-        if (xmax > in_size) xmax = in_size;
-        xmax -= xmin;
+        double in_center = ((double)o + 0.5) * scale;
+        double fmin = floor(in_center - radius), fmax = ceil(in_center + radius);
+        int32_t xmin = fmin > 0.0 ? (int32_t)fmin : 0;
+        int32_t xmax = fmax < (double)in_size ? (int32_t)fmax : in_size;
+        double center = in_center - 0.5;
+        double *wo = w + (int64_t)o * ksize;
         double ww = 0.0;
-        for (int32_t x = 0; x < xmax; ++x) {
-            w[x] = lanczos3(((double)(x + xmin) - center + 0.5) / fs);
-            ww += w[x];
+        for (int32_t x = xmin; x < xmax; ++x) {
+            wo[x - xmin] = lanczos3(((double)x - center) * recip);
+            ww += wo[x - xmin];
         }
-        for (int32_t x = 0; x < xmax; ++x) {
-            double v = ww != 0.0 ? w[x] / ww : 0.0;
-            double f = v * (double)(1 << ORC_PREC);
-            k[(int64_t)o * ksize + x] = (int32_t)(f < 0.0 ? f - 0.5 : f + 0.5);
-        }
+        if (ww != 0.0)
+            for (int32_t t = 0; t < xmax - xmin; ++t) wo[t] /= ww;
+        for (int32_t t = 0; t < xmax - xmin; ++t)
+            if (wo[t] > max_w) max_w = wo[t];
         lo[o] = xmin;
-        cnt[o] = xmax;
+        cnt[o] = xmax - xmin;
     }
+    int32_t prec = 0;
+    for (int32_t p = 0; p < 22; ++p) {
+        prec = p;
+        if (round(max_w * (double)(1 << (p + 1))) >= 32768.0) break;
+    }
+    for (int64_t i = 0; i < (int64_t)out_size * ksize; ++i)
+        k[i] = (int32_t)round(w[i] * (double)(1 << prec)); /* half away from zero */
     free(w);
     *kk = k;
+    *prec_out = prec;
     return ksize;
 }
 
-static uint8_t clip8(int32_t acc) {
-    int32_t v = acc >> ORC_PREC;
+static uint8_t clip8(int32_t acc, int32_t prec) {
+    int32_t v = acc >> prec;
     return (uint8_t)(v < 0 ? 0 : (v > 255 ? 255 : v));
 }
 
@@ -763,25 +779,25 @@ void orc_resize_lanczos3(const uint8_t *src, int32_t sw, int32_t sh, uint8_t *ds
     int32_t *xc = (int32_t *)malloc(sizeof(int32_t) * (size_t)dw);
     int32_t *ylo = (int32_t *)malloc(sizeof(int32_t) * (size_t)dh);
     int32_t *yc = (int32_t *)malloc(sizeof(int32_t) * (size_t)dh);
-    int32_t *kx, *ky;
-    int32_t ksx = lz_taps(sw, dw, xlo, xc, &kx);
-    int32_t ksy = lz_taps(sh, dh, ylo, yc, &ky);
+    int32_t *kx, *ky, px, py;
+    int32_t ksx = lz_taps(sw, dw, xlo, xc, &kx, &px);
+    int32_t ksy = lz_taps(sh, dh, ylo, yc, &ky, &py);
     uint8_t *tmp = (uint8_t *)malloc((size_t)sh * (size_t)dw * 3);
     for (int32_t y = 0; y < sh; ++y)
         for (int32_t o = 0; o < dw; ++o)
             for (int c = 0; c < 3; ++c) {
-                int32_t acc = 1 << (ORC_PREC - 1);
+                int32_t acc = 1 << (px - 1);
                 for (int32_t t = 0; t < xc[o]; ++t)
                     acc += (int32_t)src[((int64_t)y * sw + xlo[o] + t) * 3 + c] * kx[(int64_t)o * ksx + t];
-                tmp[((int64_t)y * dw + o) * 3 + c] = clip8(acc);
+                tmp[((int64_t)y * dw + o) * 3 + c] = clip8(acc, px);
             }
     for (int32_t o = 0; o < dh; ++o)
         for (int32_t x = 0; x < dw; ++x)
             for (int c = 0; c < 3; ++c) {
-                int32_t acc = 1 << (ORC_PREC - 1);
+                int32_t acc = 1 << (py - 1);
                 for (int32_t t = 0; t < yc[o]; ++t)
                     acc += (int32_t)tmp[((int64_t)(ylo[o] + t) * dw + x) * 3 + c] * ky[(int64_t)o * ksy + t];
-                dst[((int64_t)o * dw + x) * 3 + c] = clip8(acc);
+                dst[((int64_t)o * dw + x) * 3 + c] = clip8(acc, py);
             }
     free(tmp);
     free(kx);

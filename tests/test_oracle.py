@@ -527,7 +527,22 @@ def test_resize_matches_float_lanczos3(orc, sw, sh, dw, dh):
     src = _rng(sw + dw).integers(0, 256, (sh, sw, 3)).astype(np.uint8)
     got = orc.resize(src, dw, dh).astype(np.float64)
     want = np.clip(_lanczos_f64(src, dw, dh), 0, 255)
-    assert np.max(np.abs(got - want)) <= 2.0  # two u8 roundings + Q22 taps
+    assert np.max(np.abs(got - want)) <= 2.0  # two u8 roundings + 16-bit taps
+
+
+@pytest.mark.parametrize("sw,sh,dw,dh", [(100, 100, 50, 50), (128, 128, 24, 18), (30, 20, 61, 43),
+                                         (7, 300, 3, 11)])
+def test_resize_keeps_every_solid_level(orc, sw, sh, dw, dh):
+    # the rounded taps of an output pixel sum to 2^p within a few units, and
+    # the 2^(p-1) offset absorbs that: a flat image stays flat at every level
+    # (the reference's own test checks red, src/main.rs:892-908)
+    lv = np.arange(256, dtype=np.uint8)
+    for c in range(3):
+        src = np.zeros((sh, sw, 3), np.uint8)
+        for v in lv[::5]:
+            src[..., c] = v
+            out = orc.resize(src, dw, dh)
+            assert np.all(out[..., c] == v) and np.all(np.delete(out, c, axis=2) == 0)
 
 
 def test_resize_identity_and_solid(orc):
