@@ -165,6 +165,8 @@ __device__ float total_cost_wave(const G* grid, int32_t gw, int32_t gh, const TS
 __host__ __device__ __forceinline__ int32_t stamp_words(int32_t cells) {
     return (cells + 8) & ~3;
 }
+// One word per lane for packing a round's accepted deltas.
+constexpr int32_t kPackWords = 64;
 
 template <typename G>
 __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
@@ -176,7 +178,8 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
     const int32_t cells = gw * gh;
     const int32_t stamps = stamp_words(cells);
     uint32_t* stamp = sm;
-    G* grid_s = reinterpret_cast<G*>(sm + stamps);
+    float* pack_s = reinterpret_cast<float*>(sm + stamps);
+    G* grid_s = reinterpret_cast<G*>(sm + stamps + kPackWords);
 
     for (int32_t i = lane; i < cells; i += 64) grid_s[i] = (G)gload(job.grid, i);
     __syncthreads();
@@ -277,27 +280,34 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
             if (job.trace_delta) gstore(job.trace_delta, it, skip ? __int_as_float(0x7fc00000) : delta);
             if (job.trace_accept) gstore(job.trace_accept, it, (uint8_t)(acc ? 1 : 0));
         }
-        // running cost in proposal order; every lane runs the same uniform
-        // loop, so cost/best/improved stay identical across the wave.  The
-        // loop body has no branch: a new best is a compare and two selects.
+        // Running cost in proposal order (f32, as the reference's
+        // `current_cost += delta`).  The committed accepted deltas are packed
+        // into lanes 0..n-1 of `packed` in proposal order (the other lanes hold
+        // +0.0), so the serial sum reads lanes by a counter instead of walking
+        // the mask.  Every lane runs the same uniform loop, so cost / best /
+        // improved stay identical across the wave.  Adding a +0.0 past the last
+        // delta changes nothing: the cost is never -0.0 and never below best.
+        pack_s[lane] = 0.0f;
+        if ((accmask >> lane) & 1ull) pack_s[__popcll(accmask & lanes_below)] = delta;
+        __syncthreads();  // other lanes' writes, not this lane's own stores, are read below
+        const float packed = pack_s[lane];
+        const int32_t n_acc = __popcll(accmask);
         if (job.greedy) {  // optimize_greedy counts every accepted swap
-            uint64_t m = accmask;
-            while (m) {
-                const int j = __ffsll((unsigned long long)m) - 1;
-                m &= m - 1;
-                cost = cost + __int_as_float(__builtin_amdgcn_readlane(__float_as_int(delta), j));
+            for (int32_t q = 0; q < n_acc; q += 8) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) cost = cost + readlane_f(packed, q + i);
             }
-            improved += __popcll(accmask);
+            improved += n_acc;
         } else {
             uint32_t better_n = 0;
-            uint64_t m = accmask;
-            while (m) {
-                const int j = __ffsll((unsigned long long)m) - 1;
-                m &= m - 1;
-                cost = cost + __int_as_float(__builtin_amdgcn_readlane(__float_as_int(delta), j));
-                const bool better = cost < best;
-                best = better ? cost : best;
-                better_n += better ? 1u : 0u;
+            for (int32_t q = 0; q < n_acc; q += 8) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    cost = cost + readlane_f(packed, q + i);
+                    const bool better = cost < best;
+                    best = better ? cost : best;
+                    better_n += better ? 1u : 0u;
+                }
             }
             improved += better_n;
         }
@@ -332,7 +342,7 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
 }
 
 size_t anneal_lds_bytes(int32_t cells, bool narrow) {
-    return (size_t)stamp_words(cells) * sizeof(uint32_t) +
+    return (size_t)(stamp_words(cells) + kPackWords) * sizeof(uint32_t) +
            (size_t)cells * (narrow ? sizeof(int16_t) : sizeof(int32_t));
 }
 
