@@ -27,6 +27,8 @@ int fail(int code, const char* fmt, ...) {
 }
 
 int set_device(mh_context* ctx) {
+    // a library or batch whose context was destroyed carries no context
+    if (!ctx) return fail(MH_ERR_INVALID_ARGUMENT, "the handle's context was destroyed");
     HIPCHK(hipSetDevice(ctx->device));
     return MH_OK;
 }
@@ -456,6 +458,33 @@ void to_result(const AnnealOut& o, mh_anneal_result* r) {
     r->rounds = o.rounds;
 }
 
+void library_release(mh_library* lib) {
+    mh_context* ctx = lib->ctx;
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    if (lib->d_lab4) hipFree(lib->d_lab4);
+    if (lib->d_L) hipFree(lib->d_L);
+    if (lib->d_A) hipFree(lib->d_A);
+    if (lib->d_B) hipFree(lib->d_B);
+    if (lib->d_sim) hipFree(lib->d_sim);
+    if (lib->d_term) hipFree(lib->d_term);
+    if (lib->d_pixels) hipFree(lib->d_pixels);
+    lib->d_lab4 = nullptr;
+    lib->d_L = lib->d_A = lib->d_B = lib->d_sim = lib->d_term = nullptr;
+    lib->d_pixels = nullptr;
+    for (PixelGroup& g : lib->groups) {
+        if (g.d_index) hipFree(g.d_index);
+        g.d_index = nullptr;
+    }
+    for (auto& kv : lib->tiles) {
+        if (kv.second.tiles) hipFree(kv.second.tiles);
+        if (kv.second.avg) hipFree(kv.second.avg);
+    }
+    lib->tiles.clear();
+    lib->ctx = nullptr;
+}
+
 }  // namespace host
 }  // namespace mh
 
@@ -514,6 +543,19 @@ void mh_context_destroy(mh_context* ctx) {
     if (!ctx) return;
     hipSetDevice(ctx->device);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    {
+        // handles still open on this context lose their device memory now;
+        // destroying them later frees only their host memory
+        std::vector<mh_batch*> batches;
+        std::vector<mh_library*> libs;
+        {
+            std::lock_guard<std::mutex> g(ctx->dep_mu);
+            batches.swap(ctx->batches);
+            libs.swap(ctx->libs);
+        }
+        for (mh_batch* b : batches) batch_release(b);
+        for (mh_library* lib : libs) library_release(lib);
+    }
     for (auto& b : ctx->slots)
         if (b.p) hipFree(b.p);
     for (auto& kv : ctx->temps)
@@ -658,6 +700,10 @@ int mh_copy_to_host(mh_context* ctx, void* dst, const void* src, size_t bytes) {
 static int library_alloc(mh_context* ctx, int32_t n, mh_library** out) {
     mh_library* lib = new mh_library();
     lib->ctx = ctx;
+    {
+        std::lock_guard<std::mutex> g(ctx->dep_mu);
+        ctx->libs.push_back(lib);
+    }
     lib->n = n;
     lib->h_lab.assign((size_t)n * 3, 0.0f);
     const size_t m = n > 0 ? (size_t)n : 1;
@@ -901,22 +947,17 @@ int mh_library_from_images(mh_context* ctx, const uint8_t* rgb, const int64_t* o
 
 void mh_library_destroy(mh_library* lib) {
     if (!lib) return;
-    if (lib->ctx) {
-        hipSetDevice(lib->ctx->device);
-        hipStreamSynchronize(lib->ctx->stream);
-    }
-    if (lib->d_lab4) hipFree(lib->d_lab4);
-    if (lib->d_L) hipFree(lib->d_L);
-    if (lib->d_A) hipFree(lib->d_A);
-    if (lib->d_B) hipFree(lib->d_B);
-    if (lib->d_sim) hipFree(lib->d_sim);
-    if (lib->d_term) hipFree(lib->d_term);
-    if (lib->d_pixels) hipFree(lib->d_pixels);
-    for (PixelGroup& g : lib->groups)
-        if (g.d_index) hipFree(g.d_index);
-    for (auto& kv : lib->tiles) {
-        if (kv.second.tiles) hipFree(kv.second.tiles);
-        if (kv.second.avg) hipFree(kv.second.avg);
+    if (mh_context* ctx = lib->ctx) {
+        {
+            std::lock_guard<std::mutex> g(ctx->dep_mu);
+            for (size_t i = 0; i < ctx->libs.size(); ++i)
+                if (ctx->libs[i] == lib) {
+                    ctx->libs[i] = ctx->libs.back();
+                    ctx->libs.pop_back();
+                    break;
+                }
+        }
+        library_release(lib);
     }
     delete lib;
 }

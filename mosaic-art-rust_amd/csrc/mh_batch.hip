@@ -389,6 +389,33 @@ void enqueue_batch(mh_batch* b) {
 
 }  // namespace
 
+namespace mh {
+namespace host {
+
+void batch_release(mh_batch* b) {
+    mh_context* ctx = b->ctx;
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    if (b->side) hipStreamSynchronize(b->side);
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    if (b->d_jobs) hipFree(b->d_jobs);
+    if (b->d_plan) hipFree(b->d_plan);
+    if (b->fork) hipEventDestroy(b->fork);
+    if (b->join) hipEventDestroy(b->join);
+    if (b->side) hipStreamDestroy(b->side);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->d_jobs = b->d_plan = nullptr;
+    b->fork = b->join = nullptr;
+    b->side = nullptr;
+    b->ctx = nullptr;
+}
+
+}  // namespace host
+}  // namespace mh
+
 extern "C" {
 
 int64_t mh_job_footprint(mh_context* ctx, const mh_job* job) {
@@ -400,16 +427,18 @@ int64_t mh_job_footprint(mh_context* ctx, const mh_job* job) {
 
 void mh_batch_destroy(mh_batch* b) {
     if (!b) return;
-    hipSetDevice(b->ctx->device);
-    hipStreamSynchronize(b->ctx->stream);
-    if (b->side) hipStreamSynchronize(b->side);
-    if (b->exec) hipGraphExecDestroy(b->exec);
-    if (b->graph) hipGraphDestroy(b->graph);
-    if (b->d_jobs) hipFree(b->d_jobs);
-    if (b->d_plan) hipFree(b->d_plan);
-    if (b->fork) hipEventDestroy(b->fork);
-    if (b->join) hipEventDestroy(b->join);
-    if (b->side) hipStreamDestroy(b->side);
+    if (mh_context* ctx = b->ctx) {
+        {
+            std::lock_guard<std::mutex> g(ctx->dep_mu);
+            for (size_t i = 0; i < ctx->batches.size(); ++i)
+                if (ctx->batches[i] == b) {
+                    ctx->batches[i] = ctx->batches.back();
+                    ctx->batches.pop_back();
+                    break;
+                }
+        }
+        batch_release(b);
+    }
     delete b;
 }
 
@@ -420,6 +449,10 @@ int mh_batch_create(mh_context* ctx, const mh_job* jobs, int32_t n_jobs, mh_batc
     CHK(set_device(ctx));
     mh_batch* b = new mh_batch();
     b->ctx = ctx;
+    {
+        std::lock_guard<std::mutex> g(ctx->dep_mu);
+        ctx->batches.push_back(b);
+    }
     b->specs.resize((size_t)n_jobs);
     size_t total = 0;
     std::vector<size_t> offs((size_t)n_jobs);

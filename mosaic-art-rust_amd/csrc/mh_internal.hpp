@@ -70,6 +70,12 @@ struct PhaseMark {
 
 struct mh_context {
     int32_t device = 0;
+    // libraries and batches still open on this context: destroying the
+    // context releases their device memory and detaches them, so destroying
+    // them afterwards only frees host memory
+    std::vector<mh_library*> libs;
+    std::vector<mh_batch*> batches;
+    std::mutex dep_mu;
     // per-phase timing (HIP events on `stream`)
     bool timing = false;
     std::vector<hipEvent_t> event_pool;
@@ -137,6 +143,10 @@ void phase_end(mh_context* ctx, int phase);
 int ensure(mh_context* ctx, DevBuf& b, size_t bytes);
 bool finite3(const float* v, int64_t count);
 int get_taps(mh_context* ctx, int32_t in_size, int32_t out_size, ResizeTaps* t);
+// Device memory of a library / batch freed, the handle detached from its
+// context (the context is being destroyed).
+void library_release(mh_library* lib);
+void batch_release(mh_batch* b);
 int get_temps(mh_context* ctx, float t0, float decay, int64_t len, const float** out);
 void fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
 // Term matrix 1/(sim+1) of the library (nullptr above kTermMatrixMaxN

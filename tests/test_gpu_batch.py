@@ -193,3 +193,25 @@ def test_server_rejects_bad_jobs(ctx, mh, synth):
     srv.close()
     lab_only.close()
     s.close()
+
+
+def test_handles_outlive_their_context(mh, synth):
+    # A context of its own, destroyed while a library and a batch made on it
+    # are still open: their device memory goes with it, later calls on them
+    # fail cleanly, and destroying them afterwards is safe.
+    c = mh.Context(0)
+    s = Setup(c, mh, synth, n_jobs=2, seed=7)
+    b = c.batch([s.job(k, own_outputs=False) for k in range(len(s.specs))])
+    b.run()
+    b.results()
+    for p in s.d_can:
+        c.device_free(p)
+    s.d_can = []
+    c.close()
+    with pytest.raises(mh.MhError):
+        b.run()
+    with pytest.raises(mh.MhError):
+        s.libs[0].similarities()
+    b.close()
+    for lib in s.libs:
+        lib.close()
