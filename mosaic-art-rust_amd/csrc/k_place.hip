@@ -560,6 +560,86 @@ __device__ __forceinline__ NarrowStats raster_narrow(const PlaceJob& job, bool s
     return st;
 }
 
+// The look-ahead chain over a grid at least three columns wide (kSq: the
+// GUI score, whose distances are square-rooted as the records are used).
+template <bool kOutputs, bool kSq>
+__device__ __forceinline__ void raster_wide(const PlaceJob& job, uint32_t* usage_s,
+                                            int32_t* row_mat, int32_t* lane_scratch,
+                                            RasterState& st, int lane) {
+    const int32_t gw = job.grid_w, n = job.n;
+    const int32_t cells = gw * job.grid_h;
+    CommitPorts port;
+    port.mul = lane == 0 ? 4u : 0u;
+    port.usage = lane == 0 ? usage_s : reinterpret_cast<uint32_t*>(lane_scratch);
+    port.row = lane == 0 ? row_mat : lane_scratch;
+    // Candidate records for kGroup cells (and the two after them) are loaded
+    // at the top of each group and consumed in the same loop iteration, so
+    // the compiler's wait counts stay exact (a load consumed after the
+    // loop's back edge would make it wait for every load in flight); one
+    // memory latency per group.  The previous group's winners are stored
+    // just before the loads.  24 cells per group: one load latency per 24
+    // cells for 256 VGPRs (a few spilled to AGPRs, one wave per SIMD); 12
+    // cells cost 6% more chain time.
+    constexpr int kGroup = 24;
+    static_assert(kGroup <= 64, "a group's winners wait in one lane each");
+    CandSet q[kGroup + 2];
+    st.x = 0;
+    st.pen2 = job.weight > 0.0f ? 2.0f * job.weight : -__builtin_inff();
+    st.left_m = kEmpty;
+    st.up_m = kEmpty;
+    st.lane_zero = job.n < 0 ? lane : 0;  // n > 0 here
+    st.prev_w = kNoMat;
+    {
+        CandSet h0, h1;
+        load_cands(h0, job, 0, cells, lane);
+        load_cands(h1, job, 1, cells, lane);
+        const uint32_t u0 = usage_s[h0.m0 >= 0 ? h0.m0 : 0];
+        st.ob0 = wave_ballot(h0.m0 >= 0 && u0 < (uint32_t)job.max_usage);
+        st.pre_u = usage_s[h1.m0];  // a sentinel reads the guard
+        st.pre_up = kEmpty;  // cell 1 is in row 0
+    }
+    // Winners of the group starting at g0 (at most kGroup cells) to the
+    // grid: cell g0 + i's winner waits in lane i of `wins`.
+    int32_t wins = kEmpty;
+    auto flush = [&](int32_t g0) {
+        if (lane < kGroup && g0 + lane < cells) gstore(job.grid, g0 + lane, wins);
+    };
+    int32_t c = 0;
+    for (; c + kGroup <= cells; c += kGroup) {
+        if (c > 0) flush(c - kGroup);
+#pragma unroll
+        for (int i = 0; i < kGroup + 2; ++i) load_cands(q[i], job, c + i, cells, lane);
+        if (kSq) to_distance(q[0]);
+        static_for<0, kGroup>([&](auto slot) __attribute__((always_inline)) {
+            constexpr int i = decltype(slot)::value;
+            raster_step<kOutputs, i>(q[i], q[i + 1], q[i + 2], c + i, st, job, usage_s,
+                                     row_mat, port, wins, n, lane);
+            // the step waited for the records after q[i + 1]'s, so these are in
+            if (kSq && i + 1 < kGroup) to_distance(q[i + 1]);
+        });
+    }
+    int32_t last = c - kGroup;  // start of the last group run (negative: none)
+    if (c < cells) {
+        // the last, partial group: the same loads (one memory latency), with
+        // the steps past the grid skipped
+        if (c > 0) flush(c - kGroup);
+        const int32_t rem = cells - c;
+#pragma unroll
+        for (int i = 0; i < kGroup + 2; ++i) load_cands(q[i], job, c + i, cells, lane);
+        if (kSq) to_distance(q[0]);
+        static_for<0, kGroup>([&](auto slot) __attribute__((always_inline)) {
+            constexpr int i = decltype(slot)::value;
+            if (i < rem) {
+                raster_step<kOutputs, i>(q[i], q[i + 1], q[i + 2], c + i, st, job,
+                                         usage_s, row_mat, port, wins, n, lane);
+                if (kSq && i + 1 < kGroup) to_distance(q[i + 1]);
+            }
+        });
+        last = c;
+    }
+    if (last >= 0) flush(last);
+}
+
 // kScore: kScoreSq / kScoreDist fix the score for every job of the launch,
 // kScoreMixed reads each job's score_sqrt.
 template <bool kOutputs, int kScore>
@@ -598,76 +678,13 @@ __global__ __launch_bounds__(64) void place_raster_kernel(PlaceJob single,
         st.st_near = ns.near;
         st.st_resets = ns.resets;
     } else {
-        CommitPorts port;
-        port.mul = lane == 0 ? 4u : 0u;
-        port.usage = lane == 0 ? usage_s : reinterpret_cast<uint32_t*>(lane_scratch);
-        port.row = lane == 0 ? row_mat : lane_scratch;
-        // Candidate records for kGroup cells (and the two after them) are loaded
-        // at the top of each group and consumed in the same loop iteration, so
-        // the compiler's wait counts stay exact (a load consumed after the
-        // loop's back edge would make it wait for every load in flight); one
-        // memory latency per group.  The previous group's winners are stored
-        // just before the loads.  24 cells per group: one load latency per 24
-        // cells for 256 VGPRs (a few spilled to AGPRs, one wave per SIMD); 12
-        // cells cost 6% more chain time.
-        constexpr int kGroup = 24;
-        static_assert(kGroup <= 64, "a group's winners wait in one lane each");
-        CandSet q[kGroup + 2];
-        st.x = 0;
-        st.pen2 = job.weight > 0.0f ? 2.0f * job.weight : -__builtin_inff();
-        st.left_m = kEmpty;
-        st.up_m = kEmpty;
-        st.lane_zero = job.n < 0 ? lane : 0;  // n > 0 here
-        st.prev_w = kNoMat;
-        {
-            CandSet h0, h1;
-            load_cands(h0, job, 0, cells, lane);
-            load_cands(h1, job, 1, cells, lane);
-            const uint32_t u0 = usage_s[h0.m0 >= 0 ? h0.m0 : 0];
-            st.ob0 = wave_ballot(h0.m0 >= 0 && u0 < (uint32_t)job.max_usage);
-            st.pre_u = usage_s[h1.m0];  // a sentinel reads the guard
-            st.pre_up = kEmpty;  // cell 1 is in row 0
-        }
-        // Winners of the group starting at g0 (at most kGroup cells) to the
-        // grid: cell g0 + i's winner waits in lane i of `wins`.
-        int32_t wins = kEmpty;
-        auto flush = [&](int32_t g0) {
-            if (lane < kGroup && g0 + lane < cells) gstore(job.grid, g0 + lane, wins);
-        };
-        int32_t c = 0;
-        for (; c + kGroup <= cells; c += kGroup) {
-            if (c > 0) flush(c - kGroup);
-#pragma unroll
-            for (int i = 0; i < kGroup + 2; ++i) load_cands(q[i], job, c + i, cells, lane);
-            if (sq) to_distance(q[0]);
-            static_for<0, kGroup>([&](auto slot) __attribute__((always_inline)) {
-                constexpr int i = decltype(slot)::value;
-                raster_step<kOutputs, i>(q[i], q[i + 1], q[i + 2], c + i, st, job, usage_s,
-                                         row_mat, port, wins, n, lane);
-                // the step waited for the records after q[i + 1]'s, so these are in
-                if (sq && i + 1 < kGroup) to_distance(q[i + 1]);
-            });
-        }
-        int32_t last = c - kGroup;  // start of the last group run (negative: none)
-        if (c < cells) {
-            // the last, partial group: the same loads (one memory latency), with
-            // the steps past the grid skipped
-            if (c > 0) flush(c - kGroup);
-            const int32_t rem = cells - c;
-#pragma unroll
-            for (int i = 0; i < kGroup + 2; ++i) load_cands(q[i], job, c + i, cells, lane);
-            if (sq) to_distance(q[0]);
-            static_for<0, kGroup>([&](auto slot) __attribute__((always_inline)) {
-                constexpr int i = decltype(slot)::value;
-                if (i < rem) {
-                    raster_step<kOutputs, i>(q[i], q[i + 1], q[i + 2], c + i, st, job,
-                                             usage_s, row_mat, port, wins, n, lane);
-                    if (sq && i + 1 < kGroup) to_distance(q[i + 1]);
-                }
-            });
-            last = c;
-        }
-        if (last >= 0) flush(last);
+        // (kScoreMixed picks a compiled chain per job: a run-time test per
+        // step costs the chain scalar registers it does not have)
+        if (kScore == kScoreMixed && job.score_sqrt)
+            raster_wide<kOutputs, true>(job, usage_s, row_mat, lane_scratch, st, lane);
+        else
+            raster_wide<kOutputs, kScore == kScoreDist>(job, usage_s, row_mat, lane_scratch, st,
+                                                        lane);
     }
     for (int32_t i = lane; i < n; i += 64) gstore(job.usage, i, usage_s[i]);
     if (lane == 0 && job.stats) {
