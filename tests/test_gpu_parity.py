@@ -52,6 +52,18 @@ def test_every_rgb_value_converts_like_oracle(ctx, orc):
     assert_bits_equal(got, want)
 
 
+def test_every_colour_converts_like_oracle(ctx, orc):
+    # all 2^24 sRGB colours, once each, as the 1x1 cells of a 4096 x 4096
+    # image: the per-pixel conversion is bit-exact everywhere it can be fed
+    v = np.arange(1 << 24, dtype=np.uint32)
+    img = np.stack([(v >> 16) & 255, (v >> 8) & 255, v & 255], -1).astype(np.uint8)
+    img = img.reshape(4096, 4096, 3)
+    got = ctx.cell_lab_means(img, 4096, 4096)
+    want = orc.cell_lab_means(img, 4096, 4096)
+    bad = np.nonzero(np.any(got.view(np.uint32) != want.view(np.uint32), axis=1))[0]
+    assert bad.size == 0, f"{bad.size} colours differ, first {bad[:8]}"
+
+
 @pytest.mark.parametrize("w,h,gw,gh", [(640, 480, 40, 30), (101, 77, 7, 5), (64, 64, 1, 1),
                                        (1000, 37, 250, 37)])
 def test_cell_lab_means(ctx, orc, w, h, gw, gh):
