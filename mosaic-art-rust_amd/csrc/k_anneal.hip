@@ -167,6 +167,11 @@ __host__ __device__ __forceinline__ int32_t stamp_words(int32_t cells) {
 }
 // One word per lane for packing a round's accepted deltas.
 constexpr int32_t kPackWords = 64;
+// Proposal draws, made ahead in blocks of 128 iterations (two per lane) into
+// a ring of 256 slots of three words: x1 | y1 << 16, x2 | y2 << 16, u (the
+// grid lives in LDS, so both sides are far below 2^16).
+constexpr int32_t kRing = 256;
+constexpr int32_t kRingWords = 3 * kRing;
 
 template <typename G>
 __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
@@ -179,7 +184,10 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
     const int32_t stamps = stamp_words(cells);
     uint32_t* stamp = sm;
     float* pack_s = reinterpret_cast<float*>(sm + stamps);
-    G* grid_s = reinterpret_cast<G*>(sm + stamps + kPackWords);
+    uint32_t* ring_xy1 = sm + stamps + kPackWords;
+    uint32_t* ring_xy2 = ring_xy1 + kRing;
+    float* ring_u = reinterpret_cast<float*>(ring_xy2 + kRing);
+    G* grid_s = reinterpret_cast<G*>(sm + stamps + kPackWords + kRingWords);
 
     for (int32_t i = lane; i < cells; i += 64) grid_s[i] = (G)gload(job.grid, i);
     __syncthreads();
@@ -202,16 +210,47 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
     const int64_t max_iter = job.max_iter;
     const uint64_t lanes_below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
 
+    // Proposal `it` is drawn once, in a block of 128 made by the whole wave
+    // before the round that first needs it; a round re-evaluating proposals
+    // after a conflict reads them again instead of drawing them again.  A
+    // block never overwrites a slot of the round's window [s, s + 64):
+    // filled <= s + 192 < s + 256.
+    int64_t filled = 0;
     while (s < max_iter) {
+        if (filled < s + 64) {
+            do {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int64_t j = filled + 64 * h + lane;
+                    const uint32_t slot = (uint32_t)j & (kRing - 1);
+                    uint32_t a = 0, b = 0;
+                    float uj = 0.0f;
+                    if (j < max_iter) {
+                        a = (uint32_t)draw_range(job.seed, (uint64_t)j, 0, gw) |
+                            ((uint32_t)draw_range(job.seed, (uint64_t)j, 1, gh) << 16);
+                        b = (uint32_t)draw_range(job.seed, (uint64_t)j, 2, gw) |
+                            ((uint32_t)draw_range(job.seed, (uint64_t)j, 3, gh) << 16);
+                        if (!job.greedy) uj = draw_unit(job.seed, (uint64_t)j);
+                    }
+                    ring_xy1[slot] = a;
+                    ring_xy2[slot] = b;
+                    ring_u[slot] = uj;
+                }
+                filled += 128;
+            } while (filled < s + 64);
+            __syncthreads();  // other lanes' draws are read below
+        }
         const int64_t it = s + lane;
         const bool active = it < max_iter;
+        const uint32_t slot = (uint32_t)it & (kRing - 1);
+        const uint32_t xy1 = ring_xy1[slot], xy2 = ring_xy2[slot];
         int32_t x1 = 0, y1 = 0, x2 = 0, y2 = 0;
         bool skip = true;
         if (active) {
-            x1 = draw_range(job.seed, (uint64_t)it, 0, gw);
-            y1 = draw_range(job.seed, (uint64_t)it, 1, gh);
-            x2 = draw_range(job.seed, (uint64_t)it, 2, gw);
-            y2 = draw_range(job.seed, (uint64_t)it, 3, gh);
+            x1 = (int32_t)(xy1 & 0xffffu);
+            y1 = (int32_t)(xy1 >> 16);
+            x2 = (int32_t)(xy2 & 0xffffu);
+            y2 = (int32_t)(xy2 >> 16);
             skip = (x1 == x2 && y1 == y2) || gget(grid_s, y1 * gw + x1) == kEmpty ||
                    gget(grid_s, y2 * gw + x2) == kEmpty;
         }
@@ -224,7 +263,7 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
             float temp = 1.0f, u = 0.0f;
             if (!job.greedy) {
                 temp = gload(job.temps, kbase + __popcll(nonskip & lanes_below));
-                u = draw_unit(job.seed, (uint64_t)it);
+                u = ring_u[slot];
             }
             delta = job.term ? swap_delta_dev(grid_s, gw, gh, TM, x1, y1, x2, y2)
                              : swap_delta_dev(grid_s, gw, gh, TL, x1, y1, x2, y2);
@@ -342,7 +381,7 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
 }
 
 size_t anneal_lds_bytes(int32_t cells, bool narrow) {
-    return (size_t)(stamp_words(cells) + kPackWords) * sizeof(uint32_t) +
+    return (size_t)(stamp_words(cells) + kPackWords + kRingWords) * sizeof(uint32_t) +
            (size_t)cells * (narrow ? sizeof(int16_t) : sizeof(int32_t));
 }
 
