@@ -285,9 +285,11 @@ __global__ __launch_bounds__(64) void topk_batch_kernel(const TopkSeg* __restric
 }
 
 // Query cells per wave: runs reuse the previous answer as a bound, but a
-// launch wants enough waves to fill the chip (~8 per CU).
+// launch wants enough waves to hide the scan's memory latency (~32 per CU):
+// one cell per wave up to 8,192 cells (4,800 cells: 26 us against 30 us with
+// runs of 2), then longer runs.
 int32_t topk_run_length(int64_t total_queries) {
-    int64_t r = total_queries / (256 * 8);
+    int64_t r = total_queries / (256 * 32);
     return (int32_t)(r < 1 ? 1 : (r > kMaxRun ? kMaxRun : r));
 }
 
