@@ -119,6 +119,27 @@ def test_nearest_n(ctx, orc, n, k):
     lib.close()
 
 
+@pytest.mark.parametrize("n,nq,walk", [(1500, 20000, True), (1500, 20000, False),
+                                       (300, 70000, True), (5000, 17000, True)])
+def test_nearest_n_runs_of_cells(ctx, orc, n, nq, walk):
+    # enough queries that each wave answers a run of cells, the later ones
+    # bounded by the previous cell's answers; neighbouring cells of an image
+    # are close in Lab (a random walk) or, in the worst case, unrelated
+    rng = _rng(n + nq)
+    lab = (rng.random((n, 3)) * [100, 200, 200] - [0, 100, 100]).astype(np.float32)
+    if walk:
+        q = np.cumsum(rng.normal(0, 1.5, (nq, 3)), axis=0) % [100, 200, 200] - [0, 100, 100]
+    else:
+        q = rng.random((nq, 3)) * [100, 200, 200] - [0, 100, 100]
+    q = q.astype(np.float32)
+    lib = ctx.library_from_lab(lab)
+    gi, gd = ctx.nearest_n(lib, q, 100)
+    oi, od = orc.nearest_n(lab, q, 100)
+    np.testing.assert_array_equal(gi, oi)
+    assert_bits_equal(gd, od)
+    lib.close()
+
+
 def test_nearest_n_ties_by_index(ctx, orc):
     # many exact duplicates: equal squared distances must come out by index
     rng = _rng(7)
