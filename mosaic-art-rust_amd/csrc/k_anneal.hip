@@ -273,15 +273,17 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
                 acc = true;
             } else {
                 // Metropolis test u < exp(x), x = -delta/T (src/optimizer.rs:77-84).
-                // The hardware exp2 estimate is within 2^-16 of exp(x), relative,
-                // so it decides alone unless u lies within 2^-10 of it (or it
-                // is 0 or not finite); those lanes compute the exact exp.
-                const float x = -delta / temp;
-                const float e = __builtin_amdgcn_exp2f(x * 1.44269504f);
+                // An estimate from the hardware reciprocal and exp2 is within
+                // 2^-15 of exp(x), relative, down to 2^-126 (below, it may be 0,
+                // and u is either 0 or at least 2^-24), so it decides alone
+                // unless u lies within 2^-10 of it (or it is 0 or not finite);
+                // those lanes compute x and exp exactly.
+                const float e = __builtin_amdgcn_exp2f(-delta * __builtin_amdgcn_rcpf(temp) *
+                                                       1.44269504f);
                 if (__builtin_fabsf(u - e) > e * (1.0f / 1024.0f))
                     acc = u < e;
                 else
-                    acc = u < exp_f32(x);
+                    acc = u < exp_f32(-delta / temp);
             }
         }
         const int32_t p1 = y1 * gw + x1, p2 = y2 * gw + x2;
