@@ -215,3 +215,26 @@ def test_handles_outlive_their_context(mh, synth):
     b.close()
     for lib in s.libs:
         lib.close()
+
+
+def test_batch_narrow_grids(ctx, mh, synth):
+    # grids of one to three columns (the one-cell-at-a-time chain below three
+    # columns, the look-ahead chain from three) in one batch, both scores
+    s = Setup(ctx, mh, synth, n_jobs=6, seed=11)
+    shapes = [(1, 30), (2, 25), (3, 20), (1, 1), (2, 1), (3, 2)]
+    for k, (gw, gh) in enumerate(shapes):
+        j = s.specs[k]
+        j["grid_w"], j["grid_h"] = gw, gh
+        j["width"], j["height"] = gw * 16, gh * 16
+        j["ox"], j["oy"] = 0, 0
+        j["score_mode"] = k % 2
+    jobs = [s.job(k, own_outputs=True) for k in range(len(s.specs))]
+    b = ctx.batch(jobs)
+    b.run()
+    res = b.results()
+    for k in range(len(s.specs)):
+        print(f"narrow batch job {k}: {s.specs[k]['grid_w']}x{s.specs[k]['grid_h']}", flush=True)
+        g, im = s.fetch(k, *b.outputs(k))
+        _same(s.alone(k), g, im, res[k], s.specs[k]["sa"] and s.specs[k]["weight"] > 0)
+    b.close()
+    s.close()
