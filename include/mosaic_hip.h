@@ -89,7 +89,8 @@ void *mh_context_stream(mh_context *ctx);
 /* Per-phase device timing: when enabled, every phase the library enqueues
  * is bracketed by HIP events on the context stream.  mh_context_timing
  * synchronises, returns the accumulated milliseconds and launch counts per
- * phase since the last call, and clears them.  Phase names: mh_phase_name. */
+ * phase since the last call, and clears them.  Phase names: mh_phase_name.
+ * (No counterpart in the reference, whose time_tracker.rs times on the host.) */
 #define MH_PHASE_CELL_LAB 0
 #define MH_PHASE_TOPK 1
 #define MH_PHASE_PLACE 2
@@ -100,6 +101,9 @@ void *mh_context_stream(mh_context *ctx);
 #define MH_PHASE_SIM 7
 #define MH_PHASES 8
 int mh_context_set_timing(mh_context *ctx, int32_t enabled);
+/* Times only the phases whose bit (1 << MH_PHASE_x) is set in phase_mask;
+ * each timed phase adds two event records to the stream. */
+int mh_context_set_timing_phases(mh_context *ctx, uint32_t phase_mask);
 int mh_context_timing(mh_context *ctx, double ms[MH_PHASES], int64_t launches[MH_PHASES]);
 const char *mh_phase_name(int32_t phase);
 

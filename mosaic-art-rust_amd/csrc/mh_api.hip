@@ -46,13 +46,13 @@ hipEvent_t take_event(mh_context* ctx) {
 
 // Bracket the kernels of one phase with events (no-ops unless timing is on).
 void phase_begin(mh_context* ctx, int phase) {
-    if (!ctx->timing) return;
+    if (!((ctx->timing >> phase) & 1u)) return;
     hipEvent_t e = take_event(ctx);
     if (e && hipEventRecord(e, ctx->stream) == hipSuccess) ctx->open_event[phase] = e;
 }
 
 void phase_end(mh_context* ctx, int phase) {
-    if (!ctx->timing || !ctx->open_event[phase]) return;
+    if (!((ctx->timing >> phase) & 1u) || !ctx->open_event[phase]) return;
     hipEvent_t e = take_event(ctx);
     if (e && hipEventRecord(e, ctx->stream) == hipSuccess)
         ctx->marks.push_back(PhaseMark{phase, ctx->open_event[phase], e});
@@ -588,7 +588,13 @@ void* mh_context_stream(mh_context* ctx) { return ctx ? (void*)ctx->stream : nul
 
 int mh_context_set_timing(mh_context* ctx, int32_t enabled) {
     if (!ctx) return fail(MH_ERR_INVALID_ARGUMENT, "ctx is NULL");
-    ctx->timing = enabled != 0;
+    ctx->timing = enabled != 0 ? (1u << MH_PHASES) - 1u : 0u;
+    return MH_OK;
+}
+
+int mh_context_set_timing_phases(mh_context* ctx, uint32_t phase_mask) {
+    if (!ctx) return fail(MH_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    ctx->timing = phase_mask & ((1u << MH_PHASES) - 1u);
     return MH_OK;
 }
 

@@ -77,7 +77,7 @@ struct mh_context {
     std::vector<mh_batch*> batches;
     std::mutex dep_mu;
     // per-phase timing (HIP events on `stream`)
-    bool timing = false;
+    uint32_t timing = 0;  // bit p: phase p is bracketed by events
     std::vector<hipEvent_t> event_pool;
     std::vector<mh::host::PhaseMark> marks;
     hipEvent_t open_event[MH_PHASES] = {};

@@ -30,6 +30,7 @@ MH_PHASES = 8
 EXPORTS = [
     "mh_last_error", "mh_abi_version", "mh_context_create", "mh_context_destroy",
     "mh_context_synchronize", "mh_context_stream", "mh_context_set_timing",
+    "mh_context_set_timing_phases",
     "mh_context_timing", "mh_phase_name", "mh_device_alloc", "mh_device_free",
     "mh_copy_to_device", "mh_copy_to_host", "mh_copy_on_device", "mh_copy_2d_on_device",
     "mh_library_from_lab",
@@ -145,6 +146,7 @@ def load(path=LIB_PATH):
         "mh_context_synchronize": (ctypes.c_int, [P]),
         "mh_context_stream": (P, [P]),
         "mh_context_set_timing": (ctypes.c_int, [P, i32]),
+        "mh_context_set_timing_phases": (ctypes.c_int, [P, ctypes.c_uint32]),
         "mh_context_timing": (ctypes.c_int, [P, P, P]),
         "mh_phase_name": (ctypes.c_char_p, [i32]),
         "mh_device_alloc": (ctypes.c_int, [P, ctypes.c_size_t, ctypes.POINTER(P)]),
@@ -250,6 +252,13 @@ class Context:
 
     def set_timing(self, enabled):
         _check(self.lib.mh_context_set_timing(self.h, 1 if enabled else 0))
+
+    def set_timing_phases(self, phases):
+        """Times only the named phases (each adds two event records per launch)."""
+        mask = 0
+        for name in phases:
+            mask |= 1 << [self.lib.mh_phase_name(i).decode() for i in range(MH_PHASES)].index(name)
+        _check(self.lib.mh_context_set_timing_phases(self.h, mask))
 
     def timing(self):
         """{phase: (total_ms, launches)} since the last call (synchronises)."""
