@@ -251,12 +251,14 @@ void launch_cell_adjust(const int32_t* grid, int32_t cells, const uint8_t* tile_
 }
 
 // Output pixel idx of a gw*tw x gh*th mosaic.
+// Output pixel (X, Y); Y is the same for the whole block (a block covers
+// 256 pixels of one row), so only X needs a per-pixel division.
 __device__ __forceinline__ void composite_pixel(const int32_t* __restrict__ grid, int32_t gw,
-                                                int32_t tw, int32_t th, int32_t ow, int64_t idx,
-                                                const uint8_t* __restrict__ tiles,
+                                                int32_t tw, int32_t th, int32_t ow, int32_t X,
+                                                int32_t Y, const uint8_t* __restrict__ tiles,
                                                 const float4* __restrict__ adj, int32_t adjust,
                                                 uint8_t* __restrict__ out) {
-    const int32_t X = (int32_t)(idx % ow), Y = (int32_t)(idx / ow);
+    const int64_t idx = (int64_t)Y * ow + X;
     const int32_t cx = X / tw, cy = Y / th;
     const int32_t cell = cy * gw + cx;
     const int32_t m = grid[cell];
@@ -275,24 +277,27 @@ __device__ __forceinline__ void composite_pixel(const int32_t* __restrict__ grid
     d[2] = (uint8_t)b;
 }
 
-// One thread per output pixel (ow = gw*tw, oh = gh*th).
-__global__ void composite_kernel(const int32_t* __restrict__ grid, int32_t gw, int32_t tw,
-                                 int32_t th, int32_t ow, int32_t oh,
-                                 const uint8_t* __restrict__ tiles,
-                                 const float4* __restrict__ adj, int32_t adjust,
-                                 uint8_t* __restrict__ out) {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (int64_t)ow * oh) return;
-    composite_pixel(grid, gw, tw, th, ow, idx, tiles, adj, adjust, out);
+// One thread per output pixel (ow = gw*tw, oh = gh*th); block b covers
+// pixels [256 (b % per_row), 256 (b % per_row) + 256) of row b / per_row.
+__global__ __launch_bounds__(256) void composite_kernel(const int32_t* __restrict__ grid,
+                                                        int32_t gw, int32_t tw, int32_t th,
+                                                        int32_t ow, const uint8_t* __restrict__ tiles,
+                                                        const float4* __restrict__ adj,
+                                                        int32_t adjust, uint8_t* __restrict__ out) {
+    const int32_t per_row = (ow + 255) / 256;
+    const int32_t Y = (int32_t)blockIdx.x / per_row;
+    const int32_t X = ((int32_t)blockIdx.x - Y * per_row) * 256 + (int32_t)threadIdx.x;
+    if (X >= ow) return;
+    composite_pixel(grid, gw, tw, th, ow, X, Y, tiles, adj, adjust, out);
 }
 
 void launch_composite(const int32_t* grid, int32_t gw, int32_t gh, int32_t tw, int32_t th,
                       const uint8_t* tiles, const float4* adj, int32_t adjust, uint8_t* out,
                       hipStream_t stream) {
-    const int64_t total = (int64_t)gw * tw * gh * th;
-    if (total <= 0) return;
-    hipLaunchKernelGGL(composite_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream,
-                       grid, gw, tw, th, gw * tw, gh * th, tiles, adj, adjust, out);
+    const int32_t blocks = composite_blocks(gw * tw, gh * th);
+    if (blocks <= 0) return;
+    hipLaunchKernelGGL(composite_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, grid, gw, tw,
+                       th, gw * tw, tiles, adj, adjust, out);
 }
 
 // Batched finish of many jobs: per-cell adjustments (a thread per cell), then
@@ -315,15 +320,18 @@ __global__ __launch_bounds__(256) void composite_batch_kernel(const CompSeg* __r
     const int32_t j = find_segment(starts, n_segs, blockIdx.x);
     const CompSeg& s = segs[j];
     const int32_t ow = s.gw * s.tw;
-    const int64_t idx = (int64_t)(blockIdx.x - starts[j]) * 256 + threadIdx.x;
-    if (idx >= (int64_t)ow * s.gh * s.th) return;
-    composite_pixel(s.grid, s.gw, s.tw, s.th, ow, idx, s.tiles, s.adj, s.adjust, s.out);
+    const int32_t per_row = (ow + 255) / 256;
+    const int32_t b = (int32_t)blockIdx.x - starts[j];
+    const int32_t Y = b / per_row;
+    const int32_t X = (b - Y * per_row) * 256 + (int32_t)threadIdx.x;
+    if (X >= ow) return;
+    composite_pixel(s.grid, s.gw, s.tw, s.th, ow, X, Y, s.tiles, s.adj, s.adjust, s.out);
 }
 
 int32_t cell_adjust_blocks(int32_t cells) { return cells > 0 ? (cells + 255) / 256 : 0; }
 
-int32_t composite_blocks(int64_t pixels) {
-    return pixels > 0 ? (int32_t)((pixels + 255) / 256) : 0;
+int32_t composite_blocks(int32_t ow, int32_t oh) {
+    return ow > 0 && oh > 0 ? ((ow + 255) / 256) * oh : 0;
 }
 
 void launch_cell_adjust_batch(const CompSeg* d_segs, const int32_t* d_starts, int32_t n_segs,

@@ -282,7 +282,7 @@ int build_plan(const std::vector<JobSpec>& specs, const std::vector<uint8_t*>& j
                                   s.adjust ? 1 : 0, adj, image};
         ++p->n_comp;
         adj_blocks += s.adjust ? cell_adjust_blocks(s.cells) : 0;
-        comp_blocks += composite_blocks((int64_t)s.gw * s.tw * s.gh * s.th);
+        comp_blocks += composite_blocks(s.gw * s.tw, s.gh * s.th);
     }
     for (int32_t a = 0; a < p->n_anneal; ++a) {
         const JobSpec& s = specs[p->anneal_job[a]];

@@ -127,7 +127,7 @@ int32_t topk_blocks(int32_t nq, int32_t run);
 void launch_topk_batch(const TopkSeg* d_segs, const int32_t* d_starts, int32_t n_segs,
                        int32_t total_blocks, int32_t run, hipStream_t stream);
 int32_t cell_adjust_blocks(int32_t cells);
-int32_t composite_blocks(int64_t pixels);
+int32_t composite_blocks(int32_t ow, int32_t oh);  // blocks of 256 pixels of one row
 void launch_cell_adjust_batch(const CompSeg* d_segs, const int32_t* d_starts, int32_t n_segs,
                               int32_t total_blocks, hipStream_t stream);
 void launch_composite_batch(const CompSeg* d_segs, const int32_t* d_starts, int32_t n_segs,
