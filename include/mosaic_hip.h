@@ -334,6 +334,31 @@ int mh_batch_results(mh_batch *batch, mh_job_result *results);
 int mh_batch_job_outputs(mh_batch *batch, int32_t i, int32_t **d_grid, uint8_t **d_image);
 void mh_batch_destroy(mh_batch *batch);
 
+/* Launch shapes of a batch's wide stages, per job group (0: the long jobs,
+ * 1: the rest; zeros where a group or stage is absent) and the average
+ * launch time the shape measured. */
+typedef struct {
+    int32_t n_groups;
+    int32_t topk_cells_per_wave[2]; /* query cells one nearest-material wave scans for */
+    int32_t composite_threads[2];   /* output pixels per composite workgroup */
+    float topk_ms[2], composite_ms[2];
+} mh_batch_tuning;
+/* Autotuner (no counterpart in the reference, which has no launch shapes):
+ * runs the batch once, then times `repeats` launches of each candidate shape
+ * of the nearest-material stage (1, 2, 4, 8 cells per wave) and the composite
+ * stage (64, 128, 256 threads) on the batch's own data, keeps the fastest
+ * and drops a captured graph so the next run captures the tuned shapes.
+ * Every shape computes the same outputs.  tuning may be NULL; 1 <= repeats
+ * <= 100.  Synchronises the context stream. */
+int mh_batch_tune(mh_batch *batch, int32_t repeats, mh_batch_tuning *tuning);
+/* The shapes in use (the heuristics' until mh_batch_tune runs; times 0). */
+int mh_batch_get_tuning(mh_batch *batch, mh_batch_tuning *tuning);
+/* Use the given shapes (say, a tuned batch's for a later batch of the same
+ * kind): topk_cells_per_wave in [1, 8] (0 keeps the heuristic's),
+ * composite_threads 64, 128 or 256 (0 keeps the current); times and
+ * n_groups are ignored.  Drops a captured graph. */
+int mh_batch_set_tuning(mh_batch *batch, const mh_batch_tuning *tuning);
+
 /* Continuous-batching request loop.  A scheduler thread admits queued jobs
  * (FIFO) while their working memory fits memory_budget, and launches the
  * admitted jobs as one batch on a free stream slot (at most streams batches

@@ -294,7 +294,7 @@ __global__ __launch_bounds__(256) void composite_kernel(const int32_t* __restric
 void launch_composite(const int32_t* grid, int32_t gw, int32_t gh, int32_t tw, int32_t th,
                       const uint8_t* tiles, const float4* adj, int32_t adjust, uint8_t* out,
                       hipStream_t stream) {
-    const int32_t blocks = composite_blocks(gw * tw, gh * th);
+    const int32_t blocks = composite_blocks(gw * tw, gh * th, 256);
     if (blocks <= 0) return;
     hipLaunchKernelGGL(composite_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, grid, gw, tw,
                        th, gw * tw, tiles, adj, adjust, out);
@@ -314,24 +314,26 @@ __global__ __launch_bounds__(256) void cell_adjust_batch_kernel(const CompSeg* _
     cell_adjust_one(s.grid, c, s.tile_avg, s.target_avg, s.strength, s.adj);
 }
 
+// A block covers blockDim.x (64, 128 or 256) pixels of one output row.
 __global__ __launch_bounds__(256) void composite_batch_kernel(const CompSeg* __restrict__ segs,
                                                               const int32_t* __restrict__ starts,
                                                               int32_t n_segs) {
     const int32_t j = find_segment(starts, n_segs, blockIdx.x);
     const CompSeg& s = segs[j];
     const int32_t ow = s.gw * s.tw;
-    const int32_t per_row = (ow + 255) / 256;
+    const int32_t bs = (int32_t)blockDim.x;
+    const int32_t per_row = (ow + bs - 1) / bs;
     const int32_t b = (int32_t)blockIdx.x - starts[j];
     const int32_t Y = b / per_row;
-    const int32_t X = (b - Y * per_row) * 256 + (int32_t)threadIdx.x;
+    const int32_t X = (b - Y * per_row) * bs + (int32_t)threadIdx.x;
     if (X >= ow) return;
     composite_pixel(s.grid, s.gw, s.tw, s.th, ow, X, Y, s.tiles, s.adj, s.adjust, s.out);
 }
 
 int32_t cell_adjust_blocks(int32_t cells) { return cells > 0 ? (cells + 255) / 256 : 0; }
 
-int32_t composite_blocks(int32_t ow, int32_t oh) {
-    return ow > 0 && oh > 0 ? ((ow + 255) / 256) * oh : 0;
+int32_t composite_blocks(int32_t ow, int32_t oh, int32_t threads) {
+    return ow > 0 && oh > 0 ? ((ow + threads - 1) / threads) * oh : 0;
 }
 
 void launch_cell_adjust_batch(const CompSeg* d_segs, const int32_t* d_starts, int32_t n_segs,
@@ -342,9 +344,9 @@ void launch_cell_adjust_batch(const CompSeg* d_segs, const int32_t* d_starts, in
 }
 
 void launch_composite_batch(const CompSeg* d_segs, const int32_t* d_starts, int32_t n_segs,
-                            int32_t total_blocks, hipStream_t stream) {
+                            int32_t total_blocks, int32_t threads, hipStream_t stream) {
     if (n_segs <= 0 || total_blocks <= 0) return;
-    hipLaunchKernelGGL(composite_batch_kernel, dim3(total_blocks), dim3(256), 0, stream, d_segs,
+    hipLaunchKernelGGL(composite_batch_kernel, dim3(total_blocks), dim3(threads), 0, stream, d_segs,
                        d_starts, n_segs);
 }
 

@@ -141,6 +141,10 @@ struct BatchPlan {
     int32_t adj_blocks = 0;
     int32_t n_comp = 0, comp_blocks = 0;
     std::vector<int32_t> anneal_job;  // job index of each anneal entry
+    // Launch shapes a tuner may set (build_plan keeps them): query cells per
+    // top-K wave (0: topk_run_length's choice) and composite threads per block.
+    int32_t tune_topk_run = 0;
+    int32_t comp_threads = 256;
 };
 
 constexpr int32_t kStatsInts = 8;
@@ -200,7 +204,7 @@ int build_plan(const std::vector<JobSpec>& specs, const std::vector<uint8_t*>& j
     int64_t topk_queries = 0;
     for (const JobSpec& s : specs)
         if (s.n > 0) topk_queries += s.cells;
-    p->topk_run = topk_run_length(topk_queries);
+    p->topk_run = p->tune_topk_run > 0 ? p->tune_topk_run : topk_run_length(topk_queries);
     p->place_lds = 0;
     p->place_score = kScoreSq;
     p->anneal_lds = 0;
@@ -282,7 +286,7 @@ int build_plan(const std::vector<JobSpec>& specs, const std::vector<uint8_t*>& j
                                   s.adjust ? 1 : 0, adj, image};
         ++p->n_comp;
         adj_blocks += s.adjust ? cell_adjust_blocks(s.cells) : 0;
-        comp_blocks += composite_blocks(s.gw * s.tw, s.gh * s.th);
+        comp_blocks += composite_blocks(s.gw * s.tw, s.gh * s.th, p->comp_threads);
     }
     for (int32_t a = 0; a < p->n_anneal; ++a) {
         const JobSpec& s = specs[p->anneal_job[a]];
@@ -319,7 +323,7 @@ void enqueue_chains(const BatchPlan& p, uint8_t* d, hipStream_t s) {
         launch_cell_adjust_batch(at<CompSeg>(d, p.o_comp), at<int32_t>(d, p.o_adj_st), p.n_comp,
                                  p.adj_blocks, s);
     launch_composite_batch(at<CompSeg>(d, p.o_comp), at<int32_t>(d, p.o_comp_st), p.n_comp,
-                           p.comp_blocks, s);
+                           p.comp_blocks, p.comp_threads, s);
 }
 
 void enqueue_plan(const BatchPlan& p, uint8_t* d, const float* lut, hipStream_t s) {
@@ -369,6 +373,18 @@ struct mh_batch {
 };
 
 namespace {
+
+// Group g's descriptor tables, built into the host image h_plan.
+int build_group(mh_batch* b, int g) {
+    std::vector<JobSpec> sp;
+    std::vector<uint8_t*> mem;
+    for (int32_t i : b->members[g]) {
+        sp.push_back(b->specs[i]);
+        mem.push_back(b->job_mem[i]);
+    }
+    return build_plan(sp, mem, b->h_plan.data() + b->plan_off[g], b->d_plan + b->plan_off[g],
+                      &b->plan[g]);
+}
 
 void enqueue_batch(mh_batch* b) {
     mh_context* ctx = b->ctx;
@@ -506,16 +522,7 @@ int mh_batch_create(mh_context* ctx, const mh_job* jobs, int32_t n_jobs, mh_batc
         b->job_mem.resize((size_t)n_jobs);
         for (int32_t i = 0; i < n_jobs; ++i) b->job_mem[i] = b->d_jobs + offs[i];
         b->h_plan.assign(plan_total, 0);
-        for (int g = 0; g < b->n_groups && !rc; ++g) {
-            std::vector<JobSpec> sp;
-            std::vector<uint8_t*> mem;
-            for (int32_t i : b->members[g]) {
-                sp.push_back(b->specs[i]);
-                mem.push_back(b->job_mem[i]);
-            }
-            rc = build_plan(sp, mem, b->h_plan.data() + b->plan_off[g], b->d_plan + b->plan_off[g],
-                            &b->plan[g]);
-        }
+        for (int g = 0; g < b->n_groups && !rc; ++g) rc = build_group(b, g);
     }
     if (!rc) {
         e = hipMemcpy(b->d_plan, b->h_plan.data(), plan_total, hipMemcpyHostToDevice);
@@ -526,6 +533,154 @@ int mh_batch_create(mh_context* ctx, const mh_job* jobs, int32_t n_jobs, mh_batc
         return rc;
     }
     *out = b;
+    return MH_OK;
+}
+
+extern "C++" {
+namespace {
+
+// Average launch time (ms) of `reps` launches after one untimed launch, all
+// on stream s.
+template <typename F>
+int time_launch(hipStream_t s, hipEvent_t t0, hipEvent_t t1, int32_t reps, F&& launch, float* ms) {
+    launch();
+    HIPCHK(hipEventRecord(t0, s));
+    for (int32_t r = 0; r < reps; ++r) launch();
+    HIPCHK(hipEventRecord(t1, s));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventSynchronize(t1));
+    float total = 0.0f;
+    HIPCHK(hipEventElapsedTime(&total, t0, t1));
+    *ms = total / (float)reps;
+    return MH_OK;
+}
+
+// Rebuild group g's tables with its current launch shapes and upload the
+// descriptor part (the per-job results that follow it keep their values).
+int refresh_group(mh_batch* b, int g) {
+    CHK(build_group(b, g));
+    HIPCHK(hipMemcpy(b->d_plan + b->plan_off[g], b->h_plan.data() + b->plan_off[g],
+                     b->plan[g].o_stats, hipMemcpyHostToDevice));
+    return MH_OK;
+}
+
+// A captured graph holds the launch shapes it was captured with.
+int drop_graph(mh_batch* b) {
+    if (b->exec) {
+        HIPCHK(hipGraphExecDestroy(b->exec));
+        b->exec = nullptr;
+    }
+    if (b->graph) {
+        HIPCHK(hipGraphDestroy(b->graph));
+        b->graph = nullptr;
+    }
+    return MH_OK;
+}
+
+void report_tuning(const mh_batch* b, mh_batch_tuning* t) {
+    memset(t, 0, sizeof(*t));
+    t->n_groups = b->n_groups;
+    for (int g = 0; g < b->n_groups; ++g) {
+        t->topk_cells_per_wave[g] = b->plan[g].n_topk > 0 ? b->plan[g].topk_run : 0;
+        t->composite_threads[g] = b->plan[g].comp_threads;
+    }
+}
+
+struct EventPair {
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    ~EventPair() {
+        if (t0) hipEventDestroy(t0);
+        if (t1) hipEventDestroy(t1);
+    }
+};
+
+}  // namespace
+}  // extern "C++"
+
+int mh_batch_tune(mh_batch* b, int32_t repeats, mh_batch_tuning* tuning) {
+    if (!b) return fail(MH_ERR_INVALID_ARGUMENT, "batch is NULL");
+    if (repeats < 1 || repeats > 100)
+        return fail(MH_ERR_INVALID_ARGUMENT, "mh_batch_tune: repeats %d outside [1, 100]", repeats);
+    mh_context* ctx = b->ctx;
+    CHK(set_device(ctx));
+    const hipStream_t s = ctx->stream;
+    EventPair ev;
+    HIPCHK(hipEventCreate(&ev.t0));
+    HIPCHK(hipEventCreate(&ev.t1));
+    // the timed stages read cell Labs and grids: one run of the batch first
+    enqueue_batch(b);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    CHK(drop_graph(b));  // captured with the old shapes
+    mh_batch_tuning t{};
+    t.n_groups = b->n_groups;
+    static const int32_t kRuns[] = {1, 2, 4, 8};       // topk_run accepts any run >= 1
+    static const int32_t kThreads[] = {64, 128, 256};  // composite_batch_kernel's bounds
+    for (int g = 0; g < b->n_groups; ++g) {
+        BatchPlan& p = b->plan[g];
+        uint8_t* d = b->d_plan + b->plan_off[g];
+        if (p.n_topk > 0) {
+            float best = 0.0f;
+            int32_t pick = 0;
+            for (int32_t run : kRuns) {
+                p.tune_topk_run = run;
+                CHK(refresh_group(b, g));
+                float ms = 0.0f;
+                CHK(time_launch(s, ev.t0, ev.t1, repeats, [&] {
+                    launch_topk_batch(at<TopkSeg>(d, p.o_topk), at<int32_t>(d, p.o_topk_st), p.n_topk,
+                                      p.topk_blocks, p.topk_run, s);
+                }, &ms));
+                if (!pick || ms < best) best = ms, pick = run;
+            }
+            p.tune_topk_run = pick;
+            t.topk_ms[g] = best;
+        }
+        float best = 0.0f;
+        int32_t pick = 0;
+        for (int32_t threads : kThreads) {
+            p.comp_threads = threads;
+            CHK(refresh_group(b, g));
+            float ms = 0.0f;
+            CHK(time_launch(s, ev.t0, ev.t1, repeats, [&] {
+                launch_composite_batch(at<CompSeg>(d, p.o_comp), at<int32_t>(d, p.o_comp_st), p.n_comp,
+                                       p.comp_blocks, p.comp_threads, s);
+            }, &ms));
+            if (!pick || ms < best) best = ms, pick = threads;
+        }
+        p.comp_threads = pick;
+        t.composite_ms[g] = best;
+        CHK(refresh_group(b, g));
+        t.topk_cells_per_wave[g] = p.n_topk > 0 ? p.topk_run : 0;
+        t.composite_threads[g] = p.comp_threads;
+    }
+    if (tuning) *tuning = t;
+    return MH_OK;
+}
+
+int mh_batch_set_tuning(mh_batch* b, const mh_batch_tuning* tuning) {
+    if (!b || !tuning) return fail(MH_ERR_INVALID_ARGUMENT, "NULL argument");
+    for (int g = 0; g < 2; ++g) {
+        const int32_t run = tuning->topk_cells_per_wave[g], thr = tuning->composite_threads[g];
+        if (run < 0 || run > 8)
+            return fail(MH_ERR_INVALID_ARGUMENT, "topk_cells_per_wave %d outside [0, 8]", run);
+        if (thr != 0 && thr != 64 && thr != 128 && thr != 256)
+            return fail(MH_ERR_INVALID_ARGUMENT, "composite_threads %d not 0, 64, 128 or 256", thr);
+    }
+    mh_context* ctx = b->ctx;
+    CHK(set_device(ctx));
+    HIPCHK(hipStreamSynchronize(ctx->stream));  // the tables may be in use
+    CHK(drop_graph(b));
+    for (int g = 0; g < b->n_groups; ++g) {
+        b->plan[g].tune_topk_run = tuning->topk_cells_per_wave[g];
+        if (tuning->composite_threads[g]) b->plan[g].comp_threads = tuning->composite_threads[g];
+        CHK(refresh_group(b, g));
+    }
+    return MH_OK;
+}
+
+int mh_batch_get_tuning(mh_batch* b, mh_batch_tuning* tuning) {
+    if (!b || !tuning) return fail(MH_ERR_INVALID_ARGUMENT, "NULL argument");
+    report_tuning(b, tuning);
     return MH_OK;
 }
 

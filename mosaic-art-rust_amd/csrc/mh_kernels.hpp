@@ -127,11 +127,12 @@ int32_t topk_blocks(int32_t nq, int32_t run);
 void launch_topk_batch(const TopkSeg* d_segs, const int32_t* d_starts, int32_t n_segs,
                        int32_t total_blocks, int32_t run, hipStream_t stream);
 int32_t cell_adjust_blocks(int32_t cells);
-int32_t composite_blocks(int32_t ow, int32_t oh);  // blocks of 256 pixels of one row
+// blocks of `threads` pixels of one output row (threads: 64, 128 or 256)
+int32_t composite_blocks(int32_t ow, int32_t oh, int32_t threads);
 void launch_cell_adjust_batch(const CompSeg* d_segs, const int32_t* d_starts, int32_t n_segs,
                               int32_t total_blocks, hipStream_t stream);
 void launch_composite_batch(const CompSeg* d_segs, const int32_t* d_starts, int32_t n_segs,
-                            int32_t total_blocks, hipStream_t stream);
+                            int32_t total_blocks, int32_t threads, hipStream_t stream);
 // General chain: any cell order, pre-filled grid and usage (state in LDS).
 size_t place_lds_bytes(int32_t cells, int32_t n);
 // Raster chain: generate_mosaic's raster order from an empty grid, where

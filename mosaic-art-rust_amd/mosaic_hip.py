@@ -42,7 +42,7 @@ EXPORTS = [
     "mh_adjust_pixels", "mh_resize", "mh_composite", "mh_generate", "mh_generate_device",
     "mh_generate_device_result", "mh_job_footprint", "mh_batch_create", "mh_batch_set_graph",
     "mh_batch_run", "mh_batch_results", "mh_batch_job_outputs", "mh_batch_destroy",
-    "mh_server_create", "mh_server_submit", "mh_server_wait", "mh_server_poll", "mh_server_drain",
+    "mh_batch_tune", "mh_batch_get_tuning", "mh_batch_set_tuning", "mh_server_create", "mh_server_submit", "mh_server_wait", "mh_server_poll", "mh_server_drain",
     "mh_server_job_times", "mh_server_get_stats", "mh_server_destroy",
 ]
 
@@ -103,6 +103,20 @@ class Job(ctypes.Structure):
                 ("target_pitch", ctypes.c_int64), ("width", ctypes.c_int32),
                 ("height", ctypes.c_int32), ("params", JobParams),
                 ("d_grid_out", ctypes.c_void_p), ("d_image_out", ctypes.c_void_p)]
+
+
+class BatchTuning(ctypes.Structure):
+    _fields_ = [("n_groups", ctypes.c_int32), ("topk_cells_per_wave", ctypes.c_int32 * 2),
+                ("composite_threads", ctypes.c_int32 * 2), ("topk_ms", ctypes.c_float * 2),
+                ("composite_ms", ctypes.c_float * 2)]
+
+    def as_dict(self):
+        g = self.n_groups
+        return {"groups": g,
+                "topk_cells_per_wave": list(self.topk_cells_per_wave)[:g],
+                "composite_threads": list(self.composite_threads)[:g],
+                "topk_ms": [round(v, 5) for v in list(self.topk_ms)[:g]],
+                "composite_ms": [round(v, 5) for v in list(self.composite_ms)[:g]]}
 
 
 class ServerParams(ctypes.Structure):
@@ -192,6 +206,9 @@ def load(path=LIB_PATH):
         "mh_batch_results": (ctypes.c_int, [P, P]),
         "mh_batch_job_outputs": (ctypes.c_int, [P, i32, ctypes.POINTER(P), ctypes.POINTER(P)]),
         "mh_batch_destroy": (None, [P]),
+        "mh_batch_tune": (ctypes.c_int, [P, i32, P]),
+        "mh_batch_get_tuning": (ctypes.c_int, [P, P]),
+        "mh_batch_set_tuning": (ctypes.c_int, [P, P]),
         "mh_server_create": (ctypes.c_int, [P, ctypes.POINTER(ServerParams), ctypes.POINTER(P)]),
         "mh_server_submit": (ctypes.c_int, [P, ctypes.POINTER(Job), ctypes.POINTER(i64)]),
         "mh_server_wait": (ctypes.c_int, [P, i64, ctypes.POINTER(JobResult),
@@ -577,6 +594,24 @@ class Batch:
 
     def run(self):
         _check(self.lib.mh_batch_run(self.h))
+
+    def tune(self, repeats=5):
+        """Times each candidate launch shape on this batch and keeps the fastest."""
+        t = BatchTuning()
+        _check(self.lib.mh_batch_tune(self.h, repeats, ctypes.byref(t)))
+        return t.as_dict()
+
+    def tuning(self):
+        t = BatchTuning()
+        _check(self.lib.mh_batch_get_tuning(self.h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, topk_cells_per_wave=(0, 0), composite_threads=(0, 0)):
+        t = BatchTuning()
+        for g in range(2):
+            t.topk_cells_per_wave[g] = topk_cells_per_wave[g]
+            t.composite_threads[g] = composite_threads[g]
+        _check(self.lib.mh_batch_set_tuning(self.h, ctypes.byref(t)))
 
     def results(self):
         res = (JobResult * self.n)()

@@ -238,3 +238,50 @@ def test_batch_narrow_grids(ctx, mh, synth):
         _same(s.alone(k), g, im, res[k], s.specs[k]["sa"] and s.specs[k]["weight"] > 0)
     b.close()
     s.close()
+
+
+def test_every_tuned_shape_gives_the_same_outputs(ctx, mh, synth):
+    """mh_batch_tune / mh_batch_set_tuning change launch shapes only: every
+    candidate shape (top-K cells per wave, composite threads) and the tuner's
+    pick, with graph capture after each change, reproduce the jobs alone."""
+    s = Setup(ctx, mh, synth, n_jobs=8, seed=11)
+    jobs = [s.job(k, own_outputs=True) for k in range(len(s.specs))]
+    want = [s.alone(k) for k in range(len(s.specs))]
+    b = ctx.batch(jobs)
+    b.set_graph(True)
+    base = b.tuning()
+    assert all(1 <= r <= 8 for r in base["topk_cells_per_wave"]), base
+    assert all(t == 256 for t in base["composite_threads"]), base
+
+    def check(label):
+        res = b.results()
+        for k in range(len(s.specs)):
+            g, im = s.fetch(k, *b.outputs(k))
+            _same(want[k], g, im, res[k], s.specs[k]["sa"] and s.specs[k]["weight"] > 0)
+        print(f"tuned shape {label}: ok", flush=True)
+
+    b.run()
+    check("heuristic")
+    for run, thr in ((1, 64), (2, 128), (4, 256), (8, 64)):
+        b.set_tuning((run, run), (thr, thr))
+        got = b.tuning()
+        assert got["topk_cells_per_wave"] == [run] * got["groups"], got
+        assert got["composite_threads"] == [thr] * got["groups"], got
+        b.run()
+        check(f"run {run}, {thr} threads")
+    t = b.tune(repeats=2)
+    assert t == {**b.tuning(), "topk_ms": t["topk_ms"], "composite_ms": t["composite_ms"]}
+    assert all(r in (1, 2, 4, 8) for r in t["topk_cells_per_wave"]), t
+    assert all(x in (64, 128, 256) for x in t["composite_threads"]), t
+    assert all(ms > 0 for ms in t["composite_ms"]), t
+    check("after tune")  # the tuner's own run
+    b.run()
+    check("tuned, captured")
+    with pytest.raises(mh.MhError):
+        b.set_tuning((9, 0), (0, 0))
+    with pytest.raises(mh.MhError):
+        b.set_tuning((0, 0), (96, 0))
+    with pytest.raises(mh.MhError):
+        b.tune(repeats=0)
+    b.close()
+    s.close()
