@@ -262,7 +262,10 @@ __device__ __forceinline__ void rgb_to_hsv(float r, float g, float b, float& h, 
 __device__ __forceinline__ void hsv_to_rgb(float hdeg, float s, float v, float& r, float& g,
                                            float& b) {
     float c = v * s;
-    float hp = hdeg - floorf(hdeg / 360.0f) * 360.0f;
+    // hdeg - floor(hdeg/360)*360: for 0 <= hdeg < 360 the quotient rounds
+    // below 1, so the floor is +0 (-0 for hdeg = -0) and the result is
+    // hdeg + 0.0f (which turns -0 into +0, as the subtraction does)
+    float hp = hdeg >= 0.0f && hdeg < 360.0f ? hdeg + 0.0f : hdeg - floorf(hdeg / 360.0f) * 360.0f;
     float h = hp / 60.0f;
     float hm2 = h - 2.0f * floorf(h * 0.5f);
     float x = c * (1.0f - fabsf(hm2 - 1.0f));
@@ -290,18 +293,38 @@ __device__ __forceinline__ uint32_t to_u8(float v) {
     return (uint32_t)t;
 }
 
+// x / 255.0f for an integer 0 <= x <= 255, correctly rounded: the product
+// with RN(1/255), corrected once by its exact fused residual.  The result
+// equals the division for all 256 inputs (an exhaustive host check).
+__device__ __forceinline__ float u8_unit(uint32_t x) {
+    const float a = (float)x;
+    const float c = 1.0f / 255.0f;
+    const float q = a * c;
+    const float r = __builtin_fmaf(-q, 255.0f, a);
+    return __builtin_fmaf(r, c, q);
+}
+
+// fmodf(a, 360.0f).  For 360 <= |a| < 720 the remainder is a -/+ 360,
+// exact by Sterbenz's lemma; a zero remainder keeps the sign of a.
+__device__ __forceinline__ float fmod_360(float a) {
+    const float m = __builtin_fabsf(a);
+    if (!(m < 720.0f)) return fmodf(a, 360.0f);  // NaN and infinities too
+    const float r = m >= 360.0f ? m - 360.0f : m;
+    return __builtin_copysignf(r, a);
+}
+
 // ColorAdjustment::adjust_pixel (src/color_adjustment.rs:53-87).
 // adj = {brightness, contrast, hue_shift, saturation}.
 __device__ __forceinline__ void adjust_pixel(float4 adj, uint32_t r8, uint32_t g8, uint32_t b8,
                                              uint32_t& ro, uint32_t& go, uint32_t& bo) {
-    float r = (float)r8 / 255.0f, g = (float)g8 / 255.0f, b = (float)b8 / 255.0f;
+    float r = u8_unit(r8), g = u8_unit(g8), b = u8_unit(b8);
     float br = bright_contrast(r, adj.x, adj.y);
     float bg = bright_contrast(g, adj.x, adj.y);
     float bb = bright_contrast(b, adj.x, adj.y);
     if (adj.z != 0.0f || adj.w != 1.0f) {
         float h, s, v;
         rgb_to_hsv(br, bg, bb, h, s, v);
-        float hr = fmodf(h + adj.z, 360.0f);  // rem_euclid(360.0)
+        float hr = fmod_360(h + adj.z);  // rem_euclid(360.0)
         if (hr < 0.0f) hr = hr + 360.0f;
         float s2 = clampf(s * adj.w, 0.0f, 1.0f);
         hsv_to_rgb(hr, s2, v, br, bg, bb);

@@ -403,11 +403,18 @@ def test_optimal_adjustments(ctx, orc):
 
 def test_adjust_pixels(ctx, orc):
     rng = _rng(22)
-    px = rng.integers(0, 256, (20000, 3)).astype(np.uint8)
+    levels = np.arange(256, dtype=np.uint8)
+    px = np.concatenate([rng.integers(0, 256, (20000, 3)).astype(np.uint8),
+                         np.stack([levels, levels, levels], 1),  # every u8 value per channel
+                         np.stack([levels, levels[::-1], np.roll(levels, 85)], 1)])
     adjs = [orc.adjustment_new(0.0, 1.0, 0.0, 1.0), orc.adjustment_new(0.2, 1.0, 0.0, 1.0),
             orc.adjustment_new(0.0, 1.5, 0.0, 1.0), orc.adjustment_new(0.0, 1.0, 120.0, 1.0),
             orc.adjustment_new(-0.3, 0.7, -47.5, 0.5), orc.adjustment_new(0.5, 1.5, 60.0, 1.5),
             orc.adjustment_new(0.05, 1.0, 179.9, 2.0)]
+    # raw hue shifts beyond ColorAdjustment::new's clamp: hue + shift crosses
+    # +-360 and +-720 (the exact and the fmodf branches of the reduction)
+    adjs += [np.array([0.0, 1.0, hs, 1.0], np.float32)
+             for hs in (-180.0, -360.0, 359.9, 400.0, 700.0, 725.5, -1000.25, 1e6)]
     for a in adjs:
         got = ctx.adjust_pixels(a, px)
         want = orc.apply_adjustment(a, px)
