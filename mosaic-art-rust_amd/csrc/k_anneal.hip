@@ -337,6 +337,8 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
         // the mask.  Every lane runs the same uniform loop, so cost / best /
         // improved stay identical across the wave.  Adding a +0.0 past the last
         // delta changes nothing: the cost is never -0.0 and never below best.
+        // best = fminf(best, cost) equals `cost < best ? cost : best` here (no
+        // NaN, no -0.0) and keeps the compare off the dependent chain.
         pack_s[lane] = 0.0f;
         if ((accmask >> lane) & 1ull) pack_s[__popcll(accmask & lanes_below)] = delta;
         __syncthreads();  // other lanes' writes, not this lane's own stores, are read below
@@ -354,9 +356,8 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
                     cost = cost + readlane_f(packed, q + i);
-                    const bool better = cost < best;
-                    best = better ? cost : best;
-                    better_n += better ? 1u : 0u;
+                    better_n += cost < best ? 1u : 0u;
+                    best = __builtin_fminf(best, cost);
                 }
             }
             improved += better_n;
