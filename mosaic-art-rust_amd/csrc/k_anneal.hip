@@ -332,9 +332,9 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
         }
         // Running cost in proposal order (f32, as the reference's
         // `current_cost += delta`).  The committed accepted deltas are packed
-        // into lanes 0..n-1 of `packed` in proposal order (the other lanes hold
-        // +0.0), so the serial sum reads lanes by a counter instead of walking
-        // the mask.  Every lane runs the same uniform loop, so cost / best /
+        // into words 0..n-1 of pack_s in proposal order (the other words hold
+        // +0.0; pack_s is 16-byte aligned), so the serial sum reads them by a
+        // counter instead of walking the mask.  Every lane runs the same uniform loop, so cost / best /
         // improved stay identical across the wave.  Adding a +0.0 past the last
         // delta changes nothing: the cost is never -0.0 and never below best.
         // best = fminf(best, cost) equals `cost < best ? cost : best` here (no
@@ -342,20 +342,26 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
         pack_s[lane] = 0.0f;
         if ((accmask >> lane) & 1ull) pack_s[__popcll(accmask & lanes_below)] = delta;
         __syncthreads();  // other lanes' writes, not this lane's own stores, are read below
-        const float packed = pack_s[lane];
+        // Eight deltas at a time come back as two uniform 16-byte LDS reads, so
+        // the chain adds VGPRs (no lane reads into SGPRs between the adds).
+        const float4* pack4 = reinterpret_cast<const float4*>(pack_s);
         const int32_t n_acc = __popcll(accmask);
         if (job.greedy) {  // optimize_greedy counts every accepted swap
             for (int32_t q = 0; q < n_acc; q += 8) {
+                const float4 u0 = pack4[q >> 2], u1 = pack4[(q >> 2) + 1];
+                const float v[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
 #pragma unroll
-                for (int i = 0; i < 8; ++i) cost = cost + readlane_f(packed, q + i);
+                for (int i = 0; i < 8; ++i) cost = cost + v[i];
             }
             improved += n_acc;
         } else {
             uint32_t better_n = 0;
             for (int32_t q = 0; q < n_acc; q += 8) {
+                const float4 u0 = pack4[q >> 2], u1 = pack4[(q >> 2) + 1];
+                const float v[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
-                    cost = cost + readlane_f(packed, q + i);
+                    cost = cost + v[i];
                     better_n += cost < best ? 1u : 0u;
                     best = __builtin_fminf(best, cost);
                 }
