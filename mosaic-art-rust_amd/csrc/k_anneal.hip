@@ -59,11 +59,30 @@ struct TermLab {
 // a valid dummy entry -- so the gathers cost one memory latency instead of
 // one each.  Absent terms are added as +0.0: the sums start at +0 and only
 // grow, and x + (+0) == x bit for bit, so this equals skipping them.
-template <typename G, typename TS>
-__device__ float swap_delta_dev(const G* grid, int32_t gw, int32_t gh, const TS& T,
-                                int32_t x1, int32_t y1, int32_t x2, int32_t y2) {
+//
+// The neighbour slots (Up, Down, Left, Right of p1, then of p2) are read by
+// swap_slots, which callers may issue together with their own reads of p1
+// and p2; a slot outside the grid reads its own cell, always in bounds, and
+// swap_delta_core masks it out.
+template <typename G>
+__device__ __forceinline__ void swap_slots(const G* grid, int32_t gw, int32_t gh, int32_t x1,
+                                           int32_t y1, int32_t x2, int32_t y2, int32_t nb[8]) {
     const int32_t p1 = y1 * gw + x1, p2 = y2 * gw + x2;
-    const int32_t m1 = gget(grid, p1), m2 = gget(grid, p2);
+    nb[0] = gget(grid, y1 > 0 ? p1 - gw : p1);
+    nb[1] = gget(grid, y1 < gh - 1 ? p1 + gw : p1);
+    nb[2] = gget(grid, x1 > 0 ? p1 - 1 : p1);
+    nb[3] = gget(grid, x1 < gw - 1 ? p1 + 1 : p1);
+    nb[4] = gget(grid, y2 > 0 ? p2 - gw : p2);
+    nb[5] = gget(grid, y2 < gh - 1 ? p2 + gw : p2);
+    nb[6] = gget(grid, x2 > 0 ? p2 - 1 : p2);
+    nb[7] = gget(grid, x2 < gw - 1 ? p2 + 1 : p2);
+}
+
+template <typename TS>
+__device__ __forceinline__ float swap_delta_core(int32_t gw, int32_t gh, const TS& T, int32_t x1,
+                                                 int32_t y1, int32_t x2, int32_t y2, int32_t m1,
+                                                 int32_t m2, const int32_t nb[8]) {
+    const int32_t p1 = y1 * gw + x1, p2 = y2 * gw + x2;
     if (m1 == kEmpty || m2 == kEmpty) return 0.0f;
     if (m1 == m2) return 0.0f;
     const bool in[8] = {y1 > 0, y1 < gh - 1, x1 > 0, x1 < gw - 1,
@@ -77,7 +96,7 @@ __device__ float swap_delta_dev(const G* grid, int32_t gw, int32_t gh, const TS&
         const int32_t other = i < 4 ? p2 : p1;
         const bool self = in[i] && pos[i] == other;  // the swap partner
         if (i < 4) is_adj = is_adj || self;
-        const int32_t g = in[i] ? gget(grid, pos[i]) : kEmpty;
+        const int32_t g = in[i] ? nb[i] : kEmpty;
         ok[i] = in[i] && !self && g != kEmpty;
         a[i] = ok[i] ? g : 0;
     }
@@ -102,6 +121,15 @@ __device__ float swap_delta_dev(const G* grid, int32_t gw, int32_t gh, const TS&
     old_cost += is_adj ? tm : 0.0f;
     new_cost += is_adj ? tm : 0.0f;
     return new_cost - old_cost;
+}
+
+template <typename G, typename TS>
+__device__ float swap_delta_dev(const G* grid, int32_t gw, int32_t gh, const TS& T,
+                                int32_t x1, int32_t y1, int32_t x2, int32_t y2) {
+    int32_t nb[8];
+    swap_slots(grid, gw, gh, x1, y1, x2, y2, nb);
+    return swap_delta_core(gw, gh, T, x1, y1, x2, y2, gget(grid, y1 * gw + x1),
+                           gget(grid, y2 * gw + x2), nb);
 }
 
 // calculate_total_cost (src/adjacency.rs:91-130) by one wave: the terms of
@@ -244,16 +272,15 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
         const bool active = it < max_iter;
         const uint32_t slot = (uint32_t)it & (kRing - 1);
         const uint32_t xy1 = ring_xy1[slot], xy2 = ring_xy2[slot];
-        int32_t x1 = 0, y1 = 0, x2 = 0, y2 = 0;
-        bool skip = true;
-        if (active) {
-            x1 = (int32_t)(xy1 & 0xffffu);
-            y1 = (int32_t)(xy1 >> 16);
-            x2 = (int32_t)(xy2 & 0xffffu);
-            y2 = (int32_t)(xy2 >> 16);
-            skip = (x1 == x2 && y1 == y2) || gget(grid_s, y1 * gw + x1) == kEmpty ||
-                   gget(grid_s, y2 * gw + x2) == kEmpty;
-        }
+        // Every ring slot up to `filled` holds in-grid coordinates (zeros past
+        // max_iter), so all lanes read both cells and their neighbour slots
+        // at once, before the skip test.
+        const int32_t x1 = (int32_t)(xy1 & 0xffffu), y1 = (int32_t)(xy1 >> 16);
+        const int32_t x2 = (int32_t)(xy2 & 0xffffu), y2 = (int32_t)(xy2 >> 16);
+        const int32_t m1 = gget(grid_s, y1 * gw + x1), m2 = gget(grid_s, y2 * gw + x2);
+        int32_t nb[8];
+        swap_slots(grid_s, gw, gh, x1, y1, x2, y2, nb);
+        const bool skip = !active || (x1 == x2 && y1 == y2) || m1 == kEmpty || m2 == kEmpty;
         const uint64_t nonskip = wave_ballot(!skip);
         float delta = 0.0f;
         bool acc = false;
@@ -265,8 +292,8 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
                 temp = gload(job.temps, kbase + __popcll(nonskip & lanes_below));
                 u = ring_u[slot];
             }
-            delta = job.term ? swap_delta_dev(grid_s, gw, gh, TM, x1, y1, x2, y2)
-                             : swap_delta_dev(grid_s, gw, gh, TL, x1, y1, x2, y2);
+            delta = job.term ? swap_delta_core(gw, gh, TM, x1, y1, x2, y2, m1, m2, nb)
+                             : swap_delta_core(gw, gh, TL, x1, y1, x2, y2, m1, m2, nb);
             if (job.greedy) {
                 acc = delta < 0.0f;
             } else if (delta < 0.0f) {
