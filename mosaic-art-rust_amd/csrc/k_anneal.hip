@@ -350,9 +350,10 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
         __syncthreads();
         if (lane < first) {
             if (acc) {
-                const G a = grid_s[p1];
-                grid_s[p1] = grid_s[p2];
-                grid_s[p2] = a;
+                // no lower lane of this round wrote p1 or p2 (that would be a
+                // conflict), so the round-start reads m1, m2 are still current
+                grid_s[p1] = (G)m2;
+                grid_s[p2] = (G)m1;
             }
             if (job.trace_delta) gstore(job.trace_delta, it, skip ? __int_as_float(0x7fc00000) : delta);
             if (job.trace_accept) gstore(job.trace_accept, it, (uint8_t)(acc ? 1 : 0));
