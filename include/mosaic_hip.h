@@ -296,6 +296,29 @@ int mh_generate_device(mh_context *ctx, const mh_library *lib, const uint8_t *d_
                        int32_t *d_grid_out, uint8_t *d_image_out);
 int mh_generate_device_result(mh_context *ctx, mh_job_result *result);
 
+/* Allocation-free form of mh_generate_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, outputs and result (read with
+ * mh_generate_device_result) as mh_generate_device, bit for bit.  Its working
+ * memory (cell Labs, candidate rows, usage counts, adjustments, temperature
+ * table, result words, and the grid when d_grid_out is NULL) lives in one
+ * static device array of MH_NOALLOC_SCRATCH_BYTES per GPU, held until the
+ * process ends; a job that needs more fails with MH_ERR_UNSUPPORTED (about
+ * 2.6 KB per cell: 24,000 cells fit).  It calls no hipMalloc or hipFree, so
+ * the library's resized tiles for the job's cell size must already exist
+ * when d_image_out is given (any mh_generate / mh_generate_device /
+ * mh_composite at that cell size builds them; else MH_ERR_UNSUPPORTED), and
+ * the adjacency terms come from the Lab table instead of the library's term
+ * matrix (the same bits).  Phase timing is not recorded.  Every call on one
+ * GPU shares that GPU's array and leaves its contents behind: calls that
+ * overlap on one GPU (two contexts, or two host threads) race, so callers
+ * order them. */
+#define MH_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int mh_generate_device_noalloc(mh_context *ctx, const mh_library *lib, const uint8_t *d_target,
+                               int32_t width, int32_t height, const mh_job_params *params,
+                               int32_t *d_grid_out, uint8_t *d_image_out);
+
 /* ------------------------------------------- batched and served jobs */
 /* One mosaic job for the batch and server entry points.  d_target is a
  * device RGB8 image with target_pitch bytes per row (0 = width*3).  Outputs

@@ -205,22 +205,25 @@ void fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s) {
 // Device-resident placement from cell Labs already on the device.  raster:
 // the whole grid in raster order from an empty grid (generate_mosaic's case,
 // the fast chain); otherwise `order` (or raster) over the given grid/usage.
+// The candidate rows go to cidx_buf / clab_buf (cells x kCandStride each)
+// when given, else to the context's scratch.
 int run_place(mh_context* ctx, const mh_library* lib, const float* d_cell_lab, int32_t gw,
               int32_t gh, int32_t max_usage, float weight, int32_t score_mode, bool raster,
               const int32_t* d_order,
               int32_t n_order, int32_t* d_grid, uint32_t* d_usage, bool zero_usage,
-              uint8_t* d_stage, float* d_dist, float* d_score, int32_t* d_stats) {
+              uint8_t* d_stage, float* d_dist, float* d_score, int32_t* d_stats,
+              int32_t* cidx_buf = nullptr, float4* clab_buf = nullptr) {
     const int32_t cells = gw * gh;
     const size_t lds = raster ? place_raster_lds_bytes(gw, lib->n) : place_lds_bytes(cells, lib->n);
     if (lds > ctx->max_lds)
         return fail(MH_ERR_UNSUPPORTED,
                     "placement state (%d cells + %d materials) needs %zu B of LDS, limit %zu",
                     cells, lib->n, lds, ctx->max_lds);
-    int32_t* cidx = nullptr;
-    float4* clab = nullptr;
+    int32_t* cidx = cidx_buf;
+    float4* clab = clab_buf;
     int rc;
-    if ((rc = slot(ctx, S_CAND_IDX, (size_t)cells * kCandStride, &cidx))) return rc;
-    if ((rc = slot(ctx, S_CAND_D, (size_t)cells * kCandStride, &clab))) return rc;
+    if (!cidx && (rc = slot(ctx, S_CAND_IDX, (size_t)cells * kCandStride, &cidx))) return rc;
+    if (!clab && (rc = slot(ctx, S_CAND_D, (size_t)cells * kCandStride, &clab))) return rc;
     const int32_t K = lib->n < kMaxCand ? lib->n : kMaxCand;
     phase_begin(ctx, MH_PHASE_TOPK);
     launch_topk(lib->d_L, lib->d_A, lib->d_B, lib->n, d_cell_lab, cells, K, cidx, nullptr, clab,
@@ -329,24 +332,45 @@ int library_tiles(const mh_library* clib, int32_t tw, int32_t th, hipStream_t st
     return MH_OK;
 }
 
+int library_tiles_cached(const mh_library* clib, int32_t tw, int32_t th, const uint8_t** tiles,
+                         const uint8_t** avg) {
+    mh_library* lib = const_cast<mh_library*>(clib);
+    std::lock_guard<std::mutex> g(lib->cache_mu);
+    auto it = lib->tiles.find(std::make_pair(tw, th));
+    if (it == lib->tiles.end())
+        return fail(MH_ERR_UNSUPPORTED,
+                    "the library holds no resized tiles for %d x %d cells (an mh_generate, "
+                    "mh_generate_device or mh_composite call at this cell size builds them)",
+                    tw, th);
+    *tiles = it->second.tiles;
+    *avg = it->second.avg;
+    return MH_OK;
+}
+
+// temps_ready: the temperature table, already on the device (else the
+// context's cached table).  noalloc: allocate nothing -- the adjacency terms
+// come from the Lab table (bit-identical to the library's term matrix) and
+// temps_ready must be given whenever a proposal can read it.
 int run_anneal(mh_context* ctx, const mh_library* lib, int32_t* d_grid, int32_t gw, int32_t gh,
                const mh_anneal_params* p, float* d_trace_delta, uint8_t* d_trace_accept,
-               AnnealOut* d_out) {
+               AnnealOut* d_out, const float* temps_ready, bool noalloc) {
     const int32_t cells = gw * gh;
     const bool narrow = lib->n < 32768;
     const size_t lds = anneal_lds_bytes(cells, narrow);
     if (lds > ctx->max_lds)
         return fail(MH_ERR_UNSUPPORTED, "annealing state (%d cells) needs %zu B of LDS, limit %zu",
                     cells, lds, ctx->max_lds);
-    const float* temps = nullptr;
+    const float* temps = temps_ready;
     int rc;
-    if (!p->greedy) {
+    if (!p->greedy && !temps && !noalloc) {
         if ((rc = get_temps(ctx, p->initial_temperature, p->temperature_decay, p->max_iterations,
                             &temps)))
             return rc;
     }
+    if (!p->greedy && !temps && p->max_iterations > 0)
+        return fail(MH_ERR_INVALID_ARGUMENT, "annealing without a temperature table");
     const float* term = nullptr;
-    if ((rc = library_terms(lib, ctx->stream, &term))) return rc;
+    if (!noalloc && (rc = library_terms(lib, ctx->stream, &term))) return rc;
     AnnealJob job;
     job.lab4 = lib->d_lab4;
     job.term = term;
@@ -370,15 +394,21 @@ int run_anneal(mh_context* ctx, const mh_library* lib, int32_t* d_grid, int32_t 
 
 
 // tavg_ready: the target cells' average RGB, already computed (by the cell
-// Lab pass), or nullptr to compute it here.
+// Lab pass), or nullptr to compute it here.  adj_buf: room for the cells'
+// adjustments (else the context's scratch).  cached_only: the library's
+// resized tiles for this cell size must exist already (nothing is built).
 int run_composite(mh_context* ctx, const mh_library* lib, const uint8_t* d_target, int32_t width,
                   int32_t height, const int32_t* d_grid, int32_t gw, int32_t gh, float strength,
-                  uint8_t* d_out, const uint8_t* tavg_ready) {
+                  uint8_t* d_out, const uint8_t* tavg_ready, float4* adj_buf, bool cached_only) {
     const int32_t tw = width / gw, th = height / gh;
     if (tw <= 0 || th <= 0) return MH_OK;
     const uint8_t *tiles, *tile_avg;
     int rc;
-    if ((rc = library_tiles(lib, tw, th, ctx->stream, &tiles, &tile_avg))) return rc;
+    if (cached_only) {
+        if ((rc = library_tiles_cached(lib, tw, th, &tiles, &tile_avg))) return rc;
+    } else if ((rc = library_tiles(lib, tw, th, ctx->stream, &tiles, &tile_avg))) {
+        return rc;
+    }
     phase_begin(ctx, MH_PHASE_ADJUST);
     const int32_t cells = gw * gh;
     float4* adj = nullptr;
@@ -392,7 +422,8 @@ int run_composite(mh_context* ctx, const mh_library* lib, const uint8_t* d_targe
                            (int64_t)tw * 3, tw, th, cells, t, ctx->stream);
             tavg = t;
         }
-        if ((rc = slot(ctx, S_ADJ, (size_t)cells, &adj))) return rc;
+        adj = adj_buf;
+        if (!adj && (rc = slot(ctx, S_ADJ, (size_t)cells, &adj))) return rc;
         launch_cell_adjust(d_grid, cells, tile_avg, tavg, strength, adj, ctx->stream);
     }
     phase_end(ctx, MH_PHASE_ADJUST);
@@ -534,6 +565,11 @@ int mh_context_create(int32_t device, mh_context** out) {
         mh_context_destroy(ctx);
         return fail(MH_ERR_HIP, "raising the dynamic LDS limit to %zu B: %s", ctx->max_lds,
                     hipGetErrorString(e));
+    }
+    // this device's copy of the allocation-free entry point's scratch array
+    if (const int rc = noalloc_scratch_init(ctx)) {
+        mh_context_destroy(ctx);
+        return rc;
     }
     *out = ctx;
     return MH_OK;
@@ -1253,7 +1289,7 @@ int mh_anneal(mh_context* ctx, const mh_library* lib, int32_t* grid, int32_t gw,
     CHK(slot(ctx, S_ANNEAL_OUT, 1, &d_out));
     if (trace_delta) CHK(slot(ctx, S_OUT0, iters, &d_td));
     if (trace_accept) CHK(slot(ctx, S_OUT1, iters, &d_ta));
-    CHK(run_anneal(ctx, lib, d_grid, gw, gh, p, d_td, d_ta, d_out));
+    CHK(run_anneal(ctx, lib, d_grid, gw, gh, p, d_td, d_ta, d_out, nullptr, false));
     AnnealOut o;
     CHK(download(ctx, grid, d_grid, (size_t)cells));
     CHK(download(ctx, &o, d_out, 1));
@@ -1346,15 +1382,19 @@ int mh_composite(mh_context* ctx, const mh_library* lib, const uint8_t* target, 
     CHK(upload(ctx, S_IN0, target, (size_t)width * height * 3, &d_t));
     CHK(upload(ctx, S_GRID, grid, (size_t)cells, &d_grid));
     CHK(slot(ctx, S_OUT0, out_bytes, &d_out));
-    CHK(run_composite(ctx, lib, d_t, width, height, d_grid, gw, gh, strength, d_out, nullptr));
+    CHK(run_composite(ctx, lib, d_t, width, height, d_grid, gw, gh, strength, d_out, nullptr, nullptr,
+                      false));
     CHK(download(ctx, out, d_out, out_bytes));
     return sync(ctx);
 }
 
 // -------------------------------------------------------------- whole job
-int mh_generate_device(mh_context* ctx, const mh_library* lib, const uint8_t* d_target,
-                       int32_t width, int32_t height, const mh_job_params* p,
-                       int32_t* d_grid_out, uint8_t* d_image_out) {
+}  // extern "C"
+
+namespace {
+
+int check_job_params(mh_context* ctx, const mh_library* lib, const uint8_t* d_target, int32_t width,
+                     int32_t height, const mh_job_params* p, const uint8_t* d_image_out) {
     if (!ctx || !lib || !p || !d_target) return fail(MH_ERR_INVALID_ARGUMENT, "mh_generate: NULL argument");
     CHK(check_geometry(p->grid_w, p->grid_h));
     if (p->max_usage < 0) return fail(MH_ERR_INVALID_ARGUMENT, "max_usage must be >= 0");
@@ -1362,42 +1402,77 @@ int mh_generate_device(mh_context* ctx, const mh_library* lib, const uint8_t* d_
         return fail(MH_ERR_INVALID_ARGUMENT, "optimization_iterations < 0");
     if (!isfinite(p->weight)) return fail(MH_ERR_INVALID_ARGUMENT, "weight must be finite");
     CHK(check_score_mode(p->score_mode));
-    const int32_t gw = p->grid_w, gh = p->grid_h, cells = gw * gh;
-    const int32_t tw = width / gw, th = height / gh;
+    const int32_t tw = width / p->grid_w, th = height / p->grid_h;
     if (tw <= 0 || th <= 0)
         return fail(MH_ERR_INVALID_ARGUMENT, "image %d x %d smaller than grid %d x %d", width, height,
-                    gw, gh);
+                    p->grid_w, p->grid_h);
     if (d_image_out && !lib->d_pixels)
         return fail(MH_ERR_INVALID_ARGUMENT, "compositing needs a library with pixels");
-    CHK(set_device(ctx));
-    float* d_lab;
-    int32_t* d_grid = d_grid_out;
-    uint32_t* d_usage;
-    int32_t* d_stats;
-    AnnealOut* d_aout;
-    CHK(slot(ctx, S_CELL_LAB, (size_t)cells * 3, &d_lab));
-    if (!d_grid) CHK(slot(ctx, S_GRID, (size_t)cells, &d_grid));
-    CHK(slot(ctx, S_USAGE, (size_t)(lib->n > 0 ? lib->n : 1), &d_usage));
-    CHK(slot(ctx, S_STATS, 8, &d_stats));
-    CHK(slot(ctx, S_ANNEAL_OUT, 1, &d_aout));
+    return MH_OK;
+}
+
+// Device working memory of one job (all device pointers).  temps is only
+// read when `own_temps` is set (the allocation-free form fills it itself).
+struct JobBufs {
+    float* cell_lab = nullptr;
+    int32_t* cand_idx = nullptr;
+    float4* cand_lab = nullptr;
+    uint32_t* usage = nullptr;
+    int32_t* stats = nullptr;
+    AnnealOut* aout = nullptr;
+    uint8_t* tavg = nullptr;
+    float4* adj = nullptr;
+    int32_t* grid = nullptr;
+    float* temps = nullptr;
+};
+
+// T_0 = T0, T_{k+1} = T_k * decay in f32 (src/optimizer.rs:54, :102): the
+// sequential recurrence, one lane (the host table get_temps builds holds the
+// same values).
+__global__ __launch_bounds__(64) void temps_kernel(float t0, float decay, int64_t len,
+                                                  float* __restrict__ out) {
+    if (threadIdx.x != 0) return;
+    float v = t0;
+    for (int64_t i = 0; i < len; ++i) {
+        out[i] = v;
+        v = v * decay;
+    }
+}
+
+// generate_mosaic minus file I/O on the context stream into `b`.  noalloc:
+// no device allocation anywhere (no context scratch, term matrix, cached
+// temperature table or tile build; no phase events).
+int enqueue_generate(mh_context* ctx, const mh_library* lib, const uint8_t* d_target, int32_t width,
+                     int32_t height, const mh_job_params* p, const JobBufs& b,
+                     uint8_t* d_image_out, bool noalloc) {
+    const int32_t gw = p->grid_w, gh = p->grid_h, cells = gw * gh;
+    const int32_t tw = width / gw, th = height / gh;
+    const uint32_t timing = ctx->timing;
+    if (noalloc) ctx->timing = 0;
+    struct Restore {
+        mh_context* c;
+        uint32_t t;
+        ~Restore() { c->timing = t; }
+    } restore{ctx, timing};
     // a1: target cell Lab means (src/main.rs:536-541); the same pixel pass
     // gives the cells' average RGB for the colour adjustment (a13)
     float strength = p->color_strength;
     strength = strength < 0.0f ? 0.0f : (strength > 1.0f ? 1.0f : strength);  // src/main.rs:162
-    uint8_t* tavg = nullptr;
-    if (d_image_out && strength > 0.0f) CHK(slot(ctx, S_TARGET_AVG, (size_t)cells * 3, &tavg));
+    uint8_t* tavg = d_image_out && strength > 0.0f ? b.tavg : nullptr;
     phase_begin(ctx, MH_PHASE_CELL_LAB);
     launch_cell_lab(d_target, (int64_t)width * 3, gw, (int64_t)th * width * 3, (int64_t)tw * 3, tw,
-                    th, cells, ctx->d_lut, d_lab, ctx->stream, tavg);
+                    th, cells, ctx->d_lut, b.cell_lab, ctx->stream, tavg);
     phase_end(ctx, MH_PHASE_CELL_LAB);
     // a3-a9: candidates + placement chain (raster order, empty grid, zero usage)
-    CHK(run_place(ctx, lib, d_lab, gw, gh, p->max_usage, p->weight, p->score_mode, true, nullptr,
-                  cells, d_grid,
-                  d_usage, true, nullptr, nullptr, nullptr, d_stats));
+    CHK(run_place(ctx, lib, b.cell_lab, gw, gh, p->max_usage, p->weight, p->score_mode, true,
+                  nullptr, cells, b.grid, b.usage, true, nullptr, nullptr, nullptr, b.stats,
+                  b.cand_idx, b.cand_lab));
     // a10-a12: SA when enabled and weight > 0 (src/main.rs:608)
     const bool sa = p->enable_optimization && p->weight > 0.0f;
     ctx->last_ran_anneal = sa ? 1 : 0;
     ctx->last_iters = p->optimization_iterations;
+    ctx->last_stats = b.stats;
+    ctx->last_aout = b.aout;
     if (sa) {
         mh_anneal_params ap;
         ap.max_iterations = p->optimization_iterations;
@@ -1405,16 +1480,134 @@ int mh_generate_device(mh_context* ctx, const mh_library* lib, const uint8_t* d_
         ap.temperature_decay = p->temperature_decay;
         ap.seed = p->seed;
         ap.greedy = 0;
-        CHK(run_anneal(ctx, lib, d_grid, gw, gh, &ap, nullptr, nullptr, d_aout));
+        if (noalloc && ap.max_iterations > 0)
+            hipLaunchKernelGGL(temps_kernel, dim3(1), dim3(64), 0, ctx->stream,
+                               ap.initial_temperature, ap.temperature_decay, ap.max_iterations,
+                               b.temps);
+        CHK(run_anneal(ctx, lib, b.grid, gw, gh, &ap, nullptr, nullptr, b.aout,
+                       noalloc ? b.temps : nullptr, noalloc));
     }
     // a13-a16: composite
     if (d_image_out)
-        CHK(run_composite(ctx, lib, d_target, width, height, d_grid, gw, gh, strength, d_image_out,
-                          tavg));
+        CHK(run_composite(ctx, lib, d_target, width, height, b.grid, gw, gh, strength, d_image_out,
+                          tavg, b.adj, noalloc));
     ctx->last_out_w = gw * tw;
     ctx->last_out_h = gh * th;
     HIPCHK(hipGetLastError());
     return MH_OK;
+}
+
+// ------------------------------------------------ allocation-free job
+// Working memory of mh_generate_device_noalloc: one array per GPU, held from
+// the first load of this library until the process ends.  Host code never
+// uses the symbol's own name as an address: mh_context_create looks up each
+// device's copy with hipGetSymbolAddress (with that device current) and
+// every launch takes that address.
+static __device__ __align__(256) uint8_t scratch[MH_NOALLOC_SCRATCH_BYTES];
+
+std::mutex g_scratch_mu;
+std::map<int32_t, uint8_t*> g_scratch;  // device -> its copy of `scratch`
+
+int scratch_address(int32_t device, uint8_t** out) {
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    auto it = g_scratch.find(device);
+    if (it == g_scratch.end()) {
+        uint8_t* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(MH_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device,
+                        hipGetErrorString(e));
+        it = g_scratch.emplace(device, ptr).first;
+    }
+    *out = it->second;
+    return MH_OK;
+}
+
+// The job's working memory carved from the scratch array (256-byte aligned
+// pieces); bytes needed in *need.
+void carve(uint8_t* base, int32_t cells, int32_t n, bool adjust, bool temps, int64_t iters,
+           bool own_grid, JobBufs* b, size_t* need) {
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        uint8_t* at = base ? base + o : nullptr;
+        o = (o + bytes + 255) & ~(size_t)255;
+        return at;
+    };
+    const size_t c = (size_t)cells;
+    b->cell_lab = reinterpret_cast<float*>(take(c * 3 * sizeof(float)));
+    b->cand_idx = reinterpret_cast<int32_t*>(take(c * kCandStride * sizeof(int32_t)));
+    b->cand_lab = reinterpret_cast<float4*>(take(c * kCandStride * sizeof(float4)));
+    b->usage = reinterpret_cast<uint32_t*>(take((size_t)(n > 0 ? n : 1) * sizeof(uint32_t)));
+    b->stats = reinterpret_cast<int32_t*>(take(8 * sizeof(int32_t)));
+    b->aout = reinterpret_cast<AnnealOut*>(take(sizeof(AnnealOut)));
+    b->tavg = adjust ? take(c * 3) : nullptr;
+    b->adj = adjust ? reinterpret_cast<float4*>(take(c * sizeof(float4))) : nullptr;
+    b->temps = temps ? reinterpret_cast<float*>(take((size_t)iters * sizeof(float))) : nullptr;
+    b->grid = own_grid ? reinterpret_cast<int32_t*>(take(c * sizeof(int32_t))) : nullptr;
+    *need = o;
+}
+
+}  // namespace
+
+namespace mh {
+namespace host {
+int noalloc_scratch_init(mh_context* ctx) { return scratch_address(ctx->device, &ctx->noalloc_scratch); }
+}  // namespace host
+}  // namespace mh
+
+extern "C" {
+
+int mh_generate_device(mh_context* ctx, const mh_library* lib, const uint8_t* d_target,
+                       int32_t width, int32_t height, const mh_job_params* p,
+                       int32_t* d_grid_out, uint8_t* d_image_out) {
+    CHK(check_job_params(ctx, lib, d_target, width, height, p, d_image_out));
+    CHK(set_device(ctx));
+    const int32_t cells = p->grid_w * p->grid_h;
+    JobBufs b;
+    b.grid = d_grid_out;
+    CHK(slot(ctx, S_CELL_LAB, (size_t)cells * 3, &b.cell_lab));
+    if (!b.grid) CHK(slot(ctx, S_GRID, (size_t)cells, &b.grid));
+    CHK(slot(ctx, S_USAGE, (size_t)(lib->n > 0 ? lib->n : 1), &b.usage));
+    CHK(slot(ctx, S_STATS, 8, &b.stats));
+    CHK(slot(ctx, S_ANNEAL_OUT, 1, &b.aout));
+    CHK(slot(ctx, S_CAND_IDX, (size_t)cells * kCandStride, &b.cand_idx));
+    CHK(slot(ctx, S_CAND_D, (size_t)cells * kCandStride, &b.cand_lab));
+    const float strength = p->color_strength;
+    if (d_image_out && strength > 0.0f) {
+        CHK(slot(ctx, S_TARGET_AVG, (size_t)cells * 3, &b.tavg));
+        CHK(slot(ctx, S_ADJ, (size_t)cells, &b.adj));
+    }
+    return enqueue_generate(ctx, lib, d_target, width, height, p, b, d_image_out, false);
+}
+
+int mh_generate_device_noalloc(mh_context* ctx, const mh_library* lib, const uint8_t* d_target,
+                               int32_t width, int32_t height, const mh_job_params* p,
+                               int32_t* d_grid_out, uint8_t* d_image_out) {
+    CHK(check_job_params(ctx, lib, d_target, width, height, p, d_image_out));
+    CHK(set_device(ctx));
+    if (!ctx->noalloc_scratch)
+        return fail(MH_ERR_HIP, "the scratch array of device %d was not found at set-up",
+                    ctx->device);
+    const int32_t cells = p->grid_w * p->grid_h;
+    const bool adjust = d_image_out && p->color_strength > 0.0f;
+    const bool sa = p->enable_optimization && p->weight > 0.0f && p->optimization_iterations > 0;
+    JobBufs b;
+    size_t need = 0;
+    carve(nullptr, cells, lib->n, adjust, sa, p->optimization_iterations, !d_grid_out, &b, &need);
+    if (need > (size_t)MH_NOALLOC_SCRATCH_BYTES)
+        return fail(MH_ERR_UNSUPPORTED,
+                    "job of %d cells, %d materials and %lld annealing iterations needs %zu bytes "
+                    "of working memory; the allocation-free entry point holds %llu",
+                    cells, lib->n, (long long)p->optimization_iterations, need,
+                    (unsigned long long)MH_NOALLOC_SCRATCH_BYTES);
+    if (d_image_out) {  // refuse before anything is enqueued
+        const uint8_t *tiles, *tile_avg;
+        CHK(library_tiles_cached(lib, width / p->grid_w, height / p->grid_h, &tiles, &tile_avg));
+    }
+    carve(ctx->noalloc_scratch, cells, lib->n, adjust, sa, p->optimization_iterations,
+          !d_grid_out, &b, &need);
+    if (d_grid_out) b.grid = d_grid_out;
+    return enqueue_generate(ctx, lib, d_target, width, height, p, b, d_image_out, true);
 }
 
 int mh_generate_device_result(mh_context* ctx, mh_job_result* r) {
@@ -1423,8 +1616,9 @@ int mh_generate_device_result(mh_context* ctx, mh_job_result* r) {
     int32_t st[6];
     AnnealOut o;
     memset(&o, 0, sizeof(o));
-    CHK(download(ctx, st, (const int32_t*)ctx->slots[S_STATS].p, 6));
-    if (ctx->last_ran_anneal) CHK(download(ctx, &o, (const AnnealOut*)ctx->slots[S_ANNEAL_OUT].p, 1));
+    if (!ctx->last_stats) return fail(MH_ERR_INVALID_ARGUMENT, "no job has run on this context");
+    CHK(download(ctx, st, ctx->last_stats, 6));
+    if (ctx->last_ran_anneal) CHK(download(ctx, &o, ctx->last_aout, 1));
     CHK(sync(ctx));
     memset(r, 0, sizeof(*r));
     r->place.placed = st[0];

@@ -96,10 +96,15 @@ struct mh_context {
     // Lanczos taps keyed by (in, out)
     std::map<std::pair<int32_t, int32_t>, mh::host::TapsBuf> taps;
     std::mutex taps_mu;
-    // last device job geometry for mh_generate_device_result
+    // last device job geometry and result buffers for mh_generate_device_result
     int32_t last_out_w = 0, last_out_h = 0;
     int32_t last_ran_anneal = 0;
     int64_t last_iters = 0;
+    const int32_t* last_stats = nullptr;
+    const mh::AnnealOut* last_aout = nullptr;
+    // this device's copy of mh_generate_device_noalloc's scratch array
+    // (hipGetSymbolAddress at context creation)
+    uint8_t* noalloc_scratch = nullptr;
 };
 
 // Material pixels of one size: `count` RGB8 images of w x h stored back to
@@ -159,6 +164,9 @@ int library_terms(const mh_library* lib, hipStream_t stream, const float** out);
 int library_tiles(const mh_library* lib, int32_t tw, int32_t th, hipStream_t stream,
                   const uint8_t** tiles, const uint8_t** avg);
 int sync(mh_context* ctx);
+// Looks up the device's scratch array of the allocation-free entry point
+// (ctx->noalloc_scratch); the context's device must be current.
+int noalloc_scratch_init(mh_context* ctx);
 int check_geometry(int32_t gw, int32_t gh);
 int check_score_mode(int32_t mode);
 void to_result(const AnnealOut& o, mh_anneal_result* r);

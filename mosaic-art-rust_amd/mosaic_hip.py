@@ -40,7 +40,7 @@ EXPORTS = [
     "mh_average_lab", "mh_cell_lab_means", "mh_nearest_n", "mh_place", "mh_total_cost",
     "mh_penalties", "mh_swap_deltas", "mh_anneal", "mh_average_rgb", "mh_optimal_adjustments",
     "mh_adjust_pixels", "mh_resize", "mh_composite", "mh_generate", "mh_generate_device",
-    "mh_generate_device_result", "mh_job_footprint", "mh_batch_create", "mh_batch_set_graph",
+    "mh_generate_device_result", "mh_generate_device_noalloc", "mh_job_footprint", "mh_batch_create", "mh_batch_set_graph",
     "mh_batch_run", "mh_batch_results", "mh_batch_job_outputs", "mh_batch_destroy",
     "mh_batch_tune", "mh_batch_get_tuning", "mh_batch_set_tuning", "mh_server_create", "mh_server_submit", "mh_server_wait", "mh_server_poll", "mh_server_drain",
     "mh_server_job_times", "mh_server_get_stats", "mh_server_destroy",
@@ -199,6 +199,8 @@ def load(path=LIB_PATH):
                                        ctypes.POINTER(JobResult)]),
         "mh_generate_device": (ctypes.c_int, [P, P, P, i32, i32, ctypes.POINTER(JobParams), P, P]),
         "mh_generate_device_result": (ctypes.c_int, [P, ctypes.POINTER(JobResult)]),
+        "mh_generate_device_noalloc": (ctypes.c_int, [P, P, P, i32, i32,
+                                                      ctypes.POINTER(JobParams), P, P]),
         "mh_job_footprint": (i64, [P, ctypes.POINTER(Job)]),
         "mh_batch_create": (ctypes.c_int, [P, P, i32, ctypes.POINTER(P)]),
         "mh_batch_set_graph": (ctypes.c_int, [P, i32]),
@@ -484,6 +486,14 @@ class Context:
         _check(self.lib.mh_generate_device(self.h, lib.h, ctypes.c_void_p(d_target), width, height,
                                            ctypes.byref(params), ctypes.c_void_p(d_grid),
                                            ctypes.c_void_p(d_image) if d_image else None))
+
+    def generate_device_noalloc(self, lib, d_target, width, height, params, d_grid, d_image):
+        """mh_generate_device_noalloc: generate_device with its working memory
+        in the library's static per-GPU scratch array (no device allocation)."""
+        _check(self.lib.mh_generate_device_noalloc(
+            self.h, lib.h, ctypes.c_void_p(d_target), width, height, ctypes.byref(params),
+            ctypes.c_void_p(d_grid) if d_grid else None,
+            ctypes.c_void_p(d_image) if d_image else None))
 
     def generate_device_result(self):
         res = JobResult()
