@@ -205,6 +205,13 @@ void orc_sim_full(const float *lab, int32_t n, float *out) {
         for (int32_t j = 0; j < n; ++j) out[(int64_t)i * n + j] = orc_sim(lab, i, j);
 }
 
+/* Rows `rows[0..nrows)` of the full matrix (out = nrows x n), for sampled
+ * checks of large libraries. */
+void orc_sim_rows(const float *lab, int32_t n, const int32_t *rows, int32_t nrows, float *out) {
+    for (int32_t r = 0; r < nrows; ++r)
+        for (int32_t j = 0; j < n; ++j) out[(int64_t)r * n + j] = orc_sim(lab, rows[r], j);
+}
+
 /* build_similarities, src/similarity.rs:69-82: packed upper triangle in
  * (i, j>i) order; position i*n - i*(i+1)/2 + j - i - 1 (:100). */
 void orc_sim_packed(const float *lab, int32_t n, float *out) {
@@ -286,6 +293,189 @@ void orc_nearest_n(const float *lab, int32_t n, const float *query, int32_t k, i
     free(heap);
 }
 
+/* The same K nearest from a k-d tree, as the reference finds them (kiddo's
+ * KdTree<f32, u64, 3, 256, u32>, src/main.rs:145-151; buckets of 256).  The
+ * answer equals orc_nearest_n's exactly, ties included:
+ *   * every point's key is the same (sqdist, idx) from the same sqdist();
+ *   * a far subtree is skipped only when fl((q_a - split)^2) is strictly
+ *     greater than the current K-th smallest sqdist.  Rounding is monotone,
+ *     so every point beyond the split plane has an f32 sqdist at least that
+ *     bound, hence a key above the K-th smallest: skipping it loses nothing,
+ *     and points at exactly the bound (possible ties) are still visited.
+ * Nodes split at the median of their widest axis; points equal to the split
+ * may sit on either side, which the bound above allows. */
+typedef struct {
+    int32_t axis; /* -1: leaf */
+    float split;
+    int32_t lo, hi;       /* children (inner) */
+    int32_t start, count; /* points[start .. start+count) (leaf) */
+} kd_node;
+
+struct orc_kdtree {
+    const float *lab;
+    int32_t n;
+    int32_t *idx;
+    kd_node *nodes;
+    int32_t n_nodes;
+};
+
+#define KD_BUCKET 256
+
+static float kd_coord(const orc_kdtree *t, int32_t i, int32_t a) { return t->lab[3 * (int64_t)i + a]; }
+
+/* Partial sort of idx[lo..hi) so position k holds its order statistic by
+ * coordinate a (quickselect). */
+static void kd_select(orc_kdtree *t, int32_t lo, int32_t hi, int32_t k, int32_t a) {
+    int32_t *v = t->idx;
+    while (hi - lo > 1) {
+        float pivot = kd_coord(t, v[lo + (hi - lo) / 2], a);
+        int32_t i = lo, j = hi - 1;
+        while (i <= j) {
+            while (kd_coord(t, v[i], a) < pivot) ++i;
+            while (kd_coord(t, v[j], a) > pivot) --j;
+            if (i <= j) {
+                int32_t tmp = v[i];
+                v[i] = v[j];
+                v[j] = tmp;
+                ++i;
+                --j;
+            }
+        }
+        if (k <= j)
+            hi = j + 1;
+        else if (k >= i)
+            lo = i;
+        else
+            return;
+    }
+}
+
+static int32_t kd_build_node(orc_kdtree *t, int32_t start, int32_t count) {
+    int32_t id = t->n_nodes++;
+    kd_node *nd = &t->nodes[id];
+    nd->start = start;
+    nd->count = count;
+    nd->axis = -1;
+    if (count <= KD_BUCKET) return id;
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int32_t i = start; i < start + count; ++i)
+        for (int32_t a = 0; a < 3; ++a) {
+            float c = kd_coord(t, t->idx[i], a);
+            lo[a] = c < lo[a] ? c : lo[a];
+            hi[a] = c > hi[a] ? c : hi[a];
+        }
+    int32_t axis = 0;
+    for (int32_t a = 1; a < 3; ++a)
+        if (hi[a] - lo[a] > hi[axis] - lo[axis]) axis = a;
+    if (!(hi[axis] > lo[axis])) return id; /* all points equal: one leaf */
+    int32_t mid = start + count / 2;
+    kd_select(t, start, start + count, mid, axis);
+    float split = kd_coord(t, t->idx[mid], axis);
+    int32_t l = kd_build_node(t, start, mid - start);
+    int32_t r = kd_build_node(t, mid, start + count - mid);
+    nd = &t->nodes[id]; /* (t->nodes does not move: sized up front) */
+    nd->axis = axis;
+    nd->split = split;
+    nd->lo = l;
+    nd->hi = r;
+    return id;
+}
+
+orc_kdtree *orc_kdtree_build(const float *lab, int32_t n) {
+    orc_kdtree *t = (orc_kdtree *)calloc(1, sizeof(orc_kdtree));
+    t->lab = lab;
+    t->n = n;
+    t->idx = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    for (int32_t i = 0; i < n; ++i) t->idx[i] = i;
+    /* a binary tree over n points with leaves of at most KD_BUCKET points
+     * has fewer than 2n/KD_BUCKET*2 + 1 nodes; 2n + 1 is ample */
+    t->nodes = (kd_node *)malloc(sizeof(kd_node) * (size_t)(2 * (int64_t)n + 1));
+    if (n > 0) kd_build_node(t, 0, n);
+    return t;
+}
+
+void orc_kdtree_free(orc_kdtree *t) {
+    if (!t) return;
+    free(t->idx);
+    free(t->nodes);
+    free(t);
+}
+
+typedef struct {
+    uint64_t *heap; /* max-heap of the best keys so far */
+    int32_t hn, k;
+} kd_query;
+
+static void kd_offer(kd_query *q, uint64_t key) {
+    if (q->hn < q->k) {
+        int32_t c = q->hn++;
+        q->heap[c] = key;
+        while (c > 0) {
+            int32_t p = (c - 1) / 2;
+            if (q->heap[p] >= q->heap[c]) break;
+            uint64_t t = q->heap[p];
+            q->heap[p] = q->heap[c];
+            q->heap[c] = t;
+            c = p;
+        }
+    } else if (key < q->heap[0]) {
+        q->heap[0] = key;
+        heap_sift_down(q->heap, q->hn, 0);
+    }
+}
+
+static void kd_search(const orc_kdtree *t, int32_t id, const float *query, kd_query *q) {
+    const kd_node *nd = &t->nodes[id];
+    if (nd->axis < 0) {
+        for (int32_t i = nd->start; i < nd->start + nd->count; ++i) {
+            int32_t m = t->idx[i];
+            kd_offer(q, fkey(sqdist(query, t->lab + 3 * (int64_t)m), m));
+        }
+        return;
+    }
+    float d = query[nd->axis] - nd->split;
+    int32_t near = d < 0.0f ? nd->lo : nd->hi, far = d < 0.0f ? nd->hi : nd->lo;
+    kd_search(t, near, query, q);
+    float bound = d * d;
+    if (q->hn < q->k) {
+        kd_search(t, far, query, q);
+    } else {
+        uint32_t wbits = (uint32_t)(q->heap[0] >> 32);
+        float worst;
+        memcpy(&worst, &wbits, 4);
+        if (!(bound > worst)) kd_search(t, far, query, q);
+    }
+}
+
+void orc_kdtree_nearest_n(const orc_kdtree *t, const float *query, int32_t k, int32_t *out_idx,
+                          float *out_sqdist) {
+    if (k > t->n) k = t->n;
+    if (k <= 0) return;
+    kd_query q;
+    q.heap = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)k);
+    q.hn = 0;
+    q.k = k;
+    kd_search(t, 0, query, &q);
+    qsort(q.heap, (size_t)q.hn, sizeof(uint64_t), cmp_u64);
+    for (int32_t r = 0; r < q.hn; ++r) {
+        uint32_t bits = (uint32_t)(q.heap[r] >> 32);
+        out_idx[r] = (int32_t)(uint32_t)(q.heap[r] & 0xffffffffu);
+        memcpy(&out_sqdist[r], &bits, 4);
+    }
+    free(q.heap);
+}
+
+/* orc_kdtree_nearest_n for nq queries (out: nq x min(k, n)). */
+void orc_nearest_n_kd(const float *lab, int32_t n, const float *queries, int32_t nq, int32_t k,
+                      int32_t *out_idx, float *out_sqdist) {
+    int32_t kk = k < n ? k : n;
+    orc_kdtree *t = orc_kdtree_build(lab, n);
+    for (int32_t i = 0; i < nq; ++i)
+        orc_kdtree_nearest_n(t, queries + 3 * (int64_t)i, k, out_idx + (int64_t)i * kk,
+                             out_sqdist + (int64_t)i * kk);
+    orc_kdtree_free(t);
+}
+
 /* ------------------------------------------------------------------------ */
 /* a4-a9: placement                                                          */
 /* ------------------------------------------------------------------------ */
@@ -340,6 +530,9 @@ void orc_place(const float *lab, int32_t n, const float *cell_lab, int32_t gw, i
     int32_t K = n < 100 ? n : 100;
     int32_t cidx[100];
     float cd[100];
+    /* the K nearest from a k-d tree, as the reference's kiddo tree
+     * (src/main.rs:145-151); exactly orc_nearest_n's answer */
+    orc_kdtree *tree = orc_kdtree_build(lab, n);
     for (int32_t c = 0; c < cells; ++c) {
         int32_t cell = order ? order[c] : c;
         int32_t x = cell % gw, y = cell / gw;
@@ -347,7 +540,7 @@ void orc_place(const float *lab, int32_t n, const float *cell_lab, int32_t gw, i
             s.empty++;
             continue;
         }
-        orc_nearest_n(lab, n, cell_lab + 3 * (int64_t)cell, K, cidx, cd);
+        orc_kdtree_nearest_n(tree, cell_lab + 3 * (int64_t)cell, K, cidx, cd);
         if (score_mode == 1) /* app_full.rs:1436-1439: sqrt of the same f32 sum */
             for (int32_t r = 0; r < K; ++r) cd[r] = sqrtf(cd[r]);
         int32_t best = -1;
@@ -400,6 +593,7 @@ void orc_place(const float *lab, int32_t n, const float *cell_lab, int32_t gw, i
             s.fallback_near++;
         s.placed++;
     }
+    orc_kdtree_free(tree);
     if (st) *st = s;
 }
 
@@ -812,6 +1006,28 @@ void orc_resize_lanczos3(const uint8_t *src, int32_t sw, int32_t sh, uint8_t *ds
  * :629-661): per filled cell, resize the material to tw x th, and when
  * strength > 0 adjust it towards the target cell's average RGB; write at
  * (x*tw, y*th) into a gw*tw x gh*th image (zero elsewhere, :515-517). */
+/* One cell of the composite (src/main.rs:548-590, :625-664): the resized
+ * tile, colour-adjusted toward the target cell's average RGB when strength
+ * > 0, copied to (x*tw, y*th).  adj_buf holds tw*th*3 bytes. */
+static void composite_cell(const uint8_t *target, int64_t tstride, const uint8_t *tile, int32_t x,
+                           int32_t y, int32_t tw, int32_t th, int32_t ow, float strength,
+                           uint8_t *adj_buf, uint8_t *out) {
+    const uint8_t *src = tile;
+    if (strength > 0.0f) {
+        uint8_t tavg[3], savg[3];
+        float a[4];
+        orc_average_rgb(target + (int64_t)(y * th) * tstride + (int64_t)(x * tw) * 3, tw, th,
+                        tstride, tavg);
+        orc_average_rgb(tile, tw, th, (int64_t)tw * 3, savg);
+        orc_optimal_adjustment(savg, tavg, strength, a);
+        orc_apply_adjustment(a, tile, (int64_t)tw * th, adj_buf);
+        src = adj_buf;
+    }
+    for (int32_t dy = 0; dy < th; ++dy)
+        memcpy(out + ((int64_t)(y * th + dy) * ow + (int64_t)x * tw) * 3,
+               src + (int64_t)dy * tw * 3, (size_t)tw * 3);
+}
+
 void orc_composite(const uint8_t *target, int32_t width, int32_t height, const uint8_t *materials,
                    int32_t n, int32_t mw, int32_t mh, const int32_t *grid, int32_t gw, int32_t gh,
                    float strength, uint8_t *out) {
@@ -833,22 +1049,32 @@ void orc_composite(const uint8_t *target, int32_t width, int32_t height, const u
                 orc_resize_lanczos3(materials + (int64_t)m * mw * mh * 3, mw, mh, tile, tw, th);
                 have[m] = 1;
             }
-            const uint8_t *src = tile;
-            if (strength > 0.0f) {
-                uint8_t tavg[3], savg[3];
-                float a[4];
-                orc_average_rgb(target + (int64_t)(y * th) * tstride + (int64_t)(x * tw) * 3, tw, th,
-                                tstride, tavg);
-                orc_average_rgb(tile, tw, th, (int64_t)tw * 3, savg);
-                orc_optimal_adjustment(savg, tavg, strength, a);
-                orc_apply_adjustment(a, tile, (int64_t)tw * th, adj);
-                src = adj;
-            }
-            for (int32_t dy = 0; dy < th; ++dy)
-                memcpy(out + ((int64_t)(y * th + dy) * ow + (int64_t)x * tw) * 3,
-                       src + (int64_t)dy * tw * 3, (size_t)tw * 3);
+            composite_cell(target, tstride, tile, x, y, tw, th, ow, strength, adj, out);
         }
     free(cache);
     free(have);
+    free(adj);
+}
+
+/* The composite from tiles already resized to the cell size (n x th x tw x 3,
+ * tw = width/gw, th = height/gh): a library prepared once, as a resident
+ * library's cached tiles are. */
+void orc_composite_tiles(const uint8_t *target, int32_t width, int32_t height,
+                         const uint8_t *tiles, const int32_t *grid, int32_t gw, int32_t gh,
+                         float strength, uint8_t *out) {
+    int32_t tw = width / gw, th = height / gh;
+    int32_t ow = gw * tw, oh = gh * th;
+    memset(out, 0, (size_t)ow * (size_t)oh * 3);
+    if (tw <= 0 || th <= 0) return;
+    size_t tile_bytes = (size_t)tw * (size_t)th * 3;
+    uint8_t *adj = (uint8_t *)malloc(tile_bytes);
+    int64_t tstride = (int64_t)width * 3;
+    for (int32_t y = 0; y < gh; ++y)
+        for (int32_t x = 0; x < gw; ++x) {
+            int32_t m = grid[y * gw + x];
+            if (m == ORC_EMPTY) continue;
+            composite_cell(target, tstride, tiles + tile_bytes * (size_t)m, x, y, tw, th, ow,
+                           strength, adj, out);
+        }
     free(adj);
 }

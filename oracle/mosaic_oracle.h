@@ -65,10 +65,20 @@ float orc_lab_distance(const float *lab1, const float *lab2);
 float orc_sim(const float *lab, int32_t i, int32_t j);
 void orc_sim_full(const float *lab, int32_t n, float *out);
 void orc_sim_packed(const float *lab, int32_t n, float *out);
+void orc_sim_rows(const float *lab, int32_t n, const int32_t *rows, int32_t nrows, float *out);
 
 /* ---- a3: K nearest materials by (sqdist, index) ---- */
 void orc_nearest_n(const float *lab, int32_t n, const float *query, int32_t k, int32_t *out_idx,
                    float *out_sqdist);
+/* The same K nearest from a k-d tree (kiddo's structure, src/main.rs:145-151):
+ * identical answers, ties included; orc_place queries one. */
+typedef struct orc_kdtree orc_kdtree;
+orc_kdtree *orc_kdtree_build(const float *lab, int32_t n);
+void orc_kdtree_free(orc_kdtree *t);
+void orc_kdtree_nearest_n(const orc_kdtree *t, const float *query, int32_t k, int32_t *out_idx,
+                          float *out_sqdist);
+void orc_nearest_n_kd(const float *lab, int32_t n, const float *queries, int32_t nq, int32_t k,
+                      int32_t *out_idx, float *out_sqdist);
 
 /* ---- a4-a9: placement ---- */
 typedef struct {
@@ -118,6 +128,9 @@ void orc_resize_lanczos3(const uint8_t *src, int32_t sw, int32_t sh, uint8_t *ds
 void orc_composite(const uint8_t *target, int32_t width, int32_t height, const uint8_t *materials,
                    int32_t n, int32_t mw, int32_t mh, const int32_t *grid, int32_t grid_w,
                    int32_t grid_h, float strength, uint8_t *out);
+void orc_composite_tiles(const uint8_t *target, int32_t width, int32_t height,
+                         const uint8_t *tiles, const int32_t *grid, int32_t gw, int32_t gh,
+                         float strength, uint8_t *out);
 
 #ifdef __cplusplus
 }

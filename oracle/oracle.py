@@ -65,6 +65,9 @@ def load():
         "orc_sim_full": (None, [P, i32, P]),
         "orc_sim_packed": (None, [P, i32, P]),
         "orc_nearest_n": (None, [P, i32, P, i32, P, P]),
+        "orc_nearest_n_kd": (None, [P, i32, P, i32, i32, P, P]),
+        "orc_sim_rows": (None, [P, i32, P, i32, P]),
+        "orc_composite_tiles": (None, [P, i32, i32, P, P, i32, i32, f32, P]),
         "orc_place": (None, [P, i32, P, i32, i32, i32, f32, i32, P, i32, P, P, P, P, P,
                              ctypes.POINTER(PlaceStats)]),
         "orc_total_cost": (f32, [P, i32, P, i32, i32]),
@@ -175,6 +178,27 @@ def nearest_n(lab, query, k):
     for i in range(q.shape[0]):
         L.orc_nearest_n(_p(lab), n, _p(q[i]), kk, _p(idx[i]), _p(d[i]))
     return idx, d
+
+
+def nearest_n_kd(lab, query, k):
+    """nearest_n from the oracle's k-d tree (the reference's structure)."""
+    lab = _c(lab, np.float32).reshape(-1, 3)
+    q = _c(query, np.float32).reshape(-1, 3)
+    n = lab.shape[0]
+    kk = min(k, n)
+    idx = np.zeros((q.shape[0], kk), np.int32)
+    d = np.zeros((q.shape[0], kk), np.float32)
+    load().orc_nearest_n_kd(_p(lab), n, _p(q), q.shape[0], k, _p(idx), _p(d))
+    return idx, d
+
+
+def sim_rows(lab, rows):
+    """Rows `rows` of the full similarity matrix (len(rows) x n)."""
+    lab = _c(lab, np.float32).reshape(-1, 3)
+    r = _c(rows, np.int32)
+    out = np.zeros((r.size, lab.shape[0]), np.float32)
+    load().orc_sim_rows(_p(lab), lab.shape[0], _p(r), r.size, _p(out))
+    return out
 
 
 def place(lab, cell_lab, grid_w, grid_h, max_usage, weight, order=None, grid=None, usage=None,
@@ -320,4 +344,114 @@ def generate(target, materials, grid_w, grid_h, max_usage, weight, color_strengt
     strength = min(max(color_strength, 0.0), 1.0)
     img = composite(target, materials, grid, grid_w, grid_h, strength) if image else None
     return {"grid": grid, "image": img, "place": pl, "anneal": ann, "lab": lab,
+            "cell_lab": cell_lab}
+
+
+# ----------------------------------------------------------- prepared jobs
+def _pool(fn, items, threads):
+    """fn over items on `threads` threads (the C calls release the GIL)."""
+    import concurrent.futures
+    if threads <= 1 or len(items) <= 1:
+        return [fn(x) for x in items]
+    with concurrent.futures.ThreadPoolExecutor(threads) as ex:
+        return list(ex.map(fn, items))
+
+
+def stack_lab_means_par(materials, threads=8):
+    """stack_lab_means over chunks of materials on several threads (the
+    same values: each material is computed on its own)."""
+    m = _c(materials, np.uint8)
+    n = m.shape[0]
+    step = max(1, (n + threads - 1) // threads)
+    parts = _pool(lambda s: stack_lab_means(m[s:s + step]), list(range(0, n, step)), threads)
+    return np.concatenate(parts) if parts else np.zeros((0, 3), np.float32)
+
+
+def cell_lab_means_par(target, grid_w, grid_h, threads=8):
+    """cell_lab_means over bands of cell rows on several threads (each band is
+    the same cells of the same pixels)."""
+    t = _c(target, np.uint8)
+    th = t.shape[0] // grid_h
+    step = max(1, (grid_h + threads - 1) // threads)
+    bands = list(range(0, grid_h, step))
+    parts = _pool(lambda y0: cell_lab_means(t[y0 * th:min(grid_h, y0 + step) * th], grid_w,
+                                            min(grid_h, y0 + step) - y0), bands, threads)
+    return np.concatenate(parts)
+
+
+def resize_tiles(materials, tw, th, threads=8):
+    """Every material resized to tw x th (n x th x tw x 3), once per cell
+    size, as a resident library caches them."""
+    m = _c(materials, np.uint8)
+    n = m.shape[0]
+    out = np.zeros((n, th, tw, 3), np.uint8)
+    step = max(1, (n + threads - 1) // threads)
+
+    def part(s):
+        L = load()
+        mh, mw = m.shape[1:3]
+        for i in range(s, min(n, s + step)):
+            L.orc_resize_lanczos3(_p(m[i]), mw, mh, _p(out[i]), tw, th)
+
+    _pool(part, list(range(0, n, step)), threads)
+    return out
+
+
+def composite_tiles(target, tiles, grid, grid_w, grid_h, strength):
+    """composite from prepared tiles (n x th x tw x 3)."""
+    t = _c(target, np.uint8)
+    tl = _c(tiles, np.uint8)
+    h, w = t.shape[:2]
+    tw, th = w // grid_w, h // grid_h
+    out = np.zeros((grid_h * th, grid_w * tw, 3), np.uint8)
+    g = _c(grid, np.int32)
+    load().orc_composite_tiles(_p(t), w, h, _p(tl), _p(g), grid_w, grid_h, strength, _p(out))
+    return out
+
+
+def prepare_library(materials, cell_sizes=(), threads=8):
+    """A library prepared once, as the GPU's resident library is: material
+    Lab means (calculate_average_lab of each, src/main.rs:263-301) and the
+    materials resized to each cell size (tw, th)."""
+    lab = stack_lab_means_par(materials, threads)
+    return {"lab": lab, "tiles": {tuple(c): resize_tiles(materials, c[0], c[1], threads)
+                                   for c in cell_sizes}}
+
+
+def generate_prepared(target, lib, grid_w, grid_h, max_usage, weight, color_strength,
+                      enable_optimization=True, optimization_iterations=1000,
+                      initial_temperature=100.0, temperature_decay=0.99995, seed=0, image=True,
+                      score_mode=0, times=None, threads=1):
+    """generate (src/main.rs:529-665) on a prepared library: the per-job work
+    only.  `times`, when a dict, accumulates seconds per phase: cell_lab,
+    place (K nearest + chain), anneal, composite."""
+    import time
+    h, w = target.shape[:2]
+    tw, th = w // grid_w, h // grid_h
+    t0 = time.perf_counter()
+    cell_lab = (cell_lab_means_par(target, grid_w, grid_h, threads) if threads > 1
+                else cell_lab_means(target, grid_w, grid_h))
+    t1 = time.perf_counter()
+    pl = place(lib["lab"], cell_lab, grid_w, grid_h, max_usage, weight, score_mode=score_mode)
+    grid = pl["grid"]
+    t2 = time.perf_counter()
+    ann = None
+    if enable_optimization and weight > 0.0:
+        ann = anneal(lib["lab"], grid, grid_w, grid_h, optimization_iterations,
+                     initial_temperature, temperature_decay, seed)
+        grid = ann["grid"]
+    t3 = time.perf_counter()
+    strength = min(max(color_strength, 0.0), 1.0)
+    img = None
+    if image:
+        tiles = lib["tiles"].get((tw, th))
+        if tiles is None:
+            raise KeyError(f"library not prepared for {tw}x{th} cells")
+        img = composite_tiles(target, tiles, grid, grid_w, grid_h, strength)
+    t4 = time.perf_counter()
+    if times is not None:
+        for k, v in (("cell_lab", t1 - t0), ("place", t2 - t1), ("anneal", t3 - t2),
+                     ("composite", t4 - t3)):
+            times[k] = times.get(k, 0.0) + v
+    return {"grid": grid, "image": img, "place": pl, "anneal": ann, "lab": lib["lab"],
             "cell_lab": cell_lab}

@@ -2,8 +2,8 @@
 
 Every job of a fused batch or of the request loop must give exactly what
 mh_generate_device gives for the same job alone (grid, composited image,
-placement counters, annealing result), which the parity tests pin to the
-oracle.  Jobs mix grid sizes, libraries (including one with fewer than K=100
+placement counters, annealing result), and what the oracle's generate_mosaic
+(src/main.rs:477-685) gives for it.  Jobs mix grid sizes, libraries (including one with fewer than K=100
 materials and an empty one), SA on/off, colour strength 0 and weight 0, and
 read their targets as crops of shared canvases (row pitch != width*3);
 every third job places with the GUI generator's Euclidean score.
@@ -86,6 +86,34 @@ class Setup:
             ctx.device_free(p)
         return grid, img, res
 
+    def oracle(self, k, orc):
+        """The oracle's whole job on the same crop and materials (cached)."""
+        if not hasattr(self, "_want"):
+            self._want = {}
+        if k not in self._want:
+            j = self.specs[k]
+            c = self.canvases[j["canvas"]][j["oy"]:j["oy"] + j["height"],
+                                           j["ox"]:j["ox"] + j["width"]]
+            mats = (self.st["libs"][j["lib"]] if j["lib"] < len(self.st["libs"])
+                    else np.zeros((0, 64, 64, 3), np.uint8))
+            self._want[k] = orc.generate(np.ascontiguousarray(c), mats, j["grid_w"], j["grid_h"],
+                                         j["max_usage"], j["weight"], j["color"], bool(j["sa"]),
+                                         j["iters"], 100.0, 0.99995, j["seed"],
+                                         score_mode=j.get("score_mode", 0))
+        return self._want[k]
+
+    def check_oracle(self, k, orc, grid, img, res):
+        w = self.oracle(k, orc)
+        np.testing.assert_array_equal(grid, w["grid"])
+        np.testing.assert_array_equal(img, w["image"])
+        assert res["place"] == w["place"]["stats"]
+        if w["anneal"] is not None:
+            a, b = res["anneal"], w["anneal"]["result"]
+            for key in ("initial_cost", "final_cost", "best_cost"):
+                assert np.float32(a[key]).view(np.uint32) == np.float32(b[key]).view(np.uint32), key
+            for key in ("improved_count", "accepted_count", "iterations", "evaluated"):
+                assert a[key] == b[key], key
+
     def fetch(self, k, d_g, d_i):
         j = self.specs[k]
         grid = self.ctx.to_host(d_g, j["grid_w"] * j["grid_h"], np.int32)
@@ -109,7 +137,7 @@ def _same(want, grid, img, res, sa):
         assert res["anneal"] == wr["anneal"]
 
 
-def test_batch_matches_jobs_alone(ctx, mh, synth):
+def test_batch_matches_jobs_alone(ctx, mh, synth, orc):
     s = Setup(ctx, mh, synth)
     jobs = [s.job(k, own_outputs=(k % 2 == 1)) for k in range(len(s.specs))]
     b = ctx.batch(jobs)
@@ -122,6 +150,7 @@ def test_batch_matches_jobs_alone(ctx, mh, synth):
         g, im = s.fetch(k, *b.outputs(k))
         sa = s.specs[k]["sa"] and s.specs[k]["weight"] > 0
         _same(s.alone(k), g, im, res[k], sa)
+        s.check_oracle(k, orc, g, im, res[k])
     b.close()
     s.close()
 
@@ -152,7 +181,7 @@ def test_batch_graph_replays(ctx, mh, synth):
     s.close()
 
 
-def test_server_results_equal_alone_under_memory_pressure(ctx, mh, synth):
+def test_server_results_equal_alone_under_memory_pressure(ctx, mh, synth, orc):
     s = Setup(ctx, mh, synth, n_jobs=16, seed=7)
     jobs = [s.job(k, own_outputs=True) for k in range(len(s.specs))]
     need = max(ctx.job_footprint(j) for j in jobs)
@@ -170,6 +199,7 @@ def test_server_results_equal_alone_under_memory_pressure(ctx, mh, synth):
         assert got[k]["latency_ms"] >= 0.0
         g, im = s.fetch(k, jobs[k].d_grid_out, jobs[k].d_image_out)
         _same(s.alone(k), g, im, got[k], s.specs[k]["sa"] and s.specs[k]["weight"] > 0)
+        s.check_oracle(k, orc, g, im, got[k])
     s.close()
 
 
@@ -217,7 +247,7 @@ def test_handles_outlive_their_context(mh, synth):
         lib.close()
 
 
-def test_batch_narrow_grids(ctx, mh, synth):
+def test_batch_narrow_grids(ctx, mh, synth, orc):
     # grids of one to three columns (the one-cell-at-a-time chain below three
     # columns, the look-ahead chain from three) in one batch, both scores
     s = Setup(ctx, mh, synth, n_jobs=6, seed=11)
@@ -236,6 +266,7 @@ def test_batch_narrow_grids(ctx, mh, synth):
         print(f"narrow batch job {k}: {s.specs[k]['grid_w']}x{s.specs[k]['grid_h']}", flush=True)
         g, im = s.fetch(k, *b.outputs(k))
         _same(s.alone(k), g, im, res[k], s.specs[k]["sa"] and s.specs[k]["weight"] > 0)
+        s.check_oracle(k, orc, g, im, res[k])
     b.close()
     s.close()
 

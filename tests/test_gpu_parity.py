@@ -422,6 +422,26 @@ def test_adjust_pixels(ctx, orc):
         np.testing.assert_array_equal(got, want)
 
 
+def test_adjust_pixels_u8_conversion_truncates(ctx):
+    # The brightness/contrast-only path (no HSV) over all 256 levels against
+    # a numpy float32 closed form of src/color_adjustment.rs:82-96, which ends
+    # in Rust's truncating `as u8`; independent of the oracle.
+    f = np.float32
+    levels = np.arange(256, dtype=np.uint8)
+    px = np.repeat(levels[:, None], 3, axis=1)
+    for brightness, contrast in ((0.1, 1.0), (-0.139, 1.0), (0.0, 1.3), (0.05, 0.77),
+                                 (0.3, 1.9), (-0.42, 0.5)):
+        v = levels.astype(f) / f(255.0)
+        c = np.clip((v - f(0.5)) * f(contrast) + f(0.5), f(0.0), f(1.0)).astype(f)
+        c = np.clip(c + f(brightness), f(0.0), f(1.0)).astype(f)
+        t = np.clip(c * f(255.0), f(0.0), f(255.0)).astype(f)
+        want = np.trunc(t).astype(np.uint8)
+        assert np.count_nonzero(np.rint(t) != np.trunc(t)) > 20  # separates the two
+        got = ctx.adjust_pixels(np.array([brightness, contrast, 0.0, 1.0], np.float32), px)
+        for ch in range(3):
+            np.testing.assert_array_equal(got[:, ch], want, err_msg=f"b={brightness} c={contrast}")
+
+
 @pytest.mark.parametrize("sw,sh,dw,dh", [(64, 64, 16, 16), (128, 128, 24, 18), (64, 64, 20, 20),
                                          (100, 100, 50, 50), (100, 100, 200, 100), (7, 5, 13, 3),
                                          (256, 256, 30, 20), (61, 47, 19, 23)])

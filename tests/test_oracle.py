@@ -153,6 +153,65 @@ def test_nearest_one_known_answer(orc):
                     [32.30, 79.20, -107.86]], np.float32)
     idx, _ = orc.nearest_n(lab, np.array([53.0, 80.0, 67.0], np.float32), 1)
     assert idx[0, 0] == 1
+    kidx, _ = orc.nearest_n_kd(lab, np.array([53.0, 80.0, 67.0], np.float32), 1)
+    assert kidx[0, 0] == 1
+
+
+@pytest.mark.parametrize("n", [0, 1, 7, 100, 101, 256, 257, 1500, 10000])
+def test_kd_tree_nearest_equals_brute_force(orc, n):
+    # The placement queries a k-d tree (the reference's kiddo structure,
+    # src/main.rs:145-151); its K nearest must be the brute-force lexsort
+    # answer exactly, including exact ties (integer coordinates, duplicated
+    # points, queries on split planes) and the order among them.
+    rng = _rng(n + 17)
+    lab = np.round(rng.uniform(-30, 30, (n, 3))).astype(np.float32)
+    if n > 10:
+        lab[n // 2:n // 2 + 5] = lab[3]  # duplicates
+    q = np.concatenate([np.round(rng.uniform(-35, 35, (150, 3))),
+                        rng.uniform(-35, 35, (150, 3)),
+                        lab[:min(n, 20)]]).astype(np.float32)
+    for k in (1, 7, 100):
+        bi, bd = orc.nearest_n(lab, q, k) if n else (np.zeros((q.shape[0], 0), np.int32),) * 2
+        ki, kd = orc.nearest_n_kd(lab, q, k)
+        np.testing.assert_array_equal(ki, bi)
+        if n:
+            np.testing.assert_array_equal(kd.view(np.uint32), bd.view(np.uint32))
+
+
+def test_kd_tree_clustered_library(orc, synth):
+    # dense mixture clusters with near-ties, as the large library has
+    rng = _rng(31)
+    cols = synth.mixture_colours(4000, rng)
+    lab = np.stack([orc.rgb8_to_lab(*np.clip(np.rint(c), 0, 255).astype(int)) for c in cols])
+    q = lab[rng.integers(0, 4000, 300)] + rng.normal(0, 0.5, (300, 3)).astype(np.float32)
+    bi, bd = orc.nearest_n(lab, q, 100)
+    ki, kd = orc.nearest_n_kd(lab, q, 100)
+    np.testing.assert_array_equal(ki, bi)
+    np.testing.assert_array_equal(kd.view(np.uint32), bd.view(np.uint32))
+
+
+def test_sim_rows_are_rows_of_the_full_matrix(orc):
+    lab = _rng(33).uniform(-50, 100, (57, 3)).astype(np.float32)
+    rows = np.array([0, 5, 56, 5], np.int32)
+    np.testing.assert_array_equal(orc.sim_rows(lab, rows), orc.sim_full(lab)[rows])
+
+
+def test_prepared_job_equals_the_whole_job(orc, synth):
+    # a library prepared once (Lab means, tiles resized per cell size) gives
+    # the whole job's result: the per-job work the CPU baseline times
+    p, target, mats = synth.make("small")
+    prep = orc.prepare_library(mats, [(16, 16)], threads=4)
+    np.testing.assert_array_equal(prep["lab"], orc.stack_lab_means(mats))
+    np.testing.assert_array_equal(orc.cell_lab_means_par(target, 40, 30, threads=3),
+                                  orc.cell_lab_means(target, 40, 30))
+    times = {}
+    a = orc.generate_prepared(target, prep, 40, 30, 3, 0.25, 0.4, True, 500, seed=2, times=times)
+    b = orc.generate(target, mats, 40, 30, 3, 0.25, 0.4, True, 500, seed=2)
+    np.testing.assert_array_equal(a["grid"], b["grid"])
+    np.testing.assert_array_equal(a["image"], b["image"])
+    assert a["place"]["stats"] == b["place"]["stats"]
+    assert a["anneal"]["result"] == b["anneal"]["result"]
+    assert set(times) == {"cell_lab", "place", "anneal", "composite"}
 
 
 # ------------------------------------------------------------ a4-a9 place
@@ -474,6 +533,45 @@ def test_adjust_pixel_reference_cases(orc):
     ext = orc.adjustment_new(0.5, 1.5, 60.0, 1.5)  # :476-498
     assert np.all(orc.adjust_pixel(ext, [255, 255, 255]) >= 250)
     assert np.all(orc.adjust_pixel(ext, [0, 0, 0]) > 0)
+
+
+def _bc_closed_form(levels, brightness, contrast):
+    """adjust_pixel with hue_shift 0 and saturation 1 (no HSV round trip),
+    src/color_adjustment.rs:53-96, restated in numpy float32 one rounded
+    operation at a time: v = x/255; c = clamp((v - 0.5)*contrast + 0.5, 0, 1);
+    c = clamp(c + brightness, 0, 1); then `(c * 255).clamp(0, 255) as u8`,
+    Rust's saturating float->int cast, which truncates toward zero.  Returns
+    the u8 result and the pre-cast float."""
+    f = np.float32
+    v = levels.astype(f) / f(255.0)
+    c = np.clip((v - f(0.5)) * f(contrast) + f(0.5), f(0.0), f(1.0)).astype(f)
+    c = np.clip(c + f(brightness), f(0.0), f(1.0)).astype(f)
+    t = np.clip(c * f(255.0), f(0.0), f(255.0)).astype(f)
+    return np.trunc(t).astype(np.uint8), t
+
+
+def test_adjust_pixel_u8_conversion_truncates(orc):
+    # All 256 levels on the brightness/contrast-only path against a numpy
+    # closed form of src/color_adjustment.rs:82-96.  The cases are chosen so
+    # that rounding and truncation disagree on many levels: an oracle that
+    # rounded (or took floor(t + 0.5)) fails here.
+    levels = np.arange(256, dtype=np.uint8)
+    px = np.repeat(levels[:, None], 3, axis=1)
+    disagree = 0
+    for brightness, contrast in ((0.1, 1.0), (-0.139, 1.0), (0.0, 1.3), (0.05, 0.77),
+                                 (0.3, 1.9), (-0.42, 0.5), (0.0, 1.0)):
+        adj = orc.adjustment_new(brightness, contrast, 0.0, 1.0)
+        want, t = _bc_closed_form(levels, adj[0], adj[1])
+        got = orc.apply_adjustment(adj, px)
+        np.testing.assert_array_equal(got[:, 0], want, err_msg=f"b={brightness} c={contrast}")
+        np.testing.assert_array_equal(got[:, 1], want)
+        np.testing.assert_array_equal(got[:, 2], want)
+        disagree += int(np.count_nonzero(np.rint(t) != np.trunc(t)))
+    assert disagree > 200  # the cases separate truncation from rounding
+    # identity: x/255*255 truncates back to x for every level (the reference's
+    # identity test allows +-1, src/color_adjustment.rs:178-192)
+    ident = orc.apply_adjustment(orc.adjustment_new(0.0, 1.0, 0.0, 1.0), px)
+    np.testing.assert_array_equal(ident[:, 0], _bc_closed_form(levels, 0.0, 1.0)[0])
 
 
 def test_optimal_adjustment_reference_cases(orc):
