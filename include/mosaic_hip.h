@@ -414,6 +414,15 @@ int mh_server_drain(mh_server *server);
 /* Host steady-clock times (ms) of submission, launch and completion. */
 int mh_server_job_times(mh_server *server, int64_t ticket, double times_ms[3]);
 int mh_server_get_stats(mh_server *server, mh_server_stats *stats);
+/* Per-phase device timing of the server's batches (no counterpart in the
+ * reference): phases whose bit (1 << MH_PHASE_x) is set are bracketed by HIP
+ * events on the stream slot that runs each batch launched from now on.
+ * mh_server_timing returns, per phase, the milliseconds, launches and cells
+ * (summed over each timed launch's jobs) of the batches completed since the
+ * last call, and clears them; any output may be NULL. */
+int mh_server_set_timing_phases(mh_server *server, uint32_t phase_mask);
+int mh_server_timing(mh_server *server, double ms[MH_PHASES], int64_t launches[MH_PHASES],
+                     int64_t cells[MH_PHASES]);
 /* Finishes queued and running jobs, then frees the server. */
 void mh_server_destroy(mh_server *server);
 

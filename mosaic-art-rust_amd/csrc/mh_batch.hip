@@ -135,9 +135,11 @@ struct BatchPlan {
     int32_t n_topk = 0, topk_blocks = 0, topk_run = 1;
     size_t place_lds = 0;
     int32_t place_score = kScoreSq;  // kScoreSq / kScoreDist / kScoreMixed over the jobs
-    int32_t n_anneal = 0;
-    size_t anneal_lds = 0;
-    bool narrow = true;
+    // annealing entries grouped by grid width in LDS (narrow: 16-bit
+    // material ids, for libraries under 32,768 materials; wide: 32-bit), one
+    // launch per group with its own LDS size
+    int32_t n_anneal = 0, n_anneal_narrow = 0;
+    size_t anneal_lds[2] = {0, 0};
     int32_t adj_blocks = 0;
     int32_t n_comp = 0, comp_blocks = 0;
     std::vector<int32_t> anneal_job;  // job index of each anneal entry
@@ -207,9 +209,10 @@ int build_plan(const std::vector<JobSpec>& specs, const std::vector<uint8_t*>& j
     p->topk_run = p->tune_topk_run > 0 ? p->tune_topk_run : topk_run_length(topk_queries);
     p->place_lds = 0;
     p->place_score = kScoreSq;
-    p->anneal_lds = 0;
-    p->narrow = true;
+    p->anneal_lds[0] = p->anneal_lds[1] = 0;
     p->anneal_job.clear();
+    std::vector<AnnealJob> wide_anneal;
+    std::vector<int32_t> wide_job;
     for (int32_t i = 0; i < n; ++i) {
         const JobSpec& s = specs[i];
         const mh_job& j = s.job;
@@ -273,10 +276,17 @@ int build_plan(const std::vector<JobSpec>& specs, const std::vector<uint8_t*>& j
             aj.seed = j.params.seed;
             aj.greedy = 0;
             aj.result = d_aout + i;
-            anneal[p->n_anneal] = aj;
-            p->anneal_job.push_back(i);
-            ++p->n_anneal;
-            if (s.n >= 32768) p->narrow = false;
+            const bool narrow = s.n < 32768;
+            const size_t al = anneal_lds_bytes(s.cells, narrow);
+            if (narrow) {
+                anneal[p->n_anneal_narrow++] = aj;
+                p->anneal_job.push_back(i);
+            } else {
+                wide_anneal.push_back(aj);
+                wide_job.push_back(i);
+            }
+            size_t& group_lds = p->anneal_lds[narrow ? 0 : 1];
+            if (al > group_lds) group_lds = al;
         }
         // a13-a16: adjustments, composite
         float4* adj = s.adjust ? at<float4>(m, s.o_adj) : nullptr;
@@ -288,11 +298,12 @@ int build_plan(const std::vector<JobSpec>& specs, const std::vector<uint8_t*>& j
         adj_blocks += s.adjust ? cell_adjust_blocks(s.cells) : 0;
         comp_blocks += composite_blocks(s.gw * s.tw, s.gh * s.th, p->comp_threads);
     }
-    for (int32_t a = 0; a < p->n_anneal; ++a) {
-        const JobSpec& s = specs[p->anneal_job[a]];
-        const size_t al = anneal_lds_bytes(s.cells, p->narrow);
-        if (al > p->anneal_lds) p->anneal_lds = al;
+    // the wide entries after the narrow ones
+    for (size_t a = 0; a < wide_anneal.size(); ++a) {
+        anneal[p->n_anneal_narrow + a] = wide_anneal[a];
+        p->anneal_job.push_back(wide_job[a]);
     }
+    p->n_anneal = p->n_anneal_narrow + (int32_t)wide_anneal.size();
     if (lab_blocks > INT32_MAX || topk_blocks > INT32_MAX || comp_blocks > INT32_MAX)
         return fail(MH_ERR_UNSUPPORTED, "batch too large for one launch");
     lab_st[n] = (int32_t)lab_blocks;
@@ -306,29 +317,58 @@ int build_plan(const std::vector<JobSpec>& specs, const std::vector<uint8_t*>& j
     return MH_OK;
 }
 
+// Optional HIP events bracketing each stage of a launch sequence (phases
+// whose bit is set in `mask`; events ev[phase][0] before, [1] after).
+struct PhaseEvents {
+    uint32_t mask = 0;
+    hipEvent_t (*ev)[2] = nullptr;
+    void mark(int phase, int which, hipStream_t s) const {
+        if ((mask >> phase) & 1u) hipEventRecord(ev[phase][which], s);
+    }
+};
+
 // The stages up to the candidate lists (cell Lab means, nearest materials).
-void enqueue_candidates(const BatchPlan& p, uint8_t* d, const float* lut, hipStream_t s) {
+void enqueue_candidates(const BatchPlan& p, uint8_t* d, const float* lut, hipStream_t s,
+                        const PhaseEvents& pe = PhaseEvents()) {
+    pe.mark(MH_PHASE_CELL_LAB, 0, s);
     launch_cell_lab_batch(at<CellSeg>(d, p.o_lab), at<int32_t>(d, p.o_lab_st), p.n_jobs, p.lab_blocks,
                           lut, s);
+    pe.mark(MH_PHASE_CELL_LAB, 1, s);
+    pe.mark(MH_PHASE_TOPK, 0, s);
     launch_topk_batch(at<TopkSeg>(d, p.o_topk), at<int32_t>(d, p.o_topk_st), p.n_topk, p.topk_blocks,
                       p.topk_run, s);
+    pe.mark(MH_PHASE_TOPK, 1, s);
 }
 
 // The stages from placement on (chains, adjustment, composite).
-void enqueue_chains(const BatchPlan& p, uint8_t* d, hipStream_t s) {
+void enqueue_chains(const BatchPlan& p, uint8_t* d, hipStream_t s,
+                    const PhaseEvents& pe = PhaseEvents()) {
+    pe.mark(MH_PHASE_PLACE, 0, s);
     launch_place_batch(at<PlaceJob>(d, p.o_place), p.n_jobs, true, p.place_score, p.place_lds, s);
-    if (p.n_anneal > 0)
-        launch_anneal_batch(at<AnnealJob>(d, p.o_anneal), p.n_anneal, p.narrow, p.anneal_lds, s);
+    pe.mark(MH_PHASE_PLACE, 1, s);
+    pe.mark(MH_PHASE_ANNEAL, 0, s);
+    const AnnealJob* aj = at<AnnealJob>(d, p.o_anneal);
+    if (p.n_anneal_narrow > 0)
+        launch_anneal_batch(aj, p.n_anneal_narrow, true, p.anneal_lds[0], s);
+    if (p.n_anneal > p.n_anneal_narrow)
+        launch_anneal_batch(aj + p.n_anneal_narrow, p.n_anneal - p.n_anneal_narrow, false,
+                            p.anneal_lds[1], s);
+    pe.mark(MH_PHASE_ANNEAL, 1, s);
+    pe.mark(MH_PHASE_ADJUST, 0, s);
     if (p.adj_blocks > 0)  // the target averages came from the cell Lab pass
         launch_cell_adjust_batch(at<CompSeg>(d, p.o_comp), at<int32_t>(d, p.o_adj_st), p.n_comp,
                                  p.adj_blocks, s);
+    pe.mark(MH_PHASE_ADJUST, 1, s);
+    pe.mark(MH_PHASE_COMPOSITE, 0, s);
     launch_composite_batch(at<CompSeg>(d, p.o_comp), at<int32_t>(d, p.o_comp_st), p.n_comp,
                            p.comp_blocks, p.comp_threads, s);
+    pe.mark(MH_PHASE_COMPOSITE, 1, s);
 }
 
-void enqueue_plan(const BatchPlan& p, uint8_t* d, const float* lut, hipStream_t s) {
-    enqueue_candidates(p, d, lut, s);
-    enqueue_chains(p, d, s);
+void enqueue_plan(const BatchPlan& p, uint8_t* d, const float* lut, hipStream_t s,
+                  const PhaseEvents& pe = PhaseEvents()) {
+    enqueue_candidates(p, d, lut, s, pe);
+    enqueue_chains(p, d, s, pe);
 }
 
 void fill_result(const JobSpec& s, const int32_t* st, const AnnealOut* ao, mh_job_result* r) {
@@ -822,6 +862,10 @@ struct StreamSlot {
     BatchPlan plan;
     std::vector<int64_t> tickets;
     bool busy = false;
+    // per-phase timing of the batch on this slot (events made at creation)
+    hipEvent_t ev[MH_PHASES][2] = {};
+    uint32_t timed = 0;   // phases bracketed in the running batch
+    int64_t cells = 0;    // cells of the running batch
 };
 
 }  // namespace
@@ -840,6 +884,10 @@ struct mh_server {
     std::atomic<bool> stop{false};
     int64_t batches = 0, batched_jobs = 0, completed = 0;
     int32_t max_seen_batch = 0;
+    // per-phase device timing (mh_server_set_timing_phases / mh_server_timing)
+    uint32_t timing = 0;
+    double phase_ms[MH_PHASES] = {};
+    int64_t phase_n[MH_PHASES] = {}, phase_cells[MH_PHASES] = {};
     int fatal = MH_OK;
     std::string fatal_msg;
 };
@@ -862,7 +910,10 @@ int launch_batch(mh_server* sv, StreamSlot& sl, const std::vector<JobRec*>& recs
     }
     CHK(build_plan(specs, mem, sl.h_plan, sl.d_plan, &sl.plan));
     HIPCHK(hipMemcpyAsync(sl.d_plan, sl.h_plan, sl.plan.o_stats, hipMemcpyHostToDevice, sl.stream));
-    enqueue_plan(sl.plan, sl.d_plan, ctx->d_lut, sl.stream);
+    PhaseEvents pe;
+    pe.mask = sl.timed;
+    pe.ev = sl.ev;
+    enqueue_plan(sl.plan, sl.d_plan, ctx->d_lut, sl.stream, pe);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(sl.h_res, sl.d_plan + sl.plan.o_stats, sl.plan.bytes - sl.plan.o_stats,
                           hipMemcpyDeviceToHost, sl.stream));
@@ -886,6 +937,16 @@ void worker_loop(mh_server* sv) {
             const int32_t* st = reinterpret_cast<const int32_t*>(sl.h_res);
             const AnnealOut* ao =
                 reinterpret_cast<const AnnealOut*>(sl.h_res + (sl.plan.o_aout - sl.plan.o_stats));
+            if (q == hipSuccess)
+                for (int ph = 0; ph < MH_PHASES; ++ph) {
+                    float ms = 0.0f;
+                    if (((sl.timed >> ph) & 1u) &&
+                        hipEventElapsedTime(&ms, sl.ev[ph][0], sl.ev[ph][1]) == hipSuccess) {
+                        sv->phase_ms[ph] += ms;
+                        ++sv->phase_n[ph];
+                        sv->phase_cells[ph] += sl.cells;
+                    }
+                }
             for (size_t i = 0; i < sl.tickets.size(); ++i) {
                 JobRec& r = sv->jobs[(size_t)sl.tickets[i]];
                 if (q == hipSuccess) {
@@ -932,6 +993,9 @@ void worker_loop(mh_server* sv) {
             if (!take.empty()) {
                 free_slot->busy = true;
                 free_slot->tickets = take;
+                free_slot->timed = sv->timing;
+                free_slot->cells = 0;
+                for (JobRec* r : recs) free_slot->cells += r->spec.cells;
                 ++sv->batches;
                 sv->batched_jobs += (int64_t)take.size();
                 if ((int32_t)take.size() > sv->max_seen_batch) sv->max_seen_batch = (int32_t)take.size();
@@ -981,6 +1045,9 @@ void server_free(mh_server* sv) {
     for (StreamSlot& sl : sv->slots) {
         if (sl.stream) hipStreamSynchronize(sl.stream);
         if (sl.done) hipEventDestroy(sl.done);
+        for (auto& pair : sl.ev)
+            for (hipEvent_t e : pair)
+                if (e) hipEventDestroy(e);
         if (sl.d_plan) hipFree(sl.d_plan);
         if (sl.h_plan) hipHostFree(sl.h_plan);
         if (sl.h_res) hipHostFree(sl.h_res);
@@ -1013,6 +1080,9 @@ int mh_server_create(mh_context* ctx, const mh_server_params* p, mh_server** out
     for (StreamSlot& sl : sv->slots) {
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
+        for (auto& pair : sl.ev)
+            for (hipEvent_t& ev : pair)
+                if (e == hipSuccess) e = hipEventCreate(&ev);
         if (e == hipSuccess) e = hipMalloc(&sl.d_plan, plan);
         if (e == hipSuccess) e = hipHostMalloc(&sl.h_plan, plan, hipHostMallocDefault);
         if (e == hipSuccess) e = hipHostMalloc(&sl.h_res, plan - tmp.o_stats, hipHostMallocDefault);
@@ -1098,6 +1168,27 @@ int mh_server_get_stats(mh_server* sv, mh_server_stats* st) {
     st->memory_in_use = (int64_t)sv->arena.used;
     st->memory_peak = (int64_t)sv->arena.peak;
     st->memory_budget = (int64_t)sv->arena.size;
+    return MH_OK;
+}
+
+int mh_server_set_timing_phases(mh_server* sv, uint32_t phase_mask) {
+    if (!sv) return fail(MH_ERR_INVALID_ARGUMENT, "server is NULL");
+    std::lock_guard<std::mutex> g(sv->mu);
+    sv->timing = phase_mask & ((1u << MH_PHASES) - 1u);
+    return MH_OK;
+}
+
+int mh_server_timing(mh_server* sv, double ms[MH_PHASES], int64_t launches[MH_PHASES],
+                     int64_t cells[MH_PHASES]) {
+    if (!sv) return fail(MH_ERR_INVALID_ARGUMENT, "server is NULL");
+    std::lock_guard<std::mutex> g(sv->mu);
+    for (int ph = 0; ph < MH_PHASES; ++ph) {
+        if (ms) ms[ph] = sv->phase_ms[ph];
+        if (launches) launches[ph] = sv->phase_n[ph];
+        if (cells) cells[ph] = sv->phase_cells[ph];
+        sv->phase_ms[ph] = 0.0;
+        sv->phase_n[ph] = sv->phase_cells[ph] = 0;
+    }
     return MH_OK;
 }
 

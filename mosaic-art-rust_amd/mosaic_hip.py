@@ -43,7 +43,8 @@ EXPORTS = [
     "mh_generate_device_result", "mh_generate_device_noalloc", "mh_job_footprint", "mh_batch_create", "mh_batch_set_graph",
     "mh_batch_run", "mh_batch_results", "mh_batch_job_outputs", "mh_batch_destroy",
     "mh_batch_tune", "mh_batch_get_tuning", "mh_batch_set_tuning", "mh_server_create", "mh_server_submit", "mh_server_wait", "mh_server_poll", "mh_server_drain",
-    "mh_server_job_times", "mh_server_get_stats", "mh_server_destroy",
+    "mh_server_job_times", "mh_server_get_stats", "mh_server_set_timing_phases",
+    "mh_server_timing", "mh_server_destroy",
 ]
 
 
@@ -219,6 +220,8 @@ def load(path=LIB_PATH):
         "mh_server_drain": (ctypes.c_int, [P]),
         "mh_server_job_times": (ctypes.c_int, [P, i64, P]),
         "mh_server_get_stats": (ctypes.c_int, [P, ctypes.POINTER(ServerStats)]),
+        "mh_server_set_timing_phases": (ctypes.c_int, [P, ctypes.c_uint32]),
+        "mh_server_timing": (ctypes.c_int, [P, P, P, P]),
         "mh_server_destroy": (None, [P]),
     }
     for name, (res, args) in sig.items():
@@ -687,6 +690,25 @@ class Server:
         st = ServerStats()
         _check(self.lib.mh_server_get_stats(self.h, ctypes.byref(st)))
         return st.as_dict()
+
+    def set_timing_phases(self, phases):
+        """Brackets the named phases of every batch launched from now on with
+        HIP events on its stream slot."""
+        names = [self.lib.mh_phase_name(i).decode() for i in range(MH_PHASES)]
+        mask = 0
+        for name in phases:
+            mask |= 1 << names.index(name)
+        _check(self.lib.mh_server_set_timing_phases(self.h, mask))
+
+    def timing(self):
+        """{phase: (total_ms, launches, cells)} of the timed batches completed
+        since the last call."""
+        ms = np.zeros(MH_PHASES, np.float64)
+        n = np.zeros(MH_PHASES, np.int64)
+        c = np.zeros(MH_PHASES, np.int64)
+        _check(self.lib.mh_server_timing(self.h, _ptr(ms), _ptr(n), _ptr(c)))
+        return {self.lib.mh_phase_name(i).decode(): (float(ms[i]), int(n[i]), int(c[i]))
+                for i in range(MH_PHASES)}
 
     def close(self):
         if self.h:
