@@ -202,7 +202,7 @@ int build_plan(const std::vector<JobSpec>& specs, const std::vector<uint8_t*>& j
     AnnealOut* d_aout = at<AnnealOut>(d, p->o_aout);
 
     int64_t lab_blocks = 0, topk_blocks = 0, adj_blocks = 0, comp_blocks = 0;
-    p->n_topk = p->n_anneal = p->n_comp = 0;
+    p->n_topk = p->n_anneal = p->n_anneal_narrow = p->n_comp = 0;
     int64_t topk_queries = 0;
     for (const JobSpec& s : specs)
         if (s.n > 0) topk_queries += s.cells;
