@@ -9,11 +9,11 @@
 // One wave (a 64-thread workgroup) answers a run of consecutive query cells.
 // Per cell it keeps a candidate buffer in LDS and a threshold tau such that
 // at least K keys are <= tau:
-//   * tau starts as the largest of this cell's keys to the previous cell's K
-//     answers (K materials have a key <= tau, so the K smallest do too);
+//   * tau starts as the largest of this cell's distances to the previous
+//     cell's K answers (K materials lie within tau, so the K nearest do too);
 //     neighbouring cells have similar Lab means, so tau is tight;
-//   * one pass over the N materials appends every key <= tau (ballot +
-//     mbcnt compaction, 64 materials per step);
+//   * one pass over the N materials appends every material within tau
+//     (ballot + popcount compaction, 64 materials per step);
 //   * when the buffer would overflow it is sorted and cut to its K smallest,
 //     whose largest becomes the new (tighter) tau;
 //   * at the end the buffer is sorted and the first K keys are the answer.
@@ -21,11 +21,16 @@
 //     registers; partners are exchanged by DPP / swizzle / permute lane
 //     moves, so no sort step touches LDS.
 // The first cell of a run (no previous answers) gets its tau from one extra
-// pass that keeps each lane's two smallest keys: the K-th smallest of those
-// 128 real keys bounds the K-th smallest key overall.  The cost is one
-// distance evaluation per material plus a few register sorts, with no
-// histograms or atomics.  The batched kernel serves many (library, query set) segments in
-// one launch: block b handles run b - starts[j] of the segment j holding b.
+// pass that keeps each lane's two smallest distances: the K-th smallest of
+// those 128 real distances bounds the K-th smallest overall.  The threshold
+// is compared on the distance bits alone (one compare per material): keys at
+// exactly the threshold distance may enter the buffer beyond the K-th, and
+// the sorts cut them.  The cost is one distance evaluation and one compare
+// per material plus a few register sorts, with no histograms or atomics.
+// The batched kernel serves many (library, query set) segments in one
+// launch: block b handles run b - starts[j] of the segment j holding b.
+#include <type_traits>
+
 #include "mh_device.hpp"
 #include "mh_kernels.hpp"
 
@@ -39,13 +44,13 @@ __device__ __forceinline__ uint64_t make_key(uint32_t dbits, int32_t i) {
     return ((uint64_t)dbits << 32) | (uint32_t)i;
 }
 
-__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
-        const uint64_t o = __shfl_xor(v, off, 64);
+        const uint32_t o = (uint32_t)__shfl_xor((int)v, off, 64);
         v = o > v ? o : v;
     }
-    return v;
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
 }
 
 // The value of lane ^ m (m a power of two below 64) for every lane: DPP
@@ -159,91 +164,96 @@ __device__ __forceinline__ void topk_run(const TopkSeg& s, int32_t q0, int32_t r
     const int32_t k = s.k < n ? s.k : n;
     const int32_t q_end = q0 + run < s.nq ? q0 + run : s.nq;
     const uint64_t lanes_below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    // materials in whole blocks of kSteps x 64, then a clamped tail block
+    constexpr int kSteps = 4;
+    const int32_t n_full = n - n % (64 * kSteps);
 
     for (int32_t q = q0; q < q_end; ++q) {
         const float ql = gload(s.query, 3 * (int64_t)q), qa = gload(s.query, 3 * (int64_t)q + 1),
                     qb = gload(s.query, 3 * (int64_t)q + 2);
-        auto dist_bits = [&](int32_t i) {
-            return __float_as_uint(sq_distance(ql, qa, qb, gload(mL, i), gload(mA, i), gload(mB, i)));
-        };
         // Four 64-material steps per load phase: all twelve loads are issued
         // before the first distance is needed (one memory latency per four
-        // steps).  Loads past n are clamped in bounds; f(d, i, valid) sees
-        // each step's distance bits, material index and in-range flag.
-        constexpr int kSteps = 4;
-        auto scan = [&](auto&& f) {
-            for (int32_t base = 0; base < n; base += 64 * kSteps) {
-                float l[kSteps], a[kSteps], b[kSteps];
+        // steps).  Whole blocks load from a uniform base at fixed lane
+        // offsets (no per-step address or bounds arithmetic); the tail block
+        // clamps its loads in bounds and marks lanes past n.  f(d, i, valid)
+        // sees each step's distance bits, the material index as a function
+        // (evaluated only where used) and the in-range lane mask.
+        auto block = [&](int32_t base, auto tail_c, auto&& f) {
+            constexpr bool tail = decltype(tail_c)::value;
+            float l[kSteps], a[kSteps], b[kSteps];
 #pragma unroll
-                for (int u = 0; u < kSteps; ++u) {
-                    const int32_t i = base + 64 * u + lane;
-                    const uint32_t off = 4u * (uint32_t)(i < n ? i : n - 1);  // n < 2^30
-                    l[u] = gload_at(mL, off);
-                    a[u] = gload_at(mA, off);
-                    b[u] = gload_at(mB, off);
-                }
-                uint32_t d[kSteps];
+            for (int u = 0; u < kSteps; ++u) {
+                int32_t i = base + 64 * u + lane;
+                if (tail) i = i < n ? i : n - 1;
+                const uint32_t off = 4u * (uint32_t)i;  // n < 2^30
+                l[u] = gload_at(mL, off);
+                a[u] = gload_at(mA, off);
+                b[u] = gload_at(mB, off);
+            }
+            uint32_t d[kSteps];
 #pragma unroll
-                for (int u = 0; u < kSteps; ++u)
-                    d[u] = __float_as_uint(sq_distance(ql, qa, qb, l[u], a[u], b[u]));
+            for (int u = 0; u < kSteps; ++u)
+                d[u] = __float_as_uint(sq_distance(ql, qa, qb, l[u], a[u], b[u]));
 #pragma unroll
-                for (int u = 0; u < kSteps; ++u) {
-                    const int32_t i = base + 64 * u + lane;
-                    f(d[u], i);
-                }
+            for (int u = 0; u < kSteps; ++u) {
+                const int32_t first = base + 64 * u;
+                const uint64_t valid = tail ? lanes_slt(first + lane, n) : ~0ull;
+                f(d[u], first, valid);
             }
         };
-        // threshold: the previous cell's answers
-        uint64_t tau = ~0ull;
+        auto scan = [&](auto&& f) {
+            for (int32_t base = 0; base < n_full; base += 64 * kSteps)
+                block(base, std::false_type{}, f);
+            if (n_full < n) block(n_full, std::true_type{}, f);
+        };
+        // Threshold on the distance bits: at least k materials lie at or
+        // below th, so the k nearest all do.  Keys at exactly th beyond the
+        // k-th may enter the buffer too; the final sort cuts them.
+        uint32_t th = 0xffffffffu;
         if (q > q0) {
-            uint64_t m = 0;
+            // the previous cell's k answers
+            uint32_t m = 0;
             for (int32_t r = lane; r < k; r += 64) {
                 const int32_t i = prev[r];
-                const uint64_t kk = make_key(dist_bits(i), i);
-                m = kk > m ? kk : m;
+                const uint32_t d = __float_as_uint(
+                    sq_distance(ql, qa, qb, gload(mL, i), gload(mA, i), gload(mB, i)));
+                m = d > m ? d : m;
             }
-            tau = wave_max_u64(m);
+            th = wave_max_u32(m);
         } else if (n > kBuf) {
             // The first cell of a run: one pass keeps every lane's two
-            // smallest keys; k of those 128 are real keys no larger than
-            // their k-th smallest, so it bounds the answer (and the scan below
-            // rarely overflows the buffer).
-            uint64_t two[2] = {~0ull, ~0ull};
-            scan([&](uint32_t d, int32_t i) {
-                const uint64_t key = i < n ? make_key(d, i) : ~0ull;
-                const bool lt0 = key < two[0], lt1 = key < two[1];
-                two[1] = lt0 ? two[0] : (lt1 ? key : two[1]);
-                two[0] = lt0 ? key : two[0];
+            // smallest distances (of distinct materials); the k-th smallest
+            // of those 128 bounds the k-th smallest overall.
+            uint32_t two0 = 0xffffffffu, two1 = 0xffffffffu;
+            scan([&](uint32_t d, int32_t, uint64_t valid) {
+                const uint32_t v = ((valid >> lane) & 1ull) ? d : 0xffffffffu;
+                const bool lt0 = v < two0, lt1 = v < two1;
+                two1 = lt0 ? two0 : (lt1 ? v : two1);
+                two0 = lt0 ? v : two0;
             });
+            uint64_t two[2] = {(uint64_t)two0 << 32 | (uint32_t)lane,
+                               (uint64_t)two1 << 32 | (uint32_t)(lane + 64)};
             reg_sort<2>(two);
             const uint64_t v = k - 1 < 64 ? two[0] : two[1];
-            const int src = (k - 1) & 63;
-            tau = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int32_t)(v >> 32), src) << 32) |
-                  (uint32_t)__builtin_amdgcn_readlane((int32_t)(uint32_t)v, src);
+            th = (uint32_t)__builtin_amdgcn_readlane((int32_t)(v >> 32), (k - 1) & 63);
         }
         int32_t count = 0;
-        // Lanes whose key (d, i) is <= tau, compared as two 32-bit halves
-        // with single-compare lane masks (the distance bits are unsigned-
-        // ordered like the distances).
-        auto passing = [&](uint32_t d, int32_t i) {
-            const uint32_t th = (uint32_t)(tau >> 32), tl = (uint32_t)tau;
-            return lanes_ult(d, th) | (lanes_eq((int32_t)d, (int32_t)th) & lanes_ule((uint32_t)i, tl));
-        };
         // appends the passing lanes of one 64-material step to the buffer,
         // first cutting the buffer to its k smallest when it would overflow
-        auto take = [&](uint32_t d, int32_t i, uint64_t mask) {
+        auto take = [&](uint32_t d, int32_t first, uint64_t mask) {
             if (count + __popcll(mask) > kBuf) {
                 sort_buffer(buf, count);
                 count = k;
-                tau = buf[k - 1];
-                mask &= passing(d, i);
+                th = (uint32_t)(buf[k - 1] >> 32);
+                mask &= lanes_ule(d, th);
             }
-            if ((mask >> lane) & 1ull) buf[count + __popcll(mask & lanes_below)] = make_key(d, i);
+            if ((mask >> lane) & 1ull)
+                buf[count + __popcll(mask & lanes_below)] = make_key(d, first + lane);
             count += __popcll(mask);
         };
-        scan([&](uint32_t d, int32_t i) {
-            const uint64_t mask = passing(d, i) & lanes_slt(i, n);
-            if (mask) take(d, i, mask);
+        scan([&](uint32_t d, int32_t first, uint64_t valid) {
+            const uint64_t mask = lanes_ule(d, th) & valid;
+            if (mask) take(d, first, mask);
         });
         sort_buffer(buf, count);
         // answer: the first k keys, ascending
