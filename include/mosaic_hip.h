@@ -386,13 +386,16 @@ int mh_batch_set_tuning(mh_batch *batch, const mh_batch_tuning *tuning);
  * (FIFO) while their working memory fits memory_budget, and launches the
  * admitted jobs as one batch on a free stream slot (at most streams batches
  * in flight, at most max_batch jobs each); jobs that arrive while batches
- * run join the next one as slots and memory free up.  Libraries stay
- * resident with their term matrices and resized tiles.  The context's other
- * entry points must not run concurrently with a live server. */
+ * run join the next one as slots and memory free up.  Working memory is
+ * allocated in chunks of 1 GiB (or one job's size, if larger) as admitted
+ * jobs need it, never beyond memory_budget; a job that finds no device
+ * memory while nothing else runs fails with MH_ERR_OUT_OF_MEMORY.  Libraries
+ * stay resident with their term matrices and resized tiles.  The context's
+ * other entry points must not run concurrently with a live server. */
 typedef struct {
     int64_t memory_budget;  /* bytes of device working memory (0 = 64 GiB) */
-    int32_t max_batch;      /* jobs per launch (0 = 256)                    */
-    int32_t streams;        /* stream slots, 1..16 (0 = 4)                  */
+    int32_t max_batch;      /* jobs per launch (0 = 128)                    */
+    int32_t streams;        /* stream slots, 1..16 (0 = 8)                  */
 } mh_server_params;
 
 typedef struct {
@@ -406,6 +409,11 @@ int mh_server_create(mh_context *ctx, const mh_server_params *params, mh_server 
 /* Queues a job (its buffers must stay valid until it completes); returns a
  * ticket.  Fails at once if the job is invalid or larger than the budget. */
 int mh_server_submit(mh_server *server, const mh_job *job, int64_t *ticket);
+/* Queues n jobs at once (tickets[i] for jobs[i]), in order: one lock and one
+ * wake-up of the scheduler instead of n.  Every job is checked first; if any
+ * is invalid or larger than the budget, none is queued and the error names
+ * it. */
+int mh_server_submit_many(mh_server *server, const mh_job *jobs, int32_t n, int64_t *tickets);
 /* Blocks until the job is done; latency_ms = completion - submission. */
 int mh_server_wait(mh_server *server, int64_t ticket, mh_job_result *result, double *latency_ms);
 int mh_server_poll(mh_server *server, int64_t ticket, int32_t *done);

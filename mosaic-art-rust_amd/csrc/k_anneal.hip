@@ -30,6 +30,24 @@ template <typename G>
 __device__ __forceinline__ int32_t gget(const G* g, int32_t i) {
     return (int32_t)g[i];
 }
+template <typename G>
+__device__ __forceinline__ void gput(G* g, int32_t i, int32_t v) {
+    g[i] = (G)v;
+}
+
+// A grid kept in HBM (annealing of grids too large for LDS).  Every access
+// is an agent-scope atomic, so a read never sees a stale cache line of a
+// cell another lane of the wave has just written.
+struct HbmGrid {
+    int32_t* p;
+};
+__device__ __forceinline__ int32_t gget(const HbmGrid* g, int32_t i) {
+    return __hip_atomic_load((MH_GAS int32_t*)g->p + i, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void gput(HbmGrid* g, int32_t i, int32_t v) {
+    __hip_atomic_store((MH_GAS int32_t*)g->p + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // One adjacency term 1/(sim(a,b)+1): a gather from the library's term
 // matrix when it is resident, else computed from the Lab table (same bits).
@@ -196,12 +214,19 @@ __host__ __device__ __forceinline__ int32_t stamp_words(int32_t cells) {
 // One word per lane for packing a round's accepted deltas.
 constexpr int32_t kPackWords = 64;
 // Proposal draws, made ahead in blocks of 128 iterations (two per lane) into
-// a ring of 256 slots of three words: x1 | y1 << 16, x2 | y2 << 16, u (the
-// grid lives in LDS, so both sides are far below 2^16).
+// a ring of 256 slots of three words: x1 | y1 << 16, x2 | y2 << 16, u (both
+// sides are below 2^16: a grid in LDS is far smaller, and the HBM variant
+// requires it).
 constexpr int32_t kRing = 256;
 constexpr int32_t kRingWords = 3 * kRing;
+// Stamp table of the HBM variant: cell p stamps word p mod kHashStamps.  Two
+// cells sharing a word can only make a lane see a conflict that is not there,
+// which ends the round early; the commits stay those of the serial chain.
+constexpr int32_t kHashStamps = 16384;
 
-template <typename G>
+// kHbm: the grid stays in HBM (the job's own grid, swapped in place) and the
+// stamps are hashed into kHashStamps LDS words, for grids too large for LDS.
+template <typename G, bool kHbm>
 __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
                                                     const AnnealJob* __restrict__ jobs) {
     extern __shared__ __align__(16) uint32_t sm[];
@@ -209,15 +234,24 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
     const int lane = threadIdx.x;
     const int32_t gw = job.grid_w, gh = job.grid_h;
     const int32_t cells = gw * gh;
-    const int32_t stamps = stamp_words(cells);
+    const int32_t stamps = kHbm ? kHashStamps : stamp_words(cells);
+    const int32_t stamp_cells = kHbm ? kHashStamps : cells;  // words cleared between uses
     uint32_t* stamp = sm;
+    auto sidx = [&](int32_t p) { return kHbm ? (p & (kHashStamps - 1)) : p; };
     float* pack_s = reinterpret_cast<float*>(sm + stamps);
     uint32_t* ring_xy1 = sm + stamps + kPackWords;
     uint32_t* ring_xy2 = ring_xy1 + kRing;
     float* ring_u = reinterpret_cast<float*>(ring_xy2 + kRing);
-    G* grid_s = reinterpret_cast<G*>(sm + stamps + kPackWords + kRingWords);
+    HbmGrid hbm{job.grid};
+    auto grid_s = [&]() {
+        if constexpr (kHbm)
+            return &hbm;
+        else
+            return reinterpret_cast<G*>(sm + stamps + kPackWords + kRingWords);
+    }();
 
-    for (int32_t i = lane; i < cells; i += 64) grid_s[i] = (G)gload(job.grid, i);
+    if constexpr (!kHbm)
+        for (int32_t i = lane; i < cells; i += 64) grid_s[i] = (G)gload(job.grid, i);
     __syncthreads();
 
     // the stamp words double as the cost scratch (2 terms per cell + padding)
@@ -227,7 +261,7 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
     const int32_t chunk = (stamps / 2 - 1) & ~1;
     float cost = job.term ? total_cost_wave(grid_s, gw, gh, TM, cost_scratch, chunk)
                           : total_cost_wave(grid_s, gw, gh, TL, cost_scratch, chunk);
-    for (int32_t i = lane; i < cells; i += 64) stamp[i] = 0u;
+    for (int32_t i = lane; i < stamp_cells; i += 64) stamp[i] = 0u;
     __syncthreads();
 
     const float initial = cost;
@@ -316,8 +350,8 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
         const int32_t p1 = y1 * gw + x1, p2 = y2 * gw + x2;
         const uint32_t mine = (tag << 6) | (uint32_t)(63 - lane);
         if (acc) {
-            atomicMax(&stamp[p1], mine);
-            atomicMax(&stamp[p2], mine);
+            atomicMax(&stamp[sidx(p1)], mine);
+            atomicMax(&stamp[sidx(p2)], mine);
         }
         __syncthreads();
         bool conflict = false;
@@ -337,7 +371,7 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
                                    x2 < gw - 1 ? p2 + 1 : p2};
 #pragma unroll
             for (int i = 0; i < 10; ++i) {
-                const uint32_t v = stamp[q[i]];
+                const uint32_t v = stamp[sidx(q[i])];
                 conflict |= (v >> 6) == tag && (int)(63 - (v & 63u)) < lane;
             }
         }
@@ -352,12 +386,15 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
             if (acc) {
                 // no lower lane of this round wrote p1 or p2 (that would be a
                 // conflict), so the round-start reads m1, m2 are still current
-                grid_s[p1] = (G)m2;
-                grid_s[p2] = (G)m1;
+                gput(grid_s, p1, m2);
+                gput(grid_s, p2, m1);
             }
             if (job.trace_delta) gstore(job.trace_delta, it, skip ? __int_as_float(0x7fc00000) : delta);
             if (job.trace_accept) gstore(job.trace_accept, it, (uint8_t)(acc ? 1 : 0));
         }
+        // an HBM grid's swaps are complete at agent scope before the next
+        // round's reads (the barriers below order the lanes)
+        if constexpr (kHbm) __threadfence();
         // Running cost in proposal order (f32, as the reference's
         // `current_cost += delta`).  The committed accepted deltas are packed
         // into words 0..n-1 of pack_s in proposal order (the other words hold
@@ -405,13 +442,14 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
         ++tag;
         if (tag >= (1u << 26)) {
             __syncthreads();
-            for (int32_t i = lane; i < cells; i += 64) stamp[i] = 0u;
+            for (int32_t i = lane; i < stamp_cells; i += 64) stamp[i] = 0u;
             tag = 1;
         }
         __syncthreads();
     }
 
-    for (int32_t i = lane; i < cells; i += 64) gstore(job.grid, i, (int32_t)grid_s[i]);
+    if constexpr (!kHbm)
+        for (int32_t i = lane; i < cells; i += 64) gstore(job.grid, i, (int32_t)grid_s[i]);
     if (lane == 0 && job.result) {
         AnnealOut r;
         r.initial_cost = initial;
@@ -426,36 +464,58 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
     }
 }
 
-size_t anneal_lds_bytes(int32_t cells, bool narrow) {
+int32_t anneal_kind(int32_t cells, int32_t n, int32_t gw, int32_t gh, size_t max_lds) {
+    if (n < 32768 && anneal_lds_bytes(cells, kAnnealNarrow) <= max_lds) return kAnnealNarrow;
+    if (anneal_lds_bytes(cells, kAnnealWide) <= max_lds) return kAnnealWide;
+    if (gw < 65536 && gh < 65536 && anneal_lds_bytes(cells, kAnnealHbm) <= max_lds)
+        return kAnnealHbm;
+    return -1;
+}
+
+size_t anneal_lds_bytes(int32_t cells, int32_t kind) {
+    if (kind == kAnnealHbm)
+        return (size_t)(kHashStamps + kPackWords + kRingWords) * sizeof(uint32_t);
     return (size_t)(stamp_words(cells) + kPackWords + kRingWords) * sizeof(uint32_t) +
-           (size_t)cells * (narrow ? sizeof(int16_t) : sizeof(int32_t));
+           (size_t)cells * (kind == kAnnealNarrow ? sizeof(int16_t) : sizeof(int32_t));
 }
 
-void launch_anneal_single(const AnnealJob& job, bool narrow, size_t lds_bytes, hipStream_t stream) {
-    if (narrow)
-        hipLaunchKernelGGL(anneal_kernel<int16_t>, dim3(1), dim3(64), lds_bytes, stream, job,
-                           (const AnnealJob*)nullptr);
+namespace {
+template <typename... A>
+void launch_anneal(int32_t kind, dim3 grid, size_t lds_bytes, hipStream_t stream, A... args) {
+    if (kind == kAnnealNarrow)
+        hipLaunchKernelGGL((anneal_kernel<int16_t, false>), grid, dim3(64), lds_bytes, stream,
+                           args...);
+    else if (kind == kAnnealWide)
+        hipLaunchKernelGGL((anneal_kernel<int32_t, false>), grid, dim3(64), lds_bytes, stream,
+                           args...);
     else
-        hipLaunchKernelGGL(anneal_kernel<int32_t>, dim3(1), dim3(64), lds_bytes, stream, job,
-                           (const AnnealJob*)nullptr);
+        hipLaunchKernelGGL((anneal_kernel<int32_t, true>), grid, dim3(64), lds_bytes, stream,
+                           args...);
+}
+}  // namespace
+
+void launch_anneal_single(const AnnealJob& job, int32_t kind, size_t lds_bytes, hipStream_t stream) {
+    launch_anneal(kind, dim3(1), lds_bytes, stream, job, (const AnnealJob*)nullptr);
 }
 
-void launch_anneal_batch(const AnnealJob* d_jobs, int32_t n_jobs, bool narrow, size_t lds_bytes,
+void launch_anneal_batch(const AnnealJob* d_jobs, int32_t n_jobs, int32_t kind, size_t lds_bytes,
                          hipStream_t stream) {
     if (n_jobs <= 0) return;
     AnnealJob unused{};
-    if (narrow)
-        hipLaunchKernelGGL(anneal_kernel<int16_t>, dim3(n_jobs), dim3(64), lds_bytes, stream, unused,
-                           d_jobs);
-    else
-        hipLaunchKernelGGL(anneal_kernel<int32_t>, dim3(n_jobs), dim3(64), lds_bytes, stream, unused,
-                           d_jobs);
+    launch_anneal(kind, dim3(n_jobs), lds_bytes, stream, unused, d_jobs);
 }
 
 hipError_t anneal_configure(int max_lds) {
-    hipError_t e = raise_dynamic_lds(reinterpret_cast<const void*>(&anneal_kernel<int16_t>), max_lds);
-    if (e != hipSuccess) return e;
-    return raise_dynamic_lds(reinterpret_cast<const void*>(&anneal_kernel<int32_t>), max_lds);
+    const void* kernels[] = {
+        reinterpret_cast<const void*>(&anneal_kernel<int16_t, false>),
+        reinterpret_cast<const void*>(&anneal_kernel<int32_t, false>),
+        reinterpret_cast<const void*>(&anneal_kernel<int32_t, true>),
+    };
+    for (const void* k : kernels) {
+        const hipError_t e = raise_dynamic_lds(k, max_lds);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 // ------------------------------------------------ cost / delta / penalty

@@ -33,13 +33,59 @@
 //    Latency-bound: a few hundred cycles per cell.
 //  * place_chain_kernel -- any cell order over a pre-filled grid and usage
 //    (find_and_use_best_tile_with_position at arbitrary cells); grid and usage
-//    in LDS, all four neighbours read per cell.
+//    in LDS, all four neighbours read per cell.  Its HBM form (launched with
+//    no LDS) keeps grid and counts in the job's own buffers, for libraries
+//    whose counts do not fit LDS; it also runs the raster order then.
 #include "mh_device.hpp"
 #include "mh_kernels.hpp"
 
 namespace mh {
 
 // ------------------------------------------------------------ general chain
+// State access of the general chain: grid and usage counts in LDS, or (kHbm,
+// for libraries whose counts do not fit LDS) the job's own buffers in HBM,
+// read and written with agent-scope atomics so no lane sees a stale cache
+// line of a word another lane has just written.
+template <bool kHbm>
+struct ChainState {
+    int32_t* grid;
+    uint32_t* usage;
+    __device__ __forceinline__ int32_t cell(int32_t i) const {
+        if constexpr (kHbm)
+            return __hip_atomic_load((MH_GAS int32_t*)grid + i, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+        else
+            return grid[i];
+    }
+    __device__ __forceinline__ void set_cell(int32_t i, int32_t v) const {
+        if constexpr (kHbm)
+            __hip_atomic_store((MH_GAS int32_t*)grid + i, v, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        else
+            grid[i] = v;
+    }
+    __device__ __forceinline__ uint32_t count(int32_t m) const {
+        if constexpr (kHbm)
+            return __hip_atomic_load((MH_GAS uint32_t*)usage + m, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+        else
+            return usage[m];
+    }
+    __device__ __forceinline__ void set_count(int32_t m, uint32_t v) const {
+        if constexpr (kHbm)
+            __hip_atomic_store((MH_GAS uint32_t*)usage + m, v, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        else
+            usage[m] = v;
+    }
+    // every lane's writes before every lane's later reads
+    __device__ __forceinline__ void sync() const {
+        if constexpr (kHbm) __threadfence();
+        __syncthreads();
+    }
+};
+
+template <bool kHbm>
 __global__ __launch_bounds__(64) void place_chain_kernel(PlaceJob single,
                                                          const PlaceJob* __restrict__ jobs) {
     extern __shared__ int32_t sm[];
@@ -47,12 +93,20 @@ __global__ __launch_bounds__(64) void place_chain_kernel(PlaceJob single,
     const int lane = threadIdx.x;
     const int32_t gw = job.grid_w, gh = job.grid_h, n = job.n;
     const int32_t cells = gw * gh;
-    int32_t* grid_s = sm;
-    uint32_t* usage_s = reinterpret_cast<uint32_t*>(sm + cells);
-
-    for (int32_t i = lane; i < cells; i += 64) grid_s[i] = gload(job.grid, i);
-    for (int32_t i = lane; i < n; i += 64) usage_s[i] = job.zero_usage ? 0u : gload(job.usage, i);
-    __syncthreads();
+    ChainState<kHbm> S;
+    if constexpr (kHbm) {
+        S.grid = job.grid;
+        S.usage = job.usage;
+        if (job.zero_usage)
+            for (int32_t i = lane; i < n; i += 64) S.set_count(i, 0u);
+    } else {
+        S.grid = sm;
+        S.usage = reinterpret_cast<uint32_t*>(sm + cells);
+        for (int32_t i = lane; i < cells; i += 64) S.grid[i] = gload(job.grid, i);
+        for (int32_t i = lane; i < n; i += 64)
+            S.usage[i] = job.zero_usage ? 0u : gload(job.usage, i);
+    }
+    S.sync();
 
     const int32_t K = n < kMaxCand ? n : kMaxCand;
     const uint32_t max_usage = (uint32_t)job.max_usage;
@@ -69,10 +123,10 @@ __global__ __launch_bounds__(64) void place_chain_kernel(PlaceJob single,
         }
         const int32_t x = cell % gw, y = cell / gw;
         // neighbours in get_adjacent_positions order: Up, Down, Left, Right
-        const int32_t nb_up = y > 0 ? grid_s[cell - gw] : kEmpty;
-        const int32_t nb_down = y < gh - 1 ? grid_s[cell + gw] : kEmpty;
-        const int32_t nb_left = x > 0 ? grid_s[cell - 1] : kEmpty;
-        const int32_t nb_right = x < gw - 1 ? grid_s[cell + 1] : kEmpty;
+        const int32_t nb_up = y > 0 ? S.cell(cell - gw) : kEmpty;
+        const int32_t nb_down = y < gh - 1 ? S.cell(cell + gw) : kEmpty;
+        const int32_t nb_left = x > 0 ? S.cell(cell - 1) : kEmpty;
+        const int32_t nb_right = x < gw - 1 ? S.cell(cell + 1) : kEmpty;
         float4 lu = make_float4(0, 0, 0, 0), ld = lu, ll = lu, lr = lu;
         if (use_pen) {
             if (nb_up != kEmpty) lu = gload4(job.lab4, nb_up);
@@ -95,7 +149,7 @@ __global__ __launch_bounds__(64) void place_chain_kernel(PlaceJob single,
                 if (job.score_sqrt) lm.w = sqrtf(lm.w);
                 const bool pl = m != nb_up && m != nb_down && m != nb_left && m != nb_right;
                 placeable[s] = pl;
-                if (pl && usage_s[m] < max_usage) {
+                if (pl && S.count(m) < max_usage) {
                     float pen = 0.0f;
                     if (use_pen) {
                         float p = 0.0f;
@@ -120,7 +174,7 @@ __global__ __launch_bounds__(64) void place_chain_kernel(PlaceJob single,
         } else {
             // fallback_tile_selection: reset every usage count, then the first
             // candidate passing the neighbour ban, else nearest_one.
-            for (int32_t i = lane; i < n; i += 64) usage_s[i] = 0;
+            for (int32_t i = lane; i < n; i += 64) S.set_count(i, 0u);
             ++st_resets;
             const uint64_t b0 = wave_ballot(placeable[0]);
             const uint64_t b1 = wave_ballot(placeable[1]);
@@ -141,20 +195,22 @@ __global__ __launch_bounds__(64) void place_chain_kernel(PlaceJob single,
         const int32_t w = gload(cidx, rank);
         const float wd_raw = gload4(clab, rank).w;
         const float wd = job.score_sqrt ? sqrtf(wd_raw) : wd_raw;
-        __syncthreads();
+        S.sync();
         if (lane == 0) {
-            usage_s[w] = usage_s[w] + 1u;
-            grid_s[cell] = w;
+            S.set_count(w, S.count(w) + 1u);
+            S.set_cell(cell, w);
             if (job.stage) gstore(job.stage, cell, (uint8_t)stage);
             if (job.dist) gstore(job.dist, cell, wd);
             if (job.score)
                 gstore(job.score, cell, stage == 0 ? __uint_as_float((uint32_t)(best >> 32)) : wd);
         }
-        __syncthreads();
+        S.sync();
     }
 
-    for (int32_t i = lane; i < cells; i += 64) gstore(job.grid, i, grid_s[i]);
-    for (int32_t i = lane; i < n; i += 64) gstore(job.usage, i, usage_s[i]);
+    if constexpr (!kHbm) {
+        for (int32_t i = lane; i < cells; i += 64) gstore(job.grid, i, S.grid[i]);
+        for (int32_t i = lane; i < n; i += 64) gstore(job.usage, i, S.usage[i]);
+    }
     if (lane == 0 && job.stats) {
         gstore(job.stats, 0, st_main + st_ban + st_near);
         gstore(job.stats, 1, st_main);
@@ -721,8 +777,11 @@ void launch_place_single(const PlaceJob& job, bool raster, size_t lds_bytes, hip
     else if (raster)
         hipLaunchKernelGGL((place_raster_kernel<false, kScoreSq>), dim3(1), dim3(64), lds_bytes,
                            stream, job, none);
+    else if (lds_bytes > 0)
+        hipLaunchKernelGGL(place_chain_kernel<false>, dim3(1), dim3(64), lds_bytes, stream, job,
+                           (const PlaceJob*)nullptr);
     else
-        hipLaunchKernelGGL(place_chain_kernel, dim3(1), dim3(64), lds_bytes, stream, job,
+        hipLaunchKernelGGL(place_chain_kernel<true>, dim3(1), dim3(64), 0, stream, job,
                            (const PlaceJob*)nullptr);
 }
 
@@ -739,14 +798,17 @@ void launch_place_batch(const PlaceJob* d_jobs, int32_t n_jobs, bool raster, int
     else if (raster)
         hipLaunchKernelGGL((place_raster_kernel<false, kScoreSq>), dim3(n_jobs), dim3(64),
                            lds_bytes, stream, unused, d_jobs);
+    else if (lds_bytes > 0)
+        hipLaunchKernelGGL(place_chain_kernel<false>, dim3(n_jobs), dim3(64), lds_bytes, stream,
+                           unused, d_jobs);
     else
-        hipLaunchKernelGGL(place_chain_kernel, dim3(n_jobs), dim3(64), lds_bytes, stream, unused,
+        hipLaunchKernelGGL(place_chain_kernel<true>, dim3(n_jobs), dim3(64), 0, stream, unused,
                            d_jobs);
 }
 
 hipError_t place_configure(int max_lds) {
     const void* kernels[] = {
-        reinterpret_cast<const void*>(&place_chain_kernel),
+        reinterpret_cast<const void*>(&place_chain_kernel<false>),
         reinterpret_cast<const void*>(&place_raster_kernel<true, kScoreSq>),
         reinterpret_cast<const void*>(&place_raster_kernel<true, kScoreDist>),
         reinterpret_cast<const void*>(&place_raster_kernel<false, kScoreSq>),

@@ -214,11 +214,14 @@ int run_place(mh_context* ctx, const mh_library* lib, const float* d_cell_lab, i
               uint8_t* d_stage, float* d_dist, float* d_score, int32_t* d_stats,
               int32_t* cidx_buf = nullptr, float4* clab_buf = nullptr) {
     const int32_t cells = gw * gh;
-    const size_t lds = raster ? place_raster_lds_bytes(gw, lib->n) : place_lds_bytes(cells, lib->n);
-    if (lds > ctx->max_lds)
-        return fail(MH_ERR_UNSUPPORTED,
-                    "placement state (%d cells + %d materials) needs %zu B of LDS, limit %zu",
-                    cells, lib->n, lds, ctx->max_lds);
+    size_t lds = raster ? place_raster_lds_bytes(gw, lib->n) : place_lds_bytes(cells, lib->n);
+    // state beyond LDS: the general chain with grid and counts in HBM (it
+    // reads the neighbours from the grid, so a raster run starts it empty)
+    const bool hbm = lds > ctx->max_lds;
+    if (hbm) {
+        lds = 0;
+        if (raster) fill_i32(d_grid, cells, kEmpty, ctx->stream);
+    }
     int32_t* cidx = cidx_buf;
     float4* clab = clab_buf;
     int rc;
@@ -249,7 +252,7 @@ int run_place(mh_context* ctx, const mh_library* lib, const float* d_cell_lab, i
     job.score = d_score;
     job.stats = d_stats;
     phase_begin(ctx, MH_PHASE_PLACE);
-    launch_place_single(job, raster, lds, ctx->stream);
+    launch_place_single(job, raster && !hbm, lds, ctx->stream);
     phase_end(ctx, MH_PHASE_PLACE);
     HIPCHK(hipGetLastError());
     return MH_OK;
@@ -355,11 +358,11 @@ int run_anneal(mh_context* ctx, const mh_library* lib, int32_t* d_grid, int32_t 
                const mh_anneal_params* p, float* d_trace_delta, uint8_t* d_trace_accept,
                AnnealOut* d_out, const float* temps_ready, bool noalloc) {
     const int32_t cells = gw * gh;
-    const bool narrow = lib->n < 32768;
-    const size_t lds = anneal_lds_bytes(cells, narrow);
-    if (lds > ctx->max_lds)
-        return fail(MH_ERR_UNSUPPORTED, "annealing state (%d cells) needs %zu B of LDS, limit %zu",
-                    cells, lds, ctx->max_lds);
+    const int32_t kind = anneal_kind(cells, lib->n, gw, gh, ctx->max_lds);
+    if (kind < 0)
+        return fail(MH_ERR_UNSUPPORTED, "annealing a %d x %d grid: a side of 65,536 or more", gw,
+                    gh);
+    const size_t lds = anneal_lds_bytes(cells, kind);
     const float* temps = temps_ready;
     int rc;
     if (!p->greedy && !temps && !noalloc) {
@@ -386,7 +389,7 @@ int run_anneal(mh_context* ctx, const mh_library* lib, int32_t* d_grid, int32_t 
     job.trace_accept = d_trace_accept;
     job.result = d_out;
     phase_begin(ctx, MH_PHASE_ANNEAL);
-    launch_anneal_single(job, narrow, lds, ctx->stream);
+    launch_anneal_single(job, kind, lds, ctx->stream);
     phase_end(ctx, MH_PHASE_ANNEAL);
     HIPCHK(hipGetLastError());
     return MH_OK;

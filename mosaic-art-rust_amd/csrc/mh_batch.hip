@@ -44,6 +44,8 @@ struct JobSpec {
     int32_t gw = 0, gh = 0, cells = 0, tw = 0, th = 0, n = 0;
     int64_t pitch = 0;
     bool sa = false, adjust = false;
+    int32_t anneal_kind = 0;  // the annealing variant (mh::anneal_kind)
+    bool place_hbm = false;   // placement state in HBM (library beyond LDS)
     float strength = 0.0f;
     const float* term = nullptr;
     const float* temps = nullptr;
@@ -85,12 +87,12 @@ int check_job(const mh_context* ctx, const mh_job& j, JobSpec* s) {
     float k = p.color_strength;
     s->strength = k < 0.0f ? 0.0f : (k > 1.0f ? 1.0f : k);  // src/main.rs:162
     s->adjust = s->strength > 0.0f;
-    const size_t place_lds = place_raster_lds_bytes(s->gw, s->n);
-    if (place_lds > ctx->max_lds)
-        return fail(MH_ERR_UNSUPPORTED, "placement state (%d columns + %d materials) needs %zu B of LDS",
-                    s->gw, s->n, place_lds);
-    if (s->sa && anneal_lds_bytes(s->cells, s->n < 32768) > ctx->max_lds)
-        return fail(MH_ERR_UNSUPPORTED, "annealing state (%d cells) exceeds LDS", s->cells);
+    // placement state beyond LDS runs the chain with grid and counts in HBM
+    s->place_hbm = place_raster_lds_bytes(s->gw, s->n) > ctx->max_lds;
+    s->anneal_kind = s->sa ? anneal_kind(s->cells, s->n, s->gw, s->gh, ctx->max_lds) : 0;
+    if (s->anneal_kind < 0)
+        return fail(MH_ERR_UNSUPPORTED, "annealing a %d x %d grid: a side of 65,536 or more", s->gw,
+                    s->gh);
     // working memory
     const size_t cells = (size_t)s->cells;
     size_t o = 0;
@@ -135,11 +137,16 @@ struct BatchPlan {
     int32_t n_topk = 0, topk_blocks = 0, topk_run = 1;
     size_t place_lds = 0;
     int32_t place_score = kScoreSq;  // kScoreSq / kScoreDist / kScoreMixed over the jobs
-    // annealing entries grouped by grid width in LDS (narrow: 16-bit
-    // material ids, for libraries under 32,768 materials; wide: 32-bit), one
-    // launch per group with its own LDS size
-    int32_t n_anneal = 0, n_anneal_narrow = 0;
-    size_t anneal_lds[2] = {0, 0};
+    // placement entries: raster chains with state in LDS first, then the
+    // chains with state in HBM, whose grids start empty (filled per launch)
+    int32_t n_place_lds = 0;
+    std::vector<std::pair<int32_t*, int32_t>> hbm_grids;
+    // annealing entries grouped by variant (anneal_kind: the grid in LDS with
+    // 16- or 32-bit material ids, or in HBM), one launch per group with its
+    // own LDS size; group g's entries start at anneal_start[g]
+    int32_t n_anneal = 0;
+    int32_t anneal_start[4] = {0, 0, 0, 0};
+    size_t anneal_lds[3] = {0, 0, 0};
     int32_t adj_blocks = 0;
     int32_t n_comp = 0, comp_blocks = 0;
     std::vector<int32_t> anneal_job;  // job index of each anneal entry
@@ -202,17 +209,20 @@ int build_plan(const std::vector<JobSpec>& specs, const std::vector<uint8_t*>& j
     AnnealOut* d_aout = at<AnnealOut>(d, p->o_aout);
 
     int64_t lab_blocks = 0, topk_blocks = 0, adj_blocks = 0, comp_blocks = 0;
-    p->n_topk = p->n_anneal = p->n_anneal_narrow = p->n_comp = 0;
+    p->n_topk = p->n_anneal = p->n_comp = 0;
     int64_t topk_queries = 0;
     for (const JobSpec& s : specs)
         if (s.n > 0) topk_queries += s.cells;
     p->topk_run = p->tune_topk_run > 0 ? p->tune_topk_run : topk_run_length(topk_queries);
     p->place_lds = 0;
     p->place_score = kScoreSq;
-    p->anneal_lds[0] = p->anneal_lds[1] = 0;
+    p->hbm_grids.clear();
+    p->n_place_lds = 0;
+    std::vector<PlaceJob> hbm_place;
+    for (size_t& l : p->anneal_lds) l = 0;
     p->anneal_job.clear();
-    std::vector<AnnealJob> wide_anneal;
-    std::vector<int32_t> wide_job;
+    std::vector<AnnealJob> by_kind[3];
+    std::vector<int32_t> job_of_kind[3];
     for (int32_t i = 0; i < n; ++i) {
         const JobSpec& s = specs[i];
         const mh_job& j = s.job;
@@ -257,9 +267,14 @@ int build_plan(const std::vector<JobSpec>& specs, const std::vector<uint8_t*>& j
         pj.weight = j.params.weight;
         pj.score_sqrt = j.params.score_mode == MH_SCORE_DISTANCE ? 1 : 0;
         pj.stats = d_stats + (size_t)i * kStatsInts;
-        place[i] = pj;
-        const size_t pl = place_raster_lds_bytes(s.gw, s.n);
-        if (pl > p->place_lds) p->place_lds = pl;
+        if (s.place_hbm) {
+            hbm_place.push_back(pj);
+            p->hbm_grids.emplace_back(grid, s.cells);
+        } else {
+            place[p->n_place_lds++] = pj;
+            const size_t pl = place_raster_lds_bytes(s.gw, s.n);
+            if (pl > p->place_lds) p->place_lds = pl;
+        }
         const int32_t kind = pj.score_sqrt ? kScoreDist : kScoreSq;
         p->place_score = i == 0 || p->place_score == kind ? kind : kScoreMixed;
         // a10-a12: annealing chain
@@ -276,17 +291,11 @@ int build_plan(const std::vector<JobSpec>& specs, const std::vector<uint8_t*>& j
             aj.seed = j.params.seed;
             aj.greedy = 0;
             aj.result = d_aout + i;
-            const bool narrow = s.n < 32768;
-            const size_t al = anneal_lds_bytes(s.cells, narrow);
-            if (narrow) {
-                anneal[p->n_anneal_narrow++] = aj;
-                p->anneal_job.push_back(i);
-            } else {
-                wide_anneal.push_back(aj);
-                wide_job.push_back(i);
-            }
-            size_t& group_lds = p->anneal_lds[narrow ? 0 : 1];
-            if (al > group_lds) group_lds = al;
+            const int32_t kind = s.anneal_kind;
+            by_kind[kind].push_back(aj);
+            job_of_kind[kind].push_back(i);
+            const size_t al = anneal_lds_bytes(s.cells, kind);
+            if (al > p->anneal_lds[kind]) p->anneal_lds[kind] = al;
         }
         // a13-a16: adjustments, composite
         float4* adj = s.adjust ? at<float4>(m, s.o_adj) : nullptr;
@@ -298,12 +307,16 @@ int build_plan(const std::vector<JobSpec>& specs, const std::vector<uint8_t*>& j
         adj_blocks += s.adjust ? cell_adjust_blocks(s.cells) : 0;
         comp_blocks += composite_blocks(s.gw * s.tw, s.gh * s.th, p->comp_threads);
     }
-    // the wide entries after the narrow ones
-    for (size_t a = 0; a < wide_anneal.size(); ++a) {
-        anneal[p->n_anneal_narrow + a] = wide_anneal[a];
-        p->anneal_job.push_back(wide_job[a]);
+    for (size_t h = 0; h < hbm_place.size(); ++h) place[p->n_place_lds + h] = hbm_place[h];
+    p->n_anneal = 0;
+    for (int kind = 0; kind < 3; ++kind) {
+        p->anneal_start[kind] = p->n_anneal;
+        for (size_t a = 0; a < by_kind[kind].size(); ++a) {
+            anneal[p->n_anneal++] = by_kind[kind][a];
+            p->anneal_job.push_back(job_of_kind[kind][a]);
+        }
     }
-    p->n_anneal = p->n_anneal_narrow + (int32_t)wide_anneal.size();
+    p->anneal_start[3] = p->n_anneal;
     if (lab_blocks > INT32_MAX || topk_blocks > INT32_MAX || comp_blocks > INT32_MAX)
         return fail(MH_ERR_UNSUPPORTED, "batch too large for one launch");
     lab_st[n] = (int32_t)lab_blocks;
@@ -344,15 +357,22 @@ void enqueue_candidates(const BatchPlan& p, uint8_t* d, const float* lut, hipStr
 void enqueue_chains(const BatchPlan& p, uint8_t* d, hipStream_t s,
                     const PhaseEvents& pe = PhaseEvents()) {
     pe.mark(MH_PHASE_PLACE, 0, s);
-    launch_place_batch(at<PlaceJob>(d, p.o_place), p.n_jobs, true, p.place_score, p.place_lds, s);
+    const PlaceJob* pj = at<PlaceJob>(d, p.o_place);
+    if (p.n_place_lds > 0)
+        launch_place_batch(pj, p.n_place_lds, true, p.place_score, p.place_lds, s);
+    if (p.n_jobs > p.n_place_lds) {
+        for (const auto& g : p.hbm_grids) fill_i32(g.first, g.second, kEmpty, s);
+        launch_place_batch(pj + p.n_place_lds, p.n_jobs - p.n_place_lds, false, p.place_score, 0,
+                           s);
+    }
     pe.mark(MH_PHASE_PLACE, 1, s);
     pe.mark(MH_PHASE_ANNEAL, 0, s);
     const AnnealJob* aj = at<AnnealJob>(d, p.o_anneal);
-    if (p.n_anneal_narrow > 0)
-        launch_anneal_batch(aj, p.n_anneal_narrow, true, p.anneal_lds[0], s);
-    if (p.n_anneal > p.n_anneal_narrow)
-        launch_anneal_batch(aj + p.n_anneal_narrow, p.n_anneal - p.n_anneal_narrow, false,
-                            p.anneal_lds[1], s);
+    for (int kind = 0; kind < 3; ++kind) {
+        const int32_t count = p.anneal_start[kind + 1] - p.anneal_start[kind];
+        if (count > 0)
+            launch_anneal_batch(aj + p.anneal_start[kind], count, kind, p.anneal_lds[kind], s);
+    }
     pe.mark(MH_PHASE_ANNEAL, 1, s);
     pe.mark(MH_PHASE_ADJUST, 0, s);
     if (p.adj_blocks > 0)  // the target averages came from the cell Lab pass
@@ -840,6 +860,67 @@ struct Arena {
     }
 };
 
+// A server's device working memory: chunks allocated as admitted jobs need
+// them, up to the budget, each sub-allocated first-fit; a job lives in one
+// chunk.  When no job holds memory and a job still does not fit, the chunks
+// are given back so that one chunk of the needed size can be made.
+struct ChunkArena {
+    static constexpr size_t kChunk = (size_t)1 << 30;
+    struct Chunk {
+        uint8_t* base = nullptr;
+        Arena arena;
+    };
+    size_t budget = 0, reserved = 0, used = 0, peak = 0;
+    std::vector<Chunk> chunks;
+
+    void release_all() {
+        for (Chunk& c : chunks) hipFree(c.base);
+        chunks.clear();
+        reserved = 0;
+    }
+    // Device memory for `bytes` (chunk index in *chunk), or nullptr when it
+    // does not fit now.
+    uint8_t* alloc(size_t bytes, int32_t* chunk) {
+        bytes = align_up(bytes ? bytes : 1);
+        for (int attempt = 0; attempt < 2; ++attempt) {
+            for (size_t c = 0; c < chunks.size(); ++c) {
+                size_t off = 0;
+                if (chunks[c].arena.alloc(bytes, &off)) {
+                    *chunk = (int32_t)c;
+                    return take(chunks[c].base + off, bytes);
+                }
+            }
+            if (bytes <= budget - reserved) {
+                const size_t size = std::min(budget - reserved, std::max(kChunk, bytes));
+                Chunk c;
+                if (hipMalloc(&c.base, size) == hipSuccess) {
+                    c.arena.reset(size);
+                    chunks.push_back(c);
+                    reserved += size;
+                    size_t off = 0;
+                    chunks.back().arena.alloc(bytes, &off);
+                    *chunk = (int32_t)chunks.size() - 1;
+                    return take(chunks.back().base + off, bytes);
+                }
+                (void)hipGetLastError();  // an allocation failure is not sticky
+            }
+            if (used != 0) return nullptr;  // wait for running jobs to give memory back
+            release_all();                  // nothing held: start from one fitting chunk
+        }
+        return nullptr;
+    }
+    uint8_t* take(uint8_t* p, size_t bytes) {
+        used += bytes;
+        if (used > peak) peak = used;
+        return p;
+    }
+    void release(int32_t chunk, const uint8_t* p, size_t bytes) {
+        Chunk& c = chunks[(size_t)chunk];
+        c.arena.release((size_t)(p - c.base), bytes);
+        used -= align_up(bytes ? bytes : 1);
+    }
+};
+
 enum JobState { J_QUEUED = 0, J_RUNNING = 1, J_DONE = 2, J_FAILED = 3 };
 
 struct JobRec {
@@ -849,7 +930,8 @@ struct JobRec {
     std::string msg;
     mh_job_result result{};
     double t_submit = 0, t_launch = 0, t_done = 0;
-    size_t off = 0;
+    uint8_t* mem = nullptr;  // working memory (in chunk `chunk` of the server's arena)
+    int32_t chunk = 0;
     bool holds_mem = false;
 };
 
@@ -872,9 +954,8 @@ struct StreamSlot {
 
 struct mh_server {
     mh_context* ctx = nullptr;
-    int32_t max_batch = 256;
-    uint8_t* d_arena = nullptr;
-    Arena arena;
+    int32_t max_batch = 128;
+    ChunkArena arena;
     std::vector<StreamSlot> slots;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -906,7 +987,7 @@ int launch_batch(mh_server* sv, StreamSlot& sl, const std::vector<JobRec*>& recs
     for (JobRec* r : recs) {
         CHK(prepare_job(ctx, &r->spec, sl.stream));
         specs.push_back(r->spec);
-        mem.push_back(sv->d_arena + r->off);
+        mem.push_back(r->mem);
     }
     CHK(build_plan(specs, mem, sl.h_plan, sl.d_plan, &sl.plan));
     HIPCHK(hipMemcpyAsync(sl.d_plan, sl.h_plan, sl.plan.o_stats, hipMemcpyHostToDevice, sl.stream));
@@ -958,7 +1039,7 @@ void worker_loop(mh_server* sv) {
                     r.msg = std::string("batch failed: ") + hipGetErrorString(q);
                 }
                 r.t_done = t;
-                if (r.holds_mem) sv->arena.release(r.off, r.spec.bytes);
+                if (r.holds_mem) sv->arena.release(r.chunk, r.mem, r.spec.bytes);
                 r.holds_mem = false;
                 ++sv->completed;
             }
@@ -980,9 +1061,24 @@ void worker_loop(mh_server* sv) {
             while (!sv->queue.empty() && (int32_t)take.size() < sv->max_batch) {
                 const int64_t t = sv->queue.front();
                 JobRec& r = sv->jobs[(size_t)t];
-                size_t off = 0;
-                if (!sv->arena.alloc(r.spec.bytes, &off)) break;  // FIFO: wait for memory
-                r.off = off;
+                uint8_t* mem = sv->arena.alloc(r.spec.bytes, &r.chunk);
+                if (!mem) {
+                    // FIFO: wait for memory -- unless nothing runs that could
+                    // give any back, when the device itself has no room
+                    bool busy = false;
+                    for (const StreamSlot& sl : sv->slots) busy |= sl.busy;
+                    if (busy || !take.empty()) break;
+                    r.state = J_FAILED;
+                    r.err = MH_ERR_OUT_OF_MEMORY;
+                    r.msg = "no device memory for the job's working set";
+                    r.t_done = now_ms();
+                    ++sv->completed;
+                    sv->queue.pop_front();
+                    sv->cv_done.notify_all();
+                    progressed = true;
+                    continue;
+                }
+                r.mem = mem;
                 r.holds_mem = true;
                 r.state = J_RUNNING;
                 r.t_launch = now_ms();
@@ -1014,7 +1110,7 @@ void worker_loop(mh_server* sv) {
                         r.err = rc;
                         r.msg = m;
                         r.t_done = now_ms();
-                        if (r.holds_mem) sv->arena.release(r.off, r.spec.bytes);
+                        if (r.holds_mem) sv->arena.release(r.chunk, r.mem, r.spec.bytes);
                         r.holds_mem = false;
                         ++sv->completed;
                     }
@@ -1053,7 +1149,7 @@ void server_free(mh_server* sv) {
         if (sl.h_res) hipHostFree(sl.h_res);
         if (sl.stream) hipStreamDestroy(sl.stream);
     }
-    if (sv->d_arena) hipFree(sv->d_arena);
+    sv->arena.release_all();
     delete sv;
 }
 
@@ -1065,8 +1161,8 @@ int mh_server_create(mh_context* ctx, const mh_server_params* p, mh_server** out
     if (!ctx || !out) return fail(MH_ERR_INVALID_ARGUMENT, "NULL argument");
     *out = nullptr;
     const int64_t budget = p && p->memory_budget > 0 ? p->memory_budget : (int64_t)64 << 30;
-    const int32_t max_batch = p && p->max_batch > 0 ? p->max_batch : 256;
-    const int32_t n_slots = p && p->streams > 0 ? p->streams : 4;
+    const int32_t max_batch = p && p->max_batch > 0 ? p->max_batch : 128;
+    const int32_t n_slots = p && p->streams > 0 ? p->streams : 8;
     if (n_slots > 16 || max_batch > 65536)
         return fail(MH_ERR_INVALID_ARGUMENT, "streams must be <= 16 and max_batch <= 65536");
     CHK(set_device(ctx));
@@ -1076,7 +1172,7 @@ int mh_server_create(mh_context* ctx, const mh_server_params* p, mh_server** out
     sv->slots.resize((size_t)n_slots);
     BatchPlan tmp;
     const size_t plan = plan_bytes(max_batch, &tmp);
-    hipError_t e = hipMalloc(&sv->d_arena, (size_t)budget);
+    hipError_t e = hipSuccess;
     for (StreamSlot& sl : sv->slots) {
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
@@ -1092,7 +1188,7 @@ int mh_server_create(mh_context* ctx, const mh_server_params* p, mh_server** out
         return fail(e == hipErrorOutOfMemory ? MH_ERR_OUT_OF_MEMORY : MH_ERR_HIP,
                     "server resources (budget %lld B): %s", (long long)budget, hipGetErrorString(e));
     }
-    sv->arena.reset((size_t)budget);
+    sv->arena.budget = (size_t)budget;
     sv->worker = std::thread(worker_loop, sv);
     *out = sv;
     return MH_OK;
@@ -1102,9 +1198,9 @@ int mh_server_submit(mh_server* sv, const mh_job* job, int64_t* ticket) {
     if (!sv || !job || !ticket) return fail(MH_ERR_INVALID_ARGUMENT, "NULL argument");
     JobSpec s;
     CHK(check_job(sv->ctx, *job, &s));
-    if (align_up(s.bytes) > sv->arena.size)
+    if (align_up(s.bytes) > sv->arena.budget)
         return fail(MH_ERR_OUT_OF_MEMORY, "job needs %zu B, more than the server budget %zu B", s.bytes,
-                    sv->arena.size);
+                    sv->arena.budget);
     std::lock_guard<std::mutex> g(sv->mu);
     if (sv->stop) return fail(MH_ERR_INVALID_ARGUMENT, "server is shutting down");
     JobRec r;
@@ -1113,6 +1209,33 @@ int mh_server_submit(mh_server* sv, const mh_job* job, int64_t* ticket) {
     sv->jobs.push_back(std::move(r));
     *ticket = (int64_t)sv->jobs.size() - 1;
     sv->queue.push_back(*ticket);
+    sv->cv_work.notify_one();
+    return MH_OK;
+}
+
+int mh_server_submit_many(mh_server* sv, const mh_job* jobs, int32_t n, int64_t* tickets) {
+    if (!sv || n < 0 || (n > 0 && (!jobs || !tickets)))
+        return fail(MH_ERR_INVALID_ARGUMENT, "mh_server_submit_many: bad arguments");
+    // every job is checked before any is queued
+    std::vector<JobSpec> specs((size_t)n);
+    for (int32_t i = 0; i < n; ++i) {
+        if (const int rc = check_job(sv->ctx, jobs[i], &specs[(size_t)i]))
+            return fail(rc, "job %d: %s", i, mh_last_error());
+        if (align_up(specs[(size_t)i].bytes) > sv->arena.budget)
+            return fail(MH_ERR_OUT_OF_MEMORY, "job %d needs %zu B, more than the server budget %zu B",
+                        i, specs[(size_t)i].bytes, sv->arena.budget);
+    }
+    std::lock_guard<std::mutex> g(sv->mu);
+    if (sv->stop) return fail(MH_ERR_INVALID_ARGUMENT, "server is shutting down");
+    const double t = now_ms();
+    for (int32_t i = 0; i < n; ++i) {
+        JobRec r;
+        r.spec = specs[(size_t)i];
+        r.t_submit = t;
+        sv->jobs.push_back(std::move(r));
+        tickets[i] = (int64_t)sv->jobs.size() - 1;
+        sv->queue.push_back(tickets[i]);
+    }
     sv->cv_work.notify_one();
     return MH_OK;
 }
@@ -1167,7 +1290,7 @@ int mh_server_get_stats(mh_server* sv, mh_server_stats* st) {
     st->max_batch_jobs = sv->max_seen_batch;
     st->memory_in_use = (int64_t)sv->arena.used;
     st->memory_peak = (int64_t)sv->arena.peak;
-    st->memory_budget = (int64_t)sv->arena.size;
+    st->memory_budget = (int64_t)sv->arena.budget;
     return MH_OK;
 }
 

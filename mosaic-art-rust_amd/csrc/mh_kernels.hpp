@@ -146,9 +146,15 @@ void launch_place_single(const PlaceJob& job, bool raster, size_t lds_bytes, hip
 enum : int32_t { kScoreSq = 0, kScoreDist = 1, kScoreMixed = 2 };
 void launch_place_batch(const PlaceJob* d_jobs, int32_t n_jobs, bool raster, int32_t score_kind,
                         size_t lds_bytes, hipStream_t stream);
-size_t anneal_lds_bytes(int32_t cells, bool narrow);
-void launch_anneal_single(const AnnealJob& job, bool narrow, size_t lds_bytes, hipStream_t stream);
-void launch_anneal_batch(const AnnealJob* d_jobs, int32_t n_jobs, bool narrow, size_t lds_bytes,
+// Annealing variants: the grid in LDS with 16-bit material ids (libraries
+// under 32,768 materials) or 32-bit ids, or in HBM with hashed LDS stamps
+// (grids too large for LDS; sides below 65,536).
+enum : int32_t { kAnnealNarrow = 0, kAnnealWide = 1, kAnnealHbm = 2 };
+// The first variant that holds the job in max_lds bytes of LDS, -1 if none.
+int32_t anneal_kind(int32_t cells, int32_t n, int32_t gw, int32_t gh, size_t max_lds);
+size_t anneal_lds_bytes(int32_t cells, int32_t kind);
+void launch_anneal_single(const AnnealJob& job, int32_t kind, size_t lds_bytes, hipStream_t stream);
+void launch_anneal_batch(const AnnealJob* d_jobs, int32_t n_jobs, int32_t kind, size_t lds_bytes,
                          hipStream_t stream);
 void launch_total_cost(const int32_t* grid, int32_t gw, int32_t gh, const float4* lab4,
                        const float* term, int32_t n, float* scratch, int32_t chunk, float* out,

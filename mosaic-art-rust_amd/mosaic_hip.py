@@ -43,7 +43,8 @@ EXPORTS = [
     "mh_generate_device_result", "mh_generate_device_noalloc", "mh_job_footprint", "mh_batch_create", "mh_batch_set_graph",
     "mh_batch_run", "mh_batch_results", "mh_batch_job_outputs", "mh_batch_destroy",
     "mh_batch_tune", "mh_batch_get_tuning", "mh_batch_set_tuning", "mh_server_create", "mh_server_submit", "mh_server_wait", "mh_server_poll", "mh_server_drain",
-    "mh_server_job_times", "mh_server_get_stats", "mh_server_set_timing_phases",
+    "mh_server_submit_many", "mh_server_job_times", "mh_server_get_stats",
+    "mh_server_set_timing_phases",
     "mh_server_timing", "mh_server_destroy",
 ]
 
@@ -214,6 +215,7 @@ def load(path=LIB_PATH):
         "mh_batch_set_tuning": (ctypes.c_int, [P, P]),
         "mh_server_create": (ctypes.c_int, [P, ctypes.POINTER(ServerParams), ctypes.POINTER(P)]),
         "mh_server_submit": (ctypes.c_int, [P, ctypes.POINTER(Job), ctypes.POINTER(i64)]),
+        "mh_server_submit_many": (ctypes.c_int, [P, P, i32, P]),
         "mh_server_wait": (ctypes.c_int, [P, i64, ctypes.POINTER(JobResult),
                                           ctypes.POINTER(ctypes.c_double)]),
         "mh_server_poll": (ctypes.c_int, [P, i64, ctypes.POINTER(i32)]),
@@ -664,6 +666,13 @@ class Server:
         t = ctypes.c_int64()
         _check(self.lib.mh_server_submit(self.h, ctypes.byref(job), ctypes.byref(t)))
         return t.value
+
+    def submit_many(self, jobs):
+        """Queues every job at once (one call); returns their tickets."""
+        arr = (Job * len(jobs))(*jobs)
+        t = np.zeros(len(jobs), np.int64)
+        _check(self.lib.mh_server_submit_many(self.h, arr, len(jobs), _ptr(t)))
+        return t.tolist()
 
     def wait(self, ticket):
         res = JobResult()

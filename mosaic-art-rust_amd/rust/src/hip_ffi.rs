@@ -217,6 +217,7 @@ extern "C" {
     pub fn mh_batch_set_tuning(batch: *mut mh_batch, tuning: *const mh_batch_tuning) -> c_int;
     pub fn mh_server_create(ctx: *mut mh_context, params: *const mh_server_params, out: *mut *mut mh_server) -> c_int;
     pub fn mh_server_submit(server: *mut mh_server, job: *const mh_job, ticket: *mut i64) -> c_int;
+    pub fn mh_server_submit_many(server: *mut mh_server, jobs: *const mh_job, n: i32, tickets: *mut i64) -> c_int;
     pub fn mh_server_wait(server: *mut mh_server, ticket: i64, result: *mut mh_job_result, latency_ms: *mut f64) -> c_int;
     pub fn mh_server_poll(server: *mut mh_server, ticket: i64, done: *mut i32) -> c_int;
     pub fn mh_server_drain(server: *mut mh_server) -> c_int;

@@ -203,6 +203,30 @@ def test_server_results_equal_alone_under_memory_pressure(ctx, mh, synth, orc):
     s.close()
 
 
+def test_server_submit_many(ctx, mh, synth, orc):
+    # every job queued by one call; a batch holding an invalid job queues none
+    s = Setup(ctx, mh, synth, n_jobs=10, seed=13)
+    jobs = [s.job(k, own_outputs=True) for k in range(len(s.specs))]
+    srv = ctx.server(memory_budget=1 << 30, max_batch=4, streams=2)
+    lab_only = ctx.library_from_lab(np.zeros((4, 3), np.float32))
+    bad = ctx.job(lab_only, jobs[0].d_target, jobs[0].width, jobs[0].height, jobs[0].params,
+                  pitch=jobs[0].target_pitch)
+    with pytest.raises(mh.MhError) as e:
+        srv.submit_many(jobs[:3] + [bad])
+    assert e.value.code == mh.MH_ERR_INVALID_ARGUMENT and "job 3" in str(e.value)
+    assert srv.stats()["submitted"] == 0
+    tickets = srv.submit_many(jobs)
+    assert tickets == list(range(len(jobs)))
+    got = [srv.wait(t) for t in tickets]
+    srv.close()
+    lab_only.close()
+    for k in range(len(s.specs)):
+        g, im = s.fetch(k, jobs[k].d_grid_out, jobs[k].d_image_out)
+        _same(s.alone(k), g, im, got[k], s.specs[k]["sa"] and s.specs[k]["weight"] > 0)
+        s.check_oracle(k, orc, g, im, got[k])
+    s.close()
+
+
 def test_server_rejects_bad_jobs(ctx, mh, synth):
     s = Setup(ctx, mh, synth, n_jobs=2, seed=9)
     j = s.job(0, own_outputs=False)
