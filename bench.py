@@ -287,7 +287,7 @@ def stream_free(ctx, S):
         lib.close()
 
 
-def run_served(mh, ctx, S, steps, warmup, budget, streams, max_batch):
+def run_served(mh, ctx, S, steps, warmup, budget, streams, max_batch, capacity_only=False):
     """The headline: K timed passes of the whole stream through the server
     with every job queued at once (capacity), after W untimed ones; then the
     stream at its Poisson arrival rate (latency); then the plain path on the
@@ -339,6 +339,21 @@ def run_served(mh, ctx, S, steps, warmup, budget, streams, max_batch):
     ctx.synchronize()
     served_grids = {k: ctx.to_host(d_gs[k], jobs[k]["grid_w"] * jobs[k]["grid_h"], np.int32)
                     for k in sample}
+    cells = sum(j["grid_w"] * j["grid_h"] for j in jobs)
+    iters = sum(j["iters"] for j in jobs)
+    cap_s = float(np.sum(spans))
+    capacity_jps = n_jobs * steps / cap_s
+    if capacity_only:
+        for ptr in d_gs.values():
+            ctx.device_free(ptr)
+        return {"jobs_per_s": capacity_jps, "step_s": spans,
+                "capacity_latency_ms": {"p50": _pct(clat, 50), "p99": _pct(clat, 99)},
+                "tiles_placed_per_s": round(cells * steps / cap_s, 1),
+                "sa_proposals_per_s": round(iters * steps / cap_s, 1),
+                "capacity_batches": cstats["batches"] if cstats else None,
+                "streams": streams,
+                "place_timing": {"ms": place_ms, "launches": place_n, "cells": place_cells},
+                "cells": cells}
     times, stats, _ = serve([j["arrival"] for j in jobs])
     lat = times[:, 2] - times[:, 0]
     span = (times[:, 2].max() - times[:, 0].min()) / 1000.0
@@ -358,15 +373,11 @@ def run_served(mh, ctx, S, steps, warmup, budget, streams, max_batch):
         last = time.perf_counter()
         plain_lat.append(1000.0 * (last - due))
         if k in d_gs:
-            cells = j["grid_w"] * j["grid_h"]
-            same &= np.array_equal(ctx.to_host(S["d_grid"], cells, np.int32), served_grids[k])
+            same &= np.array_equal(ctx.to_host(S["d_grid"], j["grid_w"] * j["grid_h"], np.int32),
+                                   served_grids[k])
     plain_span = last - (t0 + jobs[0]["arrival"])
     for ptr in d_gs.values():
         ctx.device_free(ptr)
-    cells = sum(j["grid_w"] * j["grid_h"] for j in jobs)
-    iters = sum(j["iters"] for j in jobs)
-    cap_s = float(np.sum(spans))
-    capacity_jps = n_jobs * steps / cap_s
     plain_jps = n_jobs / plain_span
     return {
         "jobs_per_s": capacity_jps,
@@ -497,6 +508,8 @@ def main():
     ap.add_argument("--serve-streams", type=int, default=8)
     ap.add_argument("--serve-max-batch", type=int, default=128)
     ap.add_argument("--serve-budget-gib", type=float, default=64.0)
+    ap.add_argument("--serve-capacity-only", action="store_true",
+                    help="only the capacity passes of the served stream (counter runs)")
     args = ap.parse_args()
     legs = set(args.legs.split(","))
 
@@ -526,7 +539,7 @@ def main():
         S = stream_setup(mh, synth, ctx, args.serve_jobs, args.serve_rate)
         S["rate"] = args.serve_rate
         sv = run_served(mh, ctx, S, args.steps, args.warmup, int(args.serve_budget_gib * 2**30),
-                        args.serve_streams, args.serve_max_batch)
+                        args.serve_streams, args.serve_max_batch, args.serve_capacity_only)
         pt = sv.pop("place_timing")
         cells = sv.pop("cells")
         out["value"] = round(sv["jobs_per_s"], 2)
@@ -542,12 +555,12 @@ def main():
             "jobs": args.serve_jobs, "cells": cells, "libraries": [1000, 2500, 5000, 10000],
             "parallelism": f"1 GPU, continuous batching on {args.serve_streams} stream slots, "
                            f"<= {args.serve_max_batch} jobs per launch"}
-        out["latency_ms"] = sv["rate"]["latency_ms"]
+        out["latency_ms"] = sv["rate"]["latency_ms"] if "rate" in sv else None
         out["tiles_placed_per_s"] = sv["tiles_placed_per_s"]
         out["sa_proposals_per_s"] = sv["sa_proposals_per_s"]
         # roofline of the dominant kernel: the placement chains of each
-        # served batch (one wave per job), SURVEY 8(d)'s 16 B + 12 K B per
-        # tile placed (K = 100 for every library of the stream)
+        # served batch (one wave per job), 16 B + 12 K B of algorithmic
+        # traffic per tile placed (K = 100 for every library of the stream)
         if pt["launches"]:
             per_launch = pt["cells"] / pt["launches"] * (16 + 12 * 100)
             avg_ms = pt["ms"] / pt["launches"]
@@ -573,14 +586,6 @@ def main():
                           "traffic": measured_traffic("place", "medium"),
                           "algorithmic_bytes": amount, "avg_ms": pk["avg_ms"]}
         pm["similarity"] = run_similarity(ctx, lib, n, 5)
-        if args.calibrate_copy > 0:
-            src = ctx.device_alloc(args.calibrate_copy)
-            dst = ctx.device_alloc(args.calibrate_copy)
-            for _ in range(3):
-                ctx.copy_on_device(dst, src, args.calibrate_copy)
-            ctx.synchronize()
-            ctx.device_free(src)
-            ctx.device_free(dst)
         if "cpu" in legs:
             # the same job on the host: library prepared once (Lab means,
             # tiles resized to the cell size), then the per-job work
@@ -630,6 +635,15 @@ def main():
         ctx.device_free(d_grid)
         ctx.device_free(d_image)
         lib.close()
+
+    if args.calibrate_copy > 0:
+        src = ctx.device_alloc(args.calibrate_copy)
+        dst = ctx.device_alloc(args.calibrate_copy)
+        for _ in range(3):
+            ctx.copy_on_device(dst, src, args.calibrate_copy)
+        ctx.synchronize()
+        ctx.device_free(src)
+        ctx.device_free(dst)
 
     if "batched" in legs:
         out["batched"] = run_batched(mh, synth, ctx, max(3, args.steps), 1)

@@ -384,9 +384,12 @@ int mh_batch_set_tuning(mh_batch *batch, const mh_batch_tuning *tuning);
 
 /* Continuous-batching request loop.  A scheduler thread admits queued jobs
  * (FIFO) while their working memory fits memory_budget, and launches the
- * admitted jobs as one batch on a free stream slot (at most streams batches
- * in flight, at most max_batch jobs each); jobs that arrive while batches
- * run join the next one as slots and memory free up.  Working memory is
+ * admitted jobs as one batch on a free slot (at most `streams` batches in
+ * flight, at most max_batch jobs each); jobs that arrive while batches run
+ * join the next one as slots and memory free up.  The batches' wide stages
+ * (cell Labs, K nearest) run in admission order on one stream; their
+ * placement chains, annealing and output stages run on up to three more,
+ * beside later batches' wide stages.  Working memory is
  * allocated in chunks of 1 GiB (or one job's size, if larger) as admitted
  * jobs need it, never beyond memory_budget; a job that finds no device
  * memory while nothing else runs fails with MH_ERR_OUT_OF_MEMORY.  Libraries
@@ -395,7 +398,7 @@ int mh_batch_set_tuning(mh_batch *batch, const mh_batch_tuning *tuning);
 typedef struct {
     int64_t memory_budget;  /* bytes of device working memory (0 = 64 GiB) */
     int32_t max_batch;      /* jobs per launch (0 = 128)                    */
-    int32_t streams;        /* stream slots, 1..16 (0 = 8)                  */
+    int32_t streams;        /* batches in flight, 1..16 (0 = 8)             */
 } mh_server_params;
 
 typedef struct {

@@ -1,10 +1,17 @@
 // k_topk.hip -- K nearest materials per cell (the kiddo k-d tree query
 // nearest_n::<SquaredEuclidean>(lab, min(N,100)) at src/main.rs:348-352).
 //
-// Brute force over all N materials, exact: each material gets the 64-bit
-// key (float_bits(sqdist) << 32 | index).  Squared distances are >= +0, so
-// the bit pattern orders like the value and keys are unique; the K smallest
-// keys are the K nearest materials ascending with ties broken by index.
+// Exact: each material gets the 64-bit key (float_bits(sqdist) << 32 |
+// index).  Squared distances are >= +0, so the bit pattern orders like the
+// value and keys are unique; the K smallest keys are the K nearest materials
+// ascending with ties broken by index.
+//
+// The library is laid out in blocks of 64 materials (MatBlocks: median
+// splits on the widest Lab axis, so each block is a compact box).  A block
+// whose box lies farther than the threshold tau holds no candidate: its
+// bound (box_bound) is a lower bound on every member's computed distance,
+// so skipping it changes nothing.  This is the k-d tree's pruning, flat:
+// one wave tests 64 block boxes at once and scans only the blocks in reach.
 //
 // One wave (a 64-thread workgroup) answers a run of consecutive query cells.
 // Per cell it keeps a candidate buffer in LDS and a threshold tau such that
@@ -12,21 +19,20 @@
 //   * tau starts as the largest of this cell's distances to the previous
 //     cell's K answers (K materials lie within tau, so the K nearest do too);
 //     neighbouring cells have similar Lab means, so tau is tight;
-//   * one pass over the N materials appends every material within tau
-//     (ballot + popcount compaction, 64 materials per step);
+//   * one pass over the blocks within tau appends every member within tau
+//     (ballot + popcount compaction, one block of 64 per step);
 //   * when the buffer would overflow it is sorted and cut to its K smallest,
 //     whose largest becomes the new (tighter) tau;
 //   * at the end the buffer is sorted and the first K keys are the answer.
 //     Sorts are bitonic networks over 2 or 4 keys per lane held in
 //     registers; partners are exchanged by DPP / swizzle / permute lane
 //     moves, so no sort step touches LDS.
-// The first cell of a run (no previous answers) gets its tau from one extra
-// pass that keeps each lane's two smallest distances: the K-th smallest of
-// those 128 real distances bounds the K-th smallest overall.  The threshold
-// is compared on the distance bits alone (one compare per material): keys at
-// exactly the threshold distance may enter the buffer beyond the K-th, and
-// the sorts cut them.  The cost is one distance evaluation and one compare
-// per material plus a few register sorts, with no histograms or atomics.
+// The first cell of a run (no previous answers) takes the blocks nearest by
+// bound until each lane's two smallest member keys number at least K; the
+// K-th smallest of those real distances bounds the K-th smallest overall.
+// The threshold is compared on the distance bits alone (one compare per
+// material): keys at exactly the threshold distance may enter the buffer
+// beyond the K-th, and the sorts cut them.
 // The batched kernel serves many (library, query set) segments in one
 // launch: block b handles run b - starts[j] of the segment j holding b.
 #include <type_traits>
@@ -153,6 +159,71 @@ __device__ __forceinline__ void sort_buffer(uint64_t* buf, int32_t count) {
     __builtin_amdgcn_wave_barrier();
 }
 
+// Keeps the buffer's keys whose distance bits are <= some t with
+// k <= #(keys <= t) <= 128 (bisection over t), compacted in key order of
+// arrival, and returns their count; returns count unchanged when ties make
+// no such t exist.  The k smallest keys are among those kept, and 128 keys
+// take the 2-per-lane sort instead of the 4-per-lane one.
+__device__ __forceinline__ int32_t cut_buffer(uint64_t* buf, int32_t count, int32_t k) {
+    const int lane = threadIdx.x;
+    uint64_t key[4];
+    uint32_t d[4];
+    uint32_t dmax = 0;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const bool in = t * 64 + lane < count;
+        key[t] = in ? buf[t * 64 + lane] : ~0ull;
+        d[t] = (uint32_t)(key[t] >> 32);
+        if (in) dmax = d[t] > dmax ? d[t] : dmax;
+    }
+    dmax = wave_max_u32(dmax);
+    auto below = [&](uint32_t x) {
+        int32_t c = 0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) c += __popcll(lanes_ule(d[t], x) & lanes_slt(t * 64 + lane, count));
+        return c;
+    };
+    uint32_t lo = 0, hi = dmax, sel = 0;
+    bool found = false;
+    while (lo <= hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        const int32_t c = below(mid);
+        if (c < k) {
+            lo = mid + 1;
+        } else if (c > 128) {
+            if (mid == 0) break;
+            hi = mid - 1;
+        } else {
+            sel = mid;
+            found = true;
+            break;
+        }
+    }
+    if (!found) return count;
+    const uint64_t lanes_below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    int32_t base = 0;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const uint64_t keep = lanes_ule(d[t], sel) & lanes_slt(t * 64 + lane, count);
+        if ((keep >> lane) & 1ull) buf[base + __popcll(keep & lanes_below)] = key[t];
+        base += __popcll(keep);
+    }
+    __builtin_amdgcn_wave_barrier();
+    return base;
+}
+
+// Lower bound, as float bits, on the squared distance sq_distance computes
+// from the query to any material in a block's box.  Per axis the smallest
+// |fl(q - x)| over x in [lo, hi] is at a face (fl(q - x) is monotone in x),
+// or 0 inside; the squares and the sum, in sq_distance's order, are
+// monotone in their operands, so no member computes a smaller distance.
+__device__ __forceinline__ uint32_t box_bound(float ql, float qa, float qb, float4 lo, float4 hi) {
+    const float tl = ql < lo.x ? ql - lo.x : (ql > hi.x ? ql - hi.x : 0.0f);
+    const float ta = qa < lo.y ? qa - lo.y : (qa > hi.y ? qa - hi.y : 0.0f);
+    const float tb = qb < lo.z ? qb - lo.z : (qb > hi.z ? qb - hi.z : 0.0f);
+    return __float_as_uint(tl * tl + ta * ta + tb * tb);
+}
+
 // Answers query cells [q0, q0 + run) of segment s.
 __device__ __forceinline__ void topk_run(const TopkSeg& s, int32_t q0, int32_t run,
                                          uint64_t* buf, int32_t* prev) {
@@ -160,51 +231,30 @@ __device__ __forceinline__ void topk_run(const TopkSeg& s, int32_t q0, int32_t r
     const float* __restrict__ mL = s.mL;
     const float* __restrict__ mA = s.mA;
     const float* __restrict__ mB = s.mB;
+    const MatBlocks& B = s.blk;
     const int32_t n = s.n;
+    const int32_t nb = B.nb;
     const int32_t k = s.k < n ? s.k : n;
     const int32_t q_end = q0 + run < s.nq ? q0 + run : s.nq;
     const uint64_t lanes_below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    // materials in whole blocks of kSteps x 64, then a clamped tail block
-    constexpr int kSteps = 4;
-    const int32_t n_full = n - n % (64 * kSteps);
+    constexpr int kSteps = 4;  // blocks per load phase
 
     for (int32_t q = q0; q < q_end; ++q) {
         const float ql = gload(s.query, 3 * (int64_t)q), qa = gload(s.query, 3 * (int64_t)q + 1),
                     qb = gload(s.query, 3 * (int64_t)q + 2);
-        // Four 64-material steps per load phase: all twelve loads are issued
-        // before the first distance is needed (one memory latency per four
-        // steps).  Whole blocks load from a uniform base at fixed lane
-        // offsets (no per-step address or bounds arithmetic); the tail block
-        // clamps its loads in bounds and marks lanes past n.  f(d, i, valid)
-        // sees each step's distance bits, the material index as a function
-        // (evaluated only where used) and the in-range lane mask.
-        auto block = [&](int32_t base, auto tail_c, auto&& f) {
-            constexpr bool tail = decltype(tail_c)::value;
-            float l[kSteps], a[kSteps], b[kSteps];
-#pragma unroll
-            for (int u = 0; u < kSteps; ++u) {
-                int32_t i = base + 64 * u + lane;
-                if (tail) i = i < n ? i : n - 1;
-                const uint32_t off = 4u * (uint32_t)i;  // n < 2^30
-                l[u] = gload_at(mL, off);
-                a[u] = gload_at(mA, off);
-                b[u] = gload_at(mB, off);
-            }
-            uint32_t d[kSteps];
-#pragma unroll
-            for (int u = 0; u < kSteps; ++u)
-                d[u] = __float_as_uint(sq_distance(ql, qa, qb, l[u], a[u], b[u]));
-#pragma unroll
-            for (int u = 0; u < kSteps; ++u) {
-                const int32_t first = base + 64 * u;
-                const uint64_t valid = tail ? lanes_slt(first + lane, n) : ~0ull;
-                f(d[u], first, valid);
-            }
+        // this lane's bound of block b (blocks past the last bound to ~0)
+        auto bound_of = [&](int32_t b) -> uint32_t {
+            const int32_t bc = b < nb ? b : nb - 1;
+            const uint32_t lb = box_bound(ql, qa, qb, gload4(B.box, 2 * bc),
+                                          gload4(B.box, 2 * bc + 1));
+            return b < nb ? lb : 0xffffffffu;
         };
-        auto scan = [&](auto&& f) {
-            for (int32_t base = 0; base < n_full; base += 64 * kSteps)
-                block(base, std::false_type{}, f);
-            if (n_full < n) block(n_full, std::true_type{}, f);
+        // member `lane` of block b: distance bits and material (-1: padding)
+        auto member = [&](int32_t b, uint32_t& d, int32_t& m) {
+            const uint32_t off = 4u * (uint32_t)(b * 64 + lane);  // slots < 2^30
+            m = gload_at(B.pidx, off);
+            d = __float_as_uint(sq_distance(ql, qa, qb, gload_at(B.pL, off), gload_at(B.pA, off),
+                                            gload_at(B.pB, off)));
         };
         // Threshold on the distance bits: at least k materials lie at or
         // below th, so the k nearest all do.  Keys at exactly th beyond the
@@ -221,40 +271,92 @@ __device__ __forceinline__ void topk_run(const TopkSeg& s, int32_t q0, int32_t r
             }
             th = wave_max_u32(m);
         } else if (n > kBuf) {
-            // The first cell of a run: one pass keeps every lane's two
-            // smallest distances (of distinct materials); the k-th smallest
-            // of those 128 bounds the k-th smallest overall.
-            uint32_t two0 = 0xffffffffu, two1 = 0xffffffffu;
-            scan([&](uint32_t d, int32_t, uint64_t valid) {
-                const uint32_t v = ((valid >> lane) & 1ull) ? d : 0xffffffffu;
-                const bool lt0 = v < two0, lt1 = v < two1;
-                two1 = lt0 ? two0 : (lt1 ? v : two1);
-                two0 = lt0 ? v : two0;
-            });
-            uint64_t two[2] = {(uint64_t)two0 << 32 | (uint32_t)lane,
-                               (uint64_t)two1 << 32 | (uint32_t)(lane + 64)};
+            // The first cell of a run: the blocks nearest by bound, one at a
+            // time, until every lane's two smallest distances among their
+            // members cover k materials; the k-th smallest of those keys
+            // bounds the k-th nearest overall.  Two full blocks give 128
+            // keys and only the last block is partial, so three suffice.
+            uint64_t two0 = ~0ull, two1 = ~0ull;
+            int32_t picked[3] = {-1, -1, -1};
+#pragma unroll
+            for (int it = 0; it < 3; ++it) {
+                uint64_t best = ~0ull;
+                for (int32_t c0 = 0; c0 < nb; c0 += 64) {
+                    const int32_t b = c0 + lane;
+                    uint32_t lb = bound_of(b);
+#pragma unroll
+                    for (int j = 0; j < it; ++j) lb = b == picked[j] ? 0xffffffffu : lb;
+                    const uint64_t key = ((uint64_t)lb << 32) | (uint32_t)b;
+                    best = key < best ? key : best;
+                }
+                best = wave_min_u64(best);
+                const int32_t blk =
+                    __builtin_amdgcn_readfirstlane((int32_t)(uint32_t)(best & 0xffffffffu));
+                picked[it] = blk;
+                uint32_t d;
+                int32_t m;
+                member(blk, d, m);
+                const uint64_t key = m >= 0 ? make_key(d, m) : ~0ull;
+                const bool lt0 = key < two0, lt1 = key < two1;
+                two1 = lt0 ? two0 : (lt1 ? key : two1);
+                two0 = lt0 ? key : two0;
+                const int32_t real = __popcll(wave_ballot(two0 != ~0ull)) +
+                                     __popcll(wave_ballot(two1 != ~0ull));
+                if (real >= k) break;
+            }
+            uint64_t two[2] = {two0, two1};
             reg_sort<2>(two);
             const uint64_t v = k - 1 < 64 ? two[0] : two[1];
             th = (uint32_t)__builtin_amdgcn_readlane((int32_t)(v >> 32), (k - 1) & 63);
         }
         int32_t count = 0;
-        // appends the passing lanes of one 64-material step to the buffer,
-        // first cutting the buffer to its k smallest when it would overflow
-        auto take = [&](uint32_t d, int32_t first, uint64_t mask) {
+        // appends the passing lanes of one block to the buffer, first
+        // cutting the buffer to its k smallest when it would overflow
+        auto take = [&](uint32_t d, int32_t m, uint64_t mask) {
             if (count + __popcll(mask) > kBuf) {
                 sort_buffer(buf, count);
                 count = k;
                 th = (uint32_t)(buf[k - 1] >> 32);
                 mask &= lanes_ule(d, th);
             }
-            if ((mask >> lane) & 1ull)
-                buf[count + __popcll(mask & lanes_below)] = make_key(d, first + lane);
+            if ((mask >> lane) & 1ull) buf[count + __popcll(mask & lanes_below)] = make_key(d, m);
             count += __popcll(mask);
         };
-        scan([&](uint32_t d, int32_t first, uint64_t valid) {
-            const uint64_t mask = lanes_ule(d, th) & valid;
-            if (mask) take(d, first, mask);
-        });
+        // every block whose bound is within th, up to kSteps per load phase
+        // (their loads all issued before the first distance is needed); th
+        // only tightens, so a chunk's remaining blocks are re-tested after
+        // each phase
+        for (int32_t c0 = 0; c0 < nb; c0 += 64) {
+            const uint32_t lb = bound_of(c0 + lane);
+            uint64_t todo = lanes_ule(lb, th) & lanes_slt(c0 + lane, nb);
+            while (todo) {
+                int32_t bs[kSteps];
+                int32_t cnt = 0;
+#pragma unroll
+                for (int u = 0; u < kSteps; ++u) {
+                    bs[u] = c0 + __ffsll((unsigned long long)todo) - 1;  // a repeat once empty
+                    if (todo) {
+                        ++cnt;
+                        todo &= todo - 1;
+                    } else {
+                        bs[u] = bs[0];
+                    }
+                }
+                uint32_t d[kSteps];
+                int32_t m[kSteps];
+#pragma unroll
+                for (int u = 0; u < kSteps; ++u) member(bs[u], d[u], m[u]);
+#pragma unroll
+                for (int u = 0; u < kSteps; ++u) {
+                    if (u < cnt) {
+                        const uint64_t mask = lanes_ule(d[u], th) & lanes_nonneg(m[u]);
+                        if (mask) take(d[u], m[u], mask);
+                    }
+                }
+                todo &= lanes_ule(lb, th);
+            }
+        }
+        if (count > 128) count = cut_buffer(buf, count, k);
         sort_buffer(buf, count);
         // answer: the first k keys, ascending
         const int32_t out_stride = s.out_stride;
@@ -305,12 +407,12 @@ int32_t topk_run_length(int64_t total_queries) {
 
 int32_t topk_blocks(int32_t nq, int32_t run) { return nq > 0 ? (nq + run - 1) / run : 0; }
 
-void launch_topk(const float* mL, const float* mA, const float* mB, int32_t n, const float* query,
-                 int32_t nq, int32_t k, int32_t* out_idx, float* out_d, float4* out_lab,
-                 int32_t out_stride, hipStream_t stream) {
+void launch_topk(const float* mL, const float* mA, const float* mB, int32_t n, const MatBlocks& blk,
+                 const float* query, int32_t nq, int32_t k, int32_t* out_idx, float* out_d,
+                 float4* out_lab, int32_t out_stride, hipStream_t stream) {
     if (k > n) k = n;
     if (nq <= 0 || n <= 0 || k <= 0) return;
-    const TopkSeg s{mL, mA, mB, query, out_idx, out_d, out_lab, n, nq, k, out_stride};
+    const TopkSeg s{mL, mA, mB, query, out_idx, out_d, out_lab, n, nq, k, out_stride, blk};
     const int32_t run = topk_run_length(nq);
     hipLaunchKernelGGL(topk_kernel, dim3(topk_blocks(nq, run)), dim3(64), 0, stream, s, run);
 }

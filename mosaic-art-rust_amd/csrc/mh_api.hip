@@ -4,6 +4,9 @@
 // context-owned device scratch (grow-only), enqueue the kernels on the
 // context stream and synchronise.  mh_generate_device is the asynchronous,
 // device-resident form used by the benchmark and by batching callers.
+#include <algorithm>
+#include <numeric>
+
 #include "mh_internal.hpp"
 
 using namespace mh;
@@ -229,8 +232,8 @@ int run_place(mh_context* ctx, const mh_library* lib, const float* d_cell_lab, i
     if (!clab && (rc = slot(ctx, S_CAND_D, (size_t)cells * kCandStride, &clab))) return rc;
     const int32_t K = lib->n < kMaxCand ? lib->n : kMaxCand;
     phase_begin(ctx, MH_PHASE_TOPK);
-    launch_topk(lib->d_L, lib->d_A, lib->d_B, lib->n, d_cell_lab, cells, K, cidx, nullptr, clab,
-                kCandStride, ctx->stream);
+    launch_topk(lib->d_L, lib->d_A, lib->d_B, lib->n, library_blocks(lib), d_cell_lab, cells, K,
+                cidx, nullptr, clab, kCandStride, ctx->stream);
     phase_end(ctx, MH_PHASE_TOPK);
     PlaceJob job;
     job.lab4 = lib->d_lab4;
@@ -501,6 +504,14 @@ void library_release(mh_library* lib) {
     if (lib->d_L) hipFree(lib->d_L);
     if (lib->d_A) hipFree(lib->d_A);
     if (lib->d_B) hipFree(lib->d_B);
+    if (lib->d_pL) hipFree(lib->d_pL);
+    if (lib->d_pA) hipFree(lib->d_pA);
+    if (lib->d_pB) hipFree(lib->d_pB);
+    if (lib->d_pidx) hipFree(lib->d_pidx);
+    if (lib->d_box) hipFree(lib->d_box);
+    lib->d_pL = lib->d_pA = lib->d_pB = nullptr;
+    lib->d_pidx = nullptr;
+    lib->d_box = nullptr;
     if (lib->d_sim) hipFree(lib->d_sim);
     if (lib->d_term) hipFree(lib->d_term);
     if (lib->d_pixels) hipFree(lib->d_pixels);
@@ -517,6 +528,10 @@ void library_release(mh_library* lib) {
     }
     lib->tiles.clear();
     lib->ctx = nullptr;
+}
+
+MatBlocks library_blocks(const mh_library* lib) {
+    return MatBlocks{lib->d_pL, lib->d_pA, lib->d_pB, lib->d_pidx, lib->d_box, lib->nb};
 }
 
 }  // namespace host
@@ -765,6 +780,85 @@ static int library_alloc(mh_context* ctx, int32_t n, mh_library** out) {
     return MH_OK;
 }
 
+// Blocks of 64 materials for the K-nearest scan (MatBlocks): ranges of the
+// permutation are split at a multiple of 64 (about half way) on the Lab axis
+// of widest extent, until a range is one block; ties on the axis value go
+// by material index, so the layout is a function of the Labs alone.  Only
+// the last block can be partial.
+static int library_build_blocks(mh_library* lib) {
+    const int32_t n = lib->n;
+    const int32_t nb = (n + 63) / 64;
+    const float* lab = lib->h_lab.data();
+    std::vector<int32_t> perm((size_t)n);
+    for (int32_t i = 0; i < n; ++i) perm[(size_t)i] = i;
+    std::vector<std::pair<int32_t, int32_t>> todo{{0, n}};
+    while (!todo.empty()) {
+        const auto [lo, hi] = todo.back();
+        todo.pop_back();
+        if (hi - lo <= 64) continue;
+        float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int32_t i = lo; i < hi; ++i)
+            for (int a = 0; a < 3; ++a) {
+                const float v = lab[3 * (size_t)perm[(size_t)i] + a];
+                mn[a] = std::min(mn[a], v);
+                mx[a] = std::max(mx[a], v);
+            }
+        int ax = 0;
+        for (int a = 1; a < 3; ++a)
+            if (mx[a] - mn[a] > mx[ax] - mn[ax]) ax = a;
+        const int32_t blocks = (hi - lo + 63) / 64;
+        const int32_t mid = lo + 64 * ((blocks + 1) / 2);
+        std::nth_element(perm.begin() + lo, perm.begin() + mid, perm.begin() + hi,
+                         [&](int32_t x, int32_t y) {
+                             const float u = lab[3 * (size_t)x + ax], v = lab[3 * (size_t)y + ax];
+                             return u < v || (u == v && x < y);
+                         });
+        todo.push_back({lo, mid});
+        todo.push_back({mid, hi});
+    }
+    const size_t slots = (size_t)(nb > 0 ? nb : 1) * 64;
+    std::vector<float> pl(slots, 0.0f), pa(slots, 0.0f), pb(slots, 0.0f);
+    std::vector<int32_t> pidx(slots, -1);
+    std::vector<float4> box((size_t)(nb > 0 ? nb : 1) * 2, make_float4(0, 0, 0, 0));
+    for (int32_t b = 0; b < nb; ++b) {
+        float4 lo4 = make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
+        float4 hi4 = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
+        for (int32_t s = 64 * b; s < std::min(n, 64 * b + 64); ++s) {
+            const int32_t m = perm[(size_t)s];
+            const float* v = lab + 3 * (size_t)m;
+            pl[(size_t)s] = v[0];
+            pa[(size_t)s] = v[1];
+            pb[(size_t)s] = v[2];
+            pidx[(size_t)s] = m;
+            lo4.x = std::min(lo4.x, v[0]);
+            lo4.y = std::min(lo4.y, v[1]);
+            lo4.z = std::min(lo4.z, v[2]);
+            hi4.x = std::max(hi4.x, v[0]);
+            hi4.y = std::max(hi4.y, v[1]);
+            hi4.z = std::max(hi4.z, v[2]);
+        }
+        box[2 * (size_t)b] = lo4;
+        box[2 * (size_t)b + 1] = hi4;
+    }
+    hipError_t e = hipMalloc(&lib->d_pL, sizeof(float) * slots);
+    if (e == hipSuccess) e = hipMalloc(&lib->d_pA, sizeof(float) * slots);
+    if (e == hipSuccess) e = hipMalloc(&lib->d_pB, sizeof(float) * slots);
+    if (e == hipSuccess) e = hipMalloc(&lib->d_pidx, sizeof(int32_t) * slots);
+    if (e == hipSuccess) e = hipMalloc(&lib->d_box, sizeof(float4) * box.size());
+    if (e == hipSuccess) e = hipMemcpy(lib->d_pL, pl.data(), sizeof(float) * slots, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(lib->d_pA, pa.data(), sizeof(float) * slots, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(lib->d_pB, pb.data(), sizeof(float) * slots, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(lib->d_pidx, pidx.data(), sizeof(int32_t) * slots, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(lib->d_box, box.data(), sizeof(float4) * box.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess)
+        return fail(e == hipErrorOutOfMemory ? MH_ERR_OUT_OF_MEMORY : MH_ERR_HIP,
+                    "material blocks: %s", hipGetErrorString(e));
+    lib->nb = nb;
+    return MH_OK;
+}
+
 static int library_upload_lab(mh_library* lib) {
     const int32_t n = lib->n;
     if (n == 0) return MH_OK;
@@ -780,7 +874,7 @@ static int library_upload_lab(mh_library* lib) {
     HIPCHK(hipMemcpy(lib->d_L, L.data(), sizeof(float) * n, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(lib->d_A, A.data(), sizeof(float) * n, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(lib->d_B, B.data(), sizeof(float) * n, hipMemcpyHostToDevice));
-    return MH_OK;
+    return library_build_blocks(lib);
 }
 
 int mh_library_from_lab(mh_context* ctx, const float* lab, int32_t n, mh_library** out) {
@@ -1117,8 +1211,8 @@ int mh_nearest_n(mh_context* ctx, const mh_library* lib, const float* query, int
     CHK(upload(ctx, S_IN0, query, (size_t)nq * 3, &d_q));
     CHK(slot(ctx, S_OUT0, (size_t)nq * kk, &d_i));
     CHK(slot(ctx, S_OUT1, (size_t)nq * kk, &d_d));
-    launch_topk(lib->d_L, lib->d_A, lib->d_B, lib->n, d_q, nq, kk, d_i, d_d, nullptr, kk,
-                ctx->stream);
+    launch_topk(lib->d_L, lib->d_A, lib->d_B, lib->n, library_blocks(lib), d_q, nq, kk, d_i, d_d,
+                nullptr, kk, ctx->stream);
     HIPCHK(hipGetLastError());
     CHK(download(ctx, idx, d_i, (size_t)nq * kk));
     CHK(download(ctx, sqdist, d_d, (size_t)nq * kk));

@@ -246,7 +246,7 @@ int build_plan(const std::vector<JobSpec>& specs, const std::vector<uint8_t*>& j
             topk_st[p->n_topk] = (int32_t)topk_blocks;
             topk[p->n_topk] = TopkSeg{lib->d_L, lib->d_A, lib->d_B, cell_lab, cand_idx, nullptr,
                                       cand_lab, s.n, s.cells, s.n < kMaxCand ? s.n : kMaxCand,
-                                      kCandStride};
+                                      kCandStride, library_blocks(lib)};
             ++p->n_topk;
             topk_blocks += topk_blocks_of(s.cells, p->topk_run);
         }
@@ -923,6 +923,8 @@ struct ChunkArena {
 
 enum JobState { J_QUEUED = 0, J_RUNNING = 1, J_DONE = 2, J_FAILED = 3 };
 
+constexpr int32_t kChainStreams = 3;  // chain streams of a server (fewer with fewer slots)
+
 struct JobRec {
     JobSpec spec;
     int state = J_QUEUED;
@@ -935,9 +937,12 @@ struct JobRec {
     bool holds_mem = false;
 };
 
+// One batch in flight: its descriptor tables, result staging and events.
+// Its wide stages run on the server's wide stream, its chains on `lane`.
 struct StreamSlot {
-    hipStream_t stream = nullptr;
-    hipEvent_t done = nullptr;
+    int32_t lane = 0;             // chain stream of the running batch
+    hipEvent_t ready = nullptr;   // candidate lists written (wide stream)
+    hipEvent_t done = nullptr;    // results copied back (chain stream)
     uint8_t* d_plan = nullptr;    // descriptor tables + results (device)
     uint8_t* h_plan = nullptr;    // pinned staging of the tables
     uint8_t* h_res = nullptr;     // pinned copy of stats + anneal results
@@ -952,10 +957,23 @@ struct StreamSlot {
 
 }  // namespace
 
+// Batches go through two kinds of stream.  The wide stages up to the
+// candidate lists (cell Labs, K nearest), which fill the device, run in
+// admission order on one wide stream; each batch's chains (placement,
+// annealing: one wave per job, latency-bound) and its short output stages
+// run on one of a few chain streams, after an event from the wide stream.
+// While one batch's chains run, later batches' wide stages use the rest of
+// the device, and wide launches never compete with each other.  Every stream
+// is made with a full CU mask, which gives it a hardware queue of its own
+// (streams without one share a few queues, and a wide launch would wait
+// behind chains in a shared in-order queue).
 struct mh_server {
     mh_context* ctx = nullptr;
     int32_t max_batch = 128;
     ChunkArena arena;
+    hipStream_t wide = nullptr;
+    std::vector<hipStream_t> lanes;
+    std::vector<int32_t> lane_load;  // batches in flight per chain stream
     std::vector<StreamSlot> slots;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -980,26 +998,55 @@ namespace {
 // other threads while RUNNING).
 int launch_batch(mh_server* sv, StreamSlot& sl, const std::vector<JobRec*>& recs) {
     mh_context* ctx = sv->ctx;
+    hipStream_t w = sv->wide, c = sv->lanes[(size_t)sl.lane];
     std::vector<JobSpec> specs;
     std::vector<uint8_t*> mem;
     specs.reserve(recs.size());
     mem.reserve(recs.size());
     for (JobRec* r : recs) {
-        CHK(prepare_job(ctx, &r->spec, sl.stream));
+        CHK(prepare_job(ctx, &r->spec, w));
         specs.push_back(r->spec);
         mem.push_back(r->mem);
     }
     CHK(build_plan(specs, mem, sl.h_plan, sl.d_plan, &sl.plan));
-    HIPCHK(hipMemcpyAsync(sl.d_plan, sl.h_plan, sl.plan.o_stats, hipMemcpyHostToDevice, sl.stream));
+    HIPCHK(hipMemcpyAsync(sl.d_plan, sl.h_plan, sl.plan.o_stats, hipMemcpyHostToDevice, w));
     PhaseEvents pe;
     pe.mask = sl.timed;
     pe.ev = sl.ev;
-    enqueue_plan(sl.plan, sl.d_plan, ctx->d_lut, sl.stream, pe);
+    enqueue_candidates(sl.plan, sl.d_plan, ctx->d_lut, w, pe);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(sl.ready, w));
+    HIPCHK(hipStreamWaitEvent(c, sl.ready, 0));
+    enqueue_chains(sl.plan, sl.d_plan, c, pe);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(sl.h_res, sl.d_plan + sl.plan.o_stats, sl.plan.bytes - sl.plan.o_stats,
-                          hipMemcpyDeviceToHost, sl.stream));
-    HIPCHK(hipEventRecord(sl.done, sl.stream));
+                          hipMemcpyDeviceToHost, c));
+    HIPCHK(hipEventRecord(sl.done, c));
     return MH_OK;
+}
+
+// The chain stream with the fewest batches in flight.
+int32_t pick_lane(const mh_server* sv) {
+    int32_t best = 0;
+    for (int32_t i = 1; i < (int32_t)sv->lane_load.size(); ++i)
+        if (sv->lane_load[(size_t)i] < sv->lane_load[(size_t)best]) best = i;
+    return best;
+}
+
+// A stream with a hardware queue of its own (full CU mask), or a plain
+// non-blocking stream where the runtime has no CU-masked streams.
+hipError_t make_stream(int device, hipStream_t* s) {
+    hipDeviceProp_t prop;
+    hipError_t e = hipGetDeviceProperties(&prop, device);
+    if (e == hipSuccess && prop.multiProcessorCount > 0) {
+        std::vector<uint32_t> mask((size_t)(prop.multiProcessorCount + 31) / 32, 0xffffffffu);
+        if (prop.multiProcessorCount % 32)
+            mask.back() = (1u << (prop.multiProcessorCount % 32)) - 1u;
+        e = hipExtStreamCreateWithCUMask(s, (uint32_t)mask.size(), mask.data());
+        if (e == hipSuccess) return e;
+        (void)hipGetLastError();
+    }
+    return hipStreamCreateWithFlags(s, hipStreamNonBlocking);
 }
 
 void worker_loop(mh_server* sv) {
@@ -1045,6 +1092,7 @@ void worker_loop(mh_server* sv) {
             }
             sl.tickets.clear();
             sl.busy = false;
+            --sv->lane_load[(size_t)sl.lane];
             progressed = true;
             sv->cv_done.notify_all();
         }
@@ -1088,6 +1136,8 @@ void worker_loop(mh_server* sv) {
             }
             if (!take.empty()) {
                 free_slot->busy = true;
+                free_slot->lane = pick_lane(sv);
+                ++sv->lane_load[(size_t)free_slot->lane];
                 free_slot->tickets = take;
                 free_slot->timed = sv->timing;
                 free_slot->cells = 0;
@@ -1100,9 +1150,10 @@ void worker_loop(mh_server* sv) {
                 const std::string m = rc ? mh_last_error() : "";
                 lk.lock();
                 if (rc) {
-                    // nothing of this batch may still run: wait for its stream
+                    // nothing of this batch may still run: wait for its streams
                     lk.unlock();
-                    hipStreamSynchronize(free_slot->stream);
+                    hipStreamSynchronize(sv->wide);
+                    hipStreamSynchronize(sv->lanes[(size_t)free_slot->lane]);
                     lk.lock();
                     for (int64_t t : take) {
                         JobRec& r = sv->jobs[(size_t)t];
@@ -1116,6 +1167,7 @@ void worker_loop(mh_server* sv) {
                     }
                     free_slot->tickets.clear();
                     free_slot->busy = false;
+                    --sv->lane_load[(size_t)free_slot->lane];
                     sv->cv_done.notify_all();
                 }
                 progressed = true;
@@ -1138,8 +1190,11 @@ void worker_loop(mh_server* sv) {
 
 void server_free(mh_server* sv) {
     hipSetDevice(sv->ctx->device);
+    if (sv->wide) hipStreamSynchronize(sv->wide);
+    for (hipStream_t c : sv->lanes)
+        if (c) hipStreamSynchronize(c);
     for (StreamSlot& sl : sv->slots) {
-        if (sl.stream) hipStreamSynchronize(sl.stream);
+        if (sl.ready) hipEventDestroy(sl.ready);
         if (sl.done) hipEventDestroy(sl.done);
         for (auto& pair : sl.ev)
             for (hipEvent_t e : pair)
@@ -1147,8 +1202,10 @@ void server_free(mh_server* sv) {
         if (sl.d_plan) hipFree(sl.d_plan);
         if (sl.h_plan) hipHostFree(sl.h_plan);
         if (sl.h_res) hipHostFree(sl.h_res);
-        if (sl.stream) hipStreamDestroy(sl.stream);
     }
+    for (hipStream_t c : sv->lanes)
+        if (c) hipStreamDestroy(c);
+    if (sv->wide) hipStreamDestroy(sv->wide);
     sv->arena.release_all();
     delete sv;
 }
@@ -1172,9 +1229,13 @@ int mh_server_create(mh_context* ctx, const mh_server_params* p, mh_server** out
     sv->slots.resize((size_t)n_slots);
     BatchPlan tmp;
     const size_t plan = plan_bytes(max_batch, &tmp);
-    hipError_t e = hipSuccess;
+    hipError_t e = make_stream(ctx->device, &sv->wide);
+    sv->lanes.assign((size_t)std::min(n_slots, kChainStreams), nullptr);
+    sv->lane_load.assign(sv->lanes.size(), 0);
+    for (hipStream_t& c : sv->lanes)
+        if (e == hipSuccess) e = make_stream(ctx->device, &c);
     for (StreamSlot& sl : sv->slots) {
-        if (e == hipSuccess) e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.ready, hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
         for (auto& pair : sl.ev)
             for (hipEvent_t& ev : pair)

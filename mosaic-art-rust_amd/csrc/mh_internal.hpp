@@ -124,6 +124,13 @@ struct mh_library {
     float* d_L = nullptr;
     float* d_A = nullptr;
     float* d_B = nullptr;
+    // the K-nearest scan's block layout (MatBlocks)
+    float* d_pL = nullptr;
+    float* d_pA = nullptr;
+    float* d_pB = nullptr;
+    int32_t* d_pidx = nullptr;
+    float4* d_box = nullptr;
+    int32_t nb = 0;
     float* d_sim = nullptr;    // n x n after build_similarities
     float* d_term = nullptr;   // n x n 1/(sim+1), built on first use by the cost kernels
     uint8_t* d_pixels = nullptr;
@@ -151,6 +158,8 @@ int get_taps(mh_context* ctx, int32_t in_size, int32_t out_size, ResizeTaps* t);
 // Device memory of a library / batch freed, the handle detached from its
 // context (the context is being destroyed).
 void library_release(mh_library* lib);
+// The library's block layout for the K-nearest kernels.
+MatBlocks library_blocks(const mh_library* lib);
 void batch_release(mh_batch* b);
 int get_temps(mh_context* ctx, float t0, float decay, int64_t len, const float** out);
 void fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);

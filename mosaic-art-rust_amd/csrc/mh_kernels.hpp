@@ -79,6 +79,20 @@ struct CellSeg {
 };
 
 // K nearest materials of nq query Labs against one library (SoA Lab table).
+// A library's materials in blocks of 64 (median splits on the widest Lab
+// axis, so each block is a compact box): slot 64 b + i of the permuted
+// arrays is member i of block b (slots past n are padding), pidx holds each
+// slot's material index, box[2 b] / box[2 b + 1] the block's low / high
+// corner (L, a, b; w unused).
+struct MatBlocks {
+    const float* pL;
+    const float* pA;
+    const float* pB;
+    const int32_t* pidx;
+    const float4* box;
+    int32_t nb;
+};
+
 struct TopkSeg {
     const float* mL;
     const float* mA;
@@ -88,6 +102,7 @@ struct TopkSeg {
     float* out_d;        // optional
     float4* out_lab;     // optional {l, a, b, sqdist}; rows padded with idx -1 up to out_stride
     int32_t n, nq, k, out_stride;
+    MatBlocks blk;
 };
 
 // Finishing one placed grid: per-cell adjustment and compositing.
@@ -113,9 +128,9 @@ void launch_sim_full(const float4* lab, int32_t n, float* out, hipStream_t strea
 void launch_sim_packed(const float4* lab, int32_t n, float* out, hipStream_t stream);
 void launch_term_full(const float4* lab, int32_t n, float* out, hipStream_t stream);
 // out_d / out_lab ({l, a, b, sqdist} of each candidate) may be null.
-void launch_topk(const float* mL, const float* mA, const float* mB, int32_t n, const float* query,
-                 int32_t nq, int32_t k, int32_t* out_idx, float* out_d, float4* out_lab,
-                 int32_t out_stride, hipStream_t stream);
+void launch_topk(const float* mL, const float* mA, const float* mB, int32_t n, const MatBlocks& blk,
+                 const float* query, int32_t nq, int32_t k, int32_t* out_idx, float* out_d,
+                 float4* out_lab, int32_t out_stride, hipStream_t stream);
 // Batched launches over segment tables: starts[j] is the first block of
 // segment j (starts[n_segs] = total).  *_blocks give each segment's count.
 int32_t cell_lab_blocks(int32_t n_cells);
