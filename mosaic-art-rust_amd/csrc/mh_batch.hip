@@ -353,7 +353,7 @@ void enqueue_candidates(const BatchPlan& p, uint8_t* d, const float* lut, hipStr
     pe.mark(MH_PHASE_TOPK, 1, s);
 }
 
-// The stages from placement on (chains, adjustment, composite).
+// The chain stages (placement, annealing).
 void enqueue_chains(const BatchPlan& p, uint8_t* d, hipStream_t s,
                     const PhaseEvents& pe = PhaseEvents()) {
     pe.mark(MH_PHASE_PLACE, 0, s);
@@ -374,6 +374,11 @@ void enqueue_chains(const BatchPlan& p, uint8_t* d, hipStream_t s,
             launch_anneal_batch(aj + p.anneal_start[kind], count, kind, p.anneal_lds[kind], s);
     }
     pe.mark(MH_PHASE_ANNEAL, 1, s);
+}
+
+// The output stages (colour adjustment, composite).
+void enqueue_outputs(const BatchPlan& p, uint8_t* d, hipStream_t s,
+                     const PhaseEvents& pe = PhaseEvents()) {
     pe.mark(MH_PHASE_ADJUST, 0, s);
     if (p.adj_blocks > 0)  // the target averages came from the cell Lab pass
         launch_cell_adjust_batch(at<CompSeg>(d, p.o_comp), at<int32_t>(d, p.o_adj_st), p.n_comp,
@@ -389,6 +394,7 @@ void enqueue_plan(const BatchPlan& p, uint8_t* d, const float* lut, hipStream_t 
                   const PhaseEvents& pe = PhaseEvents()) {
     enqueue_candidates(p, d, lut, s, pe);
     enqueue_chains(p, d, s, pe);
+    enqueue_outputs(p, d, s, pe);
 }
 
 void fill_result(const JobSpec& s, const int32_t* st, const AnnealOut* ao, mh_job_result* r) {
@@ -458,6 +464,7 @@ void enqueue_batch(mh_batch* b) {
     hipEventRecord(b->fork, ctx->stream);
     hipStreamWaitEvent(b->side, b->fork, 0);
     enqueue_chains(b->plan[0], b->d_plan + b->plan_off[0], ctx->stream);
+    enqueue_outputs(b->plan[0], b->d_plan + b->plan_off[0], ctx->stream);
     enqueue_plan(b->plan[1], b->d_plan + b->plan_off[1], ctx->d_lut, b->side);
     hipEventRecord(b->join, b->side);
     hipStreamWaitEvent(ctx->stream, b->join, 0);
@@ -923,7 +930,7 @@ struct ChunkArena {
 
 enum JobState { J_QUEUED = 0, J_RUNNING = 1, J_DONE = 2, J_FAILED = 3 };
 
-constexpr int32_t kChainStreams = 3;  // chain streams of a server (fewer with fewer slots)
+constexpr int32_t kChainStreams = 4;  // chain streams of a server (fewer with fewer slots)
 
 struct JobRec {
     JobSpec spec;
@@ -941,8 +948,9 @@ struct JobRec {
 // Its wide stages run on the server's wide stream, its chains on `lane`.
 struct StreamSlot {
     int32_t lane = 0;             // chain stream of the running batch
-    hipEvent_t ready = nullptr;   // candidate lists written (wide stream)
-    hipEvent_t done = nullptr;    // results copied back (chain stream)
+    hipEvent_t ready = nullptr;    // candidate lists written (wide stream)
+    hipEvent_t chained = nullptr;  // grids placed and annealed (chain stream)
+    hipEvent_t done = nullptr;     // images written, results copied back (output stream)
     uint8_t* d_plan = nullptr;    // descriptor tables + results (device)
     uint8_t* h_plan = nullptr;    // pinned staging of the tables
     uint8_t* h_res = nullptr;     // pinned copy of stats + anneal results
@@ -957,13 +965,14 @@ struct StreamSlot {
 
 }  // namespace
 
-// Batches go through two kinds of stream.  The wide stages up to the
+// Batches go through three kinds of stream.  The wide stages up to the
 // candidate lists (cell Labs, K nearest), which fill the device, run in
 // admission order on one wide stream; each batch's chains (placement,
-// annealing: one wave per job, latency-bound) and its short output stages
-// run on one of a few chain streams, after an event from the wide stream.
-// While one batch's chains run, later batches' wide stages use the rest of
-// the device, and wide launches never compete with each other.  Every stream
+// annealing: one wave per job, latency-bound) run on one of a few chain
+// streams after an event from the wide stream; its output stages
+// (adjustment, composite: wide again) run in admission order on one output
+// stream after an event from its chain stream.  While one batch's chains
+// run, other batches' wide stages use the rest of the device.  Every stream
 // is made with a full CU mask, which gives it a hardware queue of its own
 // (streams without one share a few queues, and a wide launch would wait
 // behind chains in a shared in-order queue).
@@ -971,7 +980,7 @@ struct mh_server {
     mh_context* ctx = nullptr;
     int32_t max_batch = 128;
     ChunkArena arena;
-    hipStream_t wide = nullptr;
+    hipStream_t wide = nullptr, out = nullptr;
     std::vector<hipStream_t> lanes;
     std::vector<int32_t> lane_load;  // batches in flight per chain stream
     std::vector<StreamSlot> slots;
@@ -1019,9 +1028,13 @@ int launch_batch(mh_server* sv, StreamSlot& sl, const std::vector<JobRec*>& recs
     HIPCHK(hipStreamWaitEvent(c, sl.ready, 0));
     enqueue_chains(sl.plan, sl.d_plan, c, pe);
     HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(sl.chained, c));
+    HIPCHK(hipStreamWaitEvent(sv->out, sl.chained, 0));
+    enqueue_outputs(sl.plan, sl.d_plan, sv->out, pe);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(sl.h_res, sl.d_plan + sl.plan.o_stats, sl.plan.bytes - sl.plan.o_stats,
-                          hipMemcpyDeviceToHost, c));
-    HIPCHK(hipEventRecord(sl.done, c));
+                          hipMemcpyDeviceToHost, sv->out));
+    HIPCHK(hipEventRecord(sl.done, sv->out));
     return MH_OK;
 }
 
@@ -1154,6 +1167,7 @@ void worker_loop(mh_server* sv) {
                     lk.unlock();
                     hipStreamSynchronize(sv->wide);
                     hipStreamSynchronize(sv->lanes[(size_t)free_slot->lane]);
+                    hipStreamSynchronize(sv->out);
                     lk.lock();
                     for (int64_t t : take) {
                         JobRec& r = sv->jobs[(size_t)t];
@@ -1193,8 +1207,10 @@ void server_free(mh_server* sv) {
     if (sv->wide) hipStreamSynchronize(sv->wide);
     for (hipStream_t c : sv->lanes)
         if (c) hipStreamSynchronize(c);
+    if (sv->out) hipStreamSynchronize(sv->out);
     for (StreamSlot& sl : sv->slots) {
         if (sl.ready) hipEventDestroy(sl.ready);
+        if (sl.chained) hipEventDestroy(sl.chained);
         if (sl.done) hipEventDestroy(sl.done);
         for (auto& pair : sl.ev)
             for (hipEvent_t e : pair)
@@ -1206,6 +1222,7 @@ void server_free(mh_server* sv) {
     for (hipStream_t c : sv->lanes)
         if (c) hipStreamDestroy(c);
     if (sv->wide) hipStreamDestroy(sv->wide);
+    if (sv->out) hipStreamDestroy(sv->out);
     sv->arena.release_all();
     delete sv;
 }
@@ -1230,12 +1247,14 @@ int mh_server_create(mh_context* ctx, const mh_server_params* p, mh_server** out
     BatchPlan tmp;
     const size_t plan = plan_bytes(max_batch, &tmp);
     hipError_t e = make_stream(ctx->device, &sv->wide);
+    if (e == hipSuccess) e = make_stream(ctx->device, &sv->out);
     sv->lanes.assign((size_t)std::min(n_slots, kChainStreams), nullptr);
     sv->lane_load.assign(sv->lanes.size(), 0);
     for (hipStream_t& c : sv->lanes)
         if (e == hipSuccess) e = make_stream(ctx->device, &c);
     for (StreamSlot& sl : sv->slots) {
         if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.ready, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.chained, hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
         for (auto& pair : sl.ev)
             for (hipEvent_t& ev : pair)
