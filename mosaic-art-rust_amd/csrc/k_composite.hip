@@ -250,28 +250,33 @@ void launch_cell_adjust(const int32_t* grid, int32_t cells, const uint8_t* tile_
                        cells, tile_avg, target_avg, strength, out);
 }
 
-// Output pixel idx of a gw*tw x gh*th mosaic.
-// Output pixel (X, Y); Y is the same for the whole block (a block covers
-// 256 pixels of one row), so only X needs a per-pixel division.
+// Output pixel (X, Y) of a gw*tw x gh*th mosaic.  Y is the same for the
+// whole block (a block covers pixels of one row), so its cell row and the
+// row's offsets are uniform; X's cell column is umulhi(X, tw_magic) with
+// tw_magic = floor((2^32 - 1) / tw) + 1, exact while X * tw < 2^32 (then
+// X m / 2^32 exceeds X / tw by less than 1 / tw), else a division.
 __device__ __forceinline__ void composite_pixel(const int32_t* __restrict__ grid, int32_t gw,
                                                 int32_t tw, int32_t th, int32_t ow, int32_t X,
                                                 int32_t Y, const uint8_t* __restrict__ tiles,
                                                 const float4* __restrict__ adj, int32_t adjust,
                                                 uint8_t* __restrict__ out) {
-    const int64_t idx = (int64_t)Y * ow + X;
-    const int32_t cx = X / tw, cy = Y / th;
+    const int32_t cy = Y / th, dy = Y - cy * th;
+    const uint32_t tw_magic = 0xffffffffu / (uint32_t)tw + 1u;
+    const bool magic_ok = (uint64_t)(uint32_t)ow * (uint32_t)tw < (1ull << 32);
+    const int32_t cx = magic_ok ? (int32_t)__umulhi((uint32_t)X, tw_magic) : X / tw;
     const int32_t cell = cy * gw + cx;
-    const int32_t m = grid[cell];
+    const int32_t m = gload(grid, cell);
     uint32_t r = 0, g = 0, b = 0;
     if (m != kEmpty) {
-        const int32_t dx = X - cx * tw, dy = Y - cy * th;
-        const uint8_t* p = tiles + (((int64_t)m * th + dy) * tw + dx) * 3;
+        const int32_t dx = X - cx * tw;
+        const MH_GAS uint8_t* p =
+            (const MH_GAS uint8_t*)tiles + (((int64_t)m * th + dy) * tw + dx) * 3;
         r = p[0];
         g = p[1];
         b = p[2];
-        if (adjust) adjust_pixel(adj[cell], r, g, b, r, g, b);
+        if (adjust) adjust_pixel(gload4(adj, cell), r, g, b, r, g, b);
     }
-    uint8_t* d = out + idx * 3;
+    MH_GAS uint8_t* d = (MH_GAS uint8_t*)out + ((int64_t)Y * ow + X) * 3;
     d[0] = (uint8_t)r;
     d[1] = (uint8_t)g;
     d[2] = (uint8_t)b;

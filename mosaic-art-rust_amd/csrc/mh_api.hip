@@ -1008,12 +1008,13 @@ static int ragged_lab(mh_context* ctx, const uint8_t* d_pixels, const RaggedPlan
     float* d_lab;
     CHK(slot(ctx, S_OUT0, (size_t)n * 3, &d_lab));
     int32_t blocks = 0;
+    const int32_t cpb = cell_lab_cells_per_block(n);
     for (int32_t g = 0; g < n_segs; ++g) {
         const PixelGroup& pg = p.groups[g];
         segs[g] = CellSeg{d_pixels + pg.offset, (int64_t)pg.w * 3, (int64_t)pg.w * pg.h * 3, 0, 1,
                           pg.w, pg.h, pg.count, d_lab + 3 * (int64_t)p.group_first[g], nullptr};
         starts[g] = blocks;
-        blocks += cell_lab_blocks(pg.count);
+        blocks += cell_lab_blocks(pg.count, cpb);
     }
     starts[n_segs] = blocks;
     CellSeg* d_segs;
@@ -1021,7 +1022,7 @@ static int ragged_lab(mh_context* ctx, const uint8_t* d_pixels, const RaggedPlan
     CHK(upload(ctx, S_IN1, segs.data(), segs.size(), &d_segs));
     CHK(upload(ctx, S_IN2, starts.data(), starts.size(), &d_starts));
     phase_begin(ctx, MH_PHASE_CELL_LAB);
-    launch_cell_lab_batch(d_segs, d_starts, n_segs, blocks, ctx->d_lut, ctx->stream);
+    launch_cell_lab_batch(d_segs, d_starts, n_segs, blocks, cpb, ctx->d_lut, ctx->stream);
     phase_end(ctx, MH_PHASE_CELL_LAB);
     HIPCHK(hipGetLastError());
     std::vector<float> packed((size_t)n * 3);

@@ -133,7 +133,7 @@ struct BatchPlan {
     size_t o_lab = 0, o_lab_st = 0, o_topk = 0, o_topk_st = 0, o_place = 0, o_anneal = 0;
     size_t o_comp = 0, o_adj_st = 0, o_comp_st = 0;
     size_t o_stats = 0, o_aout = 0, bytes = 0;
-    int32_t lab_blocks = 0;
+    int32_t lab_blocks = 0, lab_cpb = 4;
     int32_t n_topk = 0, topk_blocks = 0, topk_run = 1;
     size_t place_lds = 0;
     int32_t place_score = kScoreSq;  // kScoreSq / kScoreDist / kScoreMixed over the jobs
@@ -210,9 +210,12 @@ int build_plan(const std::vector<JobSpec>& specs, const std::vector<uint8_t*>& j
 
     int64_t lab_blocks = 0, topk_blocks = 0, adj_blocks = 0, comp_blocks = 0;
     p->n_topk = p->n_anneal = p->n_comp = 0;
-    int64_t topk_queries = 0;
-    for (const JobSpec& s : specs)
+    int64_t topk_queries = 0, all_cells = 0;
+    for (const JobSpec& s : specs) {
         if (s.n > 0) topk_queries += s.cells;
+        all_cells += s.cells;
+    }
+    p->lab_cpb = cell_lab_cells_per_block(all_cells);
     p->topk_run = p->tune_topk_run > 0 ? p->tune_topk_run : topk_run_length(topk_queries);
     p->place_lds = 0;
     p->place_score = kScoreSq;
@@ -240,7 +243,7 @@ int build_plan(const std::vector<JobSpec>& specs, const std::vector<uint8_t*>& j
         uint8_t* tavg = s.adjust ? at<uint8_t>(m, s.o_tavg) : nullptr;
         lab[i] = CellSeg{j.d_target, s.pitch, (int64_t)s.th * s.pitch, (int64_t)s.tw * 3, s.gw, s.tw,
                          s.th, s.cells, cell_lab, tavg};
-        lab_blocks += cell_lab_blocks(s.cells);
+        lab_blocks += cell_lab_blocks(s.cells, p->lab_cpb);
         // a3: K nearest (libraries with materials only)
         if (s.n > 0) {
             topk_st[p->n_topk] = (int32_t)topk_blocks;
@@ -345,7 +348,7 @@ void enqueue_candidates(const BatchPlan& p, uint8_t* d, const float* lut, hipStr
                         const PhaseEvents& pe = PhaseEvents()) {
     pe.mark(MH_PHASE_CELL_LAB, 0, s);
     launch_cell_lab_batch(at<CellSeg>(d, p.o_lab), at<int32_t>(d, p.o_lab_st), p.n_jobs, p.lab_blocks,
-                          lut, s);
+                          p.lab_cpb, lut, s);
     pe.mark(MH_PHASE_CELL_LAB, 1, s);
     pe.mark(MH_PHASE_TOPK, 0, s);
     launch_topk_batch(at<TopkSeg>(d, p.o_topk), at<int32_t>(d, p.o_topk_st), p.n_topk, p.topk_blocks,

@@ -179,9 +179,13 @@ __device__ __forceinline__ void rgb8_to_lab(const float* lut, uint32_t r8, uint3
     float x = 0.4124564f * r + 0.3575761f * g + 0.1804375f * b;
     float y = 0.2126729f * r + 0.7151522f * g + 0.0721750f * b;
     float z = 0.0193339f * r + 0.1191920f * g + 0.9503041f * b;
-    x = x / 0.95047f;
-    y = y / 1.0f;
-    z = z / 1.08883f;
+    // x / 0.95047f and z / 1.08883f as products with the rounded reciprocal,
+    // corrected once by the fused residual: equal to the divisions for the
+    // x and z of all 2^24 u8 colours (an exhaustive host check); y / 1 is y
+    const float rx = 1.0f / 0.95047f, rz = 1.0f / 1.08883f;
+    const float qx = x * rx, qz = z * rz;
+    x = __builtin_fmaf(__builtin_fmaf(-qx, 0.95047f, x), rx, qx);
+    z = __builtin_fmaf(__builtin_fmaf(-qz, 1.08883f, z), rz, qz);
     float fx = lab_f(x), fy = lab_f(y), fz = lab_f(z);
     L = clampf(fy * 116.0f - 16.0f, 0.0f, 100.0f);
     A = clampf((fx - fy) * 500.0f, -128.0f, 127.0f);
@@ -258,6 +262,17 @@ __device__ __forceinline__ void rgb_to_hsv(float r, float g, float b, float& h, 
     v = clampf(mx, 0.0f, 1.0f);
 }
 
+// x / 60.0f for x == 0 and for 2^-124 <= x < 360: the product with
+// RN(1/60), corrected once by its fused residual.  Equal to the division
+// for every f32 in that range (an exhaustive host check); below 2^-124 the
+// quotient nears the subnormal range and the shortcut can be off by one ulp.
+__device__ __forceinline__ float div60_fast(float x) {
+    const float c = 1.0f / 60.0f;
+    const float q = x * c;
+    const float r = __builtin_fmaf(-q, 60.0f, x);
+    return __builtin_fmaf(r, c, q);
+}
+
 // palette Srgb::from_color(Hsv) (src/color_adjustment.rs:76).
 __device__ __forceinline__ void hsv_to_rgb(float hdeg, float s, float v, float& r, float& g,
                                            float& b) {
@@ -265,18 +280,26 @@ __device__ __forceinline__ void hsv_to_rgb(float hdeg, float s, float v, float& 
     // hdeg - floor(hdeg/360)*360: for 0 <= hdeg < 360 the quotient rounds
     // below 1, so the floor is +0 (-0 for hdeg = -0) and the result is
     // hdeg + 0.0f (which turns -0 into +0, as the subtraction does)
-    float hp = hdeg >= 0.0f && hdeg < 360.0f ? hdeg + 0.0f : hdeg - floorf(hdeg / 360.0f) * 360.0f;
-    float h = hp / 60.0f;
+    float hp = hdeg + 0.0f;
+    if (__builtin_expect(!(hdeg >= 0.0f && hdeg < 360.0f), 0))
+        hp = hdeg - floorf(hdeg / 360.0f) * 360.0f;
+    float h = div60_fast(hp);
+    if (__builtin_expect(!(hp == 0.0f || (hp >= 0x1p-124f && hp < 360.0f)), 0)) h = hp / 60.0f;
     float hm2 = h - 2.0f * floorf(h * 0.5f);
     float x = c * (1.0f - fabsf(hm2 - 1.0f));
     float m = v - c;
-    float rr, gg, bb;
-    if (h >= 0.0f && h < 1.0f) { rr = c; gg = x; bb = 0.0f; }
-    else if (h >= 1.0f && h < 2.0f) { rr = x; gg = c; bb = 0.0f; }
-    else if (h >= 2.0f && h < 3.0f) { rr = 0.0f; gg = c; bb = x; }
-    else if (h >= 3.0f && h < 4.0f) { rr = 0.0f; gg = x; bb = c; }
-    else if (h >= 4.0f && h < 5.0f) { rr = x; gg = 0.0f; bb = c; }
-    else { rr = c; gg = 0.0f; bb = x; }
+    // the sextant of h: [0,1) .. [4,5) are disjoint, anything else
+    // (h >= 5, NaN) is the last case; each channel as selects
+    const bool s0 = h >= 0.0f && h < 1.0f, s1 = h >= 1.0f && h < 2.0f;
+    const bool s2 = h >= 2.0f && h < 3.0f, s3 = h >= 3.0f && h < 4.0f;
+    const bool s4 = h >= 4.0f && h < 5.0f;
+    //        s0  s1  s2  s3  s4  else
+    // rr:     c   x   0   0   x   c
+    // gg:     x   c   c   x   0   0
+    // bb:     0   0   x   c   c   x
+    const float rr = (s1 || s4) ? x : ((s2 || s3) ? 0.0f : c);
+    const float gg = (s0 || s3) ? x : ((s1 || s2) ? c : 0.0f);
+    const float bb = (s0 || s1) ? 0.0f : ((s3 || s4) ? c : x);
     r = clampf(rr + m, 0.0f, 1.0f);
     g = clampf(gg + m, 0.0f, 1.0f);
     b = clampf(bb + m, 0.0f, 1.0f);

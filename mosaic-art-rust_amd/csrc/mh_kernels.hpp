@@ -133,9 +133,12 @@ void launch_topk(const float* mL, const float* mA, const float* mB, int32_t n, c
                  float4* out_lab, int32_t out_stride, hipStream_t stream);
 // Batched launches over segment tables: starts[j] is the first block of
 // segment j (starts[n_segs] = total).  *_blocks give each segment's count.
-int32_t cell_lab_blocks(int32_t n_cells);
+// cells per block (4 or 16) for a launch over total_cells cells
+int32_t cell_lab_cells_per_block(int64_t total_cells);
+int32_t cell_lab_blocks(int32_t n_cells, int32_t cells_per_block);
 void launch_cell_lab_batch(const CellSeg* d_segs, const int32_t* d_starts, int32_t n_segs,
-                           int32_t total_blocks, const float* lut, hipStream_t stream);
+                           int32_t total_blocks, int32_t cells_per_block, const float* lut,
+                           hipStream_t stream);
 // one wave per run of `run` consecutive queries of a segment
 int32_t topk_run_length(int64_t total_queries);
 int32_t topk_blocks(int32_t nq, int32_t run);

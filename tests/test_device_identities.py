@@ -1,6 +1,7 @@
-"""Host checks of the exact shortcuts the device colour-adjustment code takes
+"""Host checks of the exact shortcuts the device colour code takes
 (mosaic-art-rust_amd/csrc/mh_device.hpp: u8_unit, fmod_360, hsv_to_rgb's
-hue normalisation).  Each shortcut is compared bit for bit with the IEEE
+hue normalisation and its division by 60, rgb8_to_lab's divisions by the
+white point).  Each shortcut is compared bit for bit with the IEEE
 operation it replaces, over every input that matters (all 256 u8 values) or
 over dense sweeps around the branch boundaries.  The C program below uses the
 same f32 operations as the device helpers (fused multiply-add included); the
@@ -38,6 +39,13 @@ static float hue_norm_fast(float h) {
     return h >= 0.0f && h < 360.0f ? h + 0.0f : h - floorf(h / 360.0f) * 360.0f;
 }
 static float hue_norm(float h) { return h - floorf(h / 360.0f) * 360.0f; }
+
+static float div60_fast(float x) {
+    const float c = 1.0f / 60.0f;
+    const float q = x * c;
+    const float r = fmaf(-q, 60.0f, x);
+    return fmaf(r, c, q);
+}
 
 int main(void) {
     long bad = 0, n = 0;
@@ -79,6 +87,42 @@ int main(void) {
     /* the top of the fast hue range: every f32 in [359, 360) */
     for (float a = 359.0f; a < 360.0f; a = nextafterf(a, INFINITY), ++n)
         if (bits(hue_norm_fast(a)) != bits(hue_norm(a))) { ++bad; if (bad < 10) printf("hue %a\n", a); }
+    /* div60_fast: every f32 in [2^-124, 360) and 0 */
+    long bad60 = bits(div60_fast(0.0f)) != bits(0.0f / 60.0f), n60 = 1;
+    const uint32_t lo = bits(0x1p-124f), hi = bits(360.0f);
+    #pragma omp parallel for reduction(+ : bad60, n60) schedule(static)
+    for (long u = lo; u < (long)hi; ++u) {
+        float x;
+        const uint32_t w = (uint32_t)u;
+        memcpy(&x, &w, 4);
+        if (bits(div60_fast(x)) != bits(x / 60.0f)) ++bad60;
+        ++n60;
+    }
+    if (bad60) printf("div60 mismatches %ld\n", bad60);
+    n += n60;
+    bad += bad60;
+    /* rgb8_to_lab: x / 0.95047f and z / 1.08883f through the rounded
+       reciprocal and one fused correction, for the x and z of every u8
+       colour (the linearised ramp as the context builds it) */
+    float lut[256];
+    for (int v = 0; v < 256; ++v) {
+        const float c = (float)v / 255.0f;
+        lut[v] = c <= 0.04045f ? c / 12.92f : (float)pow(((double)c + 0.055) / 1.055, 2.4);
+    }
+    const float rx = 1.0f / 0.95047f, rz = 1.0f / 1.08883f;
+    long badw = 0;
+    for (int r8 = 0; r8 < 256; ++r8)
+        for (int g8 = 0; g8 < 256; ++g8)
+            for (int b8 = 0; b8 < 256; ++b8, n += 2) {
+                const float r = lut[r8], g = lut[g8], b = lut[b8];
+                const float x = 0.4124564f * r + 0.3575761f * g + 0.1804375f * b;
+                const float z = 0.0193339f * r + 0.1191920f * g + 0.9503041f * b;
+                const float qx = x * rx, qz = z * rz;
+                if (bits(fmaf(fmaf(-qx, 0.95047f, x), rx, qx)) != bits(x / 0.95047f)) ++badw;
+                if (bits(fmaf(fmaf(-qz, 1.08883f, z), rz, qz)) != bits(z / 1.08883f)) ++badw;
+            }
+    if (badw) printf("white point mismatches %ld\n", badw);
+    bad += badw;
     printf("checked %ld bad %ld\n", n, bad);
     return bad != 0;
 }
@@ -90,8 +134,8 @@ def test_colour_adjustment_shortcuts_are_exact(tmp_path):
     src = tmp_path / "ident.c"
     src.write_text(PROG)
     exe = tmp_path / "ident"
-    subprocess.run(["gcc", "-O1", "-ffp-contract=off", "-fno-fast-math", str(src), "-o", str(exe), "-lm"],
-                   check=True)
-    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    subprocess.run(["gcc", "-O2", "-fopenmp", "-ffp-contract=off", "-fno-fast-math", str(src), "-o",
+                    str(exe), "-lm"], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout[-2000:]
     assert "bad 0" in r.stdout
