@@ -274,7 +274,9 @@ __device__ __forceinline__ void composite_pixel(const int32_t* __restrict__ grid
         r = p[0];
         g = p[1];
         b = p[2];
-        if (adjust) adjust_pixel(gload4(adj, cell), r, g, b, r, g, b);
+        // per-cell adjustments come from calculate_optimal_adjustment and
+        // ColorAdjustment::new's clamps (cell_adjust_one)
+        if (adjust) adjust_pixel<true>(gload4(adj, cell), r, g, b, r, g, b);
     }
     MH_GAS uint8_t* d = (MH_GAS uint8_t*)out + ((int64_t)Y * ow + X) * 3;
     d[0] = (uint8_t)r;
@@ -282,18 +284,28 @@ __device__ __forceinline__ void composite_pixel(const int32_t* __restrict__ grid
     d[2] = (uint8_t)b;
 }
 
-// One thread per output pixel (ow = gw*tw, oh = gh*th); block b covers
-// pixels [256 (b % per_row), 256 (b % per_row) + 256) of row b / per_row.
+// Pixels per thread: a block of `threads` covers kCompPx * threads pixels
+// of one output row, pixel X = base + k * threads + thread (k < kCompPx),
+// so the block's setup (segment, row, divisors) is shared by kCompPx
+// pixels per thread and every k-step stores a contiguous run.
+constexpr int32_t kCompPx = 4;
+
+// One output row segment per block (ow = gw*tw, oh = gh*th); block b covers
+// the 256 * kCompPx pixels from 256 * kCompPx * (b % per_row) of row
+// b / per_row.
 __global__ __launch_bounds__(256) void composite_kernel(const int32_t* __restrict__ grid,
                                                         int32_t gw, int32_t tw, int32_t th,
                                                         int32_t ow, const uint8_t* __restrict__ tiles,
                                                         const float4* __restrict__ adj,
                                                         int32_t adjust, uint8_t* __restrict__ out) {
-    const int32_t per_row = (ow + 255) / 256;
+    const int32_t per_row = (ow + 256 * kCompPx - 1) / (256 * kCompPx);
     const int32_t Y = (int32_t)blockIdx.x / per_row;
-    const int32_t X = ((int32_t)blockIdx.x - Y * per_row) * 256 + (int32_t)threadIdx.x;
-    if (X >= ow) return;
-    composite_pixel(grid, gw, tw, th, ow, X, Y, tiles, adj, adjust, out);
+    const int32_t X0 = ((int32_t)blockIdx.x - Y * per_row) * 256 * kCompPx + (int32_t)threadIdx.x;
+#pragma unroll
+    for (int32_t k = 0; k < kCompPx; ++k) {
+        const int32_t X = X0 + k * 256;
+        if (X < ow) composite_pixel(grid, gw, tw, th, ow, X, Y, tiles, adj, adjust, out);
+    }
 }
 
 void launch_composite(const int32_t* grid, int32_t gw, int32_t gh, int32_t tw, int32_t th,
@@ -319,7 +331,8 @@ __global__ __launch_bounds__(256) void cell_adjust_batch_kernel(const CompSeg* _
     cell_adjust_one(s.grid, c, s.tile_avg, s.target_avg, s.strength, s.adj);
 }
 
-// A block covers blockDim.x (64, 128 or 256) pixels of one output row.
+// A block of blockDim.x (64, 128 or 256) threads covers kCompPx pixels per
+// thread of one output row.
 __global__ __launch_bounds__(256) void composite_batch_kernel(const CompSeg* __restrict__ segs,
                                                               const int32_t* __restrict__ starts,
                                                               int32_t n_segs) {
@@ -327,18 +340,22 @@ __global__ __launch_bounds__(256) void composite_batch_kernel(const CompSeg* __r
     const CompSeg& s = segs[j];
     const int32_t ow = s.gw * s.tw;
     const int32_t bs = (int32_t)blockDim.x;
-    const int32_t per_row = (ow + bs - 1) / bs;
+    const int32_t per_row = (ow + bs * kCompPx - 1) / (bs * kCompPx);
     const int32_t b = (int32_t)blockIdx.x - starts[j];
     const int32_t Y = b / per_row;
-    const int32_t X = (b - Y * per_row) * bs + (int32_t)threadIdx.x;
-    if (X >= ow) return;
-    composite_pixel(s.grid, s.gw, s.tw, s.th, ow, X, Y, s.tiles, s.adj, s.adjust, s.out);
+    const int32_t X0 = (b - Y * per_row) * bs * kCompPx + (int32_t)threadIdx.x;
+#pragma unroll
+    for (int32_t k = 0; k < kCompPx; ++k) {
+        const int32_t X = X0 + k * bs;
+        if (X < ow)
+            composite_pixel(s.grid, s.gw, s.tw, s.th, ow, X, Y, s.tiles, s.adj, s.adjust, s.out);
+    }
 }
 
 int32_t cell_adjust_blocks(int32_t cells) { return cells > 0 ? (cells + 255) / 256 : 0; }
 
 int32_t composite_blocks(int32_t ow, int32_t oh, int32_t threads) {
-    return ow > 0 && oh > 0 ? ((ow + threads - 1) / threads) * oh : 0;
+    return ow > 0 && oh > 0 ? ((ow + threads * kCompPx - 1) / (threads * kCompPx)) * oh : 0;
 }
 
 void launch_cell_adjust_batch(const CompSeg* d_segs, const int32_t* d_starts, int32_t n_segs,

@@ -234,12 +234,21 @@ __device__ __forceinline__ float draw_unit(uint64_t seed, uint64_t it) {
 }
 
 // ------------------------------------------------------------ HSV helpers
+// clampf(v, 0, 1) for a v that is neither NaN nor -0: the median of
+// (v, 0, 1) is then the same value (+0 below, 1 above, v between).
+__device__ __forceinline__ float clamp_unit(float v) { return __builtin_amdgcn_fmed3f(v, 0.0f, 1.0f); }
+
 // palette Hsv::from_color(Srgb<f32>) (src/color_adjustment.rs:70, :122).
+// kUnit: r, g, b lie in [0, 1] and none is -0 (see adjust_pixel), so the
+// max(., 0) are identities and s, v are clamps of non-negative values.
+template <bool kUnit = false>
 __device__ __forceinline__ void rgb_to_hsv(float r, float g, float b, float& h, float& s,
                                            float& v) {
-    r = r > 0.0f ? r : 0.0f;
-    g = g > 0.0f ? g : 0.0f;
-    b = b > 0.0f ? b : 0.0f;
+    if (!kUnit) {
+        r = r > 0.0f ? r : 0.0f;
+        g = g > 0.0f ? g : 0.0f;
+        b = b > 0.0f ? b : 0.0f;
+    }
     float mx, mn, sep, coeff;
     if (r > g) {
         mx = r; mn = g; sep = g - b; coeff = 0.0f;
@@ -258,8 +267,8 @@ __device__ __forceinline__ void rgb_to_hsv(float r, float g, float b, float& h, 
         ss = d / mx;
     }
     h = hh;
-    s = clampf(ss, 0.0f, 1.0f);
-    v = clampf(mx, 0.0f, 1.0f);
+    s = kUnit ? clamp_unit(ss) : clampf(ss, 0.0f, 1.0f);
+    v = kUnit ? clamp_unit(mx) : clampf(mx, 0.0f, 1.0f);
 }
 
 // x / 60.0f for x == 0 and for 2^-124 <= x < 360: the product with
@@ -274,6 +283,8 @@ __device__ __forceinline__ float div60_fast(float x) {
 }
 
 // palette Srgb::from_color(Hsv) (src/color_adjustment.rs:76).
+// kUnit: v in [0, 1] and not -0, s in [0, 1] (see adjust_pixel).
+template <bool kUnit = false>
 __device__ __forceinline__ void hsv_to_rgb(float hdeg, float s, float v, float& r, float& g,
                                            float& b) {
     float c = v * s;
@@ -300,17 +311,29 @@ __device__ __forceinline__ void hsv_to_rgb(float hdeg, float s, float v, float& 
     const float rr = (s1 || s4) ? x : ((s2 || s3) ? 0.0f : c);
     const float gg = (s0 || s3) ? x : ((s1 || s2) ? c : 0.0f);
     const float bb = (s0 || s1) ? 0.0f : ((s3 || s4) ? c : x);
-    r = clampf(rr + m, 0.0f, 1.0f);
-    g = clampf(gg + m, 0.0f, 1.0f);
-    b = clampf(bb + m, 0.0f, 1.0f);
+    if (kUnit) {
+        r = clamp_unit(rr + m);
+        g = clamp_unit(gg + m);
+        b = clamp_unit(bb + m);
+    } else {
+        r = clampf(rr + m, 0.0f, 1.0f);
+        g = clampf(gg + m, 0.0f, 1.0f);
+        b = clampf(bb + m, 0.0f, 1.0f);
+    }
 }
 
+template <bool kUnit = false>
 __device__ __forceinline__ float bright_contrast(float v, float brightness, float contrast) {
+    if (kUnit) return clamp_unit(clamp_unit((v - 0.5f) * contrast + 0.5f) + brightness);
     float c = clampf((v - 0.5f) * contrast + 0.5f, 0.0f, 1.0f);
     return clampf(c + brightness, 0.0f, 1.0f);
 }
 
+// (v * 255).clamp(0, 255) as u8; kUnit: v in [0, 1] (not NaN), where the
+// product already lies in [0, 255].
+template <bool kUnit = false>
 __device__ __forceinline__ uint32_t to_u8(float v) {
+    if (kUnit) return (uint32_t)(v * 255.0f);
     float t = clampf(v * 255.0f, 0.0f, 255.0f);
     if (t != t) return 0;
     return (uint32_t)t;
@@ -338,23 +361,38 @@ __device__ __forceinline__ float fmod_360(float a) {
 
 // ColorAdjustment::adjust_pixel (src/color_adjustment.rs:53-87).
 // adj = {brightness, contrast, hue_shift, saturation}.
+//
+// kUnit: adj lies in ColorAdjustment::new's ranges (brightness in [-1, 1],
+// contrast in [0, 2], hue_shift in [-180, 180], saturation in [0, 2], all
+// finite), as the composite's per-cell adjustments do.  Then every value
+// clamped to [0, 1] below is finite and not -0, so each clamp is a median:
+//   * (v - 0.5) * contrast + 0.5 is a sum with +0.5 (a zero sum is +0);
+//   * c + brightness adds a finite value to c in [0, 1] that is not -0 (a
+//     zero sum of non-(-0) operands is +0);
+//   * rgb_to_hsv's inputs are those results, so its max(., 0) are
+//     identities, s = d / mx and v = mx are >= +0;
+//   * hsv_to_rgb's rr + m has m = v - c >= +0 (c = v s <= v), and a zero
+//     sum with a +0 operand is +0;
+// and every channel that reaches to_u8 lies in [0, 1].  The saturation
+// product s * adj.w may be -0 (adj.w = -0) and keeps the general clamp.
+template <bool kUnit = false>
 __device__ __forceinline__ void adjust_pixel(float4 adj, uint32_t r8, uint32_t g8, uint32_t b8,
                                              uint32_t& ro, uint32_t& go, uint32_t& bo) {
     float r = u8_unit(r8), g = u8_unit(g8), b = u8_unit(b8);
-    float br = bright_contrast(r, adj.x, adj.y);
-    float bg = bright_contrast(g, adj.x, adj.y);
-    float bb = bright_contrast(b, adj.x, adj.y);
+    float br = bright_contrast<kUnit>(r, adj.x, adj.y);
+    float bg = bright_contrast<kUnit>(g, adj.x, adj.y);
+    float bb = bright_contrast<kUnit>(b, adj.x, adj.y);
     if (adj.z != 0.0f || adj.w != 1.0f) {
         float h, s, v;
-        rgb_to_hsv(br, bg, bb, h, s, v);
+        rgb_to_hsv<kUnit>(br, bg, bb, h, s, v);
         float hr = fmod_360(h + adj.z);  // rem_euclid(360.0)
         if (hr < 0.0f) hr = hr + 360.0f;
         float s2 = clampf(s * adj.w, 0.0f, 1.0f);
-        hsv_to_rgb(hr, s2, v, br, bg, bb);
+        hsv_to_rgb<kUnit>(hr, s2, v, br, bg, bb);
     }
-    ro = to_u8(br);
-    go = to_u8(bg);
-    bo = to_u8(bb);
+    ro = to_u8<kUnit>(br);
+    go = to_u8<kUnit>(bg);
+    bo = to_u8<kUnit>(bb);
 }
 
 // calculate_optimal_adjustment (src/color_adjustment.rs:99-154), followed

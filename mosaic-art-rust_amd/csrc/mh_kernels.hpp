@@ -145,7 +145,7 @@ int32_t topk_blocks(int32_t nq, int32_t run);
 void launch_topk_batch(const TopkSeg* d_segs, const int32_t* d_starts, int32_t n_segs,
                        int32_t total_blocks, int32_t run, hipStream_t stream);
 int32_t cell_adjust_blocks(int32_t cells);
-// blocks of `threads` pixels of one output row (threads: 64, 128 or 256)
+// blocks of `threads` threads (64, 128 or 256), 4 pixels of one output row each
 int32_t composite_blocks(int32_t ow, int32_t oh, int32_t threads);
 void launch_cell_adjust_batch(const CompSeg* d_segs, const int32_t* d_starts, int32_t n_segs,
                               int32_t total_blocks, hipStream_t stream);
