@@ -305,7 +305,7 @@ int mh_generate_device_result(mh_context *ctx, mh_job_result *result);
  * table, result words, and the grid when d_grid_out is NULL) lives in one
  * static device array of MH_NOALLOC_SCRATCH_BYTES per GPU, held until the
  * process ends; a job that needs more fails with MH_ERR_UNSUPPORTED (about
- * 2.6 KB per cell: 24,000 cells fit).  It calls no hipMalloc or hipFree, so
+ * 1.1 KB per cell: 60,000 cells fit).  It calls no hipMalloc or hipFree, so
  * the library's resized tiles for the job's cell size must already exist
  * when d_image_out is given (any mh_generate / mh_generate_device /
  * mh_composite at that cell size builds them; else MH_ERR_UNSUPPORTED), and

@@ -134,8 +134,7 @@ __global__ __launch_bounds__(64) void place_chain_kernel(PlaceJob single,
             if (nb_left != kEmpty) ll = gload4(job.lab4, nb_left);
             if (nb_right != kEmpty) lr = gload4(job.lab4, nb_right);
         }
-        const int32_t* cidx = job.cand_idx + (int64_t)cell * kCandStride;
-        const float4* clab = job.cand_lab + (int64_t)cell * kCandStride;
+        const uint64_t* crow = reinterpret_cast<const uint64_t*>(job.cand) + (int64_t)cell * kCandStride;
 
         uint64_t best = ~0ull;
         bool placeable[2];
@@ -144,14 +143,16 @@ __global__ __launch_bounds__(64) void place_chain_kernel(PlaceJob single,
             const int32_t r = lane + 64 * s;
             placeable[s] = false;
             if (r < K) {
-                const int32_t m = gload(cidx, r);
-                float4 lm = gload4(clab, r);
-                if (job.score_sqrt) lm.w = sqrtf(lm.w);
+                const uint64_t rec = gload(crow, r);
+                const int32_t m = (int32_t)(uint32_t)rec;
+                float dm = __uint_as_float((uint32_t)(rec >> 32));
+                if (job.score_sqrt) dm = sqrtf(dm);
                 const bool pl = m != nb_up && m != nb_down && m != nb_left && m != nb_right;
                 placeable[s] = pl;
                 if (pl && S.count(m) < max_usage) {
                     float pen = 0.0f;
                     if (use_pen) {
+                        const float4 lm = gload4(job.lab4, m);
                         float p = 0.0f;
                         if (nb_up != kEmpty) p += adj_term(lm, lu);
                         if (nb_down != kEmpty) p += adj_term(lm, ld);
@@ -159,7 +160,7 @@ __global__ __launch_bounds__(64) void place_chain_kernel(PlaceJob single,
                         if (nb_right != kEmpty) p += adj_term(lm, lr);
                         pen = p * weight;
                     }
-                    const float score = lm.w + pen;
+                    const float score = dm + pen;
                     const uint64_t key = ((uint64_t)__float_as_uint(score) << 32) | (uint32_t)r;
                     best = key < best ? key : best;
                 }
@@ -192,8 +193,9 @@ __global__ __launch_bounds__(64) void place_chain_kernel(PlaceJob single,
                 ++st_near;
             }
         }
-        const int32_t w = gload(cidx, rank);
-        const float wd_raw = gload4(clab, rank).w;
+        const uint64_t wrec = gload(crow, rank);
+        const int32_t w = (int32_t)(uint32_t)wrec;
+        const float wd_raw = __uint_as_float((uint32_t)(wrec >> 32));
         const float wd = job.score_sqrt ? sqrtf(wd_raw) : wd_raw;
         S.sync();
         if (lane == 0) {
@@ -224,7 +226,7 @@ __global__ __launch_bounds__(64) void place_chain_kernel(PlaceJob single,
 // ------------------------------------------------------------- raster chain
 struct CandSet {
     int32_t m0, m1;   // material of rank lane / lane+64 (-1: no candidate)
-    float4 c0, c1;    // {l, a, b, d}: d = sqdist, or sqrt(sqdist) for the GUI score
+    float d0, d1;     // sqdist, or sqrt(sqdist) for the GUI score (+inf: no candidate)
 };
 
 // Loads one cell's candidate records (ranks >= K hold the -1 sentinel
@@ -235,17 +237,18 @@ struct CandSet {
 __device__ __forceinline__ void load_cands(CandSet& s, const PlaceJob& job, int32_t cell,
                                            int32_t cells, int lane) {
     const int32_t row = cell < cells ? cell : cells - 1;
-    const int64_t base = (int64_t)row * kCandStride;
-    s.m0 = gload(job.cand_idx, base + lane);
-    s.c0 = gload4(job.cand_lab, base + lane);
-    s.m1 = gload(job.cand_idx, base + lane + 64);
-    s.c1 = gload4(job.cand_lab, base + lane + 64);
+    const uint64_t* rec = reinterpret_cast<const uint64_t*>(job.cand) + (int64_t)row * kCandStride;
+    const uint64_t r0 = gload(rec, lane), r1 = gload(rec, lane + 64);
+    s.m0 = (int32_t)(uint32_t)r0;
+    s.d0 = __uint_as_float((uint32_t)(r0 >> 32));
+    s.m1 = (int32_t)(uint32_t)r1;
+    s.d1 = __uint_as_float((uint32_t)(r1 >> 32));
 }
 
 // GUI score: the Euclidean distance replaces the squared one.
 __device__ __forceinline__ void to_distance(CandSet& s) {
-    s.c0.w = sqrtf(s.c0.w);
-    s.c1.w = sqrtf(s.c1.w);
+    s.d0 = sqrtf(s.d0);
+    s.d1 = sqrtf(s.d1);
 }
 
 constexpr int32_t kNoMat = -2;    // "no previous winner" (never a material or the -1 sentinel)
@@ -404,8 +407,8 @@ __device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn,
     // +inf, which every rank 64-127 lane meets, so the check fails.
     const uint64_t rest = ob0 & (ob0 - 1ull);  // usable ranks after f
     const float pen = ob0 ? st.pen2 : __builtin_inff();
-    const float bound = readlane_f(s.c0.w, f) + pen;
-    uint64_t viol = (rest & lanes_fle(s.c0.w, bound)) | lanes_fle(s.c1.w, bound);
+    const float bound = readlane_f(s.d0, f) + pen;
+    uint64_t viol = (rest & lanes_fle(s.d0, bound)) | lanes_fle(s.d1, bound);
     if (kOutputs && job.score) viol = 1;
 
     int32_t rank = f, stage = 0;
@@ -423,12 +426,12 @@ __device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn,
         if (ob0 | ob1) {
             const int32_t f2 = ob0 ? __ffsll((unsigned long long)ob0) - 1
                                    : 64 + __ffsll((unsigned long long)ob1) - 1;
-            const float d_f = f2 < 64 ? readlane_f(s.c0.w, f2) : readlane_f(s.c1.w, f2 & 63);
+            const float d_f = f2 < 64 ? readlane_f(s.d0, f2) : readlane_f(s.d1, f2 & 63);
             rank = f2;
             if (job.weight > 0.0f) {
                 const float bound = d_f + 2.0f * job.weight;
-                const bool ct0 = ok0 && s.c0.w <= bound;
-                const bool ct1 = ok1 && s.c1.w <= bound;
+                const bool ct0 = ok0 && s.d0 <= bound;
+                const bool ct1 = ok1 && s.d1 <= bound;
                 const uint64_t cb0 = wave_ballot(ct0), cb1 = wave_ballot(ct1);
                 if (__popcll(cb0) + __popcll(cb1) > 1 || (kOutputs && job.score)) {
                     // calculate_penalty for the contenders: Up then Left (Down
@@ -437,19 +440,23 @@ __device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn,
                         up_m >= 0 ? gload4(job.lab4, up_m) : make_float4(0, 0, 0, 0);
                     const float4 left_lab =
                         x > 0 ? gload4(job.lab4, left_m) : make_float4(0, 0, 0, 0);
+                    // the contenders' Labs (other lanes read material 0,
+                    // whose value they do not use)
+                    const float4 lab0 = gload4(job.lab4, ct0 ? s.m0 : 0);
+                    const float4 lab1 = gload4(job.lab4, ct1 ? s.m1 : 0);
                     uint32_t k0 = kNone, k1 = kNone;
                     if (cb0) {
                         float p = 0.0f;
-                        if (up_m >= 0) p += adj_term(s.c0, up_lab);
-                        if (x > 0) p += adj_term(s.c0, left_lab);
-                        const float sc = s.c0.w + p * job.weight;
+                        if (up_m >= 0) p += adj_term(lab0, up_lab);
+                        if (x > 0) p += adj_term(lab0, left_lab);
+                        const float sc = s.d0 + p * job.weight;
                         k0 = ct0 ? __float_as_uint(sc) : kNone;
                     }
                     if (cb1) {
                         float p = 0.0f;
-                        if (up_m >= 0) p += adj_term(s.c1, up_lab);
-                        if (x > 0) p += adj_term(s.c1, left_lab);
-                        const float sc = s.c1.w + p * job.weight;
+                        if (up_m >= 0) p += adj_term(lab1, up_lab);
+                        if (x > 0) p += adj_term(lab1, left_lab);
+                        const float sc = s.d1 + p * job.weight;
                         k1 = ct1 ? __float_as_uint(sc) : kNone;
                     }
                     const uint32_t kmin = wave_min_u32(k0 < k1 ? k0 : k1);
@@ -506,7 +513,7 @@ __device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn,
     *port_at(port.row, x, port.mul) = w;
     wins = write_lane<kSlot>(wins, w);  // the winner waits in lane kSlot
     if (kOutputs) {
-        const float dw = rank < 64 ? readlane_f(s.c0.w, rank & 63) : readlane_f(s.c1.w, rank & 63);
+        const float dw = rank < 64 ? readlane_f(s.d0, rank & 63) : readlane_f(s.d1, rank & 63);
         if (lane == 0) {
             if (job.stage) gstore(job.stage, c, (uint8_t)stage);
             if (job.dist) gstore(job.dist, c, dw);
@@ -550,22 +557,24 @@ __device__ __forceinline__ NarrowStats raster_narrow(const PlaceJob& job, bool s
         const bool pl1 = s.m1 >= 0 && s.m1 != up_m && s.m1 != left_m;
         const bool ok0 = pl0 && usage_s[s.m0 >= 0 ? s.m0 : 0] < max_usage;
         const bool ok1 = pl1 && usage_s[s.m1 >= 0 ? s.m1 : 0] < max_usage;
-        float sc0 = s.c0.w, sc1 = s.c1.w;
+        float sc0 = s.d0, sc1 = s.d1;
         if (weight > 0.0f) {
             // calculate_penalty: Up then Left (Down and Right are still empty)
+            const float4 lab0 = gload4(job.lab4, s.m0 >= 0 ? s.m0 : 0);
+            const float4 lab1 = gload4(job.lab4, s.m1 >= 0 ? s.m1 : 0);
             float p0 = 0.0f, p1 = 0.0f;
             if (up_m >= 0) {
                 const float4 u = gload4(job.lab4, up_m);
-                p0 += adj_term(s.c0, u);
-                p1 += adj_term(s.c1, u);
+                p0 += adj_term(lab0, u);
+                p1 += adj_term(lab1, u);
             }
             if (left_m >= 0) {
                 const float4 l = gload4(job.lab4, left_m);
-                p0 += adj_term(s.c0, l);
-                p1 += adj_term(s.c1, l);
+                p0 += adj_term(lab0, l);
+                p1 += adj_term(lab1, l);
             }
-            sc0 = s.c0.w + p0 * weight;
-            sc1 = s.c1.w + p1 * weight;
+            sc0 = s.d0 + p0 * weight;
+            sc1 = s.d1 + p1 * weight;
         }
         // first strict-< winner: the minimum of (score, rank)
         const uint64_t k0 = ok0 ? ((uint64_t)__float_as_uint(sc0) << 32) | (uint32_t)lane : ~0ull;
@@ -597,7 +606,7 @@ __device__ __forceinline__ NarrowStats raster_narrow(const PlaceJob& job, bool s
         }
         const int32_t w = rank < 64 ? __builtin_amdgcn_readlane(s.m0, rank & 63)
                                     : __builtin_amdgcn_readlane(s.m1, rank & 63);
-        const float dw = rank < 64 ? readlane_f(s.c0.w, rank & 63) : readlane_f(s.c1.w, rank & 63);
+        const float dw = rank < 64 ? readlane_f(s.d0, rank & 63) : readlane_f(s.d1, rank & 63);
         __syncthreads();
         if (lane == 0) {
             usage_s[w] += 1u;

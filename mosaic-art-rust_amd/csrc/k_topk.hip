@@ -365,16 +365,15 @@ __device__ __forceinline__ void topk_run(const TopkSeg& s, int32_t q0, int32_t r
             if (r < k) {
                 const uint64_t key = buf[r];
                 const int32_t m = (int32_t)(uint32_t)(key & 0xffffffffu);
-                const float d = __uint_as_float((uint32_t)(key >> 32));
                 prev[r] = m;
-                gstore(s.out_idx, o, m);
-                if (s.out_d) gstore(s.out_d, o, d);
-                if (s.out_lab)
-                    s.out_lab[o] = make_float4(gload(mL, m), gload(mA, m), gload(mB, m), d);
-            } else if (s.out_lab) {
+                if (s.out_idx) gstore(s.out_idx, o, m);
+                if (s.out_d) gstore(s.out_d, o, __uint_as_float((uint32_t)(key >> 32)));
+                // the key is {m, d} as one little-endian word: m in the low
+                // half (CandRec::m, offset 0), the distance bits in the high
+                if (s.out_cand) gstore((uint64_t*)s.out_cand, o, key);
+            } else if (s.out_cand) {
                 // padded (chain) rows: ranks beyond k, -1 with an infinite distance
-                gstore(s.out_idx, o, (int32_t)-1);
-                s.out_lab[o] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(0x7f800000));
+                gstore((uint64_t*)s.out_cand, o, (uint64_t)0x7f800000u << 32 | 0xffffffffu);
             }
         }
         __syncthreads();
@@ -409,10 +408,10 @@ int32_t topk_blocks(int32_t nq, int32_t run) { return nq > 0 ? (nq + run - 1) / 
 
 void launch_topk(const float* mL, const float* mA, const float* mB, int32_t n, const MatBlocks& blk,
                  const float* query, int32_t nq, int32_t k, int32_t* out_idx, float* out_d,
-                 float4* out_lab, int32_t out_stride, hipStream_t stream) {
+                 CandRec* out_cand, int32_t out_stride, hipStream_t stream) {
     if (k > n) k = n;
     if (nq <= 0 || n <= 0 || k <= 0) return;
-    const TopkSeg s{mL, mA, mB, query, out_idx, out_d, out_lab, n, nq, k, out_stride, blk};
+    const TopkSeg s{mL, mA, mB, query, out_idx, out_d, out_cand, n, nq, k, out_stride, blk};
     const int32_t run = topk_run_length(nq);
     hipLaunchKernelGGL(topk_kernel, dim3(topk_blocks(nq, run)), dim3(64), 0, stream, s, run);
 }

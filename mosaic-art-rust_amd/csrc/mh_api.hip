@@ -208,14 +208,14 @@ void fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s) {
 // Device-resident placement from cell Labs already on the device.  raster:
 // the whole grid in raster order from an empty grid (generate_mosaic's case,
 // the fast chain); otherwise `order` (or raster) over the given grid/usage.
-// The candidate rows go to cidx_buf / clab_buf (cells x kCandStride each)
-// when given, else to the context's scratch.
+// The candidate rows go to cand_buf (cells x kCandStride) when given, else
+// to the context's scratch.
 int run_place(mh_context* ctx, const mh_library* lib, const float* d_cell_lab, int32_t gw,
               int32_t gh, int32_t max_usage, float weight, int32_t score_mode, bool raster,
               const int32_t* d_order,
               int32_t n_order, int32_t* d_grid, uint32_t* d_usage, bool zero_usage,
               uint8_t* d_stage, float* d_dist, float* d_score, int32_t* d_stats,
-              int32_t* cidx_buf = nullptr, float4* clab_buf = nullptr) {
+              CandRec* cand_buf = nullptr) {
     const int32_t cells = gw * gh;
     size_t lds = raster ? place_raster_lds_bytes(gw, lib->n) : place_lds_bytes(cells, lib->n);
     // state beyond LDS: the general chain with grid and counts in HBM (it
@@ -225,21 +225,18 @@ int run_place(mh_context* ctx, const mh_library* lib, const float* d_cell_lab, i
         lds = 0;
         if (raster) fill_i32(d_grid, cells, kEmpty, ctx->stream);
     }
-    int32_t* cidx = cidx_buf;
-    float4* clab = clab_buf;
+    CandRec* cand = cand_buf;
     int rc;
-    if (!cidx && (rc = slot(ctx, S_CAND_IDX, (size_t)cells * kCandStride, &cidx))) return rc;
-    if (!clab && (rc = slot(ctx, S_CAND_D, (size_t)cells * kCandStride, &clab))) return rc;
+    if (!cand && (rc = slot(ctx, S_CAND, (size_t)cells * kCandStride, &cand))) return rc;
     const int32_t K = lib->n < kMaxCand ? lib->n : kMaxCand;
     phase_begin(ctx, MH_PHASE_TOPK);
     launch_topk(lib->d_L, lib->d_A, lib->d_B, lib->n, library_blocks(lib), d_cell_lab, cells, K,
-                cidx, nullptr, clab, kCandStride, ctx->stream);
+                nullptr, nullptr, cand, kCandStride, ctx->stream);
     phase_end(ctx, MH_PHASE_TOPK);
     PlaceJob job;
     job.lab4 = lib->d_lab4;
     job.n = lib->n;
-    job.cand_idx = cidx;
-    job.cand_lab = clab;
+    job.cand = cand;
     job.order = d_order;
     job.n_order = n_order;
     job.grid = d_grid;
@@ -1513,8 +1510,7 @@ int check_job_params(mh_context* ctx, const mh_library* lib, const uint8_t* d_ta
 // read when `own_temps` is set (the allocation-free form fills it itself).
 struct JobBufs {
     float* cell_lab = nullptr;
-    int32_t* cand_idx = nullptr;
-    float4* cand_lab = nullptr;
+    CandRec* cand = nullptr;
     uint32_t* usage = nullptr;
     int32_t* stats = nullptr;
     AnnealOut* aout = nullptr;
@@ -1564,7 +1560,7 @@ int enqueue_generate(mh_context* ctx, const mh_library* lib, const uint8_t* d_ta
     // a3-a9: candidates + placement chain (raster order, empty grid, zero usage)
     CHK(run_place(ctx, lib, b.cell_lab, gw, gh, p->max_usage, p->weight, p->score_mode, true,
                   nullptr, cells, b.grid, b.usage, true, nullptr, nullptr, nullptr, b.stats,
-                  b.cand_idx, b.cand_lab));
+                  b.cand));
     // a10-a12: SA when enabled and weight > 0 (src/main.rs:608)
     const bool sa = p->enable_optimization && p->weight > 0.0f;
     ctx->last_ran_anneal = sa ? 1 : 0;
@@ -1633,8 +1629,7 @@ void carve(uint8_t* base, int32_t cells, int32_t n, bool adjust, bool temps, int
     };
     const size_t c = (size_t)cells;
     b->cell_lab = reinterpret_cast<float*>(take(c * 3 * sizeof(float)));
-    b->cand_idx = reinterpret_cast<int32_t*>(take(c * kCandStride * sizeof(int32_t)));
-    b->cand_lab = reinterpret_cast<float4*>(take(c * kCandStride * sizeof(float4)));
+    b->cand = reinterpret_cast<CandRec*>(take(c * kCandStride * sizeof(CandRec)));
     b->usage = reinterpret_cast<uint32_t*>(take((size_t)(n > 0 ? n : 1) * sizeof(uint32_t)));
     b->stats = reinterpret_cast<int32_t*>(take(8 * sizeof(int32_t)));
     b->aout = reinterpret_cast<AnnealOut*>(take(sizeof(AnnealOut)));
@@ -1668,8 +1663,7 @@ int mh_generate_device(mh_context* ctx, const mh_library* lib, const uint8_t* d_
     CHK(slot(ctx, S_USAGE, (size_t)(lib->n > 0 ? lib->n : 1), &b.usage));
     CHK(slot(ctx, S_STATS, 8, &b.stats));
     CHK(slot(ctx, S_ANNEAL_OUT, 1, &b.aout));
-    CHK(slot(ctx, S_CAND_IDX, (size_t)cells * kCandStride, &b.cand_idx));
-    CHK(slot(ctx, S_CAND_D, (size_t)cells * kCandStride, &b.cand_lab));
+    CHK(slot(ctx, S_CAND, (size_t)cells * kCandStride, &b.cand));
     const float strength = p->color_strength;
     if (d_image_out && strength > 0.0f) {
         CHK(slot(ctx, S_TARGET_AVG, (size_t)cells * 3, &b.tavg));

@@ -51,7 +51,7 @@ struct JobSpec {
     const float* temps = nullptr;
     const uint8_t* tiles = nullptr;
     const uint8_t* tile_avg = nullptr;
-    size_t o_cell_lab = 0, o_cand_idx = 0, o_cand_lab = 0, o_usage = 0, o_tavg = 0, o_adj = 0;
+    size_t o_cell_lab = 0, o_cand = 0, o_usage = 0, o_tavg = 0, o_adj = 0;
     size_t o_grid = 0, o_image = 0;  // SIZE_MAX: the caller's buffer
     size_t bytes = 0;
 };
@@ -102,8 +102,7 @@ int check_job(const mh_context* ctx, const mh_job& j, JobSpec* s) {
         return at;
     };
     s->o_cell_lab = take(cells * 3 * sizeof(float));
-    s->o_cand_idx = take(cells * kCandStride * sizeof(int32_t));
-    s->o_cand_lab = take(cells * kCandStride * sizeof(float4));
+    s->o_cand = take(cells * kCandStride * sizeof(CandRec));
     s->o_usage = take((size_t)(s->n > 0 ? s->n : 1) * sizeof(uint32_t));
     s->o_tavg = s->adjust ? take(cells * 3) : 0;
     s->o_adj = s->adjust ? take(cells * sizeof(float4)) : 0;
@@ -232,8 +231,7 @@ int build_plan(const std::vector<JobSpec>& specs, const std::vector<uint8_t*>& j
         const mh_library* lib = j.lib;
         uint8_t* m = job_mem[i];
         float* cell_lab = at<float>(m, s.o_cell_lab);
-        int32_t* cand_idx = at<int32_t>(m, s.o_cand_idx);
-        float4* cand_lab = at<float4>(m, s.o_cand_lab);
+        CandRec* cand = at<CandRec>(m, s.o_cand);
         int32_t* grid = s.o_grid == SIZE_MAX ? j.d_grid_out : at<int32_t>(m, s.o_grid);
         uint8_t* image = s.o_image == SIZE_MAX ? j.d_image_out : at<uint8_t>(m, s.o_image);
         // a1: cell Lab means of the target
@@ -247,8 +245,8 @@ int build_plan(const std::vector<JobSpec>& specs, const std::vector<uint8_t*>& j
         // a3: K nearest (libraries with materials only)
         if (s.n > 0) {
             topk_st[p->n_topk] = (int32_t)topk_blocks;
-            topk[p->n_topk] = TopkSeg{lib->d_L, lib->d_A, lib->d_B, cell_lab, cand_idx, nullptr,
-                                      cand_lab, s.n, s.cells, s.n < kMaxCand ? s.n : kMaxCand,
+            topk[p->n_topk] = TopkSeg{lib->d_L, lib->d_A, lib->d_B, cell_lab, nullptr, nullptr,
+                                      cand, s.n, s.cells, s.n < kMaxCand ? s.n : kMaxCand,
                                       kCandStride, library_blocks(lib)};
             ++p->n_topk;
             topk_blocks += topk_blocks_of(s.cells, p->topk_run);
@@ -257,8 +255,7 @@ int build_plan(const std::vector<JobSpec>& specs, const std::vector<uint8_t*>& j
         PlaceJob pj{};
         pj.lab4 = lib->d_lab4;
         pj.n = s.n;
-        pj.cand_idx = cand_idx;
-        pj.cand_lab = cand_lab;
+        pj.cand = cand;
         pj.order = nullptr;
         pj.n_order = s.cells;
         pj.grid = grid;

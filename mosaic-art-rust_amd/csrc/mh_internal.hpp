@@ -38,7 +38,7 @@ struct DevBuf {
 
 enum Slot {
     S_IN0, S_IN1, S_IN2, S_OUT0, S_OUT1, S_OUT2, S_OUT3,
-    S_CELL_LAB, S_CAND_IDX, S_CAND_D, S_GRID, S_USAGE, S_ORDER,
+    S_CELL_LAB, S_CAND, S_GRID, S_USAGE, S_ORDER,
     S_TARGET_AVG, S_ADJ, S_STATS, S_ANNEAL_OUT,
     S_COUNT
 };

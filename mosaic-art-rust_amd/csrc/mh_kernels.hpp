@@ -13,13 +13,21 @@ constexpr int32_t kEmpty = -1;       // MH_EMPTY
 constexpr int32_t kMaxCand = 100;    // K = min(n, 100), src/main.rs:348
 constexpr int32_t kCandStride = 128; // per-cell candidate row (padded)
 
+// One candidate of a cell: material and squared distance, 8 bytes (one
+// 64-bit load).  Rows of kCandStride, ranks ascending (sqdist, idx); ranks
+// >= K hold {-1, +inf}.  A candidate's Lab is read from the library's lab4
+// where a penalty needs it.
+struct CandRec {
+    int32_t m;
+    float d;
+};
+static_assert(sizeof(CandRec) == 8, "one 64-bit word per candidate");
+
 // One placement chain (one wave).  All pointers are device pointers.
 struct PlaceJob {
     const float4* lab4;      // n material Labs {l,a,b,0}
     int32_t n;
-    const int32_t* cand_idx;  // cells x kCandStride, ascending (sqdist, idx)
-    const float4* cand_lab;   // same layout: {l, a, b, sqdist} of each candidate
-                              // (ranks >= K: idx -1, sqdist +inf)
+    const CandRec* cand;      // cells x kCandStride
     const int32_t* order;     // n_order cell indices, or nullptr = raster
     int32_t n_order;
     int32_t* grid;            // cells, in/out
@@ -98,9 +106,9 @@ struct TopkSeg {
     const float* mA;
     const float* mB;
     const float* query;  // nq x 3
-    int32_t* out_idx;
+    int32_t* out_idx;    // optional
     float* out_d;        // optional
-    float4* out_lab;     // optional {l, a, b, sqdist}; rows padded with idx -1 up to out_stride
+    CandRec* out_cand;   // optional; rows padded with {-1, +inf} up to out_stride
     int32_t n, nq, k, out_stride;
     MatBlocks blk;
 };
@@ -127,10 +135,10 @@ void launch_cell_lab(const uint8_t* rgb, int64_t row_stride, int32_t per_row,
 void launch_sim_full(const float4* lab, int32_t n, float* out, hipStream_t stream);
 void launch_sim_packed(const float4* lab, int32_t n, float* out, hipStream_t stream);
 void launch_term_full(const float4* lab, int32_t n, float* out, hipStream_t stream);
-// out_d / out_lab ({l, a, b, sqdist} of each candidate) may be null.
+// out_idx / out_d / out_cand may be null.
 void launch_topk(const float* mL, const float* mA, const float* mB, int32_t n, const MatBlocks& blk,
                  const float* query, int32_t nq, int32_t k, int32_t* out_idx, float* out_d,
-                 float4* out_lab, int32_t out_stride, hipStream_t stream);
+                 CandRec* out_cand, int32_t out_stride, hipStream_t stream);
 // Batched launches over segment tables: starts[j] is the first block of
 // segment j (starts[n_segs] = total).  *_blocks give each segment's count.
 // cells per block (4 or 16) for a launch over total_cells cells
