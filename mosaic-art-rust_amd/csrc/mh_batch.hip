@@ -970,9 +970,10 @@ struct StreamSlot {
 // admission order on one wide stream; each batch's chains (placement,
 // annealing: one wave per job, latency-bound) run on one of a few chain
 // streams after an event from the wide stream; its output stages
-// (adjustment, composite: wide again) run in admission order on one output
-// stream after an event from its chain stream.  While one batch's chains
-// run, other batches' wide stages use the rest of the device.  Every stream
+// (adjustment, composite: wide again) run on that chain stream's output
+// stream after an event from the chain stream, so a batch's outputs never
+// wait for another chain stream's chains.  While one batch's chains run,
+// other batches' wide stages use the rest of the device.  Every stream
 // is made with a full CU mask, which gives it a hardware queue of its own
 // (streams without one share a few queues, and a wide launch would wait
 // behind chains in a shared in-order queue).
@@ -980,8 +981,8 @@ struct mh_server {
     mh_context* ctx = nullptr;
     int32_t max_batch = 128;
     ChunkArena arena;
-    hipStream_t wide = nullptr, out = nullptr;
-    std::vector<hipStream_t> lanes;
+    hipStream_t wide = nullptr;
+    std::vector<hipStream_t> lanes, outs;  // chain streams and their output streams
     std::vector<int32_t> lane_load;  // batches in flight per chain stream
     std::vector<StreamSlot> slots;
     std::mutex mu;
@@ -1028,13 +1029,14 @@ int launch_batch(mh_server* sv, StreamSlot& sl, const std::vector<JobRec*>& recs
     HIPCHK(hipStreamWaitEvent(c, sl.ready, 0));
     enqueue_chains(sl.plan, sl.d_plan, c, pe);
     HIPCHK(hipGetLastError());
+    hipStream_t o = sv->outs[(size_t)sl.lane];
     HIPCHK(hipEventRecord(sl.chained, c));
-    HIPCHK(hipStreamWaitEvent(sv->out, sl.chained, 0));
-    enqueue_outputs(sl.plan, sl.d_plan, sv->out, pe);
+    HIPCHK(hipStreamWaitEvent(o, sl.chained, 0));
+    enqueue_outputs(sl.plan, sl.d_plan, o, pe);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(sl.h_res, sl.d_plan + sl.plan.o_stats, sl.plan.bytes - sl.plan.o_stats,
-                          hipMemcpyDeviceToHost, sv->out));
-    HIPCHK(hipEventRecord(sl.done, sv->out));
+                          hipMemcpyDeviceToHost, o));
+    HIPCHK(hipEventRecord(sl.done, o));
     return MH_OK;
 }
 
@@ -1167,7 +1169,7 @@ void worker_loop(mh_server* sv) {
                     lk.unlock();
                     hipStreamSynchronize(sv->wide);
                     hipStreamSynchronize(sv->lanes[(size_t)free_slot->lane]);
-                    hipStreamSynchronize(sv->out);
+                    hipStreamSynchronize(sv->outs[(size_t)free_slot->lane]);
                     lk.lock();
                     for (int64_t t : take) {
                         JobRec& r = sv->jobs[(size_t)t];
@@ -1207,7 +1209,8 @@ void server_free(mh_server* sv) {
     if (sv->wide) hipStreamSynchronize(sv->wide);
     for (hipStream_t c : sv->lanes)
         if (c) hipStreamSynchronize(c);
-    if (sv->out) hipStreamSynchronize(sv->out);
+    for (hipStream_t o : sv->outs)
+        if (o) hipStreamSynchronize(o);
     for (StreamSlot& sl : sv->slots) {
         if (sl.ready) hipEventDestroy(sl.ready);
         if (sl.chained) hipEventDestroy(sl.chained);
@@ -1222,7 +1225,8 @@ void server_free(mh_server* sv) {
     for (hipStream_t c : sv->lanes)
         if (c) hipStreamDestroy(c);
     if (sv->wide) hipStreamDestroy(sv->wide);
-    if (sv->out) hipStreamDestroy(sv->out);
+    for (hipStream_t o : sv->outs)
+        if (o) hipStreamDestroy(o);
     sv->arena.release_all();
     delete sv;
 }
@@ -1247,11 +1251,13 @@ int mh_server_create(mh_context* ctx, const mh_server_params* p, mh_server** out
     BatchPlan tmp;
     const size_t plan = plan_bytes(max_batch, &tmp);
     hipError_t e = make_stream(ctx->device, &sv->wide);
-    if (e == hipSuccess) e = make_stream(ctx->device, &sv->out);
     sv->lanes.assign((size_t)std::min(n_slots, kChainStreams), nullptr);
+    sv->outs.assign(sv->lanes.size(), nullptr);
     sv->lane_load.assign(sv->lanes.size(), 0);
-    for (hipStream_t& c : sv->lanes)
-        if (e == hipSuccess) e = make_stream(ctx->device, &c);
+    for (size_t i = 0; i < sv->lanes.size(); ++i) {
+        if (e == hipSuccess) e = make_stream(ctx->device, &sv->lanes[i]);
+        if (e == hipSuccess) e = make_stream(ctx->device, &sv->outs[i]);
+    }
     for (StreamSlot& sl : sv->slots) {
         if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.ready, hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.chained, hipEventDisableTiming);
