@@ -546,13 +546,15 @@ def main():
         out["ms_per_step"] = round(1000.0 * float(np.mean(sv["step_s"])), 3)
         sv["jobs_per_s"] = out["value"]
         sv["step_s"] = [round(x, 4) for x in sv["step_s"]]
+        pairs = sum(j["grid_w"] * j["grid_h"] * len(S["st"]["libs"][j["lib"]]) for j in S["jobs"])
         out["config"] = {
             "workload": f"served Poisson stream: {args.serve_jobs} jobs, cells log-uniform "
                         f"300-24,000 ({cells} total), 4:3 / 16:9 at 16 px, 4 resident libraries "
                         f"of 1k/2.5k/5k/10k materials 64x64 (mixture colours), usage 3-5, weight "
                         f"0.25, colour 0.4, 1,000 SA proposals each; one step = the whole stream "
                         f"through the server with every job queued at once (one submission call)",
-            "jobs": args.serve_jobs, "cells": cells, "libraries": [1000, 2500, 5000, 10000],
+            "jobs": args.serve_jobs, "cells": cells, "knn_pairs": pairs,
+            "libraries": [1000, 2500, 5000, 10000],
             "parallelism": f"1 GPU, continuous batching on {args.serve_streams} stream slots, "
                            f"<= {args.serve_max_batch} jobs per launch"}
         out["latency_ms"] = sv["rate"]["latency_ms"] if "rate" in sv else None
