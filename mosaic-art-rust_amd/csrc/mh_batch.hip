@@ -948,6 +948,7 @@ struct JobRec {
 // Its wide stages run on the server's wide stream, its chains on `lane`.
 struct StreamSlot {
     int32_t lane = 0;             // chain stream of the running batch
+    hipEvent_t uploaded = nullptr; // descriptor tables on the device (upload stream)
     hipEvent_t ready = nullptr;    // candidate lists written (wide stream)
     hipEvent_t chained = nullptr;  // grids placed and annealed (chain stream)
     hipEvent_t done = nullptr;     // images written, results copied back (output stream)
@@ -981,7 +982,7 @@ struct mh_server {
     mh_context* ctx = nullptr;
     int32_t max_batch = 128;
     ChunkArena arena;
-    hipStream_t wide = nullptr;
+    hipStream_t wide = nullptr, up = nullptr;  // wide stages; descriptor uploads
     std::vector<hipStream_t> lanes, outs;  // chain streams and their output streams
     std::vector<int32_t> lane_load;  // batches in flight per chain stream
     std::vector<StreamSlot> slots;
@@ -1019,7 +1020,12 @@ int launch_batch(mh_server* sv, StreamSlot& sl, const std::vector<JobRec*>& recs
         mem.push_back(r->mem);
     }
     CHK(build_plan(specs, mem, sl.h_plan, sl.d_plan, &sl.plan));
-    HIPCHK(hipMemcpyAsync(sl.d_plan, sl.h_plan, sl.plan.o_stats, hipMemcpyHostToDevice, w));
+    // the tables go up on their own stream as soon as the batch is admitted;
+    // the wide stream, which usually still has earlier batches to run, waits
+    // for them only if the copy has not landed by then
+    HIPCHK(hipMemcpyAsync(sl.d_plan, sl.h_plan, sl.plan.o_stats, hipMemcpyHostToDevice, sv->up));
+    HIPCHK(hipEventRecord(sl.uploaded, sv->up));
+    HIPCHK(hipStreamWaitEvent(w, sl.uploaded, 0));
     PhaseEvents pe;
     pe.mask = sl.timed;
     pe.ev = sl.ev;
@@ -1167,6 +1173,7 @@ void worker_loop(mh_server* sv) {
                 if (rc) {
                     // nothing of this batch may still run: wait for its streams
                     lk.unlock();
+                    hipStreamSynchronize(sv->up);
                     hipStreamSynchronize(sv->wide);
                     hipStreamSynchronize(sv->lanes[(size_t)free_slot->lane]);
                     hipStreamSynchronize(sv->outs[(size_t)free_slot->lane]);
@@ -1206,12 +1213,14 @@ void worker_loop(mh_server* sv) {
 
 void server_free(mh_server* sv) {
     hipSetDevice(sv->ctx->device);
+    if (sv->up) hipStreamSynchronize(sv->up);
     if (sv->wide) hipStreamSynchronize(sv->wide);
     for (hipStream_t c : sv->lanes)
         if (c) hipStreamSynchronize(c);
     for (hipStream_t o : sv->outs)
         if (o) hipStreamSynchronize(o);
     for (StreamSlot& sl : sv->slots) {
+        if (sl.uploaded) hipEventDestroy(sl.uploaded);
         if (sl.ready) hipEventDestroy(sl.ready);
         if (sl.chained) hipEventDestroy(sl.chained);
         if (sl.done) hipEventDestroy(sl.done);
@@ -1225,6 +1234,7 @@ void server_free(mh_server* sv) {
     for (hipStream_t c : sv->lanes)
         if (c) hipStreamDestroy(c);
     if (sv->wide) hipStreamDestroy(sv->wide);
+    if (sv->up) hipStreamDestroy(sv->up);
     for (hipStream_t o : sv->outs)
         if (o) hipStreamDestroy(o);
     sv->arena.release_all();
@@ -1251,6 +1261,7 @@ int mh_server_create(mh_context* ctx, const mh_server_params* p, mh_server** out
     BatchPlan tmp;
     const size_t plan = plan_bytes(max_batch, &tmp);
     hipError_t e = make_stream(ctx->device, &sv->wide);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&sv->up, hipStreamNonBlocking);
     sv->lanes.assign((size_t)std::min(n_slots, kChainStreams), nullptr);
     sv->outs.assign(sv->lanes.size(), nullptr);
     sv->lane_load.assign(sv->lanes.size(), 0);
@@ -1259,6 +1270,7 @@ int mh_server_create(mh_context* ctx, const mh_server_params* p, mh_server** out
         if (e == hipSuccess) e = make_stream(ctx->device, &sv->outs[i]);
     }
     for (StreamSlot& sl : sv->slots) {
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.uploaded, hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.ready, hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.chained, hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
