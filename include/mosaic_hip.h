@@ -387,7 +387,7 @@ int mh_batch_set_tuning(mh_batch *batch, const mh_batch_tuning *tuning);
  * admitted jobs as one batch on a free slot (at most `streams` batches in
  * flight, at most max_batch jobs each); jobs that arrive while batches run
  * join the next one as slots and memory free up.  The batches' wide stages
- * (cell Labs, K nearest) run in admission order on one stream, their
+ * (cell Labs, K nearest) run on two streams, consecutive batches alternating, their
  * placement chains and annealing on up to four more beside later batches'
  * wide stages, and their output stages (adjustment, composite) on an output
  * stream per chain stream.  Working memory is

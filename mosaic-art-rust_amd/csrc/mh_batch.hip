@@ -151,8 +151,12 @@ struct BatchPlan {
     std::vector<int32_t> anneal_job;  // job index of each anneal entry
     // Launch shapes a tuner may set (build_plan keeps them): query cells per
     // top-K wave (0: topk_run_length's choice) and composite threads per block.
+    // 64 threads (256 pixels of a row per block) by default: a block holds part
+    // of one row, so wider blocks leave more lanes idle at the ends of the
+    // rows (on the served stream's mix of row widths, 93% of lanes busy with
+    // 256-pixel blocks against 77% with 1,024).
     int32_t tune_topk_run = 0;
-    int32_t comp_threads = 256;
+    int32_t comp_threads = 64;
 };
 
 constexpr int32_t kStatsInts = 8;
@@ -931,6 +935,7 @@ struct ChunkArena {
 enum JobState { J_QUEUED = 0, J_RUNNING = 1, J_DONE = 2, J_FAILED = 3 };
 
 constexpr int32_t kChainStreams = 4;  // chain streams of a server (fewer with fewer slots)
+constexpr int kWideStreams = 2;       // wide streams of a server (batches alternate)
 
 struct JobRec {
     JobSpec spec;
@@ -948,6 +953,7 @@ struct JobRec {
 // Its wide stages run on the server's wide stream, its chains on `lane`.
 struct StreamSlot {
     int32_t lane = 0;             // chain stream of the running batch
+    int32_t wide = 0;             // wide stream of the running batch
     hipEvent_t uploaded = nullptr; // descriptor tables on the device (upload stream)
     hipEvent_t ready = nullptr;    // candidate lists written (wide stream)
     hipEvent_t chained = nullptr;  // grids placed and annealed (chain stream)
@@ -967,8 +973,9 @@ struct StreamSlot {
 }  // namespace
 
 // Batches go through three kinds of stream.  The wide stages up to the
-// candidate lists (cell Labs, K nearest), which fill the device, run in
-// admission order on one wide stream; each batch's chains (placement,
+// candidate lists (cell Labs, K nearest), which fill the device, run on two
+// wide streams, consecutive batches alternating (so one batch's launch tail
+// overlaps the next batch's start); each batch's chains (placement,
 // annealing: one wave per job, latency-bound) run on one of a few chain
 // streams after an event from the wide stream; its output stages
 // (adjustment, composite: wide again) run on that chain stream's output
@@ -982,7 +989,9 @@ struct mh_server {
     mh_context* ctx = nullptr;
     int32_t max_batch = 128;
     ChunkArena arena;
-    hipStream_t wide = nullptr, up = nullptr;  // wide stages; descriptor uploads
+    hipStream_t wide[kWideStreams] = {};      // wide stages, batches alternating
+    hipStream_t up = nullptr;                 // descriptor uploads
+    int64_t launched = 0;                     // batches launched (picks the wide stream)
     std::vector<hipStream_t> lanes, outs;  // chain streams and their output streams
     std::vector<int32_t> lane_load;  // batches in flight per chain stream
     std::vector<StreamSlot> slots;
@@ -1009,7 +1018,7 @@ namespace {
 // other threads while RUNNING).
 int launch_batch(mh_server* sv, StreamSlot& sl, const std::vector<JobRec*>& recs) {
     mh_context* ctx = sv->ctx;
-    hipStream_t w = sv->wide, c = sv->lanes[(size_t)sl.lane];
+    hipStream_t w = sv->wide[sl.wide], c = sv->lanes[(size_t)sl.lane];
     std::vector<JobSpec> specs;
     std::vector<uint8_t*> mem;
     specs.reserve(recs.size());
@@ -1158,6 +1167,7 @@ void worker_loop(mh_server* sv) {
             if (!take.empty()) {
                 free_slot->busy = true;
                 free_slot->lane = pick_lane(sv);
+                free_slot->wide = (int32_t)(sv->launched++ % kWideStreams);
                 ++sv->lane_load[(size_t)free_slot->lane];
                 free_slot->tickets = take;
                 free_slot->timed = sv->timing;
@@ -1174,7 +1184,7 @@ void worker_loop(mh_server* sv) {
                     // nothing of this batch may still run: wait for its streams
                     lk.unlock();
                     hipStreamSynchronize(sv->up);
-                    hipStreamSynchronize(sv->wide);
+                    hipStreamSynchronize(sv->wide[free_slot->wide]);
                     hipStreamSynchronize(sv->lanes[(size_t)free_slot->lane]);
                     hipStreamSynchronize(sv->outs[(size_t)free_slot->lane]);
                     lk.lock();
@@ -1214,7 +1224,8 @@ void worker_loop(mh_server* sv) {
 void server_free(mh_server* sv) {
     hipSetDevice(sv->ctx->device);
     if (sv->up) hipStreamSynchronize(sv->up);
-    if (sv->wide) hipStreamSynchronize(sv->wide);
+    for (hipStream_t w : sv->wide)
+        if (w) hipStreamSynchronize(w);
     for (hipStream_t c : sv->lanes)
         if (c) hipStreamSynchronize(c);
     for (hipStream_t o : sv->outs)
@@ -1233,7 +1244,8 @@ void server_free(mh_server* sv) {
     }
     for (hipStream_t c : sv->lanes)
         if (c) hipStreamDestroy(c);
-    if (sv->wide) hipStreamDestroy(sv->wide);
+    for (hipStream_t w : sv->wide)
+        if (w) hipStreamDestroy(w);
     if (sv->up) hipStreamDestroy(sv->up);
     for (hipStream_t o : sv->outs)
         if (o) hipStreamDestroy(o);
@@ -1260,7 +1272,9 @@ int mh_server_create(mh_context* ctx, const mh_server_params* p, mh_server** out
     sv->slots.resize((size_t)n_slots);
     BatchPlan tmp;
     const size_t plan = plan_bytes(max_batch, &tmp);
-    hipError_t e = make_stream(ctx->device, &sv->wide);
+    hipError_t e = hipSuccess;
+    for (hipStream_t& w : sv->wide)
+        if (e == hipSuccess) e = make_stream(ctx->device, &w);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&sv->up, hipStreamNonBlocking);
     sv->lanes.assign((size_t)std::min(n_slots, kChainStreams), nullptr);
     sv->outs.assign(sv->lanes.size(), nullptr);

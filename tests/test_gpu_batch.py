@@ -306,7 +306,7 @@ def test_every_tuned_shape_gives_the_same_outputs(ctx, mh, synth):
     b.set_graph(True)
     base = b.tuning()
     assert all(1 <= r <= 8 for r in base["topk_cells_per_wave"]), base
-    assert all(t == 256 for t in base["composite_threads"]), base
+    assert all(t == 64 for t in base["composite_threads"]), base
 
     def check(label):
         res = b.results()
