@@ -387,10 +387,10 @@ int mh_batch_set_tuning(mh_batch *batch, const mh_batch_tuning *tuning);
  * admitted jobs as one batch on a free slot (at most `streams` batches in
  * flight, at most max_batch jobs each); jobs that arrive while batches run
  * join the next one as slots and memory free up.  The batches' wide stages
- * (cell Labs, K nearest) run on two streams, consecutive batches alternating, their
- * placement chains and annealing on up to four more beside later batches'
- * wide stages, and their output stages (adjustment, composite) on an output
- * stream per chain stream.  Working memory is
+ * (cell Labs, K nearest) run on two streams, consecutive batches
+ * alternating, their placement chains and annealing on up to four more
+ * beside later batches' wide stages, and their output stages (adjustment,
+ * composite) on an output stream per chain stream.  Working memory is
  * allocated in chunks of 1 GiB (or one job's size, if larger) as admitted
  * jobs need it, never beyond memory_budget; a job that finds no device
  * memory while nothing else runs fails with MH_ERR_OUT_OF_MEMORY.  Libraries
