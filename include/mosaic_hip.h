@@ -383,10 +383,14 @@ int mh_batch_get_tuning(mh_batch *batch, mh_batch_tuning *tuning);
 int mh_batch_set_tuning(mh_batch *batch, const mh_batch_tuning *tuning);
 
 /* Continuous-batching request loop.  A scheduler thread admits queued jobs
- * (FIFO) while their working memory fits memory_budget, and launches the
- * admitted jobs as one batch on a free slot (at most `streams` batches in
- * flight, at most max_batch jobs each); jobs that arrive while batches run
- * join the next one as slots and memory free up.  The batches' wide stages
+ * while their working memory fits memory_budget, and launches the admitted
+ * jobs as one batch on a free slot (at most `streams` batches in flight, at
+ * most max_batch jobs each); jobs that arrive while batches run join the next
+ * one as slots and memory free up.  A batch takes the oldest queued job and
+ * the queued jobs among the next 8 * max_batch nearest to it in cell count,
+ * so that one batch's placement chains are of similar length (a job's result
+ * does not depend on its batch); with at most max_batch jobs queued,
+ * admission is first-come first-served.  The batches' wide stages
  * (cell Labs, K nearest) run on two streams, consecutive batches
  * alternating, their placement chains and annealing on up to four more
  * beside later batches' wide stages, and their output stages (adjustment,

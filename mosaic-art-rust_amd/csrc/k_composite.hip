@@ -250,71 +250,137 @@ void launch_cell_adjust(const int32_t* grid, int32_t cells, const uint8_t* tile_
                        cells, tile_avg, target_avg, strength, out);
 }
 
-// Output pixel (X, Y) of a gw*tw x gh*th mosaic.  Y is the same for the
-// whole block (a block covers pixels of one row), so its cell row and the
-// row's offsets are uniform; X's cell column is umulhi(X, tw_magic) with
-// tw_magic = floor((2^32 - 1) / tw) + 1, exact while X * tw < 2^32 (then
-// X m / 2^32 exceeds X / tw by less than 1 / tw), else a division.
-__device__ __forceinline__ void composite_pixel(const int32_t* __restrict__ grid, int32_t gw,
-                                                int32_t tw, int32_t th, int32_t ow, int32_t X,
-                                                int32_t Y, const uint8_t* __restrict__ tiles,
-                                                const float4* __restrict__ adj, int32_t adjust,
-                                                uint8_t* __restrict__ out) {
-    const int32_t cy = Y / th, dy = Y - cy * th;
-    const uint32_t tw_magic = 0xffffffffu / (uint32_t)tw + 1u;
-    const bool magic_ok = (uint64_t)(uint32_t)ow * (uint32_t)tw < (1ull << 32);
-    const int32_t cx = magic_ok ? (int32_t)__umulhi((uint32_t)X, tw_magic) : X / tw;
-    const int32_t cell = cy * gw + cx;
-    const int32_t m = gload(grid, cell);
-    uint32_t r = 0, g = 0, b = 0;
-    if (m != kEmpty) {
-        const int32_t dx = X - cx * tw;
-        const MH_GAS uint8_t* p =
-            (const MH_GAS uint8_t*)tiles + (((int64_t)m * th + dy) * tw + dx) * 3;
-        r = p[0];
-        g = p[1];
-        b = p[2];
-        // per-cell adjustments come from calculate_optimal_adjustment and
-        // ColorAdjustment::new's clamps (cell_adjust_one)
-        if (adjust) adjust_pixel<true>(gload4(adj, cell), r, g, b, r, g, b);
-    }
-    MH_GAS uint8_t* d = (MH_GAS uint8_t*)out + ((int64_t)Y * ow + X) * 3;
-    d[0] = (uint8_t)r;
-    d[1] = (uint8_t)g;
-    d[2] = (uint8_t)b;
-}
+// The contrast stage of ColorAdjustment::adjust_pixel at contrast 1 for
+// each channel value (contrast1_entry); blocks copy it to LDS.
+__device__ const Contrast1Table kContrast1 = make_contrast1_table();
 
-// Pixels per thread: a block of `threads` covers kCompPx * threads pixels
-// of one output row, pixel X = base + k * threads + thread (k < kCompPx),
-// so the block's setup (segment, row, divisors) is shared by kCompPx
-// pixels per thread and every k-step stores a contiguous run.
+// Pixels per thread: a block of bs threads covers kCompPx * bs pixels of one
+// output row, pixel X = X0 + k * bs (k < kCompPx), so the row's setup is
+// shared by kCompPx pixels per thread and every k-step stores a contiguous
+// run.
 constexpr int32_t kCompPx = 4;
 
-// One output row segment per block (ow = gw*tw, oh = gh*th); block b covers
-// the 256 * kCompPx pixels from 256 * kCompPx * (b % per_row) of row
-// b / per_row.
-__global__ __launch_bounds__(256) void composite_kernel(const int32_t* __restrict__ grid,
-                                                        int32_t gw, int32_t tw, int32_t th,
-                                                        int32_t ow, const uint8_t* __restrict__ tiles,
-                                                        const float4* __restrict__ adj,
-                                                        int32_t adjust, uint8_t* __restrict__ out) {
-    const int32_t per_row = (ow + 256 * kCompPx - 1) / (256 * kCompPx);
-    const int32_t Y = (int32_t)blockIdx.x / per_row;
-    const int32_t X0 = ((int32_t)blockIdx.x - Y * per_row) * 256 * kCompPx + (int32_t)threadIdx.x;
+// Block b of segment s: row Y = b / per_row of the gw*tw x gh*th mosaic,
+// pixels X0 + k * bs with X0 = (b % per_row) * bs * kCompPx + thread and
+// per_row = ceil(ow / (bs * kCompPx)), ow = gw * tw.  Everything about the
+// row is uniform: its cell row's grid entries and adjustments, the tiles'
+// pixel row dy = Y % th, the output row.  A thread divides once for its
+// first pixel's cell column cx and byte column dx3 = 3 (X0 - cx tw) and steps
+// the others by bs = q tw + r.  kNarrow: every material index and the tile
+// size th tw 3 are below 2^24 and n th tw 3 < 2^32 (composite_narrow), so a
+// tile byte offset is one 24-bit multiply-add; otherwise 64-bit.
+template <bool kNarrow>
+__device__ __forceinline__ void composite_block(const CompSeg& s, int32_t b, float* ctab) {
+    const int32_t tw = s.tw, th = s.th, ow = s.gw * tw;
+    const int32_t bs = (int32_t)blockDim.x;
+    const int32_t per_row = (ow + bs * kCompPx - 1) / (bs * kCompPx);
+    const int32_t Y = b / per_row;
+    const int32_t cy = Y / th, dy = Y - cy * th;
+    const int64_t row_cell = (int64_t)cy * s.gw;
+    const int32_t* __restrict__ grid = s.grid + row_cell;
+    const float4* __restrict__ adj = s.adjust ? s.adj + row_cell : nullptr;
+    const uint32_t tile_bytes = (uint32_t)th * (uint32_t)tw * 3u;
+    const uint8_t* __restrict__ tiles = s.tiles + (int64_t)dy * tw * 3;
+    uint8_t* __restrict__ out = s.out + (int64_t)Y * ow * 3;
+    if (s.adjust) {
+        // blockDim.x >= 64: one float4 of the table per thread of the first 64
+        if (threadIdx.x < 64)
+            reinterpret_cast<float4*>(ctab)[threadIdx.x] =
+                gload4(reinterpret_cast<const float4*>(kContrast1.v), threadIdx.x);
+        __syncthreads();
+    }
+    int32_t X = (b - Y * per_row) * bs * kCompPx + (int32_t)threadIdx.x;
+    int32_t cx = X / tw;
+    uint32_t dx3 = (uint32_t)(X - cx * tw) * 3u;
+    const int32_t q = bs / tw;
+    const uint32_t r3 = (uint32_t)(bs - q * tw) * 3u, tw3 = (uint32_t)tw * 3u;
 #pragma unroll
     for (int32_t k = 0; k < kCompPx; ++k) {
-        const int32_t X = X0 + k * 256;
-        if (X < ow) composite_pixel(grid, gw, tw, th, ow, X, Y, tiles, adj, adjust, out);
+        if (X < ow) {
+            const int32_t m = gload_at(grid, (uint32_t)cx * 4u);
+            uint32_t r = 0, g = 0, bl = 0;
+            if (m != kEmpty) {
+                const MH_GAS uint8_t* p =
+                    kNarrow ? (const MH_GAS uint8_t*)tiles +
+                                  (__umul24((uint32_t)m, tile_bytes) + dx3)
+                            : (const MH_GAS uint8_t*)tiles +
+                                  ((int64_t)m * tile_bytes + (int64_t)dx3);
+                r = p[0];
+                g = p[1];
+                bl = p[2];
+                // per-cell adjustments come from calculate_optimal_adjustment
+                // and ColorAdjustment::new's clamps (cell_adjust_one), whose
+                // contrast is always 1
+                if (s.adjust) {
+                    const v4f_t av =
+                        gload_at(reinterpret_cast<const v4f_t*>(adj), (uint32_t)cx * 16u);
+                    const float4 a = make_float4(av.x, av.y, av.z, av.w);
+                    float br, bg, bb;
+                    if (a.y == 1.0f) {
+                        br = clamp_unit(ctab[r] + a.x);
+                        bg = clamp_unit(ctab[g] + a.x);
+                        bb = clamp_unit(ctab[bl] + a.x);
+                    } else {
+                        br = bright_contrast<true>(u8_unit(r), a.x, a.y);
+                        bg = bright_contrast<true>(u8_unit(g), a.x, a.y);
+                        bb = bright_contrast<true>(u8_unit(bl), a.x, a.y);
+                    }
+                    adjust_tail<true>(a, br, bg, bb, r, g, bl);
+                }
+            }
+            MH_GAS uint8_t* d = (MH_GAS uint8_t*)out + (uint32_t)X * 3u;
+            d[0] = (uint8_t)r;
+            d[1] = (uint8_t)g;
+            d[2] = (uint8_t)bl;
+        }
+        X += bs;
+        cx += q;
+        dx3 += r3;
+        if (dx3 >= tw3) {
+            dx3 -= tw3;
+            ++cx;
+        }
     }
+}
+
+__device__ __forceinline__ void composite_run(const CompSeg& s, int32_t b) {
+    __shared__ __align__(16) float ctab[256];
+    if (s.narrow)
+        composite_block<true>(s, b, ctab);
+    else
+        composite_block<false>(s, b, ctab);
+}
+
+// One job: blocks of 256 threads.
+__global__ __launch_bounds__(256) void composite_kernel(CompSeg s) {
+    composite_run(s, (int32_t)blockIdx.x);
+}
+
+int32_t composite_narrow(int32_t n_tiles, int32_t tw, int32_t th) {
+    const uint64_t tile = (uint64_t)tw * (uint64_t)th * 3u;
+    return n_tiles >= 0 && (uint64_t)n_tiles < (1ull << 24) && tile < (1ull << 24) &&
+                   (uint64_t)n_tiles * tile < (1ull << 32)
+               ? 1
+               : 0;
 }
 
 void launch_composite(const int32_t* grid, int32_t gw, int32_t gh, int32_t tw, int32_t th,
-                      const uint8_t* tiles, const float4* adj, int32_t adjust, uint8_t* out,
-                      hipStream_t stream) {
+                      const uint8_t* tiles, int32_t n_tiles, const float4* adj, int32_t adjust,
+                      uint8_t* out, hipStream_t stream) {
     const int32_t blocks = composite_blocks(gw * tw, gh * th, 256);
     if (blocks <= 0) return;
-    hipLaunchKernelGGL(composite_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, grid, gw, tw,
-                       th, gw * tw, tiles, adj, adjust, out);
+    CompSeg s{};
+    s.grid = grid;
+    s.gw = gw;
+    s.gh = gh;
+    s.tw = tw;
+    s.th = th;
+    s.tiles = tiles;
+    s.adjust = adjust;
+    s.adj = const_cast<float4*>(adj);
+    s.out = out;
+    s.narrow = composite_narrow(n_tiles, tw, th);
+    hipLaunchKernelGGL(composite_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, s);
 }
 
 // Batched finish of many jobs: per-cell adjustments (a thread per cell), then
@@ -332,24 +398,12 @@ __global__ __launch_bounds__(256) void cell_adjust_batch_kernel(const CompSeg* _
 }
 
 // A block of blockDim.x (64, 128 or 256) threads covers kCompPx pixels per
-// thread of one output row.
+// thread of one output row (composite_block).
 __global__ __launch_bounds__(256) void composite_batch_kernel(const CompSeg* __restrict__ segs,
                                                               const int32_t* __restrict__ starts,
                                                               int32_t n_segs) {
     const int32_t j = find_segment(starts, n_segs, blockIdx.x);
-    const CompSeg& s = segs[j];
-    const int32_t ow = s.gw * s.tw;
-    const int32_t bs = (int32_t)blockDim.x;
-    const int32_t per_row = (ow + bs * kCompPx - 1) / (bs * kCompPx);
-    const int32_t b = (int32_t)blockIdx.x - starts[j];
-    const int32_t Y = b / per_row;
-    const int32_t X0 = (b - Y * per_row) * bs * kCompPx + (int32_t)threadIdx.x;
-#pragma unroll
-    for (int32_t k = 0; k < kCompPx; ++k) {
-        const int32_t X = X0 + k * bs;
-        if (X < ow)
-            composite_pixel(s.grid, s.gw, s.tw, s.th, ow, X, Y, s.tiles, s.adj, s.adjust, s.out);
-    }
+    composite_run(segs[j], (int32_t)blockIdx.x - starts[j]);
 }
 
 int32_t cell_adjust_blocks(int32_t cells) { return cells > 0 ? (cells + 255) / 256 : 0; }

@@ -431,7 +431,7 @@ int run_composite(mh_context* ctx, const mh_library* lib, const uint8_t* d_targe
     }
     phase_end(ctx, MH_PHASE_ADJUST);
     phase_begin(ctx, MH_PHASE_COMPOSITE);
-    launch_composite(d_grid, gw, gh, tw, th, tiles, adj, adjust, d_out, ctx->stream);
+    launch_composite(d_grid, gw, gh, tw, th, tiles, lib->n, adj, adjust, d_out, ctx->stream);
     phase_end(ctx, MH_PHASE_COMPOSITE);
     HIPCHK(hipGetLastError());
     return MH_OK;

@@ -16,8 +16,10 @@
 // slot; batches on different slots overlap on the device.  A scheduler
 // thread does admission, launches and completion (event queries), so jobs
 // that arrive while others run join the next batch as slots and memory free.
+#include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <deque>
 #include <thread>
@@ -306,7 +308,7 @@ int build_plan(const std::vector<JobSpec>& specs, const std::vector<uint8_t*>& j
         adj_st[p->n_comp] = (int32_t)adj_blocks;
         comp_st[p->n_comp] = (int32_t)comp_blocks;
         comp[p->n_comp] = CompSeg{grid, s.gw, s.gh, s.tw, s.th, s.tiles, s.tile_avg, tavg, s.strength,
-                                  s.adjust ? 1 : 0, adj, image};
+                                  s.adjust ? 1 : 0, adj, image, composite_narrow(s.n, s.tw, s.th)};
         ++p->n_comp;
         adj_blocks += s.adjust ? cell_adjust_blocks(s.cells) : 0;
         comp_blocks += composite_blocks(s.gw * s.tw, s.gh * s.th, p->comp_threads);
@@ -1013,6 +1015,54 @@ struct mh_server {
 
 namespace {
 
+// Batch formation.  The oldest queued job always goes into the next batch;
+// the rest of the batch are the queued jobs, among the next
+// kWindowBatches * max_batch, whose cell counts are nearest to its own (by
+// ratio), in queue order.  A placement launch lasts as long as its longest
+// chain (one wave walks a job's cells in order), so batches of like-sized
+// jobs keep each chain stream from waiting on one long job per batch; which
+// jobs share a batch changes no job's result.  With at most max_batch jobs
+// queued this is first-come first-served.
+constexpr size_t kWindowBatches = 8;
+
+std::vector<size_t> pick_batch(const mh_server& sv) {
+    const size_t b = (size_t)sv.max_batch;
+    const size_t wn = std::min(sv.queue.size(), b * kWindowBatches);
+    std::vector<size_t> pos;
+    if (wn <= b) {
+        for (size_t i = 0; i < wn; ++i) pos.push_back(i);
+        return pos;
+    }
+    auto lcells = [&](size_t i) {
+        const int32_t c = sv.jobs[(size_t)sv.queue[i]].spec.cells;
+        return std::log((double)(c > 1 ? c : 1));
+    };
+    const double c0 = lcells(0);
+    std::vector<std::pair<double, size_t>> d(wn - 1);
+    for (size_t i = 1; i < wn; ++i) d[i - 1] = {std::fabs(lcells(i) - c0), i};
+    std::nth_element(d.begin(), d.begin() + (ptrdiff_t)(b - 1), d.end());
+    pos.push_back(0);
+    for (size_t k = 0; k + 1 < b; ++k) pos.push_back(d[k].second);
+    std::sort(pos.begin() + 1, pos.end());
+    return pos;
+}
+
+// Drops queue positions `taken` (ascending, all inside the queue).
+void remove_positions(std::deque<int64_t>* q, const std::vector<size_t>& taken) {
+    const size_t wn = taken.back() + 1;
+    std::vector<int64_t> keep;
+    keep.reserve(wn - taken.size());
+    size_t j = 0;
+    for (size_t i = 0; i < wn; ++i) {
+        if (j < taken.size() && taken[j] == i)
+            ++j;
+        else
+            keep.push_back((*q)[i]);
+    }
+    q->erase(q->begin(), q->begin() + (ptrdiff_t)wn);
+    q->insert(q->begin(), keep.begin(), keep.end());
+}
+
 // Launch the admitted jobs `tickets` as one batch on slot `sl`.  Called by
 // the worker without the lock held (the records it reads are not touched by
 // other threads while RUNNING).
@@ -1136,16 +1186,27 @@ void worker_loop(mh_server* sv) {
         if (free_slot && !sv->queue.empty()) {
             std::vector<int64_t> take;
             std::vector<JobRec*> recs;
-            while (!sv->queue.empty() && (int32_t)take.size() < sv->max_batch) {
-                const int64_t t = sv->queue.front();
+            std::vector<size_t> taken;  // queue positions, ascending
+            for (size_t i : pick_batch(*sv)) {
+                const int64_t t = sv->queue[i];
                 JobRec& r = sv->jobs[(size_t)t];
                 uint8_t* mem = sv->arena.alloc(r.spec.bytes, &r.chunk);
-                if (!mem) {
-                    // FIFO: wait for memory -- unless nothing runs that could
-                    // give any back, when the device itself has no room
-                    bool busy = false;
-                    for (const StreamSlot& sl : sv->slots) busy |= sl.busy;
-                    if (busy || !take.empty()) break;
+                if (!mem) break;  // the rest waits for memory
+                r.mem = mem;
+                r.holds_mem = true;
+                r.state = J_RUNNING;
+                r.t_launch = now_ms();
+                take.push_back(t);
+                recs.push_back(&r);  // deque elements keep their address across push_back
+                taken.push_back(i);
+            }
+            if (take.empty()) {
+                // the oldest job waits for memory -- unless nothing runs that
+                // could give any back, when the device itself has no room
+                bool busy = false;
+                for (const StreamSlot& sl : sv->slots) busy |= sl.busy;
+                if (!busy) {
+                    JobRec& r = sv->jobs[(size_t)sv->queue.front()];
                     r.state = J_FAILED;
                     r.err = MH_ERR_OUT_OF_MEMORY;
                     r.msg = "no device memory for the job's working set";
@@ -1154,15 +1215,9 @@ void worker_loop(mh_server* sv) {
                     sv->queue.pop_front();
                     sv->cv_done.notify_all();
                     progressed = true;
-                    continue;
                 }
-                r.mem = mem;
-                r.holds_mem = true;
-                r.state = J_RUNNING;
-                r.t_launch = now_ms();
-                take.push_back(t);
-                recs.push_back(&r);  // deque elements keep their address across push_back
-                sv->queue.pop_front();
+            } else {
+                remove_positions(&sv->queue, taken);
             }
             if (!take.empty()) {
                 free_slot->busy = true;

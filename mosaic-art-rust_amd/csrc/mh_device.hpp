@@ -375,13 +375,12 @@ __device__ __forceinline__ float fmod_360(float a) {
 //     sum with a +0 operand is +0;
 // and every channel that reaches to_u8 lies in [0, 1].  The saturation
 // product s * adj.w may be -0 (adj.w = -0) and keeps the general clamp.
-template <bool kUnit = false>
-__device__ __forceinline__ void adjust_pixel(float4 adj, uint32_t r8, uint32_t g8, uint32_t b8,
-                                             uint32_t& ro, uint32_t& go, uint32_t& bo) {
-    float r = u8_unit(r8), g = u8_unit(g8), b = u8_unit(b8);
-    float br = bright_contrast<kUnit>(r, adj.x, adj.y);
-    float bg = bright_contrast<kUnit>(g, adj.x, adj.y);
-    float bb = bright_contrast<kUnit>(b, adj.x, adj.y);
+//
+// adjust_tail is everything after brightness/contrast: the HSV hue and
+// saturation step and the conversion back to u8.
+template <bool kUnit>
+__device__ __forceinline__ void adjust_tail(float4 adj, float br, float bg, float bb,
+                                            uint32_t& ro, uint32_t& go, uint32_t& bo) {
     if (adj.z != 0.0f || adj.w != 1.0f) {
         float h, s, v;
         rgb_to_hsv<kUnit>(br, bg, bb, h, s, v);
@@ -393,6 +392,38 @@ __device__ __forceinline__ void adjust_pixel(float4 adj, uint32_t r8, uint32_t g
     ro = to_u8<kUnit>(br);
     go = to_u8<kUnit>(bg);
     bo = to_u8<kUnit>(bb);
+}
+
+template <bool kUnit = false>
+__device__ __forceinline__ void adjust_pixel(float4 adj, uint32_t r8, uint32_t g8, uint32_t b8,
+                                             uint32_t& ro, uint32_t& go, uint32_t& bo) {
+    float r = u8_unit(r8), g = u8_unit(g8), b = u8_unit(b8);
+    adjust_tail<kUnit>(adj, bright_contrast<kUnit>(r, adj.x, adj.y),
+                       bright_contrast<kUnit>(g, adj.x, adj.y),
+                       bright_contrast<kUnit>(b, adj.x, adj.y), ro, go, bo);
+}
+
+// The contrast stage at contrast 1 for channel value x in 0..255:
+// clamp((x / 255 - 0.5) * 1 + 0.5, 0, 1), where the product by 1 is exact,
+// so the value is RN(RN(x / 255 - 0.5) + 0.5) (not x / 255 itself below
+// 64 / 255, where the subtraction rounds).  A 256-entry table of it,
+// evaluated in f32 at compile time (round to nearest even, as at run
+// time), stands in for bright_contrast's first clamp whenever a cell's
+// contrast is exactly 1.
+constexpr float contrast1_entry(int x) {
+    const float v = (float)x / 255.0f;
+    const float t = (v - 0.5f) + 0.5f;
+    return t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+}
+
+struct Contrast1Table {
+    float v[256];
+};
+
+constexpr Contrast1Table make_contrast1_table() {
+    Contrast1Table t{};
+    for (int x = 0; x < 256; ++x) t.v[x] = contrast1_entry(x);
+    return t;
 }
 
 // calculate_optimal_adjustment (src/color_adjustment.rs:99-154), followed

@@ -124,6 +124,7 @@ struct CompSeg {
     int32_t adjust;             // 0: plain tiles (adj unused)
     float4* adj;                // cells
     uint8_t* out;               // (gw*tw) x (gh*th) x 3
+    int32_t narrow;             // composite_narrow(n, tw, th)
 };
 
 // Cell Lab means; with out_rgb, also the cells' integer average RGB
@@ -155,6 +156,8 @@ void launch_topk_batch(const TopkSeg* d_segs, const int32_t* d_starts, int32_t n
 int32_t cell_adjust_blocks(int32_t cells);
 // blocks of `threads` threads (64, 128 or 256), 4 pixels of one output row each
 int32_t composite_blocks(int32_t ow, int32_t oh, int32_t threads);
+// 1 when n tiles of tw x th fit the composite's 32-bit tile offsets
+int32_t composite_narrow(int32_t n_tiles, int32_t tw, int32_t th);
 void launch_cell_adjust_batch(const CompSeg* d_segs, const int32_t* d_starts, int32_t n_segs,
                               int32_t total_blocks, hipStream_t stream);
 void launch_composite_batch(const CompSeg* d_segs, const int32_t* d_starts, int32_t n_segs,
@@ -202,8 +205,8 @@ void launch_cell_adjust(const int32_t* grid, int32_t cells, const uint8_t* tile_
                         const uint8_t* target_avg, float strength, float4* out,
                         hipStream_t stream);
 void launch_composite(const int32_t* grid, int32_t gw, int32_t gh, int32_t tw, int32_t th,
-                      const uint8_t* tiles, const float4* adj, int32_t adjust, uint8_t* out,
-                      hipStream_t stream);
+                      const uint8_t* tiles, int32_t n_tiles, const float4* adj, int32_t adjust,
+                      uint8_t* out, hipStream_t stream);
 void launch_adjust_pixels(float4 adj, const uint8_t* in, int64_t npx, uint8_t* out,
                           hipStream_t stream);
 void launch_optimal_adjust(const uint8_t* tile, const uint8_t* target, int32_t n, float k,

@@ -227,6 +227,29 @@ def test_server_submit_many(ctx, mh, synth, orc):
     s.close()
 
 
+def test_server_groups_by_size(ctx, mh, synth, orc):
+    # more queued than one batch holds: batches take the oldest queued job and
+    # the queued jobs nearest to it in size, so jobs run out of queue order;
+    # every result is still the job's own, and the oldest job goes first
+    s = Setup(ctx, mh, synth, n_jobs=18, seed=21)
+    order = sorted(range(len(s.specs)), key=lambda k: (k % 2, k))  # sizes interleave
+    jobs = [s.job(k, own_outputs=True) for k in order]
+    srv = ctx.server(memory_budget=1 << 30, max_batch=2, streams=1)
+    tickets = srv.submit_many(jobs)
+    got = [srv.wait(t) for t in tickets]
+    launched = [srv.times(t)[1] for t in tickets]
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == len(jobs) and st["max_batch_jobs"] <= 2
+    assert launched[0] == min(launched)
+    for i, k in enumerate(order):
+        print(f"grouped job {k}", flush=True)
+        g, im = s.fetch(k, jobs[i].d_grid_out, jobs[i].d_image_out)
+        _same(s.alone(k), g, im, got[i], s.specs[k]["sa"] and s.specs[k]["weight"] > 0)
+        s.check_oracle(k, orc, g, im, got[i])
+    s.close()
+
+
 def test_server_rejects_bad_jobs(ctx, mh, synth):
     s = Setup(ctx, mh, synth, n_jobs=2, seed=9)
     j = s.job(0, own_outputs=False)

@@ -140,6 +140,33 @@ def test_nearest_n_runs_of_cells(ctx, orc, n, nq, walk):
     lib.close()
 
 
+@pytest.mark.parametrize("n,k,kind", [(255, 100, "mix"), (256, 128, "mix"), (257, 100, "mix"),
+                                      (257, 1, "mix"), (320, 65, "dup"), (4097, 100, "mix"),
+                                      (4097, 64, "far"), (10000, 128, "dup"), (2500, 100, "far")])
+def test_nearest_n_blocks(ctx, orc, n, k, kind):
+    # the block layout's edges: libraries just below, at and above the
+    # 256-material buffer and the 64-material blocks, clustered colours (most
+    # blocks out of reach), queries far from every cluster (many blocks in
+    # reach, loose thresholds) and exact duplicates split across blocks
+    rng = _rng(n * 7 + k)
+    means = rng.random((3, 3)) * [100, 200, 200] - [0, 100, 100]
+    lab = means[rng.integers(0, 3, n)] + rng.normal(0, 4.0, (n, 3))
+    if kind == "dup":
+        lab[n // 2:] = lab[: n - n // 2][rng.permutation(n - n // 2)]
+    lab = lab.astype(np.float32)
+    if kind == "far":
+        q = rng.random((2000, 3)) * [100, 200, 200] - [0, 100, 100]
+    else:
+        q = np.cumsum(rng.normal(0, 1.0, (2000, 3)), axis=0) % [100, 200, 200] - [0, 100, 100]
+    q = q.astype(np.float32)
+    lib = ctx.library_from_lab(lab)
+    gi, gd = ctx.nearest_n(lib, q, k)
+    oi, od = orc.nearest_n(lab, q, k)
+    np.testing.assert_array_equal(gi, oi)
+    assert_bits_equal(gd, od)
+    lib.close()
+
+
 def test_nearest_n_ties_by_index(ctx, orc):
     # many exact duplicates: equal squared distances must come out by index
     rng = _rng(7)
