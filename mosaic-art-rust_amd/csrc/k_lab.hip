@@ -41,7 +41,7 @@ struct LabShared {
 //       + py * row_stride + px * 3.
 // Pixel index -> (px, py) by umulhi with cw_magic = floor((2^32 - 1) / cw) + 1,
 // exact while pix * cw < 2^32 (pix < cw * ch), else by division.
-template <int kCells>
+template <int kCells, bool kOff32>
 __device__ __forceinline__ void cell_lab_block(const CellSeg& s, int32_t blk,
                                                const float* __restrict__ lut_g, LabShared& sh) {
     constexpr int kChunk = kLabPixels / kCells;   // pixels per cell and round
@@ -57,15 +57,17 @@ __device__ __forceinline__ void cell_lab_block(const CellSeg& s, int32_t blk,
     const int32_t npx = cw * ch;
     const uint32_t cw_magic = 0xffffffffu / (uint32_t)cw + 1u;
     const bool magic_ok = (uint64_t)(uint32_t)npx * (uint32_t)cw < (1ull << 32);
-    // a pixel's byte offset within its cell in 32 bits where it fits
-    const bool off32 = s.row_stride >= 0 && (uint64_t)ch * (uint64_t)s.row_stride < (1ull << 31);
     const uint32_t rs32 = (uint32_t)s.row_stride;
     // summing lane -> (cell, channel)
     const int sum_cell = tid / 3, sum_ch = tid % 3;
     const bool summer = tid < 3 * kCells && cell0 + sum_cell < n_cells;
     float acc = 0.0f;
-    // this thread's pixel p_own of cells g_j = j * kSpan + tid / kChunk
+    // this thread's pixel p_own of cells g_j = j * kSpan + wv, wv = tid / kChunk;
+    // kChunk is a multiple of 64, so wv is the same on every lane of a wave
+    // and the cells' base addresses are scalar
+    static_assert(kChunk % 64 == 0, "a wave's lanes share their cells");
     const int p_own = tid % kChunk;
+    const int wv = __builtin_amdgcn_readfirstlane(tid / kChunk);
     // integer RGB sums of the pixels this thread converts (calculate_average_rgb,
     // src/main.rs:687-708, when out_rgb is requested)
     const bool want_rgb = s.out_rgb != nullptr;
@@ -74,7 +76,7 @@ __device__ __forceinline__ void cell_lab_block(const CellSeg& s, int32_t blk,
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
         rsum[j][0] = rsum[j][1] = rsum[j][2] = 0u;
-        const int32_t cell = cell0 + j * kSpan + tid / kChunk;
+        const int32_t cell = cell0 + j * kSpan + wv;
         const int32_t c = cell < n_cells ? cell : n_cells - 1;
         cell_base[j] = (const MH_GAS uint8_t*)s.rgb + (int64_t)(c / per_row) * s.block_row_stride +
                        (int64_t)(c % per_row) * s.block_col_stride;
@@ -83,16 +85,17 @@ __device__ __forceinline__ void cell_lab_block(const CellSeg& s, int32_t blk,
     __syncthreads();
     for (int32_t base = 0; base < npx; base += kChunk) {
         const int32_t len = min(kChunk, npx - base);
+        // the pixel's offset within its cell, the same for the kPer cells
+        const int32_t pix = base + p_own;
+        const int32_t py = magic_ok ? (int32_t)__umulhi((uint32_t)pix, cw_magic) : pix / cw;
+        const int32_t px = pix - py * cw;
+        const uint32_t o32 = (uint32_t)py * rs32 + (uint32_t)px * 3u;
+        const int64_t o64 = (int64_t)py * s.row_stride + px * 3;
 #pragma unroll
         for (int j = 0; j < kPer; ++j) {
-            const int g = j * kSpan + tid / kChunk;
+            const int g = j * kSpan + wv;
             if (cell0 + g < n_cells && p_own < len) {
-                const int32_t pix = base + p_own;
-                const int32_t py = magic_ok ? (int32_t)__umulhi((uint32_t)pix, cw_magic) : pix / cw;
-                const int32_t px = pix - py * cw;
-                const MH_GAS uint8_t* src =
-                    off32 ? cell_base[j] + ((uint32_t)py * rs32 + (uint32_t)px * 3u)
-                          : cell_base[j] + (int64_t)py * s.row_stride + px * 3;
+                const MH_GAS uint8_t* src = kOff32 ? cell_base[j] + o32 : cell_base[j] + o64;
                 const uint32_t r = src[0], gg = src[1], b = src[2];
                 float L, A, B;
                 rgb8_to_lab(sh.lut, r, gg, b, L, A, B);
@@ -140,11 +143,23 @@ __device__ __forceinline__ void cell_lab_block(const CellSeg& s, int32_t blk,
     }
 }
 
+// kOff32: a pixel's byte offset within its cell fits 31 bits (a uniform
+// choice per segment), so loads take the cell's scalar base plus a 32-bit
+// offset.
+template <int kCells>
+__device__ __forceinline__ void cell_lab_run(const CellSeg& s, int32_t blk,
+                                             const float* __restrict__ lut_g, LabShared& sh) {
+    if (s.row_stride >= 0 && (uint64_t)s.ch * (uint64_t)s.row_stride < (1ull << 31))
+        cell_lab_block<kCells, true>(s, blk, lut_g, sh);
+    else
+        cell_lab_block<kCells, false>(s, blk, lut_g, sh);
+}
+
 template <int kCells>
 __global__ __launch_bounds__(kLabThreads) void cell_lab_kernel(CellSeg s,
                                                                const float* __restrict__ lut) {
     __shared__ LabShared sh;
-    cell_lab_block<kCells>(s, blockIdx.x, lut, sh);
+    cell_lab_run<kCells>(s, blockIdx.x, lut, sh);
 }
 
 template <int kCells>
@@ -153,7 +168,7 @@ __global__ __launch_bounds__(kLabThreads) void cell_lab_batch_kernel(
     const float* __restrict__ lut) {
     __shared__ LabShared sh;
     const int32_t j = find_segment(starts, n_segs, blockIdx.x);
-    cell_lab_block<kCells>(segs[j], blockIdx.x - starts[j], lut, sh);
+    cell_lab_run<kCells>(segs[j], blockIdx.x - starts[j], lut, sh);
 }
 
 int32_t cell_lab_cells_per_block(int64_t total_cells) { return total_cells >= 65536 ? 16 : 4; }

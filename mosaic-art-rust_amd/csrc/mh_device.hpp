@@ -114,7 +114,8 @@ __device__ __forceinline__ float clampf(float v, float lo, float hi) {
 
 // Cube root for Lab's f(t) (t in (216/24389, ~1.09]): a single-precision
 // estimate from the hardware log2 / exp2 (about 2^-21 relative), then two
-// Newton steps for t^3 = x in double, t <- t - (t^3 - x) * r, with
+// Newton steps for t^3 = x in double, t <- t - (t^3 - x) * r (fused:
+// t^3 - x = fma(t t, t, -x), then fma(-(t^3 - x), r, t)), with
 // r ~ 1/(3 t0^2) from the single-precision reciprocal (the correction is so
 // small that r's own error ends below double rounding), rounded once to f32.
 // The host restatement reaches the same correctly rounded value with four
@@ -127,8 +128,8 @@ __device__ __forceinline__ float cbrt_f32(float xf) {
     double t = (double)e;
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
-        const double t3 = t * t * t;
-        t = t - (t3 - x) * r;
+        const double f = __builtin_fma(t * t, t, -x);  // t^3 - x, one rounding
+        t = __builtin_fma(-f, r, t);
     }
     return (float)t;
 }
@@ -187,9 +188,12 @@ __device__ __forceinline__ void rgb8_to_lab(const float* lut, uint32_t r8, uint3
     x = __builtin_fmaf(__builtin_fmaf(-qx, 0.95047f, x), rx, qx);
     z = __builtin_fmaf(__builtin_fmaf(-qz, 1.08883f, z), rz, qz);
     float fx = lab_f(x), fy = lab_f(y), fz = lab_f(z);
-    L = clampf(fy * 116.0f - 16.0f, 0.0f, 100.0f);
-    A = clampf((fx - fy) * 500.0f, -128.0f, 127.0f);
-    B = clampf((fy - fz) * 200.0f, -128.0f, 127.0f);
+    // x, y, z >= +0 (non-negative ramp and coefficients), so f(.) is finite
+    // and positive and none of the three values below is NaN or -0 (a zero
+    // difference or sum rounds to +0): each clamp is a median
+    L = __builtin_amdgcn_fmed3f(fy * 116.0f - 16.0f, 0.0f, 100.0f);
+    A = __builtin_amdgcn_fmed3f((fx - fy) * 500.0f, -128.0f, 127.0f);
+    B = __builtin_amdgcn_fmed3f((fy - fz) * 200.0f, -128.0f, 127.0f);
 }
 
 // ------------------------------------------------------------- distances
