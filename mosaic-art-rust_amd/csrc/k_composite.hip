@@ -289,56 +289,76 @@ __device__ __forceinline__ void composite_block(const CompSeg& s, int32_t b, flo
                 gload4(reinterpret_cast<const float4*>(kContrast1.v), threadIdx.x);
         __syncthreads();
     }
-    int32_t X = (b - Y * per_row) * bs * kCompPx + (int32_t)threadIdx.x;
-    int32_t cx = X / tw;
-    uint32_t dx3 = (uint32_t)(X - cx * tw) * 3u;
-    const int32_t q = bs / tw;
-    const uint32_t r3 = (uint32_t)(bs - q * tw) * 3u, tw3 = (uint32_t)tw * 3u;
+    const int32_t X0 = (b - Y * per_row) * bs * kCompPx + (int32_t)threadIdx.x;
+    // the pixels go through in stages, so that each stage's loads are in
+    // flight together: cells, then tile pixels and adjustments, then the
+    // arithmetic and stores
+    int32_t m[kCompPx], cxk[kCompPx];
+    uint32_t dxk[kCompPx];
+    {
+        int32_t cx = X0 / tw;
+        uint32_t dx3 = (uint32_t)(X0 - cx * tw) * 3u;
+        const int32_t q = bs / tw;
+        const uint32_t r3 = (uint32_t)(bs - q * tw) * 3u, tw3 = (uint32_t)tw * 3u;
+#pragma unroll
+        for (int32_t k = 0; k < kCompPx; ++k) {
+            cxk[k] = cx;
+            dxk[k] = dx3;
+            m[k] = X0 + k * bs < ow ? gload_at(grid, (uint32_t)cx * 4u) : kEmpty;
+            cx += q;
+            dx3 += r3;
+            if (dx3 >= tw3) {
+                dx3 -= tw3;
+                ++cx;
+            }
+        }
+    }
+    // a pixel's red and green bytes as one (unaligned) 16-bit load, blue
+    // alone; the raw loaded values are kept until the last stage, so no
+    // wait is needed before the next pixel's loads
+    typedef uint16_t u16_any __attribute__((aligned(1)));
+    uint32_t rg[kCompPx], bk[kCompPx];
+    v4f_t av[kCompPx];
 #pragma unroll
     for (int32_t k = 0; k < kCompPx; ++k) {
+        rg[k] = 0u;
+        bk[k] = 0u;
+        av[k] = v4f_t{0.0f, 1.0f, 0.0f, 1.0f};
+        if (m[k] != kEmpty) {
+            const MH_GAS uint8_t* p =
+                kNarrow ? (const MH_GAS uint8_t*)tiles + (__umul24((uint32_t)m[k], tile_bytes) + dxk[k])
+                        : (const MH_GAS uint8_t*)tiles + ((int64_t)m[k] * tile_bytes + (int64_t)dxk[k]);
+            rg[k] = *(const MH_GAS u16_any*)p;
+            bk[k] = p[2];
+            if (s.adjust) av[k] = gload_at(reinterpret_cast<const v4f_t*>(adj), (uint32_t)cxk[k] * 16u);
+        }
+    }
+#pragma unroll
+    for (int32_t k = 0; k < kCompPx; ++k) {
+        const int32_t X = X0 + k * bs;
         if (X < ow) {
-            const int32_t m = gload_at(grid, (uint32_t)cx * 4u);
-            uint32_t r = 0, g = 0, bl = 0;
-            if (m != kEmpty) {
-                const MH_GAS uint8_t* p =
-                    kNarrow ? (const MH_GAS uint8_t*)tiles +
-                                  (__umul24((uint32_t)m, tile_bytes) + dx3)
-                            : (const MH_GAS uint8_t*)tiles +
-                                  ((int64_t)m * tile_bytes + (int64_t)dx3);
-                r = p[0];
-                g = p[1];
-                bl = p[2];
-                // per-cell adjustments come from calculate_optimal_adjustment
-                // and ColorAdjustment::new's clamps (cell_adjust_one), whose
-                // contrast is always 1
-                if (s.adjust) {
-                    const v4f_t av =
-                        gload_at(reinterpret_cast<const v4f_t*>(adj), (uint32_t)cx * 16u);
-                    const float4 a = make_float4(av.x, av.y, av.z, av.w);
-                    float br, bg, bb;
-                    if (a.y == 1.0f) {
-                        br = clamp_unit(ctab[r] + a.x);
-                        bg = clamp_unit(ctab[g] + a.x);
-                        bb = clamp_unit(ctab[bl] + a.x);
-                    } else {
-                        br = bright_contrast<true>(u8_unit(r), a.x, a.y);
-                        bg = bright_contrast<true>(u8_unit(g), a.x, a.y);
-                        bb = bright_contrast<true>(u8_unit(bl), a.x, a.y);
-                    }
-                    adjust_tail<true>(a, br, bg, bb, r, g, bl);
+            uint32_t r = rg[k] & 0xffu, g = rg[k] >> 8, bl = bk[k];
+            // per-cell adjustments come from calculate_optimal_adjustment and
+            // ColorAdjustment::new's clamps (cell_adjust_one), whose contrast
+            // is always 1
+            if (m[k] != kEmpty && s.adjust) {
+                const float4 a = make_float4(av[k].x, av[k].y, av[k].z, av[k].w);
+                float br, bg, bb;
+                if (a.y == 1.0f) {
+                    br = clamp_unit(ctab[r] + a.x);
+                    bg = clamp_unit(ctab[g] + a.x);
+                    bb = clamp_unit(ctab[bl] + a.x);
+                } else {
+                    br = bright_contrast<true>(u8_unit(r), a.x, a.y);
+                    bg = bright_contrast<true>(u8_unit(g), a.x, a.y);
+                    bb = bright_contrast<true>(u8_unit(bl), a.x, a.y);
                 }
+                adjust_tail<true>(a, br, bg, bb, r, g, bl);
             }
             MH_GAS uint8_t* d = (MH_GAS uint8_t*)out + (uint32_t)X * 3u;
             d[0] = (uint8_t)r;
             d[1] = (uint8_t)g;
             d[2] = (uint8_t)bl;
-        }
-        X += bs;
-        cx += q;
-        dx3 += r3;
-        if (dx3 >= tw3) {
-            dx3 -= tw3;
-            ++cx;
         }
     }
 }

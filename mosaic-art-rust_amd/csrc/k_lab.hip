@@ -91,12 +91,24 @@ __device__ __forceinline__ void cell_lab_block(const CellSeg& s, int32_t blk,
         const int32_t px = pix - py * cw;
         const uint32_t o32 = (uint32_t)py * rs32 + (uint32_t)px * 3u;
         const int64_t o64 = (int64_t)py * s.row_stride + px * 3;
+        // every pixel's loads first (raw values kept until the conversion,
+        // so they are all in flight together), then the conversions
+        typedef uint16_t u16_any __attribute__((aligned(1)));
+        uint32_t rg[kPer], bk[kPer];
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            rg[j] = bk[j] = 0u;
+            if (cell0 + j * kSpan + wv < n_cells && p_own < len) {
+                const MH_GAS uint8_t* src = kOff32 ? cell_base[j] + o32 : cell_base[j] + o64;
+                rg[j] = *(const MH_GAS u16_any*)src;
+                bk[j] = src[2];
+            }
+        }
 #pragma unroll
         for (int j = 0; j < kPer; ++j) {
             const int g = j * kSpan + wv;
             if (cell0 + g < n_cells && p_own < len) {
-                const MH_GAS uint8_t* src = kOff32 ? cell_base[j] + o32 : cell_base[j] + o64;
-                const uint32_t r = src[0], gg = src[1], b = src[2];
+                const uint32_t r = rg[j] & 0xffu, gg = rg[j] >> 8, b = bk[j];
                 float L, A, B;
                 rgb8_to_lab(sh.lut, r, gg, b, L, A, B);
                 const int slot = g * (kChunk + 1) + p_own;  // cell g's chunk, pixel order

@@ -50,14 +50,8 @@ __device__ __forceinline__ uint64_t make_key(uint32_t dbits, int32_t i) {
     return ((uint64_t)dbits << 32) | (uint32_t)i;
 }
 
-__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        const uint32_t o = (uint32_t)__shfl_xor((int)v, off, 64);
-        v = o > v ? o : v;
-    }
-    return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
-}
+// max = ~min(~v), with wave_min_u32's DPP steps; all 64 lanes active.
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) { return ~wave_min_u32(~v); }
 
 // The value of lane ^ m (m a power of two below 64) for every lane: DPP
 // lane swizzles within rows of 16 for m = 1, 2, 8, the swizzle unit for
@@ -215,12 +209,17 @@ __device__ __forceinline__ int32_t cut_buffer(uint64_t* buf, int32_t count, int3
 // Lower bound, as float bits, on the squared distance sq_distance computes
 // from the query to any material in a block's box.  Per axis the smallest
 // |fl(q - x)| over x in [lo, hi] is at a face (fl(q - x) is monotone in x),
-// or 0 inside; the squares and the sum, in sq_distance's order, are
-// monotone in their operands, so no member computes a smaller distance.
+// or 0 inside: max(fl(lo - q), fl(q - hi), 0), since fl(lo - q) = -fl(q - lo)
+// (one is positive only outside the box, on its side).  The squares and the
+// sum, in sq_distance's order, are monotone in their operands, so no member
+// computes a smaller distance.
+__device__ __forceinline__ float face_gap(float q, float lo, float hi) {
+    return __builtin_fmaxf(__builtin_fmaxf(lo - q, q - hi), 0.0f);
+}
 __device__ __forceinline__ uint32_t box_bound(float ql, float qa, float qb, float4 lo, float4 hi) {
-    const float tl = ql < lo.x ? ql - lo.x : (ql > hi.x ? ql - hi.x : 0.0f);
-    const float ta = qa < lo.y ? qa - lo.y : (qa > hi.y ? qa - hi.y : 0.0f);
-    const float tb = qb < lo.z ? qb - lo.z : (qb > hi.z ? qb - hi.z : 0.0f);
+    const float tl = face_gap(ql, lo.x, hi.x);
+    const float ta = face_gap(qa, lo.y, hi.y);
+    const float tb = face_gap(qb, lo.z, hi.z);
     return __float_as_uint(tl * tl + ta * ta + tb * tb);
 }
 
