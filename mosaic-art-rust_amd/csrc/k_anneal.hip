@@ -270,7 +270,6 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
     int64_t s = 0, kbase = 0;
     uint32_t tag = 1;
     const int64_t max_iter = job.max_iter;
-    const uint64_t lanes_below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
 
     // Proposal `it` is drawn once, in a block of 128 made by the whole wave
     // before the round that first needs it; a round re-evaluating proposals
@@ -323,7 +322,7 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
             // delta, so their latency overlaps the term gathers
             float temp = 1.0f, u = 0.0f;
             if (!job.greedy) {
-                temp = gload(job.temps, kbase + __popcll(nonskip & lanes_below));
+                temp = gload(job.temps, kbase + lanes_before(nonskip));
                 u = ring_u[slot];
             }
             delta = job.term ? swap_delta_core(gw, gh, TM, x1, y1, x2, y2, m1, m2, nb)
@@ -405,7 +404,7 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
         // best = fminf(best, cost) equals `cost < best ? cost : best` here (no
         // NaN, no -0.0) and keeps the compare off the dependent chain.
         pack_s[lane] = 0.0f;
-        if ((accmask >> lane) & 1ull) pack_s[__popcll(accmask & lanes_below)] = delta;
+        if (lane_in(accmask)) pack_s[lanes_before(accmask)] = delta;
         __syncthreads();  // other lanes' writes, not this lane's own stores, are read below
         // Eight deltas at a time come back as two uniform 16-byte LDS reads, so
         // the chain adds VGPRs (no lane reads into SGPRs between the adds).

@@ -418,7 +418,7 @@ __device__ __forceinline__ void raster_step(const CandSet& s, const CandSet& sn,
         const int32_t up_m = st.up_m;
         const int32_t left_m = x > 0 ? st.left_m : kEmpty;
         const bool pl0 = s.m0 >= 0 && s.m0 != up_m && s.m0 != left_m;
-        const bool ok0 = ((ob0 >> lane) & 1ull) != 0;
+        const bool ok0 = lane_in(ob0);
         const uint32_t u1 = usage_s[s.m1];  // a -1 sentinel reads the guard word
         const bool pl1 = s.m1 >= 0 && s.m1 != up_m && s.m1 != left_m;
         const bool ok1 = pl1 && u1 < max_usage;

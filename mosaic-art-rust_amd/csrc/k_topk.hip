@@ -194,12 +194,11 @@ __device__ __forceinline__ int32_t cut_buffer(uint64_t* buf, int32_t count, int3
         }
     }
     if (!found) return count;
-    const uint64_t lanes_below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     int32_t base = 0;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         const uint64_t keep = lanes_ule(d[t], sel) & lanes_slt(t * 64 + lane, count);
-        if ((keep >> lane) & 1ull) buf[base + __popcll(keep & lanes_below)] = key[t];
+        if (lane_in(keep)) buf[base + lanes_before(keep)] = key[t];
         base += __popcll(keep);
     }
     __builtin_amdgcn_wave_barrier();
@@ -235,7 +234,6 @@ __device__ __forceinline__ void topk_run(const TopkSeg& s, int32_t q0, int32_t r
     const int32_t nb = B.nb;
     const int32_t k = s.k < n ? s.k : n;
     const int32_t q_end = q0 + run < s.nq ? q0 + run : s.nq;
-    const uint64_t lanes_below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     constexpr int kSteps = 4;  // blocks per load phase
 
     for (int32_t q = q0; q < q_end; ++q) {
@@ -318,7 +316,7 @@ __device__ __forceinline__ void topk_run(const TopkSeg& s, int32_t q0, int32_t r
                 th = (uint32_t)(buf[k - 1] >> 32);
                 mask &= lanes_ule(d, th);
             }
-            if ((mask >> lane) & 1ull) buf[count + __popcll(mask & lanes_below)] = make_key(d, m);
+            if (lane_in(mask)) buf[count + lanes_before(mask)] = make_key(d, m);
             count += __popcll(mask);
         };
         // every block whose bound is within th, up to kSteps per load phase

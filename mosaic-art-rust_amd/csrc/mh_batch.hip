@@ -936,7 +936,11 @@ struct ChunkArena {
 
 enum JobState { J_QUEUED = 0, J_RUNNING = 1, J_DONE = 2, J_FAILED = 3 };
 
-constexpr int32_t kChainStreams = 4;  // chain streams of a server (fewer with fewer slots)
+// Chain streams of a server with n slots: one per slot up to four, then one
+// per two slots, at most eight (each has an output stream beside it).
+inline int32_t chain_streams(int32_t n_slots) {
+    return n_slots <= 4 ? n_slots : std::min(std::max(n_slots / 2, 4), 8);
+}
 constexpr int kWideStreams = 2;       // wide streams of a server (batches alternate)
 
 struct JobRec {
@@ -1331,7 +1335,7 @@ int mh_server_create(mh_context* ctx, const mh_server_params* p, mh_server** out
     for (hipStream_t& w : sv->wide)
         if (e == hipSuccess) e = make_stream(ctx->device, &w);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&sv->up, hipStreamNonBlocking);
-    sv->lanes.assign((size_t)std::min(n_slots, kChainStreams), nullptr);
+    sv->lanes.assign((size_t)chain_streams(n_slots), nullptr);
     sv->outs.assign(sv->lanes.size(), nullptr);
     sv->lane_load.assign(sv->lanes.size(), 0);
     for (size_t i = 0; i < sv->lanes.size(); ++i) {

@@ -47,6 +47,17 @@ __device__ __forceinline__ void gstore(T* p, int64_t i, T v) {
 // directly (HIP's __ballot goes through an int select and a second compare).
 __device__ __forceinline__ uint64_t wave_ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
+// This lane's bit of a lane mask, as a condition: the mask itself becomes the
+// predicate (no per-lane shift and compare).
+__device__ __forceinline__ bool lane_in(uint64_t mask) {
+    return __builtin_amdgcn_inverse_ballot_w64(mask);
+}
+// Set bits of a lane mask below this lane (v_mbcnt_lo / v_mbcnt_hi).
+__device__ __forceinline__ uint32_t lanes_before(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
 // Lane masks of one compare each (bit l set when the compare holds in lane l;
 // inactive lanes read 0).  Each is a single v_cmp into a scalar register
 // pair, so conditions combine with scalar mask logic instead of being
