@@ -388,9 +388,12 @@ int mh_batch_set_tuning(mh_batch *batch, const mh_batch_tuning *tuning);
  * most max_batch jobs each); jobs that arrive while batches run join the next
  * one as slots and memory free up.  A batch takes the oldest queued job and
  * the queued jobs among the next 8 * max_batch nearest to it in cell count,
- * so that one batch's placement chains are of similar length (a job's result
- * does not depend on its batch); with at most max_batch jobs queued,
- * admission is first-come first-served.  The batches' wide stages
+ * so that one batch's placement chains are of similar length; once the whole
+ * queue fits those 8 * max_batch, the job with the most cells takes the
+ * oldest one's place (until the oldest has been passed over 8 times), so a
+ * backlog ends on its shortest chains.  A job's result does not depend on
+ * its batch; with at most max_batch jobs queued, admission is first-come
+ * first-served.  The batches' wide stages
  * (cell Labs, K nearest) run on two streams, consecutive batches
  * alternating, their placement chains and annealing on up to four more
  * beside later batches' wide stages, and their output stages (adjustment,

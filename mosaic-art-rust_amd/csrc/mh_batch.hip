@@ -953,6 +953,7 @@ struct JobRec {
     uint8_t* mem = nullptr;  // working memory (in chunk `chunk` of the server's arena)
     int32_t chunk = 0;
     bool holds_mem = false;
+    int32_t passed_over = 0;  // batches formed without it while it was the oldest queued
 };
 
 // One batch in flight: its descriptor tables, result staging and events.
@@ -1019,14 +1020,18 @@ struct mh_server {
 
 namespace {
 
-// Batch formation.  The oldest queued job always goes into the next batch;
-// the rest of the batch are the queued jobs, among the next
-// kWindowBatches * max_batch, whose cell counts are nearest to its own (by
-// ratio), in queue order.  A placement launch lasts as long as its longest
-// chain (one wave walks a job's cells in order), so batches of like-sized
-// jobs keep each chain stream from waiting on one long job per batch; which
-// jobs share a batch changes no job's result.  With at most max_batch jobs
-// queued this is first-come first-served.
+// Batch formation over a window of the queue, its next
+// kWindowBatches * max_batch jobs.  The batch's anchor is the oldest queued
+// job -- except at the end of a backlog, when the whole queue fits the
+// window: then it is the job with the most cells (the oldest of equals),
+// so the last batches hold the shortest chains, unless the oldest job has
+// already been passed over kWindowBatches times.  The rest of the batch are
+// the window's jobs nearest to the anchor in cell count (by ratio), in queue
+// order.  A placement launch lasts as long as its longest chain (one wave
+// walks a job's cells in order), so batches of like-sized jobs keep each
+// chain stream from waiting on one long job per batch.  Which jobs share a
+// batch changes no job's result.  With at most max_batch jobs queued this is
+// first-come first-served.
 constexpr size_t kWindowBatches = 8;
 
 std::vector<size_t> pick_batch(const mh_server& sv) {
@@ -1037,17 +1042,22 @@ std::vector<size_t> pick_batch(const mh_server& sv) {
         for (size_t i = 0; i < wn; ++i) pos.push_back(i);
         return pos;
     }
-    auto lcells = [&](size_t i) {
-        const int32_t c = sv.jobs[(size_t)sv.queue[i]].spec.cells;
-        return std::log((double)(c > 1 ? c : 1));
-    };
-    const double c0 = lcells(0);
-    std::vector<std::pair<double, size_t>> d(wn - 1);
-    for (size_t i = 1; i < wn; ++i) d[i - 1] = {std::fabs(lcells(i) - c0), i};
+    auto cells = [&](size_t i) { return sv.jobs[(size_t)sv.queue[i]].spec.cells; };
+    size_t a = 0;
+    if (sv.queue.size() == wn &&
+        sv.jobs[(size_t)sv.queue[0]].passed_over < (int32_t)kWindowBatches)
+        for (size_t i = 1; i < wn; ++i)
+            if (cells(i) > cells(a)) a = i;
+    auto lcells = [&](size_t i) { return std::log((double)std::max<int32_t>(cells(i), 1)); };
+    const double c0 = lcells(a);
+    std::vector<std::pair<double, size_t>> d;
+    d.reserve(wn - 1);
+    for (size_t i = 0; i < wn; ++i)
+        if (i != a) d.push_back({std::fabs(lcells(i) - c0), i});
     std::nth_element(d.begin(), d.begin() + (ptrdiff_t)(b - 1), d.end());
-    pos.push_back(0);
+    pos.push_back(a);
     for (size_t k = 0; k + 1 < b; ++k) pos.push_back(d[k].second);
-    std::sort(pos.begin() + 1, pos.end());
+    std::sort(pos.begin(), pos.end());
     return pos;
 }
 
@@ -1191,11 +1201,15 @@ void worker_loop(mh_server* sv) {
             std::vector<int64_t> take;
             std::vector<JobRec*> recs;
             std::vector<size_t> taken;  // queue positions, ascending
+            size_t first = 0;           // the batch's first job (queue position)
             for (size_t i : pick_batch(*sv)) {
                 const int64_t t = sv->queue[i];
                 JobRec& r = sv->jobs[(size_t)t];
                 uint8_t* mem = sv->arena.alloc(r.spec.bytes, &r.chunk);
-                if (!mem) break;  // the rest waits for memory
+                if (!mem) {  // the rest waits for memory
+                    first = i;
+                    break;
+                }
                 r.mem = mem;
                 r.holds_mem = true;
                 r.state = J_RUNNING;
@@ -1205,22 +1219,24 @@ void worker_loop(mh_server* sv) {
                 taken.push_back(i);
             }
             if (take.empty()) {
-                // the oldest job waits for memory -- unless nothing runs that
-                // could give any back, when the device itself has no room
+                // the batch's first job waits for memory -- unless nothing
+                // runs that could give any back, when the device itself has
+                // no room for it
                 bool busy = false;
                 for (const StreamSlot& sl : sv->slots) busy |= sl.busy;
                 if (!busy) {
-                    JobRec& r = sv->jobs[(size_t)sv->queue.front()];
+                    JobRec& r = sv->jobs[(size_t)sv->queue[first]];
                     r.state = J_FAILED;
                     r.err = MH_ERR_OUT_OF_MEMORY;
                     r.msg = "no device memory for the job's working set";
                     r.t_done = now_ms();
                     ++sv->completed;
-                    sv->queue.pop_front();
+                    sv->queue.erase(sv->queue.begin() + (ptrdiff_t)first);
                     sv->cv_done.notify_all();
                     progressed = true;
                 }
             } else {
+                if (taken.front() != 0) ++sv->jobs[(size_t)sv->queue.front()].passed_over;
                 remove_positions(&sv->queue, taken);
             }
             if (!take.empty()) {
