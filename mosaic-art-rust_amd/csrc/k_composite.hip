@@ -296,7 +296,13 @@ __device__ __forceinline__ void composite_block(const CompSeg& s, int32_t b, flo
     int32_t m[kCompPx], cxk[kCompPx];
     uint32_t dxk[kCompPx];
     {
-        int32_t cx = X0 / tw;
+        // X0 / tw as umulhi(X0, floor((2^32 - 1) / tw) + 1), exact while
+        // X0 * tw < 2^32 (then X0 m / 2^32 exceeds X0 / tw by less than 1 / tw)
+        const uint32_t tw_magic = 0xffffffffu / (uint32_t)tw + 1u;
+        const bool magic_ok =
+            (uint64_t)((uint32_t)ow + (uint32_t)(bs * kCompPx)) * (uint32_t)tw < (1ull << 32);
+        int32_t cx = magic_ok ? (int32_t)__umulhi((uint32_t)X0, tw_magic)
+                              : (int32_t)((uint32_t)X0 / (uint32_t)tw);
         uint32_t dx3 = (uint32_t)(X0 - cx * tw) * 3u;
         const int32_t q = bs / tw;
         const uint32_t r3 = (uint32_t)(bs - q * tw) * 3u, tw3 = (uint32_t)tw * 3u;
