@@ -273,23 +273,29 @@ __device__ __forceinline__ void topk_run(const TopkSeg& s, int32_t q0, int32_t r
             // members cover k materials; the k-th smallest of those keys
             // bounds the k-th nearest overall.  Two full blocks give 128
             // keys and only the last block is partial, so three suffice.
+            // Each lane keeps its three smallest (bound, block) keys over its
+            // blocks lane, lane + 64, ..., so every bound is computed once;
+            // the wave minimum of the lanes' smallest is the next nearest
+            // block, and the lane that held it moves on to its next.
+            uint64_t lk0 = ~0ull, lk1 = ~0ull, lk2 = ~0ull;
+            for (int32_t c0 = 0; c0 < nb; c0 += 64) {
+                const int32_t b = c0 + lane;
+                const uint64_t key = ((uint64_t)bound_of(b) << 32) | (uint32_t)b;
+                const bool l0 = key < lk0, l1 = key < lk1, l2 = key < lk2;
+                lk2 = l1 ? lk1 : (l2 ? key : lk2);
+                lk1 = l0 ? lk0 : (l1 ? key : lk1);
+                lk0 = l0 ? key : lk0;
+            }
             uint64_t two0 = ~0ull, two1 = ~0ull;
-            int32_t picked[3] = {-1, -1, -1};
 #pragma unroll
             for (int it = 0; it < 3; ++it) {
-                uint64_t best = ~0ull;
-                for (int32_t c0 = 0; c0 < nb; c0 += 64) {
-                    const int32_t b = c0 + lane;
-                    uint32_t lb = bound_of(b);
-#pragma unroll
-                    for (int j = 0; j < it; ++j) lb = b == picked[j] ? 0xffffffffu : lb;
-                    const uint64_t key = ((uint64_t)lb << 32) | (uint32_t)b;
-                    best = key < best ? key : best;
-                }
-                best = wave_min_u64(best);
+                const uint64_t best = wave_min_u64(lk0);
                 const int32_t blk =
                     __builtin_amdgcn_readfirstlane((int32_t)(uint32_t)(best & 0xffffffffu));
-                picked[it] = blk;
+                const bool own = lk0 == best;  // keys are unique (the block in the low half)
+                lk0 = own ? lk1 : lk0;
+                lk1 = own ? lk2 : lk1;
+                lk2 = own ? ~0ull : lk2;
                 uint32_t d;
                 int32_t m;
                 member(blk, d, m);
@@ -301,10 +307,20 @@ __device__ __forceinline__ void topk_run(const TopkSeg& s, int32_t q0, int32_t r
                                      __popcll(wave_ballot(two1 != ~0ull));
                 if (real >= k) break;
             }
-            uint64_t two[2] = {two0, two1};
-            reg_sort<2>(two);
-            const uint64_t v = k - 1 < 64 ? two[0] : two[1];
-            th = (uint32_t)__builtin_amdgcn_readlane((int32_t)(v >> 32), (k - 1) & 63);
+            // th: the k-th smallest of those distances, rounded up to the end
+            // of its 2^12-ulp step -- the largest t of the form j * 2^12 with
+            // fewer than k keys below it, built bit by bit from the top; then
+            // at least k keys lie at or below t + 2^12 - 1.  (Padding keys
+            // have all bits set and are never counted.)
+            const uint32_t e0 = (uint32_t)(two0 >> 32), e1 = (uint32_t)(two1 >> 32);
+            uint32_t t = 0;
+#pragma unroll
+            for (int bit = 31; bit >= 12; --bit) {
+                const uint32_t cand = t | (1u << bit);
+                const int32_t below = __popcll(lanes_ult(e0, cand)) + __popcll(lanes_ult(e1, cand));
+                if (below < k) t = cand;
+            }
+            th = t | 0xfffu;
         }
         int32_t count = 0;
         // appends the passing lanes of one block to the buffer, first
