@@ -265,15 +265,27 @@ __device__ __forceinline__ void rgb_to_hsv(float r, float g, float b, float& h, 
         b = b > 0.0f ? b : 0.0f;
     }
     float mx, mn, sep, coeff;
-    if (r > g) {
-        mx = r; mn = g; sep = g - b; coeff = 0.0f;
+    if (kUnit) {
+        // the same choices as the branches below, as selects
+        const bool c1 = r > g;
+        const float mrg = c1 ? r : g, nrg = c1 ? g : r;
+        const bool c2 = b > mrg;
+        const float sep1 = c1 ? g - b : b - r;
+        sep = c2 ? r - g : sep1;
+        coeff = c2 ? 4.0f : (c1 ? 0.0f : 2.0f);
+        mx = c2 ? b : mrg;
+        mn = !c2 && b < nrg ? b : nrg;
     } else {
-        mx = g; mn = r; sep = b - r; coeff = 2.0f;
-    }
-    if (b > mx) {
-        mx = b; sep = r - g; coeff = 4.0f;
-    } else if (b < mn) {
-        mn = b;
+        if (r > g) {
+            mx = r; mn = g; sep = g - b; coeff = 0.0f;
+        } else {
+            mx = g; mn = r; sep = b - r; coeff = 2.0f;
+        }
+        if (b > mx) {
+            mx = b; sep = r - g; coeff = 4.0f;
+        } else if (b < mn) {
+            mn = b;
+        }
     }
     float hh = 0.0f, ss = 0.0f;
     if (mx != mn) {
@@ -315,10 +327,24 @@ __device__ __forceinline__ void hsv_to_rgb(float hdeg, float s, float v, float& 
     float x = c * (1.0f - fabsf(hm2 - 1.0f));
     float m = v - c;
     // the sextant of h: [0,1) .. [4,5) are disjoint, anything else
-    // (h >= 5, NaN) is the last case; each channel as selects
-    const bool s0 = h >= 0.0f && h < 1.0f, s1 = h >= 1.0f && h < 2.0f;
-    const bool s2 = h >= 2.0f && h < 3.0f, s3 = h >= 3.0f && h < 4.0f;
-    const bool s4 = h >= 4.0f && h < 5.0f;
+    // (h >= 5, NaN) is the last case; each channel as selects.  kUnit: h is
+    // not NaN and not below 0 (hp is in [0, 360)), so five thresholds fix it.
+    bool s0, s1, s2, s3, s4;
+    if (kUnit) {
+        const bool a1 = h >= 1.0f, a2 = h >= 2.0f, a3 = h >= 3.0f, a4 = h >= 4.0f;
+        const bool a5 = h >= 5.0f;
+        s0 = !a1;
+        s1 = a1 && !a2;
+        s2 = a2 && !a3;
+        s3 = a3 && !a4;
+        s4 = a4 && !a5;
+    } else {
+        s0 = h >= 0.0f && h < 1.0f;
+        s1 = h >= 1.0f && h < 2.0f;
+        s2 = h >= 2.0f && h < 3.0f;
+        s3 = h >= 3.0f && h < 4.0f;
+        s4 = h >= 4.0f && h < 5.0f;
+    }
     //        s0  s1  s2  s3  s4  else
     // rr:     c   x   0   0   x   c
     // gg:     x   c   c   x   0   0
