@@ -236,14 +236,34 @@ __device__ __forceinline__ void topk_run(const TopkSeg& s, int32_t q0, int32_t r
     const int32_t q_end = q0 + run < s.nq ? q0 + run : s.nq;
     constexpr int kSteps = 4;  // blocks per load phase
 
+    // The boxes of the first two chunks of blocks (blocks lane and 64 + lane)
+    // are the same for every cell of the run: loaded once, so a cell's first
+    // bounds wait for no memory (libraries of up to 8,192 materials need no
+    // other box).
+    float4 box_c[2][2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const int32_t b = c * 64 + lane;
+        const int32_t bc = b < nb ? b : nb - 1;
+        box_c[c][0] = gload4(B.box, 2 * bc);
+        box_c[c][1] = gload4(B.box, 2 * bc + 1);
+    }
+
     for (int32_t q = q0; q < q_end; ++q) {
         const float ql = gload(s.query, 3 * (int64_t)q), qa = gload(s.query, 3 * (int64_t)q + 1),
                     qb = gload(s.query, 3 * (int64_t)q + 2);
-        // this lane's bound of block b (blocks past the last bound to ~0)
+        // this lane's bound of block b = c0 + lane (blocks past the last bound
+        // to ~0)
         auto bound_of = [&](int32_t b) -> uint32_t {
+            const int32_t c0 = b - lane;
             const int32_t bc = b < nb ? b : nb - 1;
-            const uint32_t lb = box_bound(ql, qa, qb, gload4(B.box, 2 * bc),
-                                          gload4(B.box, 2 * bc + 1));
+            uint32_t lb;
+            if (c0 == 0)
+                lb = box_bound(ql, qa, qb, box_c[0][0], box_c[0][1]);
+            else if (c0 == 64)
+                lb = box_bound(ql, qa, qb, box_c[1][0], box_c[1][1]);
+            else
+                lb = box_bound(ql, qa, qb, gload4(B.box, 2 * bc), gload4(B.box, 2 * bc + 1));
             return b < nb ? lb : 0xffffffffu;
         };
         // member `lane` of block b: distance bits and material (-1: padding)
