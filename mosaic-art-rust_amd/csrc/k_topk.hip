@@ -29,7 +29,8 @@
 //     moves, so no sort step touches LDS.
 // The first cell of a run (no previous answers) takes the blocks nearest by
 // bound until each lane's two smallest member keys number at least K; the
-// K-th smallest of those real distances bounds the K-th smallest overall.
+// K-th smallest of those real distances, rounded up within 2^12 ulps, bounds
+// the K-th smallest overall.
 // The threshold is compared on the distance bits alone (one compare per
 // material): keys at exactly the threshold distance may enter the buffer
 // beyond the K-th, and the sorts cut them.
