@@ -245,6 +245,35 @@ __device__ __forceinline__ void load_cands(CandSet& s, const PlaceJob& job, int3
     s.d1 = __uint_as_float((uint32_t)(r1 >> 32));
 }
 
+// Candidate rows staged in LDS: the raster chain copies the rows of the next
+// group of cells into one half of a double buffer while it works through the
+// current group, with one global-to-LDS copy of 16 bytes per lane (1 KB, one
+// cell's 128 records) per cell, so a group starts with LDS reads instead of a
+// wait for global memory.
+constexpr int32_t kRowBytes = kCandStride * 8;
+static_assert(kRowBytes == 64 * 16, "one 16-byte copy per lane moves a row");
+
+// Starts the copy of cell `cell`'s row (cells past the grid copy the last
+// row) into `dst` (LDS, 16-byte aligned, uniform).
+__device__ __forceinline__ void stage_row(uint8_t* dst, const PlaceJob& job, int32_t cell,
+                                          int32_t cells, int lane) {
+    const int32_t row = cell < cells ? cell : cells - 1;
+    const uint8_t* src = reinterpret_cast<const uint8_t*>(job.cand) + (int64_t)row * kRowBytes +
+                         16 * lane;
+    __builtin_amdgcn_global_load_lds((const MH_GAS void*)src,
+                                     (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
+}
+
+// One cell's candidate records from a staged row.
+__device__ __forceinline__ void read_cands(CandSet& s, const uint8_t* row, int lane) {
+    const uint64_t* rec = reinterpret_cast<const uint64_t*>(row);
+    const uint64_t r0 = rec[lane], r1 = rec[lane + 64];
+    s.m0 = (int32_t)(uint32_t)r0;
+    s.d0 = __uint_as_float((uint32_t)(r0 >> 32));
+    s.m1 = (int32_t)(uint32_t)r1;
+    s.d1 = __uint_as_float((uint32_t)(r1 >> 32));
+}
+
 // GUI score: the Euclidean distance replaces the squared one.
 __device__ __forceinline__ void to_distance(CandSet& s) {
     s.d0 = sqrtf(s.d0);
@@ -255,6 +284,14 @@ constexpr int32_t kNoMat = -2;    // "no previous winner" (never a material or t
 // The count a -1 sentinel candidate reads (the guard word before the usage
 // counts): above every usage limit, and one more use does not wrap it.
 constexpr uint32_t kGuard = 0x7fffffffu;
+
+// Byte offset of the raster chain's staged candidate rows in its LDS: after
+// the lane scratch (64 words), the column row (gw words), the guard word and
+// the counts (n words), rounded up to 16 bytes.
+__host__ __device__ __forceinline__ size_t place_stage_offset(int32_t grid_w, int32_t n) {
+    const size_t words = 64 + (size_t)grid_w + 1 + (size_t)(n > 0 ? n : 1);
+    return (words * 4 + 15) & ~(size_t)15;
+}
 
 struct RasterState {
     int32_t x;         // current cell's column
@@ -626,26 +663,32 @@ __device__ __forceinline__ NarrowStats raster_narrow(const PlaceJob& job, bool s
 }
 
 // The look-ahead chain over a grid at least three columns wide (kSq: the
-// GUI score, whose distances are square-rooted as the records are used).
-template <bool kOutputs, bool kSq>
+// GUI score, whose distances are square-rooted as the records are used;
+// kStaged: the candidate rows come through `stage` in LDS).  The staged and
+// unstaged chains are compiled apart: merged, the compiler's wait counts
+// for the records would also wait for the staging copies in flight.
+template <bool kOutputs, bool kSq, bool kStaged>
 __device__ __forceinline__ void raster_wide(const PlaceJob& job, uint32_t* usage_s,
                                             int32_t* row_mat, int32_t* lane_scratch,
-                                            RasterState& st, int lane) {
+                                            uint8_t* stage, RasterState& st, int lane) {
     const int32_t gw = job.grid_w, n = job.n;
     const int32_t cells = gw * job.grid_h;
     CommitPorts port;
     port.mul = lane == 0 ? 4u : 0u;
     port.usage = lane == 0 ? usage_s : reinterpret_cast<uint32_t*>(lane_scratch);
     port.row = lane == 0 ? row_mat : lane_scratch;
-    // Candidate records for kGroup cells (and the two after them) are loaded
-    // at the top of each group and consumed in the same loop iteration, so
-    // the compiler's wait counts stay exact (a load consumed after the
-    // loop's back edge would make it wait for every load in flight); one
-    // memory latency per group.  The previous group's winners are stored
-    // just before the loads.  24 cells per group: one load latency per 24
-    // cells for 256 VGPRs (a few spilled to AGPRs, one wave per SIMD); 12
-    // cells cost 6% more chain time.
+    // Candidate records for kGroup cells (and the two after them) are read
+    // at the top of each group, from the LDS half the previous group filled
+    // (stage_row) while the next group's rows are requested into the other
+    // half, or (unstaged: LDS too small) loaded from global memory with
+    // one memory latency per group.  The records are consumed in the same
+    // loop iteration, so the compiler's wait counts stay exact (a load
+    // consumed after the loop's back edge would make it wait for every load
+    // in flight).  The previous group's winners are stored just after the
+    // reads.  24 cells per group (140 VGPRs, none spilled); 12 cells cost 6%
+    // more chain time.
     constexpr int kGroup = 24;
+    constexpr int32_t kHalf = (kGroup + 2) * kRowBytes;
     static_assert(kGroup <= 64, "a group's winners wait in one lane each");
     CandSet q[kGroup + 2];
     st.x = 0;
@@ -669,11 +712,32 @@ __device__ __forceinline__ void raster_wide(const PlaceJob& job, uint32_t* usage
     auto flush = [&](int32_t g0) {
         if (lane < kGroup && g0 + lane < cells) gstore(job.grid, g0 + lane, wins);
     };
-    int32_t c = 0;
-    for (; c + kGroup <= cells; c += kGroup) {
-        if (c > 0) flush(c - kGroup);
+    // rows of the group at cell g0 into half h
+    auto stage_group = [&](int32_t g0, int32_t h) {
 #pragma unroll
-        for (int i = 0; i < kGroup + 2; ++i) load_cands(q[i], job, c + i, cells, lane);
+        for (int i = 0; i < kGroup + 2; ++i)
+            stage_row(stage + h * kHalf + i * kRowBytes, job, g0 + i, cells, lane);
+    };
+    // the rows of the group at cell g0: from half h (waiting for their
+    // copies first), or straight from global memory when nothing is staged
+    auto read_group = [&](int32_t g0, int32_t h) {
+        if constexpr (kStaged) {
+            __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): the half's copies have landed
+#pragma unroll
+            for (int i = 0; i < kGroup + 2; ++i)
+                read_cands(q[i], stage + h * kHalf + i * kRowBytes, lane);
+        } else {
+#pragma unroll
+            for (int i = 0; i < kGroup + 2; ++i) load_cands(q[i], job, g0 + i, cells, lane);
+        }
+    };
+    int32_t c = 0, half = 0;
+    if (kStaged) stage_group(0, 0);
+    for (; c + kGroup <= cells; c += kGroup) {
+        read_group(c, half);
+        if (c > 0) flush(c - kGroup);
+        half ^= 1;
+        if (kStaged && c + kGroup < cells) stage_group(c + kGroup, half);
         if (kSq) to_distance(q[0]);
         static_for<0, kGroup>([&](auto slot) __attribute__((always_inline)) {
             constexpr int i = decltype(slot)::value;
@@ -685,12 +749,11 @@ __device__ __forceinline__ void raster_wide(const PlaceJob& job, uint32_t* usage
     }
     int32_t last = c - kGroup;  // start of the last group run (negative: none)
     if (c < cells) {
-        // the last, partial group: the same loads (one memory latency), with
-        // the steps past the grid skipped
+        // the last, partial group (its rows were staged by the previous
+        // group or before the loop), with the steps past the grid skipped
+        read_group(c, half);
         if (c > 0) flush(c - kGroup);
         const int32_t rem = cells - c;
-#pragma unroll
-        for (int i = 0; i < kGroup + 2; ++i) load_cands(q[i], job, c + i, cells, lane);
         if (kSq) to_distance(q[0]);
         static_for<0, kGroup>([&](auto slot) __attribute__((always_inline)) {
             constexpr int i = decltype(slot)::value;
@@ -703,13 +766,15 @@ __device__ __forceinline__ void raster_wide(const PlaceJob& job, uint32_t* usage
         last = c;
     }
     if (last >= 0) flush(last);
+    if (kStaged) __builtin_amdgcn_s_waitcnt(0x0f70);  // nothing staged is left in flight
 }
 
 // kScore: kScoreSq / kScoreDist fix the score for every job of the launch,
 // kScoreMixed reads each job's score_sqrt.
 template <bool kOutputs, int kScore>
 __global__ __launch_bounds__(64) void place_raster_kernel(PlaceJob single,
-                                                          const PlaceJob* __restrict__ jobs) {
+                                                          const PlaceJob* __restrict__ jobs,
+                                                          int32_t staged) {
     extern __shared__ __align__(16) uint32_t smr[];
     const PlaceJob job = uniform_copy(jobs ? jobs[blockIdx.x] : single);
     const bool sq = kScore == kScoreMixed ? job.score_sqrt != 0 : kScore == kScoreDist;
@@ -721,6 +786,10 @@ __global__ __launch_bounds__(64) void place_raster_kernel(PlaceJob single,
     // one guard word between the column row and the counts: a -1 sentinel
     // candidate reads it as its count
     uint32_t* usage_s = reinterpret_cast<uint32_t*>(row_mat + gw + 1);
+    // the staged candidate rows after the counts, 16-byte aligned (when the
+    // launch holds them)
+    uint8_t* stage =
+        staged ? reinterpret_cast<uint8_t*>(smr) + place_stage_offset(gw, n) : nullptr;
 
     if (n == 0) {  // src/main.rs:342-345: no tiles, every cell stays empty
         for (int32_t i = lane; i < cells; i += 64) gstore(job.grid, i, kEmpty);
@@ -745,11 +814,16 @@ __global__ __launch_bounds__(64) void place_raster_kernel(PlaceJob single,
     } else {
         // (kScoreMixed picks a compiled chain per job: a run-time test per
         // step costs the chain scalar registers it does not have)
-        if (kScore == kScoreMixed && job.score_sqrt)
-            raster_wide<kOutputs, true>(job, usage_s, row_mat, lane_scratch, st, lane);
+        constexpr bool kDist = kScore == kScoreDist;
+        const bool dist = kScore == kScoreMixed && job.score_sqrt;
+        if (stage && dist)
+            raster_wide<kOutputs, true, true>(job, usage_s, row_mat, lane_scratch, stage, st, lane);
+        else if (stage)
+            raster_wide<kOutputs, kDist, true>(job, usage_s, row_mat, lane_scratch, stage, st, lane);
+        else if (dist)
+            raster_wide<kOutputs, true, false>(job, usage_s, row_mat, lane_scratch, stage, st, lane);
         else
-            raster_wide<kOutputs, kScore == kScoreDist>(job, usage_s, row_mat, lane_scratch, st,
-                                                        lane);
+            raster_wide<kOutputs, kDist, false>(job, usage_s, row_mat, lane_scratch, stage, st, lane);
     }
     for (int32_t i = lane; i < n; i += 64) gstore(job.usage, i, usage_s[i]);
     if (lane == 0 && job.stats) {
@@ -766,26 +840,30 @@ size_t place_lds_bytes(int32_t cells, int32_t n) {
     return sizeof(int32_t) * ((size_t)cells + (size_t)n);
 }
 
-size_t place_raster_lds_bytes(int32_t grid_w, int32_t n) {
+size_t place_raster_lds_bytes(int32_t grid_w, int32_t n, bool staged) {
+    // lane scratch, the column row, the guard word, the counts, then two
+    // halves of staged candidate rows (24 + 2 cells each)
+    if (staged) return place_stage_offset(grid_w, n) + 2 * (size_t)(24 + 2) * kRowBytes;
     return sizeof(int32_t) * (64 + (size_t)grid_w + 1) +
            sizeof(uint32_t) * (size_t)(n > 0 ? n : 1);
 }
 
-void launch_place_single(const PlaceJob& job, bool raster, size_t lds_bytes, hipStream_t stream) {
+void launch_place_single(const PlaceJob& job, bool raster, bool staged, size_t lds_bytes,
+                         hipStream_t stream) {
     const bool outputs = job.stage || job.dist || job.score;
     const PlaceJob* none = nullptr;
     if (raster && outputs && job.score_sqrt)
         hipLaunchKernelGGL((place_raster_kernel<true, kScoreDist>), dim3(1), dim3(64), lds_bytes,
-                           stream, job, none);
+                           stream, job, none, (int32_t)staged);
     else if (raster && outputs)
         hipLaunchKernelGGL((place_raster_kernel<true, kScoreSq>), dim3(1), dim3(64), lds_bytes,
-                           stream, job, none);
+                           stream, job, none, (int32_t)staged);
     else if (raster && job.score_sqrt)
         hipLaunchKernelGGL((place_raster_kernel<false, kScoreDist>), dim3(1), dim3(64), lds_bytes,
-                           stream, job, none);
+                           stream, job, none, (int32_t)staged);
     else if (raster)
         hipLaunchKernelGGL((place_raster_kernel<false, kScoreSq>), dim3(1), dim3(64), lds_bytes,
-                           stream, job, none);
+                           stream, job, none, (int32_t)staged);
     else if (lds_bytes > 0)
         hipLaunchKernelGGL(place_chain_kernel<false>, dim3(1), dim3(64), lds_bytes, stream, job,
                            (const PlaceJob*)nullptr);
@@ -794,19 +872,19 @@ void launch_place_single(const PlaceJob& job, bool raster, size_t lds_bytes, hip
                            (const PlaceJob*)nullptr);
 }
 
-void launch_place_batch(const PlaceJob* d_jobs, int32_t n_jobs, bool raster, int32_t score_kind,
-                        size_t lds_bytes, hipStream_t stream) {
+void launch_place_batch(const PlaceJob* d_jobs, int32_t n_jobs, bool raster, bool staged,
+                        int32_t score_kind, size_t lds_bytes, hipStream_t stream) {
     if (n_jobs <= 0) return;
     PlaceJob unused{};
     if (raster && score_kind == kScoreMixed)
         hipLaunchKernelGGL((place_raster_kernel<false, kScoreMixed>), dim3(n_jobs), dim3(64),
-                           lds_bytes, stream, unused, d_jobs);
+                           lds_bytes, stream, unused, d_jobs, (int32_t)staged);
     else if (raster && score_kind == kScoreDist)
         hipLaunchKernelGGL((place_raster_kernel<false, kScoreDist>), dim3(n_jobs), dim3(64),
-                           lds_bytes, stream, unused, d_jobs);
+                           lds_bytes, stream, unused, d_jobs, (int32_t)staged);
     else if (raster)
         hipLaunchKernelGGL((place_raster_kernel<false, kScoreSq>), dim3(n_jobs), dim3(64),
-                           lds_bytes, stream, unused, d_jobs);
+                           lds_bytes, stream, unused, d_jobs, (int32_t)staged);
     else if (lds_bytes > 0)
         hipLaunchKernelGGL(place_chain_kernel<false>, dim3(n_jobs), dim3(64), lds_bytes, stream,
                            unused, d_jobs);

@@ -217,7 +217,10 @@ int run_place(mh_context* ctx, const mh_library* lib, const float* d_cell_lab, i
               uint8_t* d_stage, float* d_dist, float* d_score, int32_t* d_stats,
               CandRec* cand_buf = nullptr) {
     const int32_t cells = gw * gh;
-    size_t lds = raster ? place_raster_lds_bytes(gw, lib->n) : place_lds_bytes(cells, lib->n);
+    // the raster chain stages its candidate rows in LDS when they fit
+    const bool staged = raster && place_raster_lds_bytes(gw, lib->n, true) <= ctx->max_lds;
+    size_t lds = raster ? place_raster_lds_bytes(gw, lib->n, staged)
+                        : place_lds_bytes(cells, lib->n);
     // state beyond LDS: the general chain with grid and counts in HBM (it
     // reads the neighbours from the grid, so a raster run starts it empty)
     const bool hbm = lds > ctx->max_lds;
@@ -252,7 +255,7 @@ int run_place(mh_context* ctx, const mh_library* lib, const float* d_cell_lab, i
     job.score = d_score;
     job.stats = d_stats;
     phase_begin(ctx, MH_PHASE_PLACE);
-    launch_place_single(job, raster && !hbm, lds, ctx->stream);
+    launch_place_single(job, raster && !hbm, staged, lds, ctx->stream);
     phase_end(ctx, MH_PHASE_PLACE);
     HIPCHK(hipGetLastError());
     return MH_OK;

@@ -48,6 +48,7 @@ struct JobSpec {
     bool sa = false, adjust = false;
     int32_t anneal_kind = 0;  // the annealing variant (mh::anneal_kind)
     bool place_hbm = false;   // placement state in HBM (library beyond LDS)
+    bool place_staged = false;  // the LDS chain's staged candidate rows fit too
     float strength = 0.0f;
     const float* term = nullptr;
     const float* temps = nullptr;
@@ -90,7 +91,8 @@ int check_job(const mh_context* ctx, const mh_job& j, JobSpec* s) {
     s->strength = k < 0.0f ? 0.0f : (k > 1.0f ? 1.0f : k);  // src/main.rs:162
     s->adjust = s->strength > 0.0f;
     // placement state beyond LDS runs the chain with grid and counts in HBM
-    s->place_hbm = place_raster_lds_bytes(s->gw, s->n) > ctx->max_lds;
+    s->place_hbm = place_raster_lds_bytes(s->gw, s->n, false) > ctx->max_lds;
+    s->place_staged = place_raster_lds_bytes(s->gw, s->n, true) <= ctx->max_lds;
     s->anneal_kind = s->sa ? anneal_kind(s->cells, s->n, s->gw, s->gh, ctx->max_lds) : 0;
     if (s->anneal_kind < 0)
         return fail(MH_ERR_UNSUPPORTED, "annealing a %d x %d grid: a side of 65,536 or more", s->gw,
@@ -137,6 +139,7 @@ struct BatchPlan {
     int32_t lab_blocks = 0, lab_cpb = 4;
     int32_t n_topk = 0, topk_blocks = 0, topk_run = 1;
     size_t place_lds = 0;
+    bool place_staged = false;  // the LDS chains stage their candidate rows
     int32_t place_score = kScoreSq;  // kScoreSq / kScoreDist / kScoreMixed over the jobs
     // placement entries: raster chains with state in LDS first, then the
     // chains with state in HBM, whose grids start empty (filled per launch)
@@ -194,9 +197,14 @@ T* at(uint8_t* base, size_t off) {
 }
 
 // Fill the host image `h` of a batch block whose device copy lives at `d`;
-// job i's working memory is at job_mem[i].
+// job i's working memory is at job_mem[i].  stage_rows: the raster chains
+// stage their candidate rows in LDS when every job's fit (52 KB more per
+// chain).  A batch's one or two chain launches gain from it (the chain time
+// is the batch's critical path); the server keeps several batches' chains
+// and wide stages on the GPU at once, where the larger LDS footprint costs
+// more in chains and blocks resident per CU than the staging saves.
 int build_plan(const std::vector<JobSpec>& specs, const std::vector<uint8_t*>& job_mem, uint8_t* h,
-               uint8_t* d, BatchPlan* p) {
+               uint8_t* d, bool stage_rows, BatchPlan* p) {
     const int32_t n = (int32_t)specs.size();
     p->n_jobs = n;
     plan_bytes(n, p);
@@ -223,6 +231,8 @@ int build_plan(const std::vector<JobSpec>& specs, const std::vector<uint8_t*>& j
     p->lab_cpb = cell_lab_cells_per_block(all_cells);
     p->topk_run = p->tune_topk_run > 0 ? p->tune_topk_run : topk_run_length(topk_queries);
     p->place_lds = 0;
+    p->place_staged = stage_rows;  // until a job's staged rows do not fit
+    size_t lds_plain = 0, lds_staged = 0;
     p->place_score = kScoreSq;
     p->hbm_grids.clear();
     p->n_place_lds = 0;
@@ -278,8 +288,9 @@ int build_plan(const std::vector<JobSpec>& specs, const std::vector<uint8_t*>& j
             p->hbm_grids.emplace_back(grid, s.cells);
         } else {
             place[p->n_place_lds++] = pj;
-            const size_t pl = place_raster_lds_bytes(s.gw, s.n);
-            if (pl > p->place_lds) p->place_lds = pl;
+            lds_plain = std::max(lds_plain, place_raster_lds_bytes(s.gw, s.n, false));
+            lds_staged = std::max(lds_staged, place_raster_lds_bytes(s.gw, s.n, true));
+            p->place_staged = p->place_staged && s.place_staged;
         }
         const int32_t kind = pj.score_sqrt ? kScoreDist : kScoreSq;
         p->place_score = i == 0 || p->place_score == kind ? kind : kScoreMixed;
@@ -314,6 +325,7 @@ int build_plan(const std::vector<JobSpec>& specs, const std::vector<uint8_t*>& j
         comp_blocks += composite_blocks(s.gw * s.tw, s.gh * s.th, p->comp_threads);
     }
     for (size_t h = 0; h < hbm_place.size(); ++h) place[p->n_place_lds + h] = hbm_place[h];
+    p->place_lds = p->place_staged ? lds_staged : lds_plain;
     p->n_anneal = 0;
     for (int kind = 0; kind < 3; ++kind) {
         p->anneal_start[kind] = p->n_anneal;
@@ -365,11 +377,12 @@ void enqueue_chains(const BatchPlan& p, uint8_t* d, hipStream_t s,
     pe.mark(MH_PHASE_PLACE, 0, s);
     const PlaceJob* pj = at<PlaceJob>(d, p.o_place);
     if (p.n_place_lds > 0)
-        launch_place_batch(pj, p.n_place_lds, true, p.place_score, p.place_lds, s);
+        launch_place_batch(pj, p.n_place_lds, true, p.place_staged, p.place_score, p.place_lds,
+                           s);
     if (p.n_jobs > p.n_place_lds) {
         for (const auto& g : p.hbm_grids) fill_i32(g.first, g.second, kEmpty, s);
-        launch_place_batch(pj + p.n_place_lds, p.n_jobs - p.n_place_lds, false, p.place_score, 0,
-                           s);
+        launch_place_batch(pj + p.n_place_lds, p.n_jobs - p.n_place_lds, false, false,
+                           p.place_score, 0, s);
     }
     pe.mark(MH_PHASE_PLACE, 1, s);
     pe.mark(MH_PHASE_ANNEAL, 0, s);
@@ -455,7 +468,7 @@ int build_group(mh_batch* b, int g) {
         mem.push_back(b->job_mem[i]);
     }
     return build_plan(sp, mem, b->h_plan.data() + b->plan_off[g], b->d_plan + b->plan_off[g],
-                      &b->plan[g]);
+                      true, &b->plan[g]);
 }
 
 void enqueue_batch(mh_batch* b) {
@@ -1092,7 +1105,7 @@ int launch_batch(mh_server* sv, StreamSlot& sl, const std::vector<JobRec*>& recs
         specs.push_back(r->spec);
         mem.push_back(r->mem);
     }
-    CHK(build_plan(specs, mem, sl.h_plan, sl.d_plan, &sl.plan));
+    CHK(build_plan(specs, mem, sl.h_plan, sl.d_plan, false, &sl.plan));
     // the tables go up on their own stream as soon as the batch is admitted;
     // the wide stream, which usually still has earlier batches to run, waits
     // for them only if the copy has not landed by then

@@ -165,16 +165,19 @@ void launch_composite_batch(const CompSeg* d_segs, const int32_t* d_starts, int3
 // General chain: any cell order, pre-filled grid and usage (state in LDS).
 size_t place_lds_bytes(int32_t cells, int32_t n);
 // Raster chain: generate_mosaic's raster order from an empty grid, where
-// only the up and left neighbours can be filled.
-size_t place_raster_lds_bytes(int32_t grid_w, int32_t n);
+// only the up and left neighbours can be filled.  `staged`: the LDS also
+// holds two groups of candidate rows (52 KB more), copied in ahead of the
+// chain; the chain runs unstaged where that does not fit.
+size_t place_raster_lds_bytes(int32_t grid_w, int32_t n, bool staged);
 // One job passed by value (kernel argument), or n_jobs jobs from a device
-// array (one wave each).
-void launch_place_single(const PlaceJob& job, bool raster, size_t lds_bytes, hipStream_t stream);
+// array (one wave each); `staged` as for place_raster_lds_bytes.
+void launch_place_single(const PlaceJob& job, bool raster, bool staged, size_t lds_bytes,
+                         hipStream_t stream);
 // score_kind: kScoreSq or kScoreDist when every job scores alike, else
 // kScoreMixed (each job's score_sqrt is read).
 enum : int32_t { kScoreSq = 0, kScoreDist = 1, kScoreMixed = 2 };
-void launch_place_batch(const PlaceJob* d_jobs, int32_t n_jobs, bool raster, int32_t score_kind,
-                        size_t lds_bytes, hipStream_t stream);
+void launch_place_batch(const PlaceJob* d_jobs, int32_t n_jobs, bool raster, bool staged,
+                        int32_t score_kind, size_t lds_bytes, hipStream_t stream);
 // Annealing variants: the grid in LDS with 16-bit material ids (libraries
 // under 32,768 materials) or 32-bit ids, or in HBM with hashed LDS stamps
 // (grids too large for LDS; sides below 65,536).

@@ -90,8 +90,39 @@ def test_place_library_beyond_lds(ctx, synth, orc):
     lib.close()
 
 
-def test_batch_mixes_hbm_and_lds_jobs(ctx, mh, synth, orc):
-    jobs = [_big_grid_job(synth), _big_library_job(synth), synth.make("small")]
+def _mid_library_job(synth):
+    # 32,000 materials on a 40-wide grid: the usage counts fit the raster
+    # chain's 160 KB of LDS (4 B per material plus the lane scratch and the
+    # column row), the 52 KB of staged candidate rows beside them do not
+    # (they fit up to about 27,500 materials), so the chain reads its
+    # candidate rows from global memory
+    rng = np.random.default_rng(47)
+    mats = synth.flat_materials(32000, 4, rng, colours=synth.mixture_colours(32000, rng))
+    target = synth.blob_target(640, 480, 8, rng)
+    return dict(grid_w=40, grid_h=30, max_usage=2, weight=0.25, color=0.4, iters=800), target, mats
+
+
+def test_place_library_unstaged(ctx, synth, orc):
+    p, target, mats = _mid_library_job(synth)
+    print(f"library of {mats.shape[0]} materials", flush=True)
+    lib = ctx.library_from_pixels(mats)
+    got = ctx.generate(lib, target, p["grid_w"], p["grid_h"], p["max_usage"], p["weight"],
+                       p["color"], True, p["iters"], seed=4)
+    prep = orc.prepare_library(mats, [(16, 16)], threads=THREADS)
+    want = orc.generate_prepared(target, prep, p["grid_w"], p["grid_h"], p["max_usage"],
+                                 p["weight"], p["color"], True, p["iters"], seed=4)
+    _check(got, want, "32k library")
+    cl = orc.cell_lab_means(target, p["grid_w"], p["grid_h"])
+    gp = ctx.place(lib, cl, p["grid_w"], p["grid_h"], p["max_usage"], p["weight"])
+    wp = orc.place(prep["lab"], cl, p["grid_w"], p["grid_h"], p["max_usage"], p["weight"])
+    np.testing.assert_array_equal(gp["grid"], wp["grid"])
+    np.testing.assert_array_equal(gp["usage"], wp["usage"])
+    np.testing.assert_array_equal(_bits(gp["dist"]), _bits(wp["dist"]))
+    assert gp["stats"] == wp["stats"]
+    lib.close()
+
+
+def _run_batch(ctx, mh, orc, jobs):
     libs, bufs, mj, want = [], [], [], []
     for k, (p, target, mats) in enumerate(jobs):
         gw, gh = p["grid_w"], p["grid_h"]
@@ -125,3 +156,14 @@ def test_batch_mixes_hbm_and_lds_jobs(ctx, mh, synth, orc):
         ctx.device_free(ptr)
     for lib in libs:
         lib.close()
+
+
+def test_batch_mixes_hbm_and_lds_jobs(ctx, mh, synth, orc):
+    _run_batch(ctx, mh, orc, [_big_grid_job(synth), _big_library_job(synth), synth.make("small")])
+
+
+def test_batch_unstaged_chains(ctx, mh, synth, orc):
+    # one job too large for staged rows: every LDS chain of the launch runs
+    # unstaged, the small jobs' too
+    _run_batch(ctx, mh, orc, [synth.make("small"), _mid_library_job(synth),
+                              synth.make("small", seed=1)])
