@@ -154,7 +154,8 @@ int32_t topk_blocks(int32_t nq, int32_t run);
 void launch_topk_batch(const TopkSeg* d_segs, const int32_t* d_starts, int32_t n_segs,
                        int32_t total_blocks, int32_t run, hipStream_t stream);
 int32_t cell_adjust_blocks(int32_t cells);
-// blocks of `threads` threads (64, 128 or 256), 4 pixels of one output row each
+// the batched composite's blocks of `threads` threads (64, 128 or 256), 8
+// pixels of one output row each
 int32_t composite_blocks(int32_t ow, int32_t oh, int32_t threads);
 // 1 when n tiles of tw x th fit the composite's 32-bit tile offsets
 int32_t composite_narrow(int32_t n_tiles, int32_t tw, int32_t th);
