@@ -64,8 +64,11 @@ def test_every_colour_converts_like_oracle(ctx, orc):
     assert bad.size == 0, f"{bad.size} colours differ, first {bad[:8]}"
 
 
+# the last two have 262,144 and 360,000 cells (the launch shape of the
+# served batches: 16 cells per block)
 @pytest.mark.parametrize("w,h,gw,gh", [(640, 480, 40, 30), (101, 77, 7, 5), (64, 64, 1, 1),
-                                       (1000, 37, 250, 37)])
+                                       (1000, 37, 250, 37), (2048, 2048, 512, 512),
+                                       (6000, 1500, 1200, 300)])
 def test_cell_lab_means(ctx, orc, w, h, gw, gh):
     img = _rng(w * h).integers(0, 256, (h, w, 3)).astype(np.uint8)
     assert_bits_equal(ctx.cell_lab_means(img, gw, gh), orc.cell_lab_means(img, gw, gh))
