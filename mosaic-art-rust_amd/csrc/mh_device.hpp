@@ -310,7 +310,8 @@ __device__ __forceinline__ float div60_fast(float x) {
 }
 
 // palette Srgb::from_color(Hsv) (src/color_adjustment.rs:76).
-// kUnit: v in [0, 1] and not -0, s in [0, 1] (see adjust_pixel).
+// kUnit: v in [0, 1] and not -0, s in [0, 1], hdeg in [0, 360] and not -0
+// (see adjust_pixel).
 template <bool kUnit = false>
 __device__ __forceinline__ void hsv_to_rgb(float hdeg, float s, float v, float& r, float& g,
                                            float& b) {
@@ -319,8 +320,13 @@ __device__ __forceinline__ void hsv_to_rgb(float hdeg, float s, float v, float& 
     // below 1, so the floor is +0 (-0 for hdeg = -0) and the result is
     // hdeg + 0.0f (which turns -0 into +0, as the subtraction does)
     float hp = hdeg + 0.0f;
-    if (__builtin_expect(!(hdeg >= 0.0f && hdeg < 360.0f), 0))
+    if (kUnit) {
+        // hdeg in [0, 360] and not -0: only 360 leaves the range, and
+        // 360 - floor(360 / 360) * 360 is +0
+        hp = hdeg < 360.0f ? hdeg : 0.0f;
+    } else if (__builtin_expect(!(hdeg >= 0.0f && hdeg < 360.0f), 0)) {
         hp = hdeg - floorf(hdeg / 360.0f) * 360.0f;
+    }
     float h = div60_fast(hp);
     if (__builtin_expect(!(hp == 0.0f || (hp >= 0x1p-124f && hp < 360.0f)), 0)) h = hp / 60.0f;
     float hm2 = h - 2.0f * floorf(h * 0.5f);
@@ -393,9 +399,11 @@ __device__ __forceinline__ float u8_unit(uint32_t x) {
 
 // fmodf(a, 360.0f).  For 360 <= |a| < 720 the remainder is a -/+ 360,
 // exact by Sterbenz's lemma; a zero remainder keeps the sign of a.
+// kUnit: |a| < 720 is known (see adjust_pixel), so no other case is tested.
+template <bool kUnit = false>
 __device__ __forceinline__ float fmod_360(float a) {
     const float m = __builtin_fabsf(a);
-    if (!(m < 720.0f)) return fmodf(a, 360.0f);  // NaN and infinities too
+    if (!kUnit && !(m < 720.0f)) return fmodf(a, 360.0f);  // NaN and infinities too
     const float r = m >= 360.0f ? m - 360.0f : m;
     return __builtin_copysignf(r, a);
 }
@@ -414,6 +422,15 @@ __device__ __forceinline__ float fmod_360(float a) {
 //     identities, s = d / mx and v = mx are >= +0;
 //   * hsv_to_rgb's rr + m has m = v - c >= +0 (c = v s <= v), and a zero
 //     sum with a +0 operand is +0;
+//   * the hue: rgb_to_hsv's h = (sep / d + coeff) * 60 has |sep| <= d (sep
+//     is the difference of the two channels other than the maximum, d the
+//     maximum minus the minimum, and rounding keeps the order), so h lies
+//     in [-60, 300]; it is never -0 (sep is a difference of distinct values
+//     or +0, and h starts at +0).  With hue_shift in [-180, 180],
+//     h + hue_shift lies in [-240, 480] and is not -0 (a -0 sum needs two
+//     -0 operands), so fmod_360 needs no range test, and rem_euclid gives
+//     a value in [0, 360] that is not -0 (360 when a tiny negative
+//     remainder rounds up);
 // and every channel that reaches to_u8 lies in [0, 1].  The saturation
 // product s * adj.w may be -0 (adj.w = -0) and keeps the general clamp.
 //
@@ -425,7 +442,7 @@ __device__ __forceinline__ void adjust_tail(float4 adj, float br, float bg, floa
     if (adj.z != 0.0f || adj.w != 1.0f) {
         float h, s, v;
         rgb_to_hsv<kUnit>(br, bg, bb, h, s, v);
-        float hr = fmod_360(h + adj.z);  // rem_euclid(360.0)
+        float hr = fmod_360<kUnit>(h + adj.z);  // rem_euclid(360.0)
         if (hr < 0.0f) hr = hr + 360.0f;
         float s2 = clampf(s * adj.w, 0.0f, 1.0f);
         hsv_to_rgb<kUnit>(hr, s2, v, br, bg, bb);
