@@ -96,49 +96,69 @@ __device__ __forceinline__ void swap_slots(const G* grid, int32_t gw, int32_t gh
     nb[7] = gget(grid, x2 < gw - 1 ? p2 + 1 : p2);
 }
 
+// swap_delta_core in two parts, so that a caller can do other work while
+// the gathers are in flight: swap_terms requests every term (m1 and m2 must
+// be material ids), swap_sum adds them in the reference's order.
+struct SwapTerms {
+    float r1[8], r2[8];  // T(m1, a_i), T(m2, a_i)
+    float tm;            // T(m1, m2)
+    bool ok[8];          // slot i holds a neighbour that is not the partner
+    bool is_adj;
+};
+
 template <typename TS>
-__device__ __forceinline__ float swap_delta_core(int32_t gw, int32_t gh, const TS& T, int32_t x1,
-                                                 int32_t y1, int32_t x2, int32_t y2, int32_t m1,
-                                                 int32_t m2, const int32_t nb[8]) {
+__device__ __forceinline__ void swap_terms(int32_t gw, int32_t gh, const TS& T, int32_t x1,
+                                           int32_t y1, int32_t x2, int32_t y2, int32_t m1,
+                                           int32_t m2, const int32_t nb[8], SwapTerms& st) {
     const int32_t p1 = y1 * gw + x1, p2 = y2 * gw + x2;
-    if (m1 == kEmpty || m2 == kEmpty) return 0.0f;
-    if (m1 == m2) return 0.0f;
     const bool in[8] = {y1 > 0, y1 < gh - 1, x1 > 0, x1 < gw - 1,
                         y2 > 0, y2 < gh - 1, x2 > 0, x2 < gw - 1};
     const int32_t pos[8] = {p1 - gw, p1 + gw, p1 - 1, p1 + 1, p2 - gw, p2 + gw, p2 - 1, p2 + 1};
     int32_t a[8];
-    bool ok[8];
-    bool is_adj = false;
+    st.is_adj = false;
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
         const int32_t other = i < 4 ? p2 : p1;
         const bool self = in[i] && pos[i] == other;  // the swap partner
-        if (i < 4) is_adj = is_adj || self;
+        if (i < 4) st.is_adj = st.is_adj || self;
         const int32_t g = in[i] ? nb[i] : kEmpty;
-        ok[i] = in[i] && !self && g != kEmpty;
-        a[i] = ok[i] ? g : 0;
+        st.ok[i] = in[i] && !self && g != kEmpty;
+        a[i] = st.ok[i] ? g : 0;
     }
-    float r1[8], r2[8];  // T(m1, a_i), T(m2, a_i)
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-        r1[i] = T(m1, a[i]);
-        r2[i] = T(m2, a[i]);
+        st.r1[i] = T(m1, a[i]);
+        st.r2[i] = T(m2, a[i]);
     }
-    const float tm = T(m1, m2);
+    st.tm = T(m1, m2);
+}
+
+__device__ __forceinline__ float swap_sum(const SwapTerms& st) {
     float old_cost = 0.0f, new_cost = 0.0f;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {  // p1's neighbours: old with m1, new with m2
-        old_cost += ok[i] ? r1[i] : 0.0f;
-        new_cost += ok[i] ? r2[i] : 0.0f;
+        old_cost += st.ok[i] ? st.r1[i] : 0.0f;
+        new_cost += st.ok[i] ? st.r2[i] : 0.0f;
     }
 #pragma unroll
     for (int i = 4; i < 8; ++i) {  // p2's neighbours: old with m2, new with m1
-        old_cost += ok[i] ? r2[i] : 0.0f;
-        new_cost += ok[i] ? r1[i] : 0.0f;
+        old_cost += st.ok[i] ? st.r2[i] : 0.0f;
+        new_cost += st.ok[i] ? st.r1[i] : 0.0f;
     }
-    old_cost += is_adj ? tm : 0.0f;
-    new_cost += is_adj ? tm : 0.0f;
+    old_cost += st.is_adj ? st.tm : 0.0f;
+    new_cost += st.is_adj ? st.tm : 0.0f;
     return new_cost - old_cost;
+}
+
+template <typename TS>
+__device__ __forceinline__ float swap_delta_core(int32_t gw, int32_t gh, const TS& T, int32_t x1,
+                                                 int32_t y1, int32_t x2, int32_t y2, int32_t m1,
+                                                 int32_t m2, const int32_t nb[8]) {
+    if (m1 == kEmpty || m2 == kEmpty) return 0.0f;
+    if (m1 == m2) return 0.0f;
+    SwapTerms st;
+    swap_terms(gw, gh, T, x1, y1, x2, y2, m1, m2, nb, st);
+    return swap_sum(st);
 }
 
 template <typename G, typename TS>
@@ -277,6 +297,38 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
     // block never overwrites a slot of the round's window [s, s + 64):
     // filled <= s + 192 < s + 256.
     int64_t filled = 0;
+    // The serial cost sum of a round's accepted deltas (packed in proposal
+    // order into pack_s, see below) runs in the next round, after that
+    // round's term gathers are requested, so it overlaps their latency; the
+    // last round's runs after the loop.  pend: the count waiting (-1: none).
+    int32_t pend = -1;
+    auto serial_sum = [&](int32_t n_acc) {
+        // Eight deltas at a time come back as two uniform 16-byte LDS reads,
+        // so the chain adds VGPRs (no lane reads into SGPRs between the adds).
+        const float4* pack4 = reinterpret_cast<const float4*>(pack_s);
+        if (job.greedy) {  // optimize_greedy counts every accepted swap
+            for (int32_t q = 0; q < n_acc; q += 8) {
+                const float4 u0 = pack4[q >> 2], u1 = pack4[(q >> 2) + 1];
+                const float v[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
+#pragma unroll
+                for (int i = 0; i < 8; ++i) cost = cost + v[i];
+            }
+            improved += n_acc;
+        } else {
+            uint32_t better_n = 0;
+            for (int32_t q = 0; q < n_acc; q += 8) {
+                const float4 u0 = pack4[q >> 2], u1 = pack4[(q >> 2) + 1];
+                const float v[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    cost = cost + v[i];
+                    better_n += cost < best ? 1u : 0u;
+                    best = __builtin_fminf(best, cost);
+                }
+            }
+            improved += better_n;
+        }
+    };
     while (s < max_iter) {
         if (filled < s + 64) {
             do {
@@ -315,18 +367,28 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
         swap_slots(grid_s, gw, gh, x1, y1, x2, y2, nb);
         const bool skip = !active || (x1 == x2 && y1 == y2) || m1 == kEmpty || m2 == kEmpty;
         const uint64_t nonskip = wave_ballot(!skip);
+        // every lane requests its swap's terms (a skipped lane those of
+        // material 0, which exists when any lane is not skipped), then the
+        // temperature and the uniform draw, then the previous round's serial
+        // sum runs while they are in flight
+        SwapTerms st;
+        if (nonskip) {
+            const int32_t g1 = skip ? 0 : m1, g2 = skip ? 0 : m2;
+            if (job.term)
+                swap_terms(gw, gh, TM, x1, y1, x2, y2, g1, g2, nb, st);
+            else
+                swap_terms(gw, gh, TL, x1, y1, x2, y2, g1, g2, nb, st);
+        }
+        float temp = 1.0f, u = 0.0f;
+        if (!skip && !job.greedy) {
+            temp = gload(job.temps, kbase + lanes_before(nonskip));
+            u = ring_u[slot];
+        }
+        if (pend >= 0) serial_sum(pend);
         float delta = 0.0f;
         bool acc = false;
         if (!skip) {
-            // the temperature and the uniform draw are requested before the
-            // delta, so their latency overlaps the term gathers
-            float temp = 1.0f, u = 0.0f;
-            if (!job.greedy) {
-                temp = gload(job.temps, kbase + lanes_before(nonskip));
-                u = ring_u[slot];
-            }
-            delta = job.term ? swap_delta_core(gw, gh, TM, x1, y1, x2, y2, m1, m2, nb)
-                             : swap_delta_core(gw, gh, TL, x1, y1, x2, y2, m1, m2, nb);
+            delta = m1 == m2 ? 0.0f : swap_sum(st);  // swap_delta_core's value
             if (job.greedy) {
                 acc = delta < 0.0f;
             } else if (delta < 0.0f) {
@@ -398,40 +460,16 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
         // `current_cost += delta`).  The committed accepted deltas are packed
         // into words 0..n-1 of pack_s in proposal order (the other words hold
         // +0.0; pack_s is 16-byte aligned), so the serial sum reads them by a
-        // counter instead of walking the mask.  Every lane runs the same uniform loop, so cost / best /
-        // improved stay identical across the wave.  Adding a +0.0 past the last
-        // delta changes nothing: the cost is never -0.0 and never below best.
-        // best = fminf(best, cost) equals `cost < best ? cost : best` here (no
-        // NaN, no -0.0) and keeps the compare off the dependent chain.
+        // counter instead of walking the mask.  Every lane runs the same
+        // uniform loop, so cost / best / improved stay identical across the
+        // wave.  Adding a +0.0 past the last delta changes nothing: the cost is
+        // never -0.0 and never below best.  best = fminf(best, cost) equals
+        // `cost < best ? cost : best` here (no NaN, no -0.0) and keeps the
+        // compare off the dependent chain.
         pack_s[lane] = 0.0f;
         if (lane_in(accmask)) pack_s[lanes_before(accmask)] = delta;
-        __syncthreads();  // other lanes' writes, not this lane's own stores, are read below
-        // Eight deltas at a time come back as two uniform 16-byte LDS reads, so
-        // the chain adds VGPRs (no lane reads into SGPRs between the adds).
-        const float4* pack4 = reinterpret_cast<const float4*>(pack_s);
-        const int32_t n_acc = __popcll(accmask);
-        if (job.greedy) {  // optimize_greedy counts every accepted swap
-            for (int32_t q = 0; q < n_acc; q += 8) {
-                const float4 u0 = pack4[q >> 2], u1 = pack4[(q >> 2) + 1];
-                const float v[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
-#pragma unroll
-                for (int i = 0; i < 8; ++i) cost = cost + v[i];
-            }
-            improved += n_acc;
-        } else {
-            uint32_t better_n = 0;
-            for (int32_t q = 0; q < n_acc; q += 8) {
-                const float4 u0 = pack4[q >> 2], u1 = pack4[(q >> 2) + 1];
-                const float v[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
-#pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    cost = cost + v[i];
-                    better_n += cost < best ? 1u : 0u;
-                    best = __builtin_fminf(best, cost);
-                }
-            }
-            improved += better_n;
-        }
+        __syncthreads();  // other lanes' writes, not this lane's own stores, are read next round
+        pend = __popcll(accmask);
         accepted += __popcll(accmask);
         const int64_t ev = __popcll(nonskip & commit);
         evaluated += ev;
@@ -446,6 +484,8 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
         }
         __syncthreads();
     }
+
+    if (pend >= 0) serial_sum(pend);
 
     if constexpr (!kHbm)
         for (int32_t i = lane; i < cells; i += 64) gstore(job.grid, i, (int32_t)grid_s[i]);
