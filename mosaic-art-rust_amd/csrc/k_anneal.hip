@@ -292,11 +292,37 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
     const int64_t max_iter = job.max_iter;
 
     // Proposal `it` is drawn once, in a block of 128 made by the whole wave
-    // before the round that first needs it; a round re-evaluating proposals
-    // after a conflict reads them again instead of drawing them again.  A
-    // block never overwrites a slot of the round's window [s, s + 64):
-    // filled <= s + 192 < s + 256.
+    // (refill) a round or two before the round that first needs it; a round
+    // re-evaluating proposals after a conflict reads them again instead of
+    // drawing them again.  A round refills, while its term gathers are in
+    // flight, when fewer than 128 proposals past its window start are drawn;
+    // a round advances the window by at most 64, so the next window is
+    // always drawn.  A block never overwrites a slot of the round's window
+    // [s, s + 64): it fills [filled, filled + 128) with s + 64 <= filled <
+    // s + 128, inside [s + 64, s + 256).
     int64_t filled = 0;
+    auto refill = [&]() {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int64_t j = filled + 64 * h + lane;
+            const uint32_t slot = (uint32_t)j & (kRing - 1);
+            uint32_t a = 0, b = 0;
+            float uj = 0.0f;
+            if (j < max_iter) {
+                a = (uint32_t)draw_range(job.seed, (uint64_t)j, 0, gw) |
+                    ((uint32_t)draw_range(job.seed, (uint64_t)j, 1, gh) << 16);
+                b = (uint32_t)draw_range(job.seed, (uint64_t)j, 2, gw) |
+                    ((uint32_t)draw_range(job.seed, (uint64_t)j, 3, gh) << 16);
+                if (!job.greedy) uj = draw_unit(job.seed, (uint64_t)j);
+            }
+            ring_xy1[slot] = a;
+            ring_xy2[slot] = b;
+            ring_u[slot] = uj;
+        }
+        filled += 128;
+    };
+    refill();
+    __syncthreads();  // other lanes' draws are read below
     // The serial cost sum of a round's accepted deltas (packed in proposal
     // order into pack_s, see below) runs in the next round, after that
     // round's term gathers are requested, so it overlaps their latency; the
@@ -330,29 +356,6 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
         }
     };
     while (s < max_iter) {
-        if (filled < s + 64) {
-            do {
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const int64_t j = filled + 64 * h + lane;
-                    const uint32_t slot = (uint32_t)j & (kRing - 1);
-                    uint32_t a = 0, b = 0;
-                    float uj = 0.0f;
-                    if (j < max_iter) {
-                        a = (uint32_t)draw_range(job.seed, (uint64_t)j, 0, gw) |
-                            ((uint32_t)draw_range(job.seed, (uint64_t)j, 1, gh) << 16);
-                        b = (uint32_t)draw_range(job.seed, (uint64_t)j, 2, gw) |
-                            ((uint32_t)draw_range(job.seed, (uint64_t)j, 3, gh) << 16);
-                        if (!job.greedy) uj = draw_unit(job.seed, (uint64_t)j);
-                    }
-                    ring_xy1[slot] = a;
-                    ring_xy2[slot] = b;
-                    ring_u[slot] = uj;
-                }
-                filled += 128;
-            } while (filled < s + 64);
-            __syncthreads();  // other lanes' draws are read below
-        }
         const int64_t it = s + lane;
         const bool active = it < max_iter;
         const uint32_t slot = (uint32_t)it & (kRing - 1);
@@ -369,8 +372,8 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
         const uint64_t nonskip = wave_ballot(!skip);
         // every lane requests its swap's terms (a skipped lane those of
         // material 0, which exists when any lane is not skipped), then the
-        // temperature and the uniform draw, then the previous round's serial
-        // sum runs while they are in flight
+        // temperature and the uniform draw; the draws of later proposals and
+        // the previous round's serial sum run while they are in flight
         SwapTerms st;
         if (nonskip) {
             const int32_t g1 = skip ? 0 : m1, g2 = skip ? 0 : m2;
@@ -384,6 +387,7 @@ __global__ __launch_bounds__(64) void anneal_kernel(AnnealJob single,
             temp = gload(job.temps, kbase + lanes_before(nonskip));
             u = ring_u[slot];
         }
+        if (filled < s + 128) refill();  // read from the next round on, after its barriers
         if (pend >= 0) serial_sum(pend);
         float delta = 0.0f;
         bool acc = false;
