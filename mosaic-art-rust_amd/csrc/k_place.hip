@@ -26,8 +26,10 @@
 //    empty grid, so only Up and Left can be filled.  The left neighbour is
 //    the previous winner (registers), the up neighbour comes from a per-column
 //    LDS row (material + Lab of the last placement in that column), usage
-//    counts live in LDS, candidate records are loaded 24 cells at a time into
-//    registers, the min is a DPP reduction, and the winner's id / Lab / count
+//    counts live in LDS, candidate records are read 24 cells at a time into
+//    registers (from an LDS double buffer that the next group's rows are
+//    copied into meanwhile, where LDS holds it; else straight from global
+//    memory), the min is a DPP reduction, and the winner's id / Lab / count
 //    are read from its lane.  Winners wait one per lane in a register that
 //    is stored to the grid, coalesced, once per group of 24 cells.
 //    Latency-bound: a few hundred cycles per cell.
