@@ -261,7 +261,9 @@ __device__ const Contrast1Table kContrast1 = make_contrast1_table();
 // batched launch, with many jobs' rows, takes 8, which halves the per-thread
 // share of the setup (segment search, row and column divisions, the table
 // copy): the served stream 57.3k -> 59.0k jobs/s, against 16.9 us instead
-// of 14.9 us for a medium job's own launch.
+// of 14.9 us for a medium job's own launch.  12 and 16 pixels per thread
+// lose more to idle lanes at the ends of short rows than they save (57.0k
+// and 54.5-55.0k jobs/s).
 constexpr int32_t kCompPxJob = 4;
 constexpr int32_t kCompPxBatch = 8;
 
